@@ -1,0 +1,563 @@
+/*
+ * olref.c -- CPU restatement of OpenOL's diffraction / CGH path.
+ * TEST INFRASTRUCTURE ONLY (the checker); see olref.h for scope and the list of
+ * deliberate divergences.  Every routine cites the reference lines it follows.
+ */
+#include "olref.h"
+#include <complex.h>
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef float complex cf;
+typedef double complex cd;
+
+static int is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* int window limit from floor(x); "no window" when it does not fit an int */
+static int64_t win_lim(double v) {
+    double f = floor(v);
+    if (!(f == f) || f >= (double)INT_MAX) return (int64_t)INT_MAX;
+    if (f <= (double)INT_MIN) return (int64_t)INT_MIN;
+    return (int64_t)f;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FFT: radix-2 iterative (power-of-two lengths) or direct DFT (other lengths). */
+/* Twiddles are computed in double and rounded to the working precision.       */
+/* ------------------------------------------------------------------------- */
+#define DEFINE_FFT(SUF, CT, RT, COSF, SINF)                                         \
+static CT* twiddles_##SUF(int64_t n, int sign) {                                   \
+    CT* tw = (CT*)malloc(sizeof(CT) * (size_t)(n > 1 ? n : 1));                     \
+    for (int64_t k = 0; k < n; k++) {                                               \
+        double a = 2.0 * M_PI * (double)k / (double)n;                              \
+        tw[k] = (RT)cos(a) + I * (RT)(sign * sin(a));                               \
+    }                                                                               \
+    return tw;                                                                      \
+}                                                                                   \
+static void fft1d_##SUF(CT* a, int64_t n, const CT* tw, CT* tmp) {                 \
+    if (n <= 1) return;                                                             \
+    if (!is_pow2(n)) { /* direct DFT, exact index reduction */                      \
+        for (int64_t k = 0; k < n; k++) {                                           \
+            CT s = 0;                                                               \
+            for (int64_t j = 0; j < n; j++) s += a[j] * tw[(j * k) % n];            \
+            tmp[k] = s;                                                             \
+        }                                                                           \
+        memcpy(a, tmp, sizeof(CT) * (size_t)n);                                     \
+        return;                                                                     \
+    }                                                                               \
+    for (int64_t i = 1, j = 0; i < n; i++) { /* bit reversal */                     \
+        int64_t bit = n >> 1;                                                       \
+        for (; j & bit; bit >>= 1) j ^= bit;                                        \
+        j ^= bit;                                                                   \
+        if (i < j) { CT t = a[i]; a[i] = a[j]; a[j] = t; }                          \
+    }                                                                               \
+    for (int64_t len = 2; len <= n; len <<= 1) {                                    \
+        int64_t half = len >> 1, step = n / len;                                    \
+        for (int64_t i = 0; i < n; i += len) {                                      \
+            for (int64_t k = 0; k < half; k++) {                                    \
+                CT w = tw[k * step];                                                \
+                CT u = a[i + k], v = a[i + k + half] * w;                           \
+                a[i + k] = u + v;                                                   \
+                a[i + k + half] = u - v;                                            \
+            }                                                                       \
+        }                                                                           \
+    }                                                                               \
+}                                                                                   \
+static int fft2d_##SUF(CT* d, int64_t ny, int64_t nx, int sign) {                  \
+    CT* twx = twiddles_##SUF(nx, sign);                                             \
+    CT* twy = twiddles_##SUF(ny, sign);                                             \
+    _Pragma("omp parallel")                                                         \
+    {                                                                               \
+        CT* tmp = (CT*)malloc(sizeof(CT) * (size_t)(nx > ny ? nx : ny) * 17);       \
+        _Pragma("omp for schedule(static)")                                         \
+        for (int64_t r = 0; r < ny; r++) fft1d_##SUF(d + r * nx, nx, twx, tmp);      \
+        /* columns in blocks of 16 to stay cache friendly */                        \
+        _Pragma("omp for schedule(static)")                                         \
+        for (int64_t c0 = 0; c0 < nx; c0 += 16) {                                   \
+            int64_t cb = nx - c0 < 16 ? nx - c0 : 16;                               \
+            CT* col = tmp + ny;                                                     \
+            for (int64_t r = 0; r < ny; r++)                                        \
+                for (int64_t c = 0; c < cb; c++) col[c * ny + r] = d[r * nx + c0 + c];\
+            for (int64_t c = 0; c < cb; c++) fft1d_##SUF(col + c * ny, ny, twy, tmp); \
+            for (int64_t r = 0; r < ny; r++)                                        \
+                for (int64_t c = 0; c < cb; c++) d[r * nx + c0 + c] = col[c * ny + r];\
+        }                                                                           \
+        free(tmp);                                                                  \
+    }                                                                               \
+    free(twx); free(twy);                                                           \
+    return 0;                                                                       \
+}
+
+DEFINE_FFT(c64, cf, float, cosf, sinf)
+DEFINE_FFT(c128, cd, double, cos, sin)
+
+int olref_fft2d_c64(float* data, int64_t ny, int64_t nx, int sign) {
+    if (!data || ny <= 0 || nx <= 0) return -1;
+    return fft2d_c64((cf*)data, ny, nx, sign < 0 ? -1 : 1);
+}
+int olref_fft2d_c128(double* data, int64_t ny, int64_t nx, int sign) {
+    if (!data || ny <= 0 || nx <= 0) return -1;
+    return fft2d_c128((cd*)data, ny, nx, sign < 0 ? -1 : 1);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Array helpers: include/olutils.h:95-198                                    */
+/* ------------------------------------------------------------------------- */
+#define DEFINE_HELPERS(SUF, CT)                                                     \
+/* zeropadding(src,dst,in_h,in_w,out_h,out_w), olutils.h:155-178 */                 \
+static CT* pad_##SUF(const CT* in, int64_t ih, int64_t iw, int64_t oh, int64_t ow) {\
+    int64_t yl = (oh - ih) / 2, xl = (ow - iw) / 2;                                 \
+    int64_t yr = (oh - ih + 1) / 2, xr = (ow - iw + 1) / 2;                         \
+    CT* out = (CT*)calloc((size_t)(oh * ow), sizeof(CT));                           \
+    for (int64_t h = 0; h < oh; h++)                                                \
+        for (int64_t w = 0; w < ow; w++)                                            \
+            if (yl <= h && h < oh - yr && xl <= w && w < ow - xr)                   \
+                out[h * ow + w] = in[(h - yl) * iw + w - xl];                       \
+    return out;                                                                     \
+}                                                                                   \
+/* del_zero(in,out,in_h,in_w), olutils.h:181-198: keep [h/4,3h/4) x [w/4,3w/4) */    \
+static void crop_##SUF(const CT* in, CT* out, int64_t ih, int64_t iw) {            \
+    for (int64_t n = ih / 4; n < ih * 3 / 4; n++)                                   \
+        for (int64_t m = iw / 4; m < iw * 3 / 4; m++)                               \
+            out[m - iw / 4 + (n - ih / 4) * (iw / 2)] = in[m + n * iw];             \
+}                                                                                   \
+/* fftshift, olutils.h:95-117 (quadrant swap) */                                    \
+static void fftshift_##SUF(CT* u, int64_t height, int64_t width) {                 \
+    int64_t hh = height / 2, hw = width / 2;                                        \
+    for (int64_t h = 0; h < hh; h++)                                                \
+        for (int64_t w = 0; w < hw; w++) {                                          \
+            int64_t i1 = h * width + w, i2 = (h + hh) * width + (w + hw);           \
+            CT t = u[i1]; u[i1] = u[i2]; u[i2] = t;                                 \
+            i1 = h * width + (w + hw); i2 = (h + hh) * width + w;                   \
+            t = u[i1]; u[i1] = u[i2]; u[i2] = t;                                    \
+        }                                                                           \
+}
+
+DEFINE_HELPERS(c64, cf)
+DEFINE_HELPERS(c128, cd)
+
+/* ------------------------------------------------------------------------- */
+/* Angular spectrum: AsmProp olprop.h:227-248, AsmTransferF :172-189,        */
+/* band-limit window shiftedAsmTransferF :250-305 evaluated at ox = oy = 0.   */
+/* ------------------------------------------------------------------------- */
+static void asm_transfer(cd* H, int64_t ny, int64_t nx, double dv, double du,
+                         double lambda, double z, unsigned flags) {
+    double ulim = INFINITY, vlim = INFINITY;
+    if (flags & OLREF_BANDLIMIT) {
+        /* ulm = 1/sqrt(pow(ox - 0.5/du,-2) z^2 + 1)/lambda, ulp likewise; at
+         * ox = 0 the branch -sx <= ox < sx gives u0 = 0, uw = ulp + ulm */
+        double ulm = 1.0 / sqrt(pow(0.0 - 0.5f / du, -2) * z * z + 1.0f) / lambda;
+        double ulp = 1.0 / sqrt(pow(0.0 + 0.5f / du, -2) * z * z + 1.0f) / lambda;
+        double vlm = 1.0 / sqrt(pow(0.0 - 0.5f / dv, -2) * z * z + 1.0f) / lambda;
+        double vlp = 1.0 / sqrt(pow(0.0 + 0.5f / dv, -2) * z * z + 1.0f) / lambda;
+        ulim = ulp + ulm; /* condition |u - u0| * 2 < uw */
+        vlim = vlp + vlm;
+    }
+    double tmp = 1.0 / (lambda * lambda);
+    #pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < ny; n++) {
+        double v = (double)(n - ny / 2) * dv;
+        for (int64_t m = 0; m < nx; m++) {
+            int64_t idx = m + n * nx;
+            double u = (double)(m - nx / 2) * du;
+            double arg = tmp - u * u - v * v;
+            if (arg < 0.0 || !(fabs(u) * 2 < ulim && fabs(v) * 2 < vlim)) {
+                H[idx] = 0;
+                continue;
+            }
+            double w = sqrt(arg);
+            double phase = (flags & OLREF_BANDLIMIT) ? 2 * M_PI * (w * z + 0.0 * u + 0.0 * v)
+                                                     : 2 * M_PI * w * z;
+            double s = sin(phase);
+            if (flags & OLREF_ADJOINT) s = -s;
+            H[idx] = cos(phase) + I * s;
+        }
+    }
+}
+
+/* PREC_T = float emulation of AsmTransferF (informational variant) */
+static void asm_transfer_f32(cd* H, int64_t ny, int64_t nx, float dv, float du,
+                             float lambda, float z, unsigned flags) {
+    float tmp = 1 / (lambda * lambda);
+    #pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < ny; n++) {
+        float v = (float)(n - ny / 2) * dv;
+        for (int64_t m = 0; m < nx; m++) {
+            float u = (float)(m - nx / 2) * du;
+            float arg = tmp - u * u - v * v;
+            if (arg < 0) { H[m + n * nx] = 0; continue; }
+            float w = sqrtf(arg);
+            float phase = (float)(2 * M_PI * w * z);
+            float s = sinf(phase);
+            if (flags & OLREF_ADJOINT) s = -s;
+            H[m + n * nx] = cosf(phase) + I * s;
+        }
+    }
+}
+
+#define DEFINE_ASM(SUF, CT, RT)                                                     \
+int olref_asm_##SUF(RT* uin, int64_t ny, int64_t nx, double dy, double dx,          \
+                    double lambda, double z, unsigned flags) {                      \
+    if (!uin || ny <= 0 || nx <= 0 || !(lambda > 0) || !(dx > 0) || !(dy > 0))      \
+        return -1;                                                                  \
+    CT* u = (CT*)uin;                                                               \
+    int64_t ny2 = ny * 2, nx2 = nx * 2;                                             \
+    CT* buf = pad_##SUF(u, ny, nx, ny2, nx2);                                       \
+    cd* Hd = (cd*)malloc(sizeof(cd) * (size_t)(ny2 * nx2));                         \
+    if (flags & OLREF_PREC_F32) {                                                   \
+        float du = 1 / ((float)dx * nx2), dv = 1 / ((float)dy * ny2);               \
+        asm_transfer_f32(Hd, ny2, nx2, dv, du, (float)lambda, (float)z, flags);     \
+    } else {                                                                        \
+        double du = 1 / (dx * nx2), dv = 1 / (dy * ny2);                            \
+        asm_transfer(Hd, ny2, nx2, dv, du, lambda, z, flags);                       \
+    }                                                                               \
+    CT* H = (CT*)malloc(sizeof(CT) * (size_t)(ny2 * nx2));                          \
+    for (int64_t i = 0; i < ny2 * nx2; i++) H[i] = (CT)Hd[i];                       \
+    free(Hd);                                                                       \
+    fftshift_##SUF(H, ny2, nx2);                                                    \
+    fft2d_##SUF(buf, ny2, nx2, -1);                                                 \
+    RT sc = (RT)(1.0 / (double)(ny2 * nx2));                                        \
+    for (int64_t i = 0; i < ny2 * nx2; i++) { buf[i] = buf[i] * H[i]; buf[i] *= sc; }\
+    fft2d_##SUF(buf, ny2, nx2, +1);                                                 \
+    crop_##SUF(buf, u, ny2, nx2);                                                   \
+    free(H); free(buf);                                                             \
+    return 0;                                                                       \
+}
+
+DEFINE_ASM(c64, cf, float)
+DEFINE_ASM(c128, cd, double)
+
+/* ------------------------------------------------------------------------- */
+/* Fresnel: FresnelProp olprop.h:125-148 with FresnelResponse :8-24;         */
+/* FresnelPropBandLimit :150-170 with FresnelResponseBandLimit :26-54;        */
+/* GPU band limit (x-limit on both axes) golprop.h:201-238.                   */
+/* ------------------------------------------------------------------------- */
+#define DEFINE_FRESNEL(SUF, CT, RT)                                                 \
+int olref_fresnel_##SUF(RT* uin, int64_t ny, int64_t nx, double dy, double dx,      \
+                        double lambda, double z, unsigned flags) {                  \
+    if (!uin || ny <= 0 || nx <= 0 || !(lambda > 0) || z == 0) return -1;            \
+    CT* u = (CT*)uin;                                                               \
+    CT* buf = pad_##SUF(u, ny, nx, ny * 2, nx * 2);                                 \
+    int64_t height = ny * 2, width = nx * 2;                                        \
+    CT* h = (CT*)malloc(sizeof(CT) * (size_t)(height * width));                     \
+    double tmp = 1 / (lambda * z);                                                  \
+    int64_t hh = height / 2, hw = width / 2;                                        \
+    int64_t wlim = INT_MAX, hlim = INT_MAX;                                         \
+    if (flags & OLREF_FRESNEL_BL_CPU) {                                             \
+        double xlim = fabs(lambda * z * 0.5 / dx), ylim = fabs(lambda * z * 0.5 / dy);\
+        wlim = win_lim(xlim / dx); hlim = win_lim(ylim / dy);                       \
+    } else if (flags & OLREF_FRESNEL_BL_GPU) {                                      \
+        double xlim = fabs(lambda * z * 0.5 / dx);                                  \
+        wlim = hlim = win_lim(xlim / dx);                                           \
+    }                                                                               \
+    for (int64_t n = 0; n < height; n++) {                                          \
+        double y = ((double)(n - hh)) * dy;                                         \
+        for (int64_t m = 0; m < width; m++) {                                       \
+            int64_t idx = n * width + m;                                            \
+            if (llabs(m - hw) < wlim && llabs(n - hh) < hlim) {                     \
+                double x = ((double)(m - hw)) * dx;                                 \
+                double phase = M_PI * (x * x + y * y) * tmp;                        \
+                double s = sin(phase);                                              \
+                if (flags & OLREF_ADJOINT) s = -s;                                  \
+                h[idx] = (RT)cos(phase) + I * (RT)s;                                \
+            } else h[idx] = 0;                                                      \
+        }                                                                           \
+    }                                                                               \
+    fftshift_##SUF(h, height, width);                                               \
+    fft2d_##SUF(buf, height, width, -1);                                            \
+    fft2d_##SUF(h, height, width, -1);                                              \
+    RT sc = (RT)(1.0 / (double)(height * width));                                   \
+    for (int64_t i = 0; i < height * width; i++) {                                  \
+        buf[i] *= sc; h[i] *= sc; buf[i] = buf[i] * h[i];                           \
+    }                                                                               \
+    fft2d_##SUF(buf, height, width, +1);                                            \
+    crop_##SUF(buf, u, height, width);                                              \
+    free(h); free(buf);                                                             \
+    return 0;                                                                       \
+}
+
+DEFINE_FRESNEL(c64, cf, float)
+DEFINE_FRESNEL(c128, cd, double)
+
+/* ------------------------------------------------------------------------- */
+/* ARSS: ARSSFresnelProp golprop.h:941-989 with stages _u :816-859,          */
+/* _h :861-898 and _Cz :900-938.                                             */
+/* ------------------------------------------------------------------------- */
+#define DEFINE_ARSS(SUF, CT, RT)                                                    \
+int olref_arss_##SUF(const RT* srcin, RT* dstin, int64_t ny, int64_t nx, double dy, \
+                     double dx, double lambda, double z, double s, double oy,       \
+                     double ox, unsigned flags) {                                   \
+    (void)flags;                                                                    \
+    if (!srcin || !dstin || ny <= 0 || nx <= 0 || !(lambda > 0) || z == 0 || s == 0)\
+        return -1;                                                                  \
+    const CT* src = (const CT*)srcin;                                               \
+    CT* dst = (CT*)dstin;                                                           \
+    int64_t ny2 = ny * 2, nx2 = nx * 2;                                             \
+    /* stage u: pre-chirp inside window, zero outside */                            \
+    {                                                                               \
+        double xlim = lambda * fabs(z) * 0.5 / (s * fabs(s - 1) * dx);              \
+        int64_t wlim = win_lim(xlim / dx);                                          \
+        double ylim = lambda * fabs(z) * 0.5 / (s * fabs(s - 1) * dy);              \
+        int64_t hlim = win_lim(ylim / dy);                                          \
+        int64_t hny = ny / 2, hnx = nx / 2;                                         \
+        for (int64_t h = 0; h < ny; h++)                                            \
+            for (int64_t w = 0; w < nx; w++) {                                      \
+                int64_t idx = h * nx + w;                                           \
+                if (llabs(w - hnx) >= wlim || llabs(h - hny) >= hlim) { dst[idx] = 0; continue; }\
+                double tmp = M_PI / (lambda * z);                                   \
+                double y1 = (double)(h - hny) * dy;                                 \
+                double phiu_y = (s * s - s) * y1 * y1 - 2 * s * oy * y1;            \
+                double x1 = (double)(w - hnx) * dx;                                 \
+                double phiu = (s * s - s) * x1 * x1 - 2 * s * ox * x1;              \
+                phiu += phiu_y;                                                     \
+                phiu *= tmp;                                                        \
+                dst[idx] = src[idx] * ((RT)cos(phiu) + I * (RT)sin(phiu));          \
+            }                                                                       \
+    }                                                                               \
+    CT* buf1 = pad_##SUF(dst, ny, nx, ny2, nx2);                                    \
+    fft2d_##SUF(buf1, ny2, nx2, -1);                                                \
+    RT sc = (RT)(1.0f / (float)(ny2 * nx2));                                        \
+    for (int64_t i = 0; i < ny2 * nx2; i++) buf1[i] *= sc;                          \
+    /* stage h: kernel chirp inside a square window of the x-limit */               \
+    CT* buf2 = (CT*)malloc(sizeof(CT) * (size_t)(ny2 * nx2));                       \
+    {                                                                               \
+        double xlim = fabs(lambda * z * 0.5 / (dx * s));                            \
+        int64_t lim = win_lim(xlim / dx);                                           \
+        int64_t hny = ny2 / 2, hnx = nx2 / 2;                                       \
+        for (int64_t h = 0; h < ny2; h++)                                           \
+            for (int64_t w = 0; w < nx2; w++) {                                     \
+                int64_t idx = h * nx2 + w;                                          \
+                if (llabs(w - hnx) >= lim || llabs(h - hny) >= lim) { buf2[idx] = 0; continue; }\
+                double tmp = M_PI / (lambda * z);                                   \
+                double y1 = (double)(h - hny) * dy;                                 \
+                double x1 = (double)(w - hnx) * dx;                                 \
+                double phih_y = s * y1 * y1;                                        \
+                double phih = s * x1 * x1 + phih_y;                                 \
+                phih *= tmp;                                                        \
+                buf2[idx] = (RT)cos(phih) + I * (RT)sin(phih);                      \
+            }                                                                       \
+    }                                                                               \
+    fftshift_##SUF(buf2, ny2, nx2);                                                 \
+    fft2d_##SUF(buf2, ny2, nx2, -1);                                                \
+    for (int64_t i = 0; i < ny2 * nx2; i++) { buf2[i] *= sc; buf1[i] = buf1[i] * buf2[i]; }\
+    free(buf2);                                                                     \
+    fft2d_##SUF(buf1, ny2, nx2, +1);                                                \
+    crop_##SUF(buf1, dst, ny2, nx2);                                                \
+    free(buf1);                                                                     \
+    /* stage Cz: post-chirp only inside its window, outside left unmodified */      \
+    {                                                                               \
+        double xlim = lambda * fabs(z) * 0.5 / (fabs(1 - s) * dx);                  \
+        int64_t wlim = win_lim(xlim / dx);                                          \
+        double ylim = lambda * fabs(z) * 0.5 / (fabs(1 - s) * dy);                  \
+        int64_t hlim = win_lim(ylim / dy);                                          \
+        int64_t hny = ny / 2, hnx = nx / 2;                                         \
+        for (int64_t h = 0; h < ny; h++)                                            \
+            for (int64_t w = 0; w < nx; w++) {                                      \
+                if (!(llabs(w - hnx) < wlim && llabs(h - hny) < hlim)) continue;    \
+                int64_t idx = h * nx + w;                                           \
+                double tmp = M_PI / (lambda * z);                                   \
+                double y2 = (double)(h - hny) * dy;                                 \
+                double phic_y = ((1 - s) * y2 * y2 + 2 * oy * y2 + oy * oy);        \
+                double x2 = (double)(w - hnx) * dx;                                 \
+                double phic = ((1 - s) * x2 * x2 + 2 * ox * x2 + ox * ox);          \
+                phic += phic_y;                                                     \
+                phic *= tmp;                                                        \
+                dst[idx] = dst[idx] * ((RT)cos(phic) + I * (RT)sin(phic));          \
+            }                                                                       \
+    }                                                                               \
+    return 0;                                                                       \
+}
+
+DEFINE_ARSS(c64, cf, float)
+DEFINE_ARSS(c128, cd, double)
+
+/* ------------------------------------------------------------------------- */
+/* Dispatcher and checks: olprop.h:109-123, :209-225, :656-682                */
+/* ------------------------------------------------------------------------- */
+int olref_asm_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda,
+                         double z) {
+    int64_t nx2 = 2 * nx, ny2 = 2 * ny;
+    double du = 1 / (dx * nx2), dv = 1 / (dy * ny2);
+    double umax = du * nx2 / 2, vmax = dv * ny2 / 2;
+    double tmpsqrt = sqrt(1 / (lambda * lambda) - umax * umax - vmax * vmax);
+    double zmax_u = tmpsqrt / (2 * umax * du), zmax_v = tmpsqrt / (2 * vmax * dv);
+    if (zmax_u < fabs(z) || zmax_v < fabs(z)) return 0;
+    return 1;
+}
+
+int olref_fresnel_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda,
+                             double z) {
+    int64_t nx2 = 2 * nx, ny2 = 2 * ny;
+    double xmax = dx * nx2 / 2, ymax = dy * ny2 / 2;
+    double zmin_x = 2 * xmax * dx / lambda, zmin_y = 2 * ymax * dy / lambda;
+    if (zmin_x > fabs(z) || zmin_y > fabs(z)) return 0;
+    return 1;
+}
+
+int olref_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx, double lambda,
+                   double z, int mode, int* chosen) {
+    int c;
+    if (mode == 1) c = 1;
+    else if (mode == 0) c = 0;
+    else if (mode == 2) {
+        if (olref_asm_prop_check(ny, nx, dy, dx, lambda, z)) c = 0;
+        else if (!olref_fresnel_prop_check(ny, nx, dy, dx, lambda, z)) c = 2;
+        else c = 1;
+    } else return -1;
+    if (chosen) *chosen = c;
+    if (c == 0) return olref_asm_c64(u, ny, nx, dy, dx, lambda, z, 0);
+    if (c == 1) return olref_fresnel_c64(u, ny, nx, dy, dx, lambda, z, 0);
+    return olref_fresnel_c64(u, ny, nx, dy, dx, lambda, z, OLREF_FRESNEL_BL_CPU);
+}
+
+/* ------------------------------------------------------------------------- */
+/* CGH: _CghFresnel olcgh.h:296-313 (complex out), :275-295 (phase out).     */
+/* Extension (flag): exact spherical r; amp/phase per point.                  */
+/* ------------------------------------------------------------------------- */
+static inline double cgh_theta(double dxv, double dyv, double dz, double lambda,
+                               unsigned flags) {
+    if (flags & OLREF_CGH_EXACT_R)
+        return 2 * M_PI * sqrt(dxv * dxv + dyv * dyv + dz * dz) / lambda;
+    return 2 * M_PI * (dxv * dxv + dyv * dyv) / (2 * dz * lambda);
+}
+
+int olref_cgh_c64(const double* x, const double* y, const double* z, const double* amp,
+                  const double* phase, int64_t n, float* Uin, int64_t ny, int64_t nx,
+                  double py, double px, double lambda, unsigned flags) {
+    if (!x || !y || !z || !Uin || n < 0 || ny <= 0 || nx <= 0) return -1;
+    cf* U = (cf*)Uin;
+    for (int64_t h = 0; h < ny; h++) {
+        #pragma omp parallel for schedule(static)
+        for (int64_t w = 0; w < nx; w++) {
+            cf acc = 0;
+            for (int64_t j = 0; j < n; j++) {
+                double dxv, dyv, theta;
+                if (flags & OLREF_PREC_F32) {
+                    float fdx = (float)px * (float)w - (float)x[j];
+                    float fdy = (float)py * (float)h - (float)y[j];
+                    float fdz = (float)z[j];
+                    float th = (float)(2 * M_PI * (fdx * fdx + fdy * fdy) / (2 * fdz * (float)lambda));
+                    acc += (cf)(cosf(th) + I * sinf(th));
+                    continue;
+                }
+                dxv = px * w - x[j];
+                dyv = py * h - y[j];
+                theta = cgh_theta(dxv, dyv, z[j], lambda, flags);
+                if (phase) theta += phase[j];
+                double a = amp ? amp[j] : 1.0;
+                cd t = a * cos(theta) + I * (a * sin(theta));
+                acc += (cf)t; /* complex<float> += complex<double> */
+            }
+            U[h * nx + w] = acc;
+        }
+    }
+    return 0;
+}
+
+int olref_cgh_c128(const double* x, const double* y, const double* z, const double* amp,
+                   const double* phase, int64_t n, double* Uin, int64_t ny, int64_t nx,
+                   double py, double px, double lambda, unsigned flags) {
+    if (!x || !y || !z || !Uin || n < 0 || ny <= 0 || nx <= 0) return -1;
+    cd* U = (cd*)Uin;
+    #pragma omp parallel for schedule(static) collapse(2)
+    for (int64_t h = 0; h < ny; h++)
+        for (int64_t w = 0; w < nx; w++) {
+            cd acc = 0;
+            for (int64_t j = 0; j < n; j++) {
+                double theta = cgh_theta(px * w - x[j], py * h - y[j], z[j], lambda, flags);
+                if (phase) theta += phase[j];
+                double a = amp ? amp[j] : 1.0;
+                acc += a * cos(theta) + I * (a * sin(theta));
+            }
+            U[h * nx + w] = acc;
+        }
+    return 0;
+}
+
+int olref_cgh_pixels_c128(const double* x, const double* y, const double* z,
+                          const double* amp, const double* phase, int64_t n,
+                          const int64_t* ph, const int64_t* pw, int64_t npix, double* out,
+                          double py, double px, double lambda, unsigned flags) {
+    if (!x || !y || !z || !ph || !pw || !out || n < 0 || npix < 0) return -1;
+    cd* o = (cd*)out;
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = 0; i < npix; i++) {
+        cd acc = 0;
+        for (int64_t j = 0; j < n; j++) {
+            double theta = cgh_theta(px * pw[i] - x[j], py * ph[i] - y[j], z[j], lambda, flags);
+            if (phase) theta += phase[j];
+            double a = amp ? amp[j] : 1.0;
+            acc += a * cos(theta) + I * (a * sin(theta));
+        }
+        o[i] = acc;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* WRP step 1: WRPStep1Pre / WRPStep1 golcgh.h:291-367 (circle, float        */
+/* accumulation into U) and WRPStep1Pre_D / WRPStep1_D :391-468 (square,     */
+/* PREC_T accumulation).                                                     */
+/* ------------------------------------------------------------------------- */
+#define DEFINE_WRP(SUF, CT, RT)                                                     \
+int olref_wrp_##SUF(const double* x, const double* y, const double* z, int64_t n,   \
+                    RT* Uin, int64_t ny, int64_t nx, double p, double lambda,       \
+                    double plane_z, double p_limit, unsigned flags) {               \
+    if (!x || !y || !z || !Uin || n < 0 || ny <= 0 || nx <= 0) return -1;           \
+    CT* U = (CT*)Uin;                                                               \
+    double* W = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));          \
+    double rt2 = (double)sqrtf(2.0f);                                               \
+    for (int64_t j = 0; j < n; j++) {                                               \
+        double Wj = 1 / (sqrt(4 * pow((p_limit / lambda), 2) - 1)) * fabs(z[j] - plane_z);\
+        W[j] = (flags & OLREF_WRP_SQUARE) ? Wj * rt2 : Wj;                          \
+    }                                                                               \
+    _Pragma("omp parallel for schedule(static) collapse(2)")                        \
+    for (int64_t h = 0; h < ny; h++)                                                \
+        for (int64_t w = 0; w < nx; w++) {                                          \
+            CT acc = 0;                                                             \
+            cd accd = 0;                                                            \
+            for (int64_t j = 0; j < n; j++) {                                       \
+                double dxv = p * w - x[j], dyv = p * h - y[j], dz = z[j] - plane_z; \
+                double tmp = dxv * dxv + dyv * dyv;                                 \
+                if (flags & OLREF_WRP_SQUARE) {                                     \
+                    if (fabs(dxv) * 2 <= W[j] && fabs(dyv) * 2 <= W[j]) {           \
+                        double th = 2 * M_PI * tmp / (2 * dz * lambda);             \
+                        accd += (cos(th) + I * sin(th)) / fabs(dz);                 \
+                    }                                                               \
+                } else if (tmp < W[j] * W[j]) {                                     \
+                    double th = 2 * M_PI * tmp / (2 * dz * lambda);                 \
+                    acc += (CT)((cos(th) + I * sin(th)) / fabs(dz));                \
+                }                                                                   \
+            }                                                                       \
+            U[h * nx + w] = (flags & OLREF_WRP_SQUARE) ? (CT)accd : acc;            \
+        }                                                                           \
+    free(W);                                                                        \
+    return 0;                                                                       \
+}
+
+DEFINE_WRP(c64, cf, float)
+DEFINE_WRP(c128, cd, double)
+
+double olref_wrp_plane_z(int64_t ny, int64_t nx, double p, double lambda, double zmin,
+                         double p_limit, float s, int square) {
+    float plane_z;
+    if (s < 0) {
+        if (square) { /* WRPMethod_D, golcgh.h:546-550 */
+            float D = (float)(nx * p * 0.05);
+            float d = (float)(sqrt(4 * pow((p_limit / lambda), 2) - 1) / sqrt(2) * D);
+            plane_z = (float)(zmin - d);
+        } else { /* WRPMethod, golcgh.h:567-571 */
+            float Wpow2 = (float)(nx * ny * p * p / M_PI * 1 / 1024.f);
+            float d = (float)sqrt((4 * pow((p / lambda), 2) - 1) * Wpow2);
+            plane_z = (float)(zmin - d);
+        }
+    } else {
+        plane_z = (float)(zmin * s);
+    }
+    return (double)plane_z;
+}

@@ -1,0 +1,110 @@
+/*
+ * olref.h -- CPU restatement of OpenOL's diffraction / CGH path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by
+ * nothing else.  The product path (openol_amd/, include/) never links or calls it.
+ *
+ * Each function restates the reference algorithm named in its comment, keeping
+ * the reference's expression order with PREC_T = double (coordinates/phases)
+ * and COMPLEX_T = float ("c64") or double ("c128").  The FFT is this file's own
+ * radix-2 FFT (FFTW, which the reference links, is not installed here): forward
+ * uses e^{-i}, backward e^{+i}, both unnormalised (FFTW convention,
+ * include/olutils.h:27,52 in the reference).
+ *
+ * Deliberate divergences from the reference (shared with the HIP library):
+ *   - evanescent angular-spectrum components (1/lambda^2 - u^2 - v^2 < 0) are set
+ *     to zero; the reference takes sqrt of a negative number and produces NaN
+ *     (include/olprop.h:184).
+ *   - window limits that overflow int (e.g. ARSS with s == 1) mean "no window";
+ *     the reference converts +inf to int (undefined behaviour).
+ *   - CGH / WRP phases use the double-precision pi, not F32_PI.
+ */
+#ifndef OLREF_H
+#define OLREF_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* flags shared by the propagation routines */
+#define OLREF_BANDLIMIT   0x1u   /* Matsushima band limit (olprop.h:250-305 at ox=oy=0) */
+#define OLREF_ADJOINT     0x2u   /* conjugate transfer function (adjoint operator) */
+#define OLREF_PREC_F32    0x4u   /* emulate PREC_T=float coordinate/phase arithmetic */
+/* Fresnel band-limit variants */
+#define OLREF_FRESNEL_BL_CPU 0x10u /* per-axis limits, olprop.h:26-54 */
+#define OLREF_FRESNEL_BL_GPU 0x20u /* x-limit on both axes, golprop.h:223-238 */
+/* CGH flags */
+#define OLREF_CGH_EXACT_R 0x100u  /* exact spherical r instead of the Fresnel quadratic */
+/* WRP flags */
+#define OLREF_WRP_SQUARE  0x200u  /* square window D = sqrt(2) W (golcgh.h:391-468) */
+
+/* 2-D FFT, in place, row-major ny x nx, sign -1 forward / +1 backward, unnormalised */
+int olref_fft2d_c64(float* data, int64_t ny, int64_t nx, int sign);
+int olref_fft2d_c128(double* data, int64_t ny, int64_t nx, int sign);
+
+/* AsmProp, include/olprop.h:227-248 (+ AsmTransferF :172-189, band limit :250-305) */
+int olref_asm_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,
+                  double lambda, double z, unsigned flags);
+int olref_asm_c128(double* u, int64_t ny, int64_t nx, double dy, double dx,
+                   double lambda, double z, unsigned flags);
+
+/* FresnelProp / FresnelPropBandLimit, include/olprop.h:125-170 (+ :8-54);
+ * GPU band-limit variant golprop.h:258-302 */
+int olref_fresnel_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,
+                      double lambda, double z, unsigned flags);
+int olref_fresnel_c128(double* u, int64_t ny, int64_t nx, double dy, double dx,
+                       double lambda, double z, unsigned flags);
+
+/* ARSSFresnelProp, include/golprop.h:816-995 (src and dst may alias) */
+int olref_arss_c64(const float* src, float* dst, int64_t ny, int64_t nx, double dy,
+                   double dx, double lambda, double z, double s, double oy,
+                   double ox, unsigned flags);
+int olref_arss_c128(const double* src, double* dst, int64_t ny, int64_t nx, double dy,
+                    double dx, double lambda, double z, double s, double oy,
+                    double ox, unsigned flags);
+
+/* Prop dispatcher, include/olprop.h:656-682; returns chosen mode in *chosen
+ * (0 ASM, 1 FRESNEL, 2 FRESNEL band-limited) */
+int olref_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,
+                   double lambda, double z, int mode, int* chosen);
+int olref_asm_prop_check(int64_t ny, int64_t nx, double dy, double dx,
+                         double lambda, double z);       /* olprop.h:209-225 */
+int olref_fresnel_prop_check(int64_t ny, int64_t nx, double dy, double dx,
+                             double lambda, double z);   /* olprop.h:109-123 */
+
+/* Point-cloud CGH, include/olcgh.h:296-313 (complex out).  Points are SoA;
+ * amp / phase may be NULL (reference behaviour: A = 1, phi = 0).
+ * c64: float accumulation as complex<float> += complex<double>; c128: double. */
+int olref_cgh_c64(const double* x, const double* y, const double* z,
+                  const double* amp, const double* phase, int64_t n,
+                  float* U, int64_t ny, int64_t nx, double py, double px,
+                  double lambda, unsigned flags);
+int olref_cgh_c128(const double* x, const double* y, const double* z,
+                   const double* amp, const double* phase, int64_t n,
+                   double* U, int64_t ny, int64_t nx, double py, double px,
+                   double lambda, unsigned flags);
+/* same sum evaluated only at a list of pixels (h[i], w[i]) -- for sampled checks
+ * at configuration size; FP64 throughout */
+int olref_cgh_pixels_c128(const double* x, const double* y, const double* z,
+                          const double* amp, const double* phase, int64_t n,
+                          const int64_t* ph, const int64_t* pw, int64_t npix,
+                          double* out, double py, double px, double lambda,
+                          unsigned flags);
+
+/* WRP step 1, include/golcgh.h:291-378 (circle) and :391-491 (square, flag);
+ * p_limit is the pitch used for the window (p for the plain overloads). */
+int olref_wrp_c64(const double* x, const double* y, const double* z, int64_t n,
+                  float* U, int64_t ny, int64_t nx, double p, double lambda,
+                  double plane_z, double p_limit, unsigned flags);
+int olref_wrp_c128(const double* x, const double* y, const double* z, int64_t n,
+                   double* U, int64_t ny, int64_t nx, double p, double lambda,
+                   double plane_z, double p_limit, unsigned flags);
+/* WRPMethod(_D) plane position rule, golcgh.h:519-581 (returns float value) */
+double olref_wrp_plane_z(int64_t ny, int64_t nx, double p, double lambda,
+                         double zmin, double p_limit, float s, int square);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
