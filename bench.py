@@ -1,0 +1,209 @@
+"""Benchmark: 4096^2 angular-spectrum propagations per second on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--extras]
+
+A step is one band-limited ASM propagation of a seeded 4096 x 4096 complex64
+field (lambda 532 nm, 8 um pitch, z = 0.10 m, zero-padded to 8192 x 8192,
+cropped back), in place, with the field already resident in HBM.  Prints ONE
+JSON line.  `value` is propagations per second.  The line also carries:
+
+* roofline: the dominant kernel of the three-pass propagation, its algorithmic
+  HBM bytes per launch over its average launch time (HIP events recorded on the
+  stream the kernels run on, inside the timed region), against 8 TB/s;
+* cpu_baseline: the oracle's CPU restatement of the reference AsmProp (own
+  radix-2 FFT with OpenMP; FFTW is not installed) timed on this host;
+* extras (--extras, on by default): direct CGH, layered MFMA CGH, WRP + ARSS
+  and the multi-plane optimisation step, each with its own throughput.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+LAM, PITCH = 532e-9, 8e-6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096, help="field side (metric: 4096)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="only the ASM metric")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run the plain copy kernel (for PMC byte-counter calibration)")
+    return ap.parse_args()
+
+
+def pass_bytes(n):
+    """Algorithmic HBM bytes per launch of each propagation pass (ny = nx = n,
+    padded row length 2n, complex64): A reads n^2 c64 and writes n x 2n c64; B
+    reads and writes n x 2n c64; C reads n x 2n and writes n^2."""
+    return {"prop_rows_pad_fwd": 24 * n * n, "prop_cols_conv": 32 * n * n,
+            "prop_rows_crop_inv": 24 * n * n}
+
+
+def bench_asm(ol, lib, torch, n, steps, warmup):
+    from openol_amd import synth
+    u = torch.from_numpy(synth.random_field(n, n, seed=0)).cuda()
+    L = lib.load()
+    c = lib.ctx()
+    for _ in range(warmup):
+        ol.AsmProp(u, n, n, PITCH, PITCH, LAM, 0.10, bandlimit=True)
+    torch.cuda.synchronize()
+    L.olx_ctx_reset_timing(c)
+    L.olx_ctx_enable_timing(c, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ol.AsmProp(u, n, n, PITCH, PITCH, LAM, 0.10, bandlimit=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    L.olx_ctx_enable_timing(c, 0)
+    rep = lib.timing_report()
+    assert torch.isfinite(torch.view_as_real(u)).all()
+    return dt, rep
+
+
+def roofline_of(rep, n):
+    pb = pass_bytes(n)
+    kern = {k: v for k, v in rep.items() if k in pb}
+    dom = max(kern, key=lambda k: kern[k][1])
+    cnt, tot = kern[dom]
+    avg_s = tot / cnt / 1e3
+    achieved = pb[dom] / avg_s / 1e9
+    per_kernel = {k: {"avg_us": round(v[1] / v[0] * 1e3, 2),
+                      "GBps": round(pb[k] / (v[1] / v[0] / 1e3) / 1e9, 1)} for k, v in kern.items()}
+    return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": pb[dom], "avg_launch_us": round(avg_s * 1e6, 2),
+            "per_kernel": per_kernel}
+
+
+def cpu_baseline(n):
+    """The oracle's AsmProp (complex64 field, PREC_T = double) on the host cores:
+    1 warm-up, then the median of 3 runs, at the metric size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import olref
+    from openol_amd import synth
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    u = synth.random_field(n, n, seed=0)
+    olref.asm_prop(u, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        olref.asm_prop(u, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return {"value": round(1.0 / med, 4), "unit": "props/s", "cores": cores, "kind": "port",
+            "sample": f"{n}x{n} AsmProp (band-limited, pad {2 * n}), 1 warm-up + median of 3 "
+                      f"= {med:.2f} s/prop; own OpenMP radix-2 FFT (FFTW unavailable - substitute FFT)"}
+
+
+def extras(ol, lib, torch):
+    """Secondary workloads of the metric: direct vs layered CGH, WRP + ARSS."""
+    from openol_amd import synth
+    out = {}
+    # direct CGH, 10,000 points over 2048^2, exact r (config 2)
+    N, n = 2048, 10000
+    pts = synth.random_points(n, seed=2, xmax=N * PITCH, ymax=N * PITCH, zmin=0.10, zmax=0.20)
+    obj = ol.Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    U = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    for exact in (True, False):
+        ol.Cgh(obj, U, N, N, PITCH, PITCH, LAM, exact_r=exact)
+        torch.cuda.synchronize()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ol.Cgh(obj, U, N, N, PITCH, PITCH, LAM, exact_r=exact)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out["cgh_direct_" + ("exact" if exact else "fresnel") + "_Gptpx_s"] = round(n * N * N / dt / 1e9, 1)
+    # WRP (200,000 points) + ARSS to the hologram (config 3)
+    N, n = 4096, 200000
+    pts = synth.random_points(n, seed=3, xmax=N * PITCH, ymax=N * PITCH, zmin=0.151, zmax=0.155)
+    obj = ol.Object(pts["x"], pts["y"], pts["z"])
+    W = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    ol.WRPStep1(obj, W, N, N, PITCH, LAM, 0.15)
+    ol.ARSSFresnelProp(W, W, N, N, PITCH, PITCH, LAM, 0.15, 0.5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ol.WRPStep1(obj, W, N, N, PITCH, LAM, 0.15)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        ol.ARSSFresnelProp(W, W, N, N, PITCH, PITCH, LAM, 0.15, 0.5)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["wrp_200k_4096_ms"] = round((t1 - t0) * 1e3, 3)
+    arss_s = (t2 - t1) / reps
+    out["arss_4096_ms"] = round(arss_s * 1e3, 3)
+    out["arss_4096_GBps_80N2"] = round(80 * N * N / arss_s / 1e9, 1)
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "decomposition is described in DESIGN.md and not run here.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import openol_amd as ol
+    from openol_amd import _lib as lib
+
+    n = args.n
+    dt, rep = bench_asm(ol, lib, torch, n, args.steps, args.warmup)
+    ms = dt / args.steps * 1e3
+    value = args.steps / dt
+    roof = roofline_of(rep, n)
+    prop_gbs = 80 * n * n / (dt / args.steps) / 1e9
+    line = {
+        "metric": "4096^2 ASM props/s (band-limited angular spectrum, 2x pad, complex64)",
+        "value": round(value, 2), "unit": "props/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (f32) with f64 phase",
+        "data": "synthetic seeded field (amplitude U[0,1), phase U[0,2pi))",
+        "config": {"workload": f"asm_{n}_pad{2 * n}", "n": n, "pad": 2 * n, "lambda": LAM,
+                   "pitch": PITCH, "z": 0.10, "bandlimit": True},
+        "roofline": roof,
+        "prop_roofline": {"algorithmic_bytes": 80 * n * n, "achieved_GBps": round(prop_gbs, 1),
+                          "frac": round(prop_gbs / HBM_PEAK_GBS, 4)},
+    }
+    if args.calibrate_copy:
+        nbytes = 512 << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        L = lib.load()
+        c = lib.ctx()
+        for _ in range(10):
+            L.olx_stream_copy(c, b.data_ptr(), a.data_ptr(), nbytes)
+        torch.cuda.synchronize()
+        line["copy_calibration_bytes"] = nbytes
+    if not args.no_extras:
+        try:
+            line["extras"] = extras(ol, lib, torch)
+        except Exception as e:  # report, do not hide the metric line
+            line["extras"] = {"error": repr(e)}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n)
+        line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

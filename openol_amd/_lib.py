@@ -1,0 +1,141 @@
+"""ctypes binding of libolx.so (the C ABI declared in include/olx.h).
+
+The library must be built in-tree (openol_amd/libolx.so).  There is no CPU
+fallback: if the library or a HIP device is missing, every entry point raises.
+"""
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libolx.so")
+
+OK = 0
+EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED = -1, -2, -3, -4
+
+BANDLIMIT = 0x1
+ADJOINT = 0x2
+FRESNEL_BL_CPU = 0x10
+FRESNEL_BL_GPU = 0x20
+CGH_EXACT_R = 0x100
+CGH_ACCUMULATE = 0x400
+WRP_SQUARE = 0x200
+
+MODE_ASM, MODE_FRESNEL, MODE_AUTO, MODE_FRESNEL_BL = 0, 1, 2, 3
+LAYERED_F16, LAYERED_BF16, LAYERED_BF16X3 = 0, 1, 2
+
+
+class OlxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"olx error {code}: {msg}")
+        self.code = code
+
+
+class OlxInvalid(OlxError, ValueError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+_i64, _d, _u32, _p, _i = ctypes.c_int64, ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int
+SIGNATURES = {
+    "olx_ctx_create": [ctypes.POINTER(_p), _i],
+    "olx_ctx_destroy": [_p],
+    "olx_ctx_set_stream": [_p, _p],
+    "olx_ctx_synchronize": [_p],
+    "olx_version": [],
+    "olx_dev_alloc": [_p, ctypes.POINTER(_p), ctypes.c_size_t],
+    "olx_dev_free": [_p, _p],
+    "olx_copy_h2d": [_p, _p, _p, ctypes.c_size_t],
+    "olx_copy_d2h": [_p, _p, _p, ctypes.c_size_t],
+    "olx_dev_memset": [_p, _p, _i, ctypes.c_size_t],
+    "olx_asm_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_fresnel_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_arss_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _u32],
+    "olx_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
+    "olx_asm_prop_check": [_i64, _i64, _d, _d, _d, _d],
+    "olx_fresnel_prop_check": [_i64, _i64, _d, _d, _d, _d],
+    "olx_fft2d_c64": [_p, _p, _i64, _i64, _i],
+    "olx_cgh_c64": [_p, _p, _i64, _p, _p, _p, _i64, _i64, _d, _d, _d, _u32],
+    "olx_cgh_c128": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _u32],
+    "olx_encode_c64": [_p, _p, _p, _p, _i64],
+    "olx_wrp_c64": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_wrp_method_c64": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _d, ctypes.c_float, _i,
+                           _u32, ctypes.POINTER(_d)],
+    "olx_wrp_plane_z": [_i64, _i64, _d, _d, _d, _d, ctypes.c_float, _i],
+    "olx_cgh_layered_c64": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _i, _d, _d, _i, _u32],
+    "olx_layer_depth": [_d, _d, _d, _i],
+    "olx_ctx_enable_timing": [_p, _i],
+    "olx_ctx_reset_timing": [_p],
+    "olx_ctx_timing_report": [_p, ctypes.c_char_p, ctypes.c_size_t],
+    "olx_stream_copy": [_p, _p, _p, ctypes.c_size_t],
+}
+DOUBLE_RESULT = {"olx_wrp_plane_z", "olx_layer_depth"}
+
+
+def load():
+    """Load libolx.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIBPATH):
+            raise OlxError(EDEVICE, f"{LIBPATH} not built: run python -m openol_amd.build")
+        L = ctypes.CDLL(LIBPATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_double if name in DOUBLE_RESULT else ctypes.c_int
+        L.olx_last_error.argtypes = []
+        L.olx_last_error.restype = ctypes.c_char_p
+        _lib = L
+        return L
+
+
+def last_error():
+    return load().olx_last_error().decode(errors="replace")
+
+
+def check(rc, what=""):
+    if rc != OK:
+        msg = f"{what}: {last_error()}" if what else last_error()
+        if rc == EINVAL:
+            raise OlxInvalid(rc, msg)
+        raise OlxError(rc, msg)
+    return rc
+
+
+_ctx = {}
+
+
+def ctx(device=None):
+    """Per-device library context bound to torch's current stream."""
+    import torch
+    if not torch.cuda.is_available():
+        raise OlxError(EDEVICE, "no HIP device visible: the MI355X path has no CPU fallback")
+    if device is None:
+        device = torch.cuda.current_device()
+    L = load()
+    with _lock:
+        c = _ctx.get(device)
+        if c is None:
+            h = _p()
+            with torch.cuda.device(device):
+                check(L.olx_ctx_create(ctypes.byref(h), device), "olx_ctx_create")
+            c = h
+            _ctx[device] = c
+    check(L.olx_ctx_set_stream(c, _p(torch.cuda.current_stream(device).cuda_stream)))
+    return c
+
+
+def timing_report(device=None):
+    """{kernel: (launches, total_ms)} of the context's instrumented launches."""
+    L = load()
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(L.olx_ctx_timing_report(ctx(device), buf, len(buf)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        out[name] = (int(cnt), float(ms))
+    return out

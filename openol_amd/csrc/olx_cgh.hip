@@ -1,0 +1,237 @@
+// olx_cgh.hip -- direct point-cloud CGH.
+//
+// U[h,w] = sum_j A_j exp(2 pi i theta_j(h,w)),  theta in cycles:
+//   Fresnel:  theta = ((px w - x_j)^2 + (py h - y_j)^2) / (2 lambda z_j) + phi_j / 2 pi
+//             (the reference's 2 pi (dx^2+dy^2)/(2 dz lambda), include/olcgh.h:304-308)
+//   exact r:  theta = sqrt(dx^2 + dy^2 + z_j^2) / lambda + phi_j / 2 pi
+//
+// A work-group owns a 32 x 32 pixel tile (256 threads, 4 pixels each, one row of
+// 32 contiguous pixels per 32 lanes, so every store is a 256-byte line).  Points
+// are staged through LDS 256 at a time: each thread converts one point into the
+// tile's FP64 anchor (phase at the tile centre reduced mod 1, and the linear and
+// quadratic phase slopes across the tile), rounded to FP32 once.  The inner loop
+// then evaluates every (pixel, point) pair in FP32 from the anchor with the
+// pixel's small offset, so the 1e4..1e5-cycle phases never meet FP32 rounding.
+// sin/cos use the hardware transcendental unit in revolutions.
+#include "olx_common.hpp"
+
+namespace olx {
+
+constexpr int CGH_TX = 32, CGH_TY = 32, CGH_NT = 256, CGH_RPT = CGH_TY / (CGH_NT / CGH_TX);
+
+__device__ __forceinline__ void sincos_rev(float t, float* s, float* c) {
+    t = t - rintf(t);  // [-0.5, 0.5] revolutions
+    *s = __builtin_amdgcn_sinf(t);
+    *c = __builtin_amdgcn_cosf(t);
+}
+
+struct PtAnchor {  // per-(tile, point) constants, FP32
+    float th;      // phase at the anchor pixel, cycles mod 1 (phi folded in)
+    float gx, gy;  // Fresnel: 2 c dxa px, 2 c dya py (cycles per pixel); exact: 2 dxa px, 2 dya py (m^2 per pixel)
+    float qx, qy;  // Fresnel: c px^2, c py^2; exact: ra, ra^2
+    float amp;
+};
+
+template <bool EXACT>
+__global__ void __launch_bounds__(CGH_NT)
+k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
+          float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx, double py,
+          double px, double lambda, int accumulate) {
+    __shared__ PtAnchor sp[CGH_NT];
+    const int tid = threadIdx.x, tx = tid % CGH_TX, ty = tid / CGH_TX;
+    const int w0 = blockIdx.x * CGH_TX, h0 = blockIdx.y * CGH_TY;
+    const int wa = w0 + CGH_TX / 2, ha = h0 + CGH_TY / 2;  // anchor pixel
+    const float a = (float)(tx - CGH_TX / 2);               // this thread's x offset
+    float bk[CGH_RPT];
+#pragma unroll
+    for (int k = 0; k < CGH_RPT; k++) bk[k] = (float)(ty + k * (CGH_NT / CGH_TX) - CGH_TY / 2);
+    const float fpx = (float)px, fpy = (float)py;
+    const float px2 = (float)(px * px), py2 = (float)(py * py);
+    const float inv_lambda = (float)(1.0 / lambda);
+    float re[CGH_RPT], im[CGH_RPT];
+#pragma unroll
+    for (int k = 0; k < CGH_RPT; k++) re[k] = im[k] = 0.f;
+
+    for (int base = 0; base < n; base += CGH_NT) {
+        const int j = base + tid;
+        PtAnchor pa;
+        if (j < n) {
+            const olx_point p = pts[j];
+            const double dxa = px * wa - p.x, dya = py * ha - p.y;
+            double th;
+            if (!EXACT) {
+                const double c = 1.0 / (2.0 * lambda * p.z);
+                th = c * (dxa * dxa + dya * dya);
+                pa.gx = (float)(2.0 * c * dxa * px);
+                pa.gy = (float)(2.0 * c * dya * py);
+                pa.qx = (float)(c * px * px);
+                pa.qy = (float)(c * py * py);
+            } else {
+                const double ra2 = dxa * dxa + dya * dya + p.z * p.z;
+                const double ra = sqrt(ra2);
+                th = ra / lambda;
+                pa.gx = (float)(2.0 * dxa * px);
+                pa.gy = (float)(2.0 * dya * py);
+                pa.qx = (float)ra;
+                pa.qy = (float)ra2;
+            }
+            th += (double)p.phase * (0.5 / M_PI);
+            pa.th = (float)(th - rint(th));
+            pa.amp = p.amp;
+        } else {
+            pa.th = pa.gx = pa.gy = pa.qx = pa.qy = 0.f;
+            pa.amp = 0.f;
+        }
+        sp[tid] = pa;
+        __syncthreads();
+        const int cnt = min(CGH_NT, n - base);
+        for (int q = 0; q < cnt; q++) {
+            const PtAnchor P = sp[q];
+            if (!EXACT) {
+                const float thx = P.th + a * (P.gx + a * P.qx);
+#pragma unroll
+                for (int k = 0; k < CGH_RPT; k++) {
+                    const float t = thx + bk[k] * (P.gy + bk[k] * P.qy);
+                    float s, c;
+                    sincos_rev(t, &s, &c);
+                    re[k] += P.amp * c;
+                    im[k] += P.amp * s;
+                }
+            } else {
+                const float dqx = a * (P.gx + a * px2);
+#pragma unroll
+                for (int k = 0; k < CGH_RPT; k++) {
+                    const float dq = dqx + bk[k] * (P.gy + bk[k] * py2);
+                    const float dr = dq / (P.qx + sqrtf(P.qy + dq));  // r - ra
+                    const float t = P.th + dr * inv_lambda;
+                    float s, c;
+                    sincos_rev(t, &s, &c);
+                    re[k] += P.amp * c;
+                    im[k] += P.amp * s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int w = w0 + tx;
+    if (w >= nx) return;
+#pragma unroll
+    for (int k = 0; k < CGH_RPT; k++) {
+        const int h = h0 + ty + k * (CGH_NT / CGH_TX);
+        if (h >= ny) continue;
+        const size_t idx = (size_t)h * nx + w;
+        float2 v = make_float2(re[k], im[k]);
+        if (accumulate && U) {
+            const float2 o = U[idx];
+            v = make_float2(v.x + o.x, v.y + o.y);
+        }
+        if (U) U[idx] = v;
+        if (phase_out) phase_out[idx] = atan2f(v.y, v.x);
+        if (amp_out) amp_out[idx] = sqrtf(v.x * v.x + v.y * v.y);
+    }
+}
+
+// FP64 reference kernel: one pixel per thread, FP64 throughout
+template <bool EXACT>
+__global__ void __launch_bounds__(256)
+k_cgh_f64(const olx_point* __restrict__ pts, int n, double2* __restrict__ U, int ny, int nx,
+          double py, double px, double lambda) {
+    __shared__ olx_point sp[256];
+    const int w = blockIdx.x * 16 + threadIdx.x % 16, h = blockIdx.y * 16 + threadIdx.x / 16;
+    const double xw = px * w, yh = py * h;
+    double re = 0.0, im = 0.0;
+    for (int base = 0; base < n; base += 256) {
+        const int j = base + threadIdx.x;
+        if (j < n) sp[threadIdx.x] = pts[j];
+        __syncthreads();
+        const int cnt = min(256, n - base);
+        for (int q = 0; q < cnt; q++) {
+            const olx_point p = sp[q];
+            const double dx = xw - p.x, dy = yh - p.y;
+            double th = EXACT ? sqrt(dx * dx + dy * dy + p.z * p.z) / lambda
+                              : (dx * dx + dy * dy) / (2.0 * p.z * lambda);
+            th += (double)p.phase * (0.5 / M_PI);
+            th -= rint(th);
+            double s, c;
+            sincospi(2.0 * th, &s, &c);
+            re += (double)p.amp * c;
+            im += (double)p.amp * s;
+        }
+        __syncthreads();
+    }
+    if (w < nx && h < ny) U[(size_t)h * nx + w] = make_double2(re, im);
+}
+
+__global__ void k_encode(const float2* __restrict__ U, float* __restrict__ ph,
+                         float* __restrict__ am, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float2 v = U[i];
+        if (ph) ph[i] = atan2f(v.y, v.x);
+        if (am) am[i] = sqrtf(v.x * v.x + v.y * v.y);
+    }
+}
+
+static int check_cgh(olx_ctx* ctx, const olx_point* pts, int64_t n, int64_t ny, int64_t nx,
+                     double py, double px, double lambda) {
+    if (!ctx) OLX_FAIL(OLX_EINVAL, "null context");
+    if (n < 0 || (n > 0 && !pts)) OLX_FAIL(OLX_EINVAL, "cgh: bad point array");
+    if (n > INT32_MAX) OLX_FAIL(OLX_EUNSUPPORTED, "cgh: more than 2^31-1 points");
+    if (ny <= 0 || nx <= 0 || ny > (1 << 20) || nx > (1 << 20))
+        OLX_FAIL(OLX_EINVAL, "cgh: bad hologram size %lld x %lld", (long long)ny, (long long)nx);
+    if (!(py > 0) || !(px > 0) || !(lambda > 0)) OLX_FAIL(OLX_EINVAL, "cgh: pitch and lambda must be > 0");
+    return OLX_OK;
+}
+
+}  // namespace olx
+
+using namespace olx;
+
+extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
+                           float* d_phase, float* d_amp, int64_t ny, int64_t nx, double py,
+                           double px, double lambda, uint32_t flags) {
+    OLX_TRY(check_cgh(ctx, d_pts, n, ny, nx, py, px, lambda));
+    if (!d_U && !d_phase && !d_amp) OLX_FAIL(OLX_EINVAL, "cgh: no output requested");
+    if ((flags & OLX_CGH_ACCUMULATE) && !d_U) OLX_FAIL(OLX_EINVAL, "cgh: accumulate needs d_U");
+    dim3 grid((unsigned)((nx + CGH_TX - 1) / CGH_TX), (unsigned)((ny + CGH_TY - 1) / CGH_TY));
+    const int acc = (flags & OLX_CGH_ACCUMULATE) ? 1 : 0;
+    {
+        TimedLaunch tl(ctx, (flags & OLX_CGH_EXACT_R) ? "cgh_f32_exact" : "cgh_f32_fresnel");
+        if (flags & OLX_CGH_EXACT_R)
+            k_cgh_f32<true><<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U, d_phase,
+                                                              d_amp, (int)ny, (int)nx, py, px,
+                                                              lambda, acc);
+        else
+            k_cgh_f32<false><<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U,
+                                                               d_phase, d_amp, (int)ny, (int)nx,
+                                                               py, px, lambda, acc);
+    }
+    return check_launch("cgh_f32");
+}
+
+extern "C" int olx_cgh_c128(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
+                            int64_t ny, int64_t nx, double py, double px, double lambda,
+                            uint32_t flags) {
+    OLX_TRY(check_cgh(ctx, d_pts, n, ny, nx, py, px, lambda));
+    if (!d_U) OLX_FAIL(OLX_EINVAL, "cgh_c128: null output");
+    dim3 grid((unsigned)((nx + 15) / 16), (unsigned)((ny + 15) / 16));
+    {
+        TimedLaunch tl(ctx, "cgh_f64");
+        if (flags & OLX_CGH_EXACT_R)
+            k_cgh_f64<true><<<grid, 256, 0, ctx->stream>>>(d_pts, (int)n, (double2*)d_U, (int)ny,
+                                                           (int)nx, py, px, lambda);
+        else
+            k_cgh_f64<false><<<grid, 256, 0, ctx->stream>>>(d_pts, (int)n, (double2*)d_U, (int)ny,
+                                                            (int)nx, py, px, lambda);
+    }
+    return check_launch("cgh_f64");
+}
+
+extern "C" int olx_encode_c64(olx_ctx* ctx, const void* d_U, float* d_phase, float* d_amp,
+                              int64_t count) {
+    if (!ctx || !d_U || count < 0) OLX_FAIL(OLX_EINVAL, "encode: bad argument");
+    if (count == 0 || (!d_phase && !d_amp)) return OLX_OK;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 65536);
+    k_encode<<<(unsigned)blocks, 256, 0, ctx->stream>>>((const float2*)d_U, d_phase, d_amp, count);
+    return check_launch("encode");
+}

@@ -1,0 +1,97 @@
+// olx_common.hpp -- shared host/device definitions of libolx (MI355X / gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/olx.h"
+
+namespace olx {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+#define OLX_FAIL(code, ...)            \
+    do {                               \
+        ::olx::set_error(__VA_ARGS__); \
+        return (code);                 \
+    } while (0)
+#define OLX_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            OLX_FAIL(OLX_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                               \
+    } while (0)
+#define OLX_TRY(call)           \
+    do {                        \
+        int rc_ = (call);       \
+        if (rc_ != OLX_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------- device buffers
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int reserve(size_t b);  // grows (never shrinks); contents not preserved
+};
+
+// ---------------------------------------------------------------- kernel timing
+struct KernelStat {
+    int64_t count = 0;
+    double total_ms = 0.0;
+};
+
+}  // namespace olx
+
+struct olx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // twiddle table for the FFT passes (length olx::TWN, float2), built at create
+    float2* twiddle = nullptr;
+    // propagation workspace (padded intermediate)
+    olx::DevBuf work;
+    olx::DevBuf scratch2;  // second work buffer (sources aliasing, staging)
+    // cached 1-D tables (chirp spectra, pre/post chirps), keyed by parameters
+    std::map<std::string, std::shared_ptr<olx::DevBuf>> tables;
+    // WRP / CGH scratch
+    olx::DevBuf wrp_counts, wrp_lists, wrp_pts;
+    olx::DevBuf lay_pts, lay_counts;
+    // timing
+    bool timing = false;
+    std::map<std::string, olx::KernelStat> stats;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace olx {
+
+constexpr int TWN = 16384;  // twiddle table length = largest transform length
+
+// timing bracket around launches (no-op unless ctx->timing)
+struct TimedLaunch {
+    olx_ctx* ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(olx_ctx* c, const char* n);
+    ~TimedLaunch();
+};
+void flush_timing(olx_ctx* ctx);  // resolves pending events (synchronises them)
+
+int check_launch(const char* what);
+
+// table cache: returns a device buffer for key, creating it with bytes when absent;
+// *fresh is set when the caller must fill it
+float2* table(olx_ctx* ctx, const std::string& key, size_t count, bool* fresh, int* rc);
+
+inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace olx

@@ -1,0 +1,575 @@
+// olx_prop.hip -- fused 3-pass angular-spectrum / Fresnel / ARSS propagation.
+//
+// One propagation of an ny x nx field through the 2x-padded (My x Mx) transform
+// domain is three kernels over one ny x Mx intermediate W in HBM:
+//   A  rows:    load the nx input values of a row (pre-chirp for ARSS), pad in
+//               registers (the zero half is never loaded), forward FFT of
+//               length Mx, store the Mx-row of W.
+//   B  columns: load the ny stored values of a column, forward FFT of length My,
+//               multiply by the transfer function (ASM: computed on the fly with an
+//               FP64 phase; Fresnel/ARSS: product of two cached 1-D spectra),
+//               inverse FFT, store the centre ny values back into W.
+//   C  rows:    load an Mx-row of W, inverse FFT, keep the centre nx values,
+//               scale by 1/(My Mx) (post-chirp for ARSS), store.
+// The padded rows that are all zero are never materialised, so HBM traffic is
+// 8 + 16 + 16 + 16 + 16 + 8 = 80 ny nx bytes per propagation.
+#include <climits>
+#include <cstring>
+
+#include "fft_core.hpp"
+#include "olx_common.hpp"
+
+namespace olx {
+
+enum { K_ASM = 0, K_TABLE = 1 };
+
+struct PropArgs {
+    int ny, nx;  // unpadded field
+    int mode;    // K_ASM / K_TABLE
+    int adjoint;
+    int bandlimit;
+    double du, dv, inv_l2, z, uw, vw;  // ASM transfer function and band limit
+    const float2* sx;                  // K_TABLE: natural-order spectra, length Mx / My
+    const float2* sy;
+    const float2* ax;  // pre-chirp tables (length nx / ny), zero outside the window
+    const float2* ay;
+    const float2* cx;  // post-chirp tables, applied inside |w-nx/2|<cwl, |h-ny/2|<chl
+    const float2* cy;
+    int cwl, chl;
+    float scale;
+};
+
+template <int L>
+constexpr int row_rpw() { return Plan<L>::T >= 256 ? 1 : 256 / Plan<L>::T; }
+template <int L>
+constexpr int row_nt() { return row_rpw<L>() * Plan<L>::T; }
+template <int L>
+constexpr int col_cb() { return L >= 16384 ? 1 : (L >= 1024 ? 16384 / L : 16); }
+template <int L>
+constexpr int col_nt() { return col_cb<L>() * Plan<L>::T; }
+
+// ------------------------------------------------------------------ pass A
+template <int L>
+__global__ void __launch_bounds__(row_nt<L>())
+k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a,
+               const float2* __restrict__ tw) {
+    using P = Plan<L>;
+    constexpr int RPW = row_rpw<L>();
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
+    __shared__ float2 lds[RPW * P::LP];
+    const int tid = threadIdx.x, rr = tid / P::T, t = tid - rr * P::T;
+    const int row = blockIdx.x * RPW + rr;
+    const bool valid = row < a.ny;
+    const int nx = a.nx;
+    float2 ay = make_float2(1.f, 0.f);
+    if (a.ay && valid) ay = a.ay[row];
+    float2 v[P::E];
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            float2 x = make_float2(0.f, 0.f);
+            if (first_nonzero<L>(r) && valid) {
+                const int col = first_index<L>(t, b, r) - L / 4;
+                x = src[(size_t)row * nx + col];
+                if (a.ax) x = cmul(x, cmul(a.ax[col], ay));
+            }
+            v[b * R0 + r] = x;
+        }
+    run_stages<L, false, TWN, 1>(v, t, lds, rr * P::LP, tw);
+    if (valid) {
+        float2* out = W + (size_t)row * L;
+#pragma unroll
+        for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) out[last_index<L>(t, b, r)] = v[b * RL + r];
+    }
+}
+
+// transfer function value at padded-spectrum column col (natural order) and row i
+__device__ __forceinline__ float2 asm_h(const PropArgs& a, double u2, bool ucut, int i, int My) {
+    const int fy = i < My / 2 ? i : i - My;
+    const double v = (double)fy * a.dv;
+    if (ucut || (a.bandlimit && !(fabs(v) * 2 < a.vw))) return make_float2(0.f, 0.f);
+    const double arg = a.inv_l2 - u2 - v * v;
+    if (arg < 0.0) return make_float2(0.f, 0.f);  // evanescent: zero (reference: NaN)
+    const double cyc = sqrt(arg) * a.z;
+    const double f = cyc - rint(cyc);  // phase 2 pi w z reduced to [-pi, pi]
+    float s, c;
+    sincospif((float)(2.0 * f), &s, &c);
+    return make_float2(c, a.adjoint ? -s : s);
+}
+
+// ------------------------------------------------------------------ pass B
+template <int L>
+__global__ void __launch_bounds__(col_nt<L>())
+k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
+    using P = Plan<L>;
+    constexpr int CB = col_cb<L>();
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
+    __shared__ float2 lds[CB * P::LP];
+    const int tid = threadIdx.x, c = tid % CB, t = tid / CB;
+    const int Mx = 2 * a.nx;
+    const int col = blockIdx.x * CB + c;
+    const bool valid = col < Mx;
+    float2 v[P::E];
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            float2 x = make_float2(0.f, 0.f);
+            if (first_nonzero<L>(r) && valid)
+                x = W[(size_t)(first_index<L>(t, b, r) - L / 4) * Mx + col];
+            v[b * R0 + r] = x;
+        }
+    run_stages<L, false, TWN, CB>(v, t, lds, c, tw);
+#pragma unroll
+    for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) lds[c + lpad(last_index<L>(t, b, r)) * CB] = v[b * RL + r];
+    __syncthreads();
+    // transfer function, then inverse transform
+    double u2 = 0.0;
+    bool ucut = false;
+    const float2 sxc = (a.mode == K_TABLE && valid) ? a.sx[col] : make_float2(0.f, 0.f);
+    if (a.mode == K_ASM) {
+        const int fx = col < Mx / 2 ? col : col - Mx;
+        const double u = (double)fx * a.du;
+        u2 = u * u;
+        ucut = a.bandlimit && !(fabs(u) * 2 < a.uw);
+    }
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int i = first_index<L>(t, b, r);
+            float2 x = lds[c + lpad(i) * CB];
+            float2 h;
+            if (a.mode == K_TABLE) {
+                h = cmul(sxc, a.sy[i]);
+                if (a.adjoint) h = cconj(h);
+            } else {
+                h = asm_h(a, u2, ucut, i, L);
+            }
+            v[b * R0 + r] = cmul(x, h);
+        }
+    __syncthreads();
+    run_stages<L, true, TWN, CB>(v, t, lds, c, tw);
+#pragma unroll
+    for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            if (!last_maybe_kept<L>(r)) continue;
+            const int i = last_index<L>(t, b, r);
+            if (valid && i >= L / 4 && i < 3 * L / 4)
+                W[(size_t)(i - L / 4) * Mx + col] = v[b * RL + r];
+        }
+}
+
+// ------------------------------------------------------------------ pass C
+template <int L>
+__global__ void __launch_bounds__(row_nt<L>())
+k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a,
+                const float2* __restrict__ tw) {
+    using P = Plan<L>;
+    constexpr int RPW = row_rpw<L>();
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
+    __shared__ float2 lds[RPW * P::LP];
+    const int tid = threadIdx.x, rr = tid / P::T, t = tid - rr * P::T;
+    const int row = blockIdx.x * RPW + rr;
+    const bool valid = row < a.ny;
+    const int nx = a.nx;
+    float2 v[P::E];
+    const float2* in = W + (size_t)row * L;
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++)
+            v[b * R0 + r] = valid ? in[first_index<L>(t, b, r)] : make_float2(0.f, 0.f);
+    run_stages<L, true, TWN, 1>(v, t, lds, rr * P::LP, tw);
+    if (!valid) return;
+    const bool post = a.cx != nullptr && abs(row - a.ny / 2) < a.chl;
+    const float2 cy = post ? a.cy[row] : make_float2(1.f, 0.f);
+#pragma unroll
+    for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            if (!last_maybe_kept<L>(r)) continue;
+            const int i = last_index<L>(t, b, r);
+            if (i < L / 4 || i >= 3 * L / 4) continue;
+            const int col = i - L / 4;
+            float2 x = cscale(v[b * RL + r], a.scale);
+            if (post && abs(col - nx / 2) < a.cwl) x = cmul(x, cmul(a.cx[col], cy));
+            dst[(size_t)row * nx + col] = x;
+        }
+}
+
+// ------------------------------------------------------------------ plain 2-D FFT
+template <int L, bool INV>
+__global__ void __launch_bounds__(row_nt<L>())
+k_rows_fft(float2* __restrict__ u, int rows, const float2* __restrict__ tw) {
+    using P = Plan<L>;
+    constexpr int RPW = row_rpw<L>();
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
+    __shared__ float2 lds[RPW * P::LP];
+    const int tid = threadIdx.x, rr = tid / P::T, t = tid - rr * P::T;
+    const int row = blockIdx.x * RPW + rr;
+    const bool valid = row < rows;
+    float2* p = u + (size_t)row * L;
+    float2 v[P::E];
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++)
+            v[b * R0 + r] = valid ? p[first_index<L>(t, b, r)] : make_float2(0.f, 0.f);
+    run_stages<L, INV, TWN, 1>(v, t, lds, rr * P::LP, tw);
+    if (!valid) return;
+#pragma unroll
+    for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) p[last_index<L>(t, b, r)] = v[b * RL + r];
+}
+
+template <int L, bool INV>
+__global__ void __launch_bounds__(col_nt<L>())
+k_cols_fft(float2* __restrict__ u, int cols, const float2* __restrict__ tw) {
+    using P = Plan<L>;
+    constexpr int CB = col_cb<L>();
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
+    __shared__ float2 lds[CB * P::LP];
+    const int tid = threadIdx.x, c = tid % CB, t = tid / CB;
+    const int col = blockIdx.x * CB + c;
+    const bool valid = col < cols;
+    float2 v[P::E];
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++)
+            v[b * R0 + r] = valid ? u[(size_t)first_index<L>(t, b, r) * cols + col]
+                                  : make_float2(0.f, 0.f);
+    run_stages<L, INV, TWN, CB>(v, t, lds, c, tw);
+    if (!valid) return;
+#pragma unroll
+    for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) u[(size_t)last_index<L>(t, b, r) * cols + col] = v[b * RL + r];
+}
+
+// ------------------------------------------------------------------ tables
+// centred chirp in FP64: h[m] = exp(i K (qa x^2 + qb x + qc)), x = (m - n/2) d,
+// zero where |m - n/2| >= lim
+__global__ void k_chirp_f64(double2* __restrict__ out, int n, double d, double K, double qa,
+                            double qb, double qc, long long lim) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const long long off = (long long)m - n / 2;
+    if (llabs(off) >= lim) {
+        out[m] = make_double2(0.0, 0.0);
+        return;
+    }
+    const double x = (double)off * d;
+    const double ph = K * (qa * x * x + qb * x + qc);
+    double s, c;
+    sincos(ph, &s, &c);
+    out[m] = make_double2(c, s);
+}
+
+__global__ void k_to_f32(float2* __restrict__ out, const double2* __restrict__ in, int n,
+                         int adjoint) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    out[m] = make_float2((float)in[m].x, (float)(adjoint ? -in[m].y : in[m].y));
+}
+
+// natural-order spectrum of the fftshift of a centred table, FP64 direct DFT:
+// S[k] = scale * sum_n hc[(n + M/2) mod M] exp(-2 pi i n k / M)
+__global__ void k_dft_f64(float2* __restrict__ S, const double2* __restrict__ hc, int M,
+                          double scale, int adjoint) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    double re = 0.0, im = 0.0;
+    for (int n = 0; n < M; n++) {
+        const double2 h = hc[(n + M / 2) & (M - 1)];
+        if (h.x == 0.0 && h.y == 0.0) continue;
+        const long long nk = ((long long)n * k) & (M - 1);
+        double s, c;
+        sincospi(-2.0 * (double)nk / (double)M, &s, &c);
+        re += h.x * c - h.y * s;
+        im += h.x * s + h.y * c;
+    }
+    S[k] = make_float2((float)(re * scale), (float)((adjoint ? -im : im) * scale));
+}
+
+static int64_t win_limit(double v) {
+    const double f = std::floor(v);
+    if (!(f == f) || f >= (double)INT_MAX) return INT_MAX;
+    if (f <= (double)INT_MIN) return INT_MIN;
+    return (int64_t)f;
+}
+
+static std::string key_of(const char* kind, int n, double d, double K, double qa, double qb,
+                          double qc, int64_t lim, int adj) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s|%d|%.17g|%.17g|%.17g|%.17g|%.17g|%lld|%d", kind, n, d, K, qa,
+             qb, qc, (long long)lim, adj);
+    return buf;
+}
+
+// float2 chirp table of length n (pre/post chirps)
+static const float2* chirp_table(olx_ctx* ctx, int n, double d, double K, double qa, double qb,
+                                 double qc, int64_t lim, int* rc) {
+    bool fresh = false;
+    float2* t = table(ctx, key_of("chirp", n, d, K, qa, qb, qc, lim, 0), n, &fresh, rc);
+    if (!t || !fresh) return t;
+    if ((*rc = ctx->scratch2.reserve(sizeof(double2) * n)) != OLX_OK) return nullptr;
+    double2* tmp = (double2*)ctx->scratch2.p;
+    k_chirp_f64<<<(n + 255) / 256, 256, 0, ctx->stream>>>(tmp, n, d, K, qa, qb, qc, lim);
+    k_to_f32<<<(n + 255) / 256, 256, 0, ctx->stream>>>(t, tmp, n, 0);
+    if ((*rc = check_launch("chirp table")) != OLX_OK) return nullptr;
+    return t;
+}
+
+// spectrum (length M, natural order, scaled by 1/M) of the centred chirp K qa x^2
+static const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa,
+                                    int64_t lim, int adj, int* rc) {
+    bool fresh = false;
+    float2* t = table(ctx, key_of("spec", M, d, K, qa, 0, 0, lim, adj), M, &fresh, rc);
+    if (!t || !fresh) return t;
+    if ((*rc = ctx->scratch2.reserve(sizeof(double2) * M)) != OLX_OK) return nullptr;
+    double2* tmp = (double2*)ctx->scratch2.p;
+    k_chirp_f64<<<(M + 255) / 256, 256, 0, ctx->stream>>>(tmp, M, d, K, qa, 0.0, 0.0, lim);
+    k_dft_f64<<<(M + 127) / 128, 128, 0, ctx->stream>>>(t, tmp, M, 1.0 / M, adj);
+    if ((*rc = check_launch("spectrum table")) != OLX_OK) return nullptr;
+    return t;
+}
+
+// ------------------------------------------------------------------ launchers
+#define OLX_LEN_SWITCH(LOG2, ...)             \
+    switch (LOG2) {                           \
+        case 3: { constexpr int L = 8; __VA_ARGS__; } break;     \
+        case 4: { constexpr int L = 16; __VA_ARGS__; } break;    \
+        case 5: { constexpr int L = 32; __VA_ARGS__; } break;    \
+        case 6: { constexpr int L = 64; __VA_ARGS__; } break;    \
+        case 7: { constexpr int L = 128; __VA_ARGS__; } break;   \
+        case 8: { constexpr int L = 256; __VA_ARGS__; } break;   \
+        case 9: { constexpr int L = 512; __VA_ARGS__; } break;   \
+        case 10: { constexpr int L = 1024; __VA_ARGS__; } break; \
+        case 11: { constexpr int L = 2048; __VA_ARGS__; } break; \
+        case 12: { constexpr int L = 4096; __VA_ARGS__; } break; \
+        case 13: { constexpr int L = 8192; __VA_ARGS__; } break; \
+        case 14: { constexpr int L = 16384; __VA_ARGS__; } break;\
+        default: OLX_FAIL(OLX_EUNSUPPORTED, "transform length 2^%d unsupported", (int)(LOG2)); \
+    }
+
+static int ilog2(int64_t n) { return 63 - __builtin_clzll((unsigned long long)n); }
+
+static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a) {
+    const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
+    OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
+    float2* W = (float2*)ctx->work.p;
+    a.scale = (float)(1.0 / ((double)Mx * (double)My));
+    hipStream_t s = ctx->stream;
+    {
+        TimedLaunch tl(ctx, "prop_rows_pad_fwd");
+        OLX_LEN_SWITCH(ilog2(Mx), {
+            const int grid = (a.ny + row_rpw<L>() - 1) / row_rpw<L>();
+            k_rows_pad_fwd<L><<<grid, row_nt<L>(), 0, s>>>(src, W, a, ctx->twiddle);
+        });
+    }
+    OLX_TRY(check_launch("prop pass A"));
+    {
+        TimedLaunch tl(ctx, "prop_cols_conv");
+        OLX_LEN_SWITCH(ilog2(My), {
+            const int grid = (int)((Mx + col_cb<L>() - 1) / col_cb<L>());
+            k_cols_conv<L><<<grid, col_nt<L>(), 0, s>>>(W, a, ctx->twiddle);
+        });
+    }
+    OLX_TRY(check_launch("prop pass B"));
+    {
+        TimedLaunch tl(ctx, "prop_rows_crop_inv");
+        OLX_LEN_SWITCH(ilog2(Mx), {
+            const int grid = (a.ny + row_rpw<L>() - 1) / row_rpw<L>();
+            k_rows_crop_inv<L><<<grid, row_nt<L>(), 0, s>>>(W, dst, a, ctx->twiddle);
+        });
+    }
+    return check_launch("prop pass C");
+}
+
+static int check_field(olx_ctx* ctx, const void* p, int64_t ny, int64_t nx) {
+    if (!ctx) OLX_FAIL(OLX_EINVAL, "null context");
+    if (!p) OLX_FAIL(OLX_EINVAL, "null field pointer");
+    if (!is_pow2(ny) || !is_pow2(nx) || ny < 4 || nx < 4 || ny > 8192 || nx > 8192)
+        OLX_FAIL(OLX_EUNSUPPORTED,
+                 "field %lld x %lld: sides must be powers of two in [4, 8192]", (long long)ny,
+                 (long long)nx);
+    return OLX_OK;
+}
+
+static PropArgs base_args(int64_t ny, int64_t nx) {
+    PropArgs a;
+    memset(&a, 0, sizeof a);
+    a.ny = (int)ny;
+    a.nx = (int)nx;
+    return a;
+}
+
+}  // namespace olx
+
+using namespace olx;
+
+extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
+                           double dx, double lambda, double z, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_u, ny, nx));
+    if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !std::isfinite(z))
+        OLX_FAIL(OLX_EINVAL, "asm: lambda, dx, dy must be > 0 and z finite");
+    PropArgs a = base_args(ny, nx);
+    a.mode = K_ASM;
+    a.adjoint = (flags & OLX_ADJOINT) ? 1 : 0;
+    const int64_t nx2 = 2 * nx, ny2 = 2 * ny;
+    a.du = 1 / (dx * nx2);
+    a.dv = 1 / (dy * ny2);
+    a.inv_l2 = 1 / (lambda * lambda);
+    a.z = z;
+    a.bandlimit = (flags & OLX_BANDLIMIT) ? 1 : 0;
+    if (a.bandlimit) {
+        // shiftedAsmTransferF at ox = oy = 0 (include/olprop.h:258-270): uw = ulp + ulm
+        const double ulm = 1.0 / std::sqrt(std::pow(0.0 - 0.5f / a.du, -2) * z * z + 1.0f) / lambda;
+        const double ulp = 1.0 / std::sqrt(std::pow(0.0 + 0.5f / a.du, -2) * z * z + 1.0f) / lambda;
+        const double vlm = 1.0 / std::sqrt(std::pow(0.0 - 0.5f / a.dv, -2) * z * z + 1.0f) / lambda;
+        const double vlp = 1.0 / std::sqrt(std::pow(0.0 + 0.5f / a.dv, -2) * z * z + 1.0f) / lambda;
+        a.uw = ulp + ulm;
+        a.vw = vlp + vlm;
+    }
+    return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a);
+}
+
+extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
+                               double dx, double lambda, double z, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_u, ny, nx));
+    if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !(z != 0) || !std::isfinite(z))
+        OLX_FAIL(OLX_EINVAL, "fresnel: lambda, dx, dy must be > 0 and z finite, nonzero");
+    if ((flags & OLX_FRESNEL_BL_CPU) && (flags & OLX_FRESNEL_BL_GPU))
+        OLX_FAIL(OLX_EINVAL, "fresnel: choose one band-limit variant");
+    PropArgs a = base_args(ny, nx);
+    a.mode = K_TABLE;
+    const int adj = (flags & OLX_ADJOINT) ? 1 : 0;
+    int64_t wlim = INT_MAX, hlim = INT_MAX;
+    if (flags & OLX_FRESNEL_BL_CPU) {  // include/olprop.h:35-38
+        wlim = win_limit(std::fabs(lambda * z * 0.5 / dx) / dx);
+        hlim = win_limit(std::fabs(lambda * z * 0.5 / dy) / dy);
+    } else if (flags & OLX_FRESNEL_BL_GPU) {  // include/golprop.h:229-231
+        wlim = hlim = win_limit(std::fabs(lambda * z * 0.5 / dx) / dx);
+    }
+    const double K = M_PI * (1 / (lambda * z));
+    int rc = OLX_OK;
+    a.sx = spectrum_table(ctx, (int)(2 * nx), dx, K, 1.0, wlim, adj, &rc);
+    if (!a.sx) return rc;
+    a.sy = spectrum_table(ctx, (int)(2 * ny), dy, K, 1.0, hlim, adj, &rc);
+    if (!a.sy) return rc;
+    return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a);
+}
+
+extern "C" int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny,
+                            int64_t nx, double dy, double dx, double lambda, double z, double s,
+                            double oy, double ox, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_src, ny, nx));
+    if (!d_dst) OLX_FAIL(OLX_EINVAL, "arss: null destination");
+    if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !(z != 0) || !std::isfinite(z) ||
+        !(s != 0) || !std::isfinite(s))
+        OLX_FAIL(OLX_EINVAL, "arss: lambda, dx, dy > 0, z and s finite and nonzero");
+    if (flags & OLX_ADJOINT)
+        OLX_FAIL(OLX_EUNSUPPORTED, "arss: adjoint not provided (chirp order reverses)");
+    PropArgs a = base_args(ny, nx);
+    a.mode = K_TABLE;
+    const double K = M_PI / (lambda * z);
+    int rc = OLX_OK;
+    // stage u window and pre-chirp (include/golprop.h:849-854, 831-840)
+    const int64_t uwl = win_limit(lambda * std::fabs(z) * 0.5 / (s * std::fabs(s - 1) * dx) / dx);
+    const int64_t uhl = win_limit(lambda * std::fabs(z) * 0.5 / (s * std::fabs(s - 1) * dy) / dy);
+    a.ax = chirp_table(ctx, (int)nx, dx, K, s * s - s, -2 * s * ox, 0.0, uwl, &rc);
+    if (!a.ax) return rc;
+    a.ay = chirp_table(ctx, (int)ny, dy, K, s * s - s, -2 * s * oy, 0.0, uhl, &rc);
+    if (!a.ay) return rc;
+    // stage h: kernel chirp, x-limit on both axes (include/golprop.h:890-896)
+    const int64_t hl = win_limit(std::fabs(lambda * z * 0.5 / (dx * s)) / dx);
+    a.sx = spectrum_table(ctx, (int)(2 * nx), dx, K, s, hl, 0, &rc);
+    if (!a.sx) return rc;
+    a.sy = spectrum_table(ctx, (int)(2 * ny), dy, K, s, hl, 0, &rc);
+    if (!a.sy) return rc;
+    // stage Cz window and post-chirp (include/golprop.h:928-933, 911-920)
+    const int64_t cwl = win_limit(lambda * std::fabs(z) * 0.5 / (std::fabs(1 - s) * dx) / dx);
+    const int64_t chl = win_limit(lambda * std::fabs(z) * 0.5 / (std::fabs(1 - s) * dy) / dy);
+    a.cx = chirp_table(ctx, (int)nx, dx, K, 1 - s, 2 * ox, ox * ox, INT_MAX, &rc);
+    if (!a.cx) return rc;
+    a.cy = chirp_table(ctx, (int)ny, dy, K, 1 - s, 2 * oy, oy * oy, INT_MAX, &rc);
+    if (!a.cy) return rc;
+    a.cwl = (int)std::max<int64_t>(std::min<int64_t>(cwl, INT_MAX), INT_MIN);
+    a.chl = (int)std::max<int64_t>(std::min<int64_t>(chl, INT_MAX), INT_MIN);
+    return run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a);
+}
+
+extern "C" int olx_asm_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda,
+                                  double z) {
+    // AsmPropCheck, include/olprop.h:209-225
+    const int64_t nx2 = 2 * nx, ny2 = 2 * ny;
+    const double du = 1 / (dx * nx2), dv = 1 / (dy * ny2);
+    const double umax = du * nx2 / 2, vmax = dv * ny2 / 2;
+    const double tmpsqrt = std::sqrt(1 / (lambda * lambda) - umax * umax - vmax * vmax);
+    const double zmax_u = tmpsqrt / (2 * umax * du), zmax_v = tmpsqrt / (2 * vmax * dv);
+    if (zmax_u < std::fabs(z) || zmax_v < std::fabs(z)) return 0;
+    return 1;
+}
+
+extern "C" int olx_fresnel_prop_check(int64_t ny, int64_t nx, double dy, double dx,
+                                      double lambda, double z) {
+    // FresnelPropCheck, include/olprop.h:109-123
+    const int64_t nx2 = 2 * nx, ny2 = 2 * ny;
+    const double xmax = dx * nx2 / 2, ymax = dy * ny2 / 2;
+    const double zmin_x = 2 * xmax * dx / lambda, zmin_y = 2 * ymax * dy / lambda;
+    if (zmin_x > std::fabs(z) || zmin_y > std::fabs(z)) return 0;
+    return 1;
+}
+
+extern "C" int olx_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
+                            double dx, double lambda, double z, int mode, int* chosen) {
+    // Prop, include/olprop.h:656-682 (silent: the reference prints the chosen mode)
+    int c;
+    if (mode == OLX_MODE_FRESNEL) c = OLX_MODE_FRESNEL;
+    else if (mode == OLX_MODE_ASM) c = OLX_MODE_ASM;
+    else if (mode == OLX_MODE_AUTO) {
+        if (olx_asm_prop_check(ny, nx, dy, dx, lambda, z)) c = OLX_MODE_ASM;
+        else if (!olx_fresnel_prop_check(ny, nx, dy, dx, lambda, z)) c = OLX_MODE_FRESNEL_BL;
+        else c = OLX_MODE_FRESNEL;
+    } else OLX_FAIL(OLX_EINVAL, "prop: unknown mode %d", mode);
+    if (chosen) *chosen = c;
+    if (c == OLX_MODE_ASM) return olx_asm_c64(ctx, d_u, ny, nx, dy, dx, lambda, z, 0);
+    if (c == OLX_MODE_FRESNEL) return olx_fresnel_c64(ctx, d_u, ny, nx, dy, dx, lambda, z, 0);
+    return olx_fresnel_c64(ctx, d_u, ny, nx, dy, dx, lambda, z, OLX_FRESNEL_BL_CPU);
+}
+
+extern "C" int olx_fft2d_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, int sign) {
+    if (!ctx || !d_u) OLX_FAIL(OLX_EINVAL, "fft2d: null argument");
+    if (!is_pow2(ny) || !is_pow2(nx) || ny < 8 || nx < 8 || ny > 16384 || nx > 16384)
+        OLX_FAIL(OLX_EUNSUPPORTED, "fft2d: sides must be powers of two in [8, 16384]");
+    float2* u = (float2*)d_u;
+    hipStream_t s = ctx->stream;
+    const bool inv = sign > 0;
+    {
+        TimedLaunch tl(ctx, "fft_rows");
+        OLX_LEN_SWITCH(ilog2(nx), {
+            const int grid = (int)((ny + row_rpw<L>() - 1) / row_rpw<L>());
+            if (inv) k_rows_fft<L, true><<<grid, row_nt<L>(), 0, s>>>(u, (int)ny, ctx->twiddle);
+            else k_rows_fft<L, false><<<grid, row_nt<L>(), 0, s>>>(u, (int)ny, ctx->twiddle);
+        });
+    }
+    OLX_TRY(check_launch("fft rows"));
+    {
+        TimedLaunch tl(ctx, "fft_cols");
+        OLX_LEN_SWITCH(ilog2(ny), {
+            const int grid = (int)((nx + col_cb<L>() - 1) / col_cb<L>());
+            if (inv) k_cols_fft<L, true><<<grid, col_nt<L>(), 0, s>>>(u, (int)nx, ctx->twiddle);
+            else k_cols_fft<L, false><<<grid, col_nt<L>(), 0, s>>>(u, (int)nx, ctx->twiddle);
+        });
+    }
+    return check_launch("fft cols");
+}

@@ -1,0 +1,280 @@
+"""Python mirror of OpenOL's diffraction / CGH operators on MI355X.
+
+Same names, argument order and meaning as the reference's C++ templates
+(include/olprop.h, include/olcgh.h, include/golprop.h, include/golcgh.h), on
+torch tensors that live on the GPU.  Fields are complex64 tensors of shape
+(height, width), row-major (idx = h * width + w), updated in place like the
+reference's unique_ptr buffers.  Every call goes through the C ABI of libolx.so;
+errors raise (OlxInvalid for bad arguments) instead of exiting the process.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+ASM, FRESNEL, AUTO = _lib.MODE_ASM, _lib.MODE_FRESNEL, _lib.MODE_AUTO
+FRESNEL_BL = _lib.MODE_FRESNEL_BL
+
+_p = ctypes.c_void_p
+
+
+def _ptr(t):
+    return _p(t.data_ptr())
+
+
+def _field(u, height, width, name="u"):
+    if not isinstance(u, torch.Tensor) or not u.is_cuda:
+        raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be a CUDA (HIP) tensor")
+    if u.dtype != torch.complex64:
+        raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be complex64, got {u.dtype}")
+    if not u.is_contiguous():
+        raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be contiguous")
+    if tuple(u.shape[-2:]) != (int(height), int(width)):
+        raise _lib.OlxInvalid(_lib.EINVAL, f"{name} shape {tuple(u.shape)} != ({height}, {width})")
+    return u
+
+
+# ------------------------------------------------------------------ propagation
+def AsmProp(u, height, width, dy, dx, lam, z, bandlimit=False, adjoint=False):
+    """ol::AsmProp (include/olprop.h:227-248): in place, net scale 1."""
+    _field(u, height, width)
+    flags = (_lib.BANDLIMIT if bandlimit else 0) | (_lib.ADJOINT if adjoint else 0)
+    L = _lib.load()
+    check(L.olx_asm_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z, flags),
+          "AsmProp")
+    return u
+
+
+def FresnelProp(u, height, width, dy, dx, lam, z, adjoint=False):
+    """ol::FresnelProp (include/olprop.h:125-148): circconv(pad u, h) / M^2, in place."""
+    _field(u, height, width)
+    L = _lib.load()
+    check(L.olx_fresnel_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z,
+                            _lib.ADJOINT if adjoint else 0), "FresnelProp")
+    return u
+
+
+def FresnelPropBandLimit(u, height, width, dy, dx, lam, z, variant="cpu"):
+    """ol::FresnelPropBandLimit (include/olprop.h:150-170, per-axis window) or, with
+    variant="gpu", gFresnelPropBandLimit (include/golprop.h:258-302, x-limit on both axes)."""
+    _field(u, height, width)
+    flags = {"cpu": _lib.FRESNEL_BL_CPU, "gpu": _lib.FRESNEL_BL_GPU}[variant]
+    L = _lib.load()
+    check(L.olx_fresnel_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z,
+                            flags), "FresnelPropBandLimit")
+    return u
+
+
+def gFresnelPropBandLimit(u, height, width, dy, dx, lam, z):
+    return FresnelPropBandLimit(u, height, width, dy, dx, lam, z, variant="gpu")
+
+
+def ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lam, z, s, oy=0.0, ox=0.0):
+    """ol::ARSSFresnelProp (include/golprop.h:941-995); usrc and udst may alias."""
+    _field(usrc, ny, nx, "usrc")
+    _field(udst, ny, nx, "udst")
+    L = _lib.load()
+    check(L.olx_arss_c64(_lib.ctx(usrc.device.index), _ptr(usrc), _ptr(udst), ny, nx, dy, dx,
+                         lam, z, s, oy, ox, 0), "ARSSFresnelProp")
+    return udst
+
+
+def Prop(u, ny, nx, dy, dx, lam, d, propmode=AUTO):
+    """ol::Prop (include/olprop.h:656-682).  Returns the mode used (ASM, FRESNEL or
+    FRESNEL_BL when AUTO falls back to the band-limited Fresnel)."""
+    _field(u, ny, nx)
+    L = _lib.load()
+    ch = ctypes.c_int(-1)
+    check(L.olx_prop_c64(_lib.ctx(u.device.index), _ptr(u), ny, nx, dy, dx, lam, d, propmode,
+                         ctypes.byref(ch)), "Prop")
+    return ch.value
+
+
+def AsmPropCheck(ny, nx, dy, dx, lam, z):
+    """ol::AsmPropCheck (include/olprop.h:209-225)."""
+    return bool(_lib.load().olx_asm_prop_check(ny, nx, dy, dx, lam, z))
+
+
+def FresnelPropCheck(ny, nx, dy, dx, lam, z):
+    """ol::FresnelPropCheck (include/olprop.h:109-123)."""
+    return bool(_lib.load().olx_fresnel_prop_check(ny, nx, dy, dx, lam, z))
+
+
+def fft(u):
+    """Unnormalised forward 2-D FFT in place (ol::fft, include/olutils.h:19-41)."""
+    _field(u, u.shape[-2], u.shape[-1])
+    check(_lib.load().olx_fft2d_c64(_lib.ctx(u.device.index), _ptr(u), u.shape[-2], u.shape[-1],
+                                    -1), "fft")
+    return u
+
+
+def ifft(u):
+    """Unnormalised backward 2-D FFT in place (ol::ifft, include/olutils.h:44-66)."""
+    _field(u, u.shape[-2], u.shape[-1])
+    check(_lib.load().olx_fft2d_c64(_lib.ctx(u.device.index), _ptr(u), u.shape[-2], u.shape[-1],
+                                    1), "ifft")
+    return u
+
+
+# ------------------------------------------------------------------ point clouds
+POINT_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("amp", "<f4"),
+                        ("phase", "<f4")])
+
+
+class Object:
+    """Point cloud on the GPU (ol::Object / ol::gObject, include/olclass.h:54-70,
+    include/golclass.h:58-71) packed as olx_point records (x, y, z, amp, phase)."""
+
+    def __init__(self, x, y, z, amp=None, phase=None, device=None):
+        x = np.asarray(x, dtype=np.float64).ravel()
+        n = x.size
+        rec = np.zeros(n, dtype=POINT_DTYPE)
+        rec["x"] = x
+        rec["y"] = np.asarray(y, dtype=np.float64).ravel()
+        rec["z"] = np.asarray(z, dtype=np.float64).ravel()
+        rec["amp"] = 1.0 if amp is None else np.asarray(amp, dtype=np.float32).ravel()
+        rec["phase"] = 0.0 if phase is None else np.asarray(phase, dtype=np.float32).ravel()
+        self.size = n
+        self.host = rec
+        dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        buf = torch.from_numpy(rec.view(np.uint8).copy())
+        self.data = buf.to(dev) if n else torch.empty(32, dtype=torch.uint8, device=dev)
+
+    @property
+    def device(self):
+        return self.data.device
+
+    def ptr(self):
+        return _ptr(self.data)
+
+
+def Object2cuda(obj_or_xyz):
+    """ol::Object2cuda (include/golclass.h:73-78): accepts an Object or an (n, 3) array."""
+    if isinstance(obj_or_xyz, Object):
+        return obj_or_xyz
+    a = np.asarray(obj_or_xyz, dtype=np.float64)
+    return Object(a[:, 0], a[:, 1], a[:, 2])
+
+
+def _out2d(t, ny, nx, dtype, name):
+    if t is None:
+        return None
+    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous() or tuple(t.shape[-2:]) != (ny, nx):
+        raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be a contiguous CUDA {dtype} ({ny}, {nx})")
+    return t
+
+
+def Cgh(obj, out, ny, nx, py, px, lam, exact_r=False, phase_out=None, amp_out=None,
+        accumulate=False):
+    """ol::Cgh / gCgh (include/olcgh.h:351-379, include/golcgh.h:127-145).
+
+    out is a complex64 tensor (complex hologram, _CghFresnel include/olcgh.h:296-313)
+    or a float32 tensor (phase-only hologram via atan2, include/olcgh.h:275-295).
+    Optional phase_out / amp_out receive the encodings in the same pass."""
+    if out is not None and out.dtype == torch.float32:
+        phase_out, out = out, None
+    _out2d(out, ny, nx, torch.complex64, "out")
+    _out2d(phase_out, ny, nx, torch.float32, "phase_out")
+    _out2d(amp_out, ny, nx, torch.float32, "amp_out")
+    flags = (_lib.CGH_EXACT_R if exact_r else 0) | (_lib.CGH_ACCUMULATE if accumulate else 0)
+    nul = lambda t: None if t is None else _ptr(t)
+    check(_lib.load().olx_cgh_c64(_lib.ctx(obj.device.index), obj.ptr(), obj.size, nul(out),
+                                  nul(phase_out), nul(amp_out), ny, nx, py, px, lam, flags), "Cgh")
+    return out if out is not None else phase_out
+
+
+def CghFP64(obj, out, ny, nx, py, px, lam, exact_r=False):
+    """Same sum in FP64 into a complex128 tensor (the FP64 column)."""
+    _out2d(out, ny, nx, torch.complex128, "out")
+    check(_lib.load().olx_cgh_c128(_lib.ctx(obj.device.index), obj.ptr(), obj.size, _ptr(out), ny,
+                                   nx, py, px, lam, _lib.CGH_EXACT_R if exact_r else 0), "CghFP64")
+    return out
+
+
+def encode(U, phase_out=None, amp_out=None):
+    """Phase-only (atan2) and amplitude encodings of a complex64 field."""
+    if phase_out is None:
+        phase_out = torch.empty(U.shape, dtype=torch.float32, device=U.device)
+    if amp_out is None:
+        amp_out = torch.empty(U.shape, dtype=torch.float32, device=U.device)
+    check(_lib.load().olx_encode_c64(_lib.ctx(U.device.index), _ptr(U), _ptr(phase_out),
+                                     _ptr(amp_out), U.numel()), "encode")
+    return phase_out, amp_out
+
+
+# ------------------------------------------------------------------ WRP
+def WRPStep1(obj, U, ny, nx, p, lam, d, plimit=None, square=False):
+    """WRPStep1 (include/golcgh.h:369-389); square=True is WRPStep1_D (:471-491).
+    d is the WRP plane position; plimit the pitch that sets the window."""
+    _out2d(U, ny, nx, torch.complex64, "U")
+    check(_lib.load().olx_wrp_c64(_lib.ctx(U.device.index), obj.ptr(), obj.size, _ptr(U), ny, nx,
+                                  p, lam, d, p if plimit is None else plimit,
+                                  _lib.WRP_SQUARE if square else 0), "WRPStep1")
+    return U
+
+
+def WRPStep1_D(obj, U, ny, nx, p, lam, d, plimit=None):
+    return WRPStep1(obj, U, ny, nx, p, lam, d, plimit, square=True)
+
+
+def WRPStep2(U, ny, nx, p, lam, d, propmode=AUTO):
+    """WRPStep2 (include/golcgh.h:494-517): AUTO uses ASM when AsmPropCheck holds,
+    else Fresnel when FresnelPropCheck holds, else ASM.  Returns the mode used."""
+    if propmode == AUTO:
+        if AsmPropCheck(ny, nx, p, p, lam, d) or not FresnelPropCheck(ny, nx, p, p, lam, d):
+            propmode = ASM
+        else:
+            propmode = FRESNEL
+    if propmode == ASM:
+        AsmProp(U, ny, nx, p, p, lam, d)
+    elif propmode == FRESNEL:
+        FresnelProp(U, ny, nx, p, p, lam, d)
+    else:
+        raise _lib.OlxInvalid(_lib.EINVAL, f"WRPStep2: unknown mode {propmode}")
+    return propmode
+
+
+def wrp_plane_z(ny, nx, p, lam, zmin, s=0.9, plimit=None, square=False):
+    """Plane rule of WRPMethod / WRPMethod_D (include/golcgh.h:524-533, :545-554)."""
+    return _lib.load().olx_wrp_plane_z(ny, nx, p, lam, zmin, p if plimit is None else plimit,
+                                       s, 1 if square else 0)
+
+
+def WRPMethod(obj, U, ny, nx, p, lam, zmin, s=0.9, propmode=AUTO, square=False, plimit=None):
+    """WRPMethod (include/golcgh.h:561-581); square=True is WRPMethod_D (:519-559).
+    Returns the WRP plane position used."""
+    _out2d(U, ny, nx, torch.complex64, "U")
+    pz = ctypes.c_double(0.0)
+    check(_lib.load().olx_wrp_method_c64(_lib.ctx(U.device.index), obj.ptr(), obj.size, _ptr(U),
+                                         ny, nx, p, lam, zmin, p if plimit is None else plimit, s,
+                                         propmode, _lib.WRP_SQUARE if square else 0,
+                                         ctypes.byref(pz)), "WRPMethod")
+    return pz.value
+
+
+def WRPMethod_D(obj, U, ny, nx, p, lam, zmin, s=0.9, propmode=AUTO, plimit=None):
+    return WRPMethod(obj, U, ny, nx, p, lam, zmin, s, propmode, square=True, plimit=plimit)
+
+
+# ------------------------------------------------------------------ layered CGH
+_PREC = {"f16": _lib.LAYERED_F16, "bf16": _lib.LAYERED_BF16, "bf16x3": _lib.LAYERED_BF16X3}
+
+
+def CghLayered(obj, U, ny, nx, py, px, lam, nlayers, zmin, zmax, precision="f16",
+               accumulate=False):
+    """Layered Fresnel CGH as complex GEMMs on MFMA (no reference counterpart).
+    Points are snapped to the centres of nlayers uniform layers over [zmin, zmax)."""
+    _out2d(U, ny, nx, torch.complex64, "U")
+    check(_lib.load().olx_cgh_layered_c64(_lib.ctx(U.device.index), obj.ptr(), obj.size, _ptr(U),
+                                          ny, nx, py, px, lam, nlayers, zmin, zmax,
+                                          _PREC[precision],
+                                          _lib.CGH_ACCUMULATE if accumulate else 0), "CghLayered")
+    return U
+
+
+def layer_depth(z, zmin, zmax, nlayers):
+    """Depth a point is snapped to by CghLayered."""
+    return _lib.load().olx_layer_depth(z, zmin, zmax, nlayers)

@@ -1,0 +1,179 @@
+"""GPU parity: direct CGH and WRP vs the oracle (through the C ABI).
+
+CGH tolerance: relative L2 <= 1e-4 against the FP64 oracle (the bar for the CGH
+path); WRP: <= 1e-5 against the oracle restatement at PREC_T = double.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import olref
+from openol_amd import synth
+
+pytestmark = pytest.mark.gpu
+LAM = 532e-9
+P = 8e-6
+
+
+def ol():
+    import openol_amd
+    return openol_amd
+
+
+def rel(a, b):
+    return olref.rel_l2(a, b)
+
+
+def cloud(n, seed, nx, ny, zmin=0.1, zmax=0.2, pitch=P):
+    """Seeded cloud; amp and phase rounded to float32, as olx_point stores them."""
+    pts = synth.random_points(n, seed=seed, xmax=nx * pitch, ymax=ny * pitch, zmin=zmin, zmax=zmax)
+    for k in ("amp", "phase"):
+        pts[k] = pts[k].astype(np.float32).astype(np.float64)
+    return pts
+
+
+def obj_of(pts, with_ap=True):
+    if with_ap:
+        return ol().Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    return ol().Object(pts["x"], pts["y"], pts["z"])
+
+
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("shape,n", [((12, 16), 7), ((64, 96), 300), ((100, 70), 257), ((256, 256), 1000)])
+def test_cgh_matches_fp64_oracle(exact, shape, n):
+    ny, nx = shape
+    pts = cloud(n, 40 + n, nx, ny)
+    U = torch.empty(shape, dtype=torch.complex64, device="cuda")
+    ol().Cgh(obj_of(pts), U, ny, nx, P, P, LAM, exact_r=exact)
+    ref = olref.cgh(pts["x"], pts["y"], pts["z"], ny, nx, P, P, LAM, pts["amp"], pts["phase"],
+                    flags=olref.CGH_EXACT_R if exact else 0, dtype=np.complex128)
+    assert rel(U.cpu().numpy(), ref) < 1e-4
+    U64 = torch.empty(shape, dtype=torch.complex128, device="cuda")
+    ol().CghFP64(obj_of(pts), U64, ny, nx, P, P, LAM, exact_r=exact)
+    # k r reaches 3e5 cycles: FP64 rounding of the phase alone is ~1e-10 rad
+    assert rel(U64.cpu().numpy(), ref) < 1e-9
+
+
+def test_cgh_reference_semantics_and_golden():
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+    o = ol().Object(g["cgh_x"], g["cgh_y"], g["cgh_z"])
+    U = torch.empty((12, 16), dtype=torch.complex64, device="cuda")
+    ol().Cgh(o, U, 12, 16, 6e-6, 8e-6, LAM)  # A = 1, phi = 0: _CghFresnel itself
+    assert rel(U.cpu().numpy(), g["cgh_plain"]) < 1e-5
+    # the oracle's complex64 restatement (float accumulation) agrees too
+    ref32 = olref.cgh(g["cgh_x"], g["cgh_y"], g["cgh_z"], 12, 16, 6e-6, 8e-6, LAM)
+    assert rel(U.cpu().numpy(), ref32) < 1e-5
+
+
+def test_cgh_outputs_phase_amp_accumulate():
+    ny, nx, n = 48, 80, 200
+    pts = cloud(n, 41, nx, ny)
+    o = obj_of(pts)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ph = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    am = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    ol().Cgh(o, U, ny, nx, P, P, LAM, phase_out=ph, amp_out=am)
+    u = U.cpu().numpy()
+    assert np.allclose(ph.cpu().numpy(), np.angle(u), atol=1e-5)
+    assert np.allclose(am.cpu().numpy(), np.abs(u), rtol=1e-6)
+    # phase-only overload: float32 output tensor (include/olcgh.h:362-365)
+    ph2 = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    ol().Cgh(o, ph2, ny, nx, P, P, LAM)
+    assert torch.equal(ph2, ph)
+    # encode kernel
+    p3, a3 = ol().encode(U)
+    assert torch.allclose(p3, ph, rtol=0, atol=2e-6) and torch.allclose(a3, am, rtol=1e-6, atol=0)
+    # accumulate: two halves sum to the whole
+    h1 = ol().Object(pts["x"][:90], pts["y"][:90], pts["z"][:90], pts["amp"][:90], pts["phase"][:90])
+    h2 = ol().Object(pts["x"][90:], pts["y"][90:], pts["z"][90:], pts["amp"][90:], pts["phase"][90:])
+    V = torch.empty_like(U)
+    ol().Cgh(h1, V, ny, nx, P, P, LAM)
+    ol().Cgh(h2, V, ny, nx, P, P, LAM, accumulate=True)
+    assert rel(V.cpu().numpy(), u) < 1e-5
+
+
+def test_cgh_gauss_sum_closed_form():
+    z = 2 * P * P / LAM
+    o = ol().Object([5 * P], [3 * P], [z])
+    U = torch.empty((10, 12), dtype=torch.complex64, device="cuda")
+    ol().Cgh(o, U, 10, 12, P, P, LAM)
+    hh, ww = np.meshgrid(np.arange(10), np.arange(12), indexing="ij")
+    expect = 1j ** (((ww - 5) ** 2 + (hh - 3) ** 2) % 4)
+    # the FP32 tile offsets carry ~1e-5 rad of rounding at these slopes
+    assert np.max(np.abs(U.cpu().numpy() - expect)) < 1e-4
+
+
+def test_cgh_empty_cloud():
+    o = ol().Object(np.zeros(0), np.zeros(0), np.zeros(0))
+    U = torch.ones((32, 32), dtype=torch.complex64, device="cuda")
+    ol().Cgh(o, U, 32, 32, P, P, LAM)
+    assert torch.count_nonzero(U).item() == 0
+
+
+def test_cgh_config2_sampled_fp64():
+    """Config: 10,000 points, 2048^2 hologram, exact r and Fresnel; FP64 oracle on
+    4096 sampled pixels (each pixel is an independent sum)."""
+    n, N = 10000, 2048
+    pts = cloud(n, 2, N, N, 0.10, 0.20)
+    o = obj_of(pts)
+    rng = np.random.default_rng(5)
+    ph, pw = rng.integers(0, N, 4096), rng.integers(0, N, 4096)
+    for exact in (True, False):
+        U = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+        ol().Cgh(o, U, N, N, P, P, LAM, exact_r=exact)
+        got = U[torch.from_numpy(ph).cuda(), torch.from_numpy(pw).cuda()].cpu().numpy()
+        ref = olref.cgh_pixels(pts["x"], pts["y"], pts["z"], ph, pw, P, P, LAM, pts["amp"],
+                               pts["phase"], flags=olref.CGH_EXACT_R if exact else 0)
+        assert rel(got, ref) < 1e-4, exact
+
+
+@pytest.mark.parametrize("square", [False, True])
+def test_wrp_matches_oracle(square):
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+    o = ol().Object(g["wrp_x"], g["wrp_y"], g["wrp_z"])
+    U = torch.empty((24, 20), dtype=torch.complex64, device="cuda")
+    ol().WRPStep1(o, U, 24, 20, P, LAM, 0.15, square=square)
+    assert rel(U.cpu().numpy(), g["wrp_square" if square else "wrp_circle"]) < 1e-5
+    # a larger plane with many overlapping windows
+    ny, nx, n = 200, 300, 3000
+    pts = cloud(n, 42, nx, ny, 0.151, 0.155)
+    o = obj_of(pts, with_ap=False)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ol().WRPStep1(o, U, ny, nx, P, LAM, 0.15, square=square)
+    ref = olref.wrp(pts["x"], pts["y"], pts["z"], ny, nx, P, LAM, 0.15, square=square,
+                    dtype=np.complex128)
+    assert rel(U.cpu().numpy(), ref) < 1e-5
+    # deterministic summation order
+    U2 = torch.empty_like(U)
+    ol().WRPStep1(o, U2, ny, nx, P, LAM, 0.15, square=square)
+    assert torch.equal(U, U2)
+
+
+def test_wrp_plimit_and_edges():
+    # points near and beyond the plane edges, and a p_limit overload (golcgh.h:381-389)
+    ny, nx = 64, 48
+    x = np.array([-3 * P, 0.0, nx * P + 2 * P, 20.5 * P, 47 * P])
+    y = np.array([5 * P, -P, 30 * P, 63 * P, 64 * P + 4 * P])
+    z = np.array([0.153, 0.152, 0.155, 0.1505, 0.154])
+    o = ol().Object(x, y, z)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ol().WRPStep1(o, U, ny, nx, P, LAM, 0.15, plimit=2 * P)
+    ref = olref.wrp(x, y, z, ny, nx, P, LAM, 0.15, p_limit=2 * P, dtype=np.complex128)
+    assert rel(U.cpu().numpy(), ref) < 1e-5
+
+
+def test_wrp_method_pipeline():
+    ny = nx = 256
+    pts = cloud(2000, 43, nx, ny, 0.10, 0.104)
+    o = obj_of(pts, with_ap=False)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    zmin = float(pts["z"].min())
+    pz = ol().WRPMethod(o, U, ny, nx, P, LAM, zmin, s=0.99)
+    assert pz == olref.wrp_plane_z(ny, nx, P, LAM, zmin, 0.99)
+    w = olref.wrp(pts["x"], pts["y"], pts["z"], ny, nx, P, LAM, pz)
+    ch = 0 if (olref.asm_prop_check(ny, nx, P, P, LAM, pz)
+               or not olref.fresnel_prop_check(ny, nx, P, P, LAM, pz)) else 1
+    ref = olref.asm_prop(w, P, P, LAM, pz) if ch == 0 else olref.fresnel_prop(w, P, P, LAM, pz)
+    assert rel(U.cpu().numpy(), ref) < 1e-5

@@ -1,0 +1,219 @@
+"""GPU parity: fused ASM / Fresnel / ARSS propagation vs the oracle (through the C ABI).
+
+Tolerances: relative L2 <= 1e-5 against the oracle's fftw3f-path restatement
+(complex64 fields, PREC_T = double), the bar stated for the propagation path.
+"""
+import numpy as np
+import pytest
+import torch
+
+import olref
+from openol_amd import synth
+
+pytestmark = pytest.mark.gpu
+LAM = 532e-9
+TOL = 1e-5
+
+
+def ol():
+    import openol_amd
+    return openol_amd
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex64)).cuda()
+
+
+def rel(a, b):
+    return olref.rel_l2(a, b)
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (16, 64), (256, 128), (1024, 1024), (8192, 16)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft2d_matches_numpy(shape, sign):
+    u0 = synth.random_field(*shape, seed=21)
+    u = dev(u0)
+    (ol().fft if sign < 0 else ol().ifft)(u)
+    ref = np.fft.fft2(u0.astype(np.complex128)) if sign < 0 else \
+        np.fft.ifft2(u0.astype(np.complex128)) * (shape[0] * shape[1])
+    assert rel(u.cpu().numpy(), ref) < 2e-6
+
+
+@pytest.mark.parametrize("shape,dy,dx,z", [
+    ((8, 8), 6e-6, 8e-6, 0.01), ((8, 4), 6e-6, 8e-6, 0.01), ((4, 16), 8e-6, 8e-6, 0.003),
+    ((64, 64), 8e-6, 8e-6, 0.02), ((128, 256), 4e-6, 8e-6, 0.05), ((512, 512), 8e-6, 8e-6, 0.1)])
+@pytest.mark.parametrize("flags", [0, olref.BANDLIMIT])
+def test_asm_matches_oracle(shape, dy, dx, z, flags):
+    u0 = synth.random_field(*shape, seed=22)
+    u = dev(u0)
+    ol().AsmProp(u, shape[0], shape[1], dy, dx, LAM, z, bandlimit=bool(flags))
+    ref = olref.asm_prop(u0, dy, dx, LAM, z, flags)
+    assert rel(u.cpu().numpy(), ref) < TOL
+
+
+def test_asm_bandlimit_active_matches_golden():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+    for tag, shp in (("sq", (8, 8)), ("rect", (8, 4))):
+        u = dev(g[f"field_{tag}"])
+        ol().AsmProp(u, *shp, 1e-6, 1e-6, LAM, 1e-4, bandlimit=True)
+        assert rel(u.cpu().numpy(), g[f"asm_bl_{tag}"]) < TOL
+        u = dev(g[f"field_{tag}"])
+        ol().AsmProp(u, *shp, 6e-6, 8e-6, LAM, 0.01)
+        assert rel(u.cpu().numpy(), g[f"asm_{tag}"]) < TOL
+
+
+def test_asm_config1_full_size():
+    """Config: 1024^2 field, 8 um pitch, z = 0.10 m, padded to 2048^2, band-limited TF."""
+    n = 1024
+    u0 = synth.random_field(n, n, seed=1)
+    u = dev(u0)
+    ol().AsmProp(u, n, n, 8e-6, 8e-6, LAM, 0.10, bandlimit=True)
+    got = u.cpu().numpy()
+    assert rel(got, olref.asm_prop(u0, 8e-6, 8e-6, LAM, 0.10, olref.BANDLIMIT)) < TOL
+    # fftw3 (complex128) path and FP64: the FP32 floor is ~2e-7
+    ref64 = olref.asm_prop(u0.astype(np.complex128), 8e-6, 8e-6, LAM, 0.10, olref.BANDLIMIT)
+    assert rel(got, ref64) < TOL
+    # intensity output
+    assert rel(np.abs(got) ** 2, np.abs(ref64) ** 2) < 2 * TOL
+
+
+def test_asm_adjoint_matches_oracle_and_dot_product():
+    ny, nx = 128, 64
+    u0 = synth.random_field(ny, nx, seed=23)
+    v0 = synth.random_field(ny, nx, seed=24)
+    a = dev(u0)
+    ol().AsmProp(a, ny, nx, 8e-6, 6e-6, LAM, 0.03, adjoint=True)
+    assert rel(a.cpu().numpy(), olref.asm_prop(u0, 8e-6, 6e-6, LAM, 0.03, olref.ADJOINT)) < TOL
+    Au = dev(u0)
+    ol().AsmProp(Au, ny, nx, 8e-6, 6e-6, LAM, 0.03)
+    AHv = dev(v0)
+    ol().AsmProp(AHv, ny, nx, 8e-6, 6e-6, LAM, 0.03, adjoint=True)
+    lhs = np.vdot(v0.astype(np.complex128), Au.cpu().numpy().astype(np.complex128))
+    rhs = np.vdot(AHv.cpu().numpy().astype(np.complex128), u0.astype(np.complex128))
+    assert abs(lhs - rhs) < 1e-5 * abs(lhs)
+
+
+def test_asm_full_size_properties():
+    """4096^2 (the metric workload): z = 0 identity, linearity, adjoint dot product."""
+    n = 4096
+    u0 = synth.random_field(n, n, seed=25)
+    u = dev(u0)
+    ol().AsmProp(u, n, n, 8e-6, 8e-6, LAM, 0.0, bandlimit=True)
+    assert rel(u.cpu().numpy(), u0) < 1e-6
+    v0 = synth.random_field(n, n, seed=26)
+    a, b = dev(u0), dev(v0)
+    c = dev(u0 + 0.5j * v0)
+    for t in (a, b, c):
+        ol().AsmProp(t, n, n, 8e-6, 8e-6, LAM, 0.10, bandlimit=True)
+    assert torch.linalg.norm(a + 0.5j * b - c) / torch.linalg.norm(c) < 1e-6
+    vh = dev(v0)
+    ol().AsmProp(vh, n, n, 8e-6, 8e-6, LAM, 0.10, bandlimit=True, adjoint=True)
+    lhs = torch.vdot(torch.from_numpy(v0).cuda().to(torch.complex128).flatten(),
+                     a.to(torch.complex128).flatten())
+    rhs = torch.vdot(vh.to(torch.complex128).flatten(),
+                     torch.from_numpy(u0).cuda().to(torch.complex128).flatten())
+    assert abs((lhs - rhs) / lhs).item() < 1e-5
+
+
+@pytest.mark.parametrize("n", [2048])
+def test_asm_matches_oracle_large(n):
+    u0 = synth.random_field(n, n, seed=27)
+    u = dev(u0)
+    ol().AsmProp(u, n, n, 8e-6, 8e-6, LAM, 0.12, bandlimit=True)
+    assert rel(u.cpu().numpy(), olref.asm_prop(u0, 8e-6, 8e-6, LAM, 0.12, olref.BANDLIMIT)) < TOL
+
+
+@pytest.mark.parametrize("variant,flags", [(None, 0), ("cpu", olref.FRESNEL_BL_CPU),
+                                           ("gpu", olref.FRESNEL_BL_GPU)])
+@pytest.mark.parametrize("shape,dy,dx,z", [((8, 8), 6e-6, 8e-6, 0.001), ((8, 4), 6e-6, 8e-6, 0.002),
+                                           ((256, 128), 8e-6, 6e-6, 0.05), ((512, 512), 8e-6, 8e-6, 0.2)])
+def test_fresnel_matches_oracle(variant, flags, shape, dy, dx, z):
+    u0 = synth.random_field(*shape, seed=28)
+    u = dev(u0)
+    if variant is None:
+        ol().FresnelProp(u, *shape, dy, dx, LAM, z)
+    else:
+        ol().FresnelPropBandLimit(u, *shape, dy, dx, LAM, z, variant=variant)
+    ref = olref.fresnel_prop(u0, dy, dx, LAM, z, flags)
+    assert rel(u.cpu().numpy(), ref) < TOL
+
+
+def test_fresnel_adjoint():
+    u0 = synth.random_field(64, 128, seed=29)
+    u = dev(u0)
+    ol().FresnelProp(u, 64, 128, 8e-6, 8e-6, LAM, 0.04, adjoint=True)
+    ref = olref.fresnel_prop(u0, 8e-6, 8e-6, LAM, 0.04, olref.ADJOINT)
+    assert rel(u.cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("shape,dy,dx,z,s,oy,ox", [
+    ((8, 8), 6e-6, 8e-6, 0.002, 0.5, 0.0, 0.0),
+    ((8, 4), 12e-6, 16e-6, -0.0005, 0.5, 3e-5, -2e-5),
+    ((256, 256), 8e-6, 8e-6, 0.15, 0.5, 0.0, 0.0),
+    ((512, 256), 16e-6, 16e-6, -0.1, 0.25, 1e-4, -2e-4),
+    ((128, 128), 8e-6, 8e-6, 0.05, 2.0, 0.0, 0.0),
+    ((64, 64), 8e-6, 8e-6, 0.01, 1.0, 0.0, 0.0)])
+def test_arss_matches_oracle(shape, dy, dx, z, s, oy, ox):
+    u0 = synth.random_field(*shape, seed=30)
+    src = dev(u0)
+    dst = torch.empty_like(src)
+    ol().ARSSFresnelProp(src, dst, *shape, dy, dx, LAM, z, s, oy, ox)
+    ref = olref.arss_prop(u0, dy, dx, LAM, z, s, oy, ox)
+    assert rel(dst.cpu().numpy(), ref) < TOL
+    assert torch.equal(src.cpu(), torch.from_numpy(u0))  # source untouched
+    # aliasing src == dst (tests/gProp_test.cu:67 calls it in place)
+    ol().ARSSFresnelProp(src, src, *shape, dy, dx, LAM, z, s, oy, ox)
+    assert rel(src.cpu().numpy(), ref) < TOL
+
+
+def test_arss_config3_size_properties():
+    """4096^2 WRP -> ARSS s = 0.5, z = 0.15 m (config 3 shape): linearity and a
+    2048^2 oracle comparison at the same optical parameters."""
+    n = 2048
+    u0 = synth.random_field(n, n, seed=31)
+    src = dev(u0)
+    ol().ARSSFresnelProp(src, src, n, n, 8e-6, 8e-6, LAM, 0.15, 0.5)
+    assert rel(src.cpu().numpy(), olref.arss_prop(u0, 8e-6, 8e-6, LAM, 0.15, 0.5)) < TOL
+    n = 4096
+    a0 = synth.random_field(n, n, seed=32)
+    b0 = synth.random_field(n, n, seed=33)
+    a, b, c = dev(a0), dev(b0), dev(a0 - 2j * b0)
+    for t in (a, b, c):
+        ol().ARSSFresnelProp(t, t, n, n, 8e-6, 8e-6, LAM, 0.15, 0.5)
+    assert torch.linalg.norm(a - 2j * b - c) / torch.linalg.norm(c) < 1e-6
+
+
+def test_prop_dispatcher():
+    n, dx = 64, 8e-6
+    du = 1 / (dx * 2 * n)
+    umax = du * n
+    zmax = np.sqrt(1 / LAM ** 2 - 2 * umax ** 2) / (2 * umax * du)
+    zmin = 2 * (dx * n) * dx / LAM
+    u0 = synth.random_field(n, n, seed=34)
+    for z, want in ((0.5 * zmax, 0), (2 * zmin, 1), (0.5 * (zmax + zmin), 3)):
+        assert ol().AsmPropCheck(n, n, dx, dx, LAM, z) == olref.asm_prop_check(n, n, dx, dx, LAM, z)
+        assert ol().FresnelPropCheck(n, n, dx, dx, LAM, z) == olref.fresnel_prop_check(n, n, dx, dx, LAM, z)
+        u = dev(u0)
+        ch = ol().Prop(u, n, n, dx, dx, LAM, z)
+        assert ch == want
+        ref, rch = olref.prop(u0, dx, dx, LAM, z, mode=2)
+        assert rel(u.cpu().numpy(), ref) < TOL
+
+
+def test_invalid_arguments_raise():
+    from openol_amd._lib import OlxError, OlxInvalid
+    u = dev(synth.random_field(12, 16, seed=1))
+    with pytest.raises(OlxError):
+        ol().AsmProp(u, 12, 16, 8e-6, 8e-6, LAM, 0.1)  # not a power of two
+    u = dev(synth.random_field(16, 16, seed=1))
+    with pytest.raises(OlxInvalid):
+        ol().AsmProp(u, 16, 16, 8e-6, 8e-6, -LAM, 0.1)
+    with pytest.raises(OlxInvalid):
+        ol().FresnelProp(u, 16, 16, 8e-6, 8e-6, LAM, 0.0)
+    with pytest.raises(OlxInvalid):
+        ol().AsmProp(u, 16, 8, 8e-6, 8e-6, LAM, 0.1)  # shape mismatch
+    with pytest.raises(OlxInvalid):
+        ol().AsmProp(u.to(torch.complex128), 16, 16, 8e-6, 8e-6, LAM, 0.1)
+    with pytest.raises(OlxInvalid):
+        ol().Prop(u, 16, 16, 8e-6, 8e-6, LAM, 0.1, propmode=7)
