@@ -1,8 +1,8 @@
 // fft_core.hpp -- LDS-staged Stockham FFT building blocks for gfx950.
 //
 // A length-L transform (L = 2^p, 8 <= L <= 16384) is carried by T = L/E threads,
-// each holding E = min(16, L) complex values in registers.  Stages use radix 16
-// (the remainder radix 2, 4 or 8 last); between stages the data goes through LDS
+// each holding E = min(16, L) complex values in registers.  Stages use radix 16,
+// the remainder radix 2, 4 or 8 first; between stages the data goes through LDS
 // (one float2 of padding every 16 elements against bank conflicts).  The first
 // stage loads straight from HBM and the last stage stores straight to HBM, so the
 // caller can fuse zero-padding (statically skipped loads), pointwise factors and
@@ -137,7 +137,12 @@ struct Plan {
     static constexpr int REM = L < 16 ? L : (1 << (LOG % 4));
     static constexpr int NST = N16 + (REM > 1 ? 1 : 0);
     static constexpr int LP = L + L / 16;  // padded LDS length
-    __host__ __device__ static constexpr int radix(int s) { return s < N16 ? 16 : REM; }
+    // remainder radix first: with centred 2x padding its butterflies see one
+    // nonzero input each, and the final radix-16 stage of a cropped inverse
+    // transform has whole outputs that are never stored (dead code)
+    __host__ __device__ static constexpr int radix(int s) {
+        return REM > 1 ? (s == 0 ? REM : 16) : 16;
+    }
     __host__ __device__ static constexpr int ns(int s) {
         int p = 1;
         for (int i = 0; i < s; i++) p *= radix(i);
@@ -161,23 +166,33 @@ __device__ __forceinline__ void apply_twiddles(float2* v, int base, const float2
         v[2] = cmul(v[2], w2);
         v[3] = cmul(v[3], cmul(w1, w2));
     } else {
-        float2 w[R];
+        // w1, w2, w4, w8 from the table; the others as products, applied as soon as
+        // they exist so that few twiddles are live at once
         float2 w1 = tw[base], w2 = tw[2 * base], w4 = tw[4 * base];
         if (INV) { w1 = cconj(w1); w2 = cconj(w2); w4 = cconj(w4); }
-        w[1] = w1; w[2] = w2; w[4] = w4;
-        w[3] = cmul(w1, w2);
-        w[5] = cmul(w1, w4);
-        w[6] = cmul(w2, w4);
-        w[7] = cmul(w[3], w4);
+        v[1] = cmul(v[1], w1);
+        v[2] = cmul(v[2], w2);
+        v[4] = cmul(v[4], w4);
+        const float2 w3 = cmul(w1, w2);
+        v[3] = cmul(v[3], w3);
+        const float2 w5 = cmul(w1, w4);
+        v[5] = cmul(v[5], w5);
+        const float2 w6 = cmul(w2, w4);
+        v[6] = cmul(v[6], w6);
+        const float2 w7 = cmul(w3, w4);
+        v[7] = cmul(v[7], w7);
         if constexpr (R == 16) {
             float2 w8 = tw[8 * base];
             if (INV) w8 = cconj(w8);
-            w[8] = w8;
-#pragma unroll
-            for (int r = 9; r < 16; r++) w[r] = cmul(w[r - 8], w8);
+            v[8] = cmul(v[8], w8);
+            v[9] = cmul(v[9], cmul(w1, w8));
+            v[10] = cmul(v[10], cmul(w2, w8));
+            v[11] = cmul(v[11], cmul(w3, w8));
+            v[12] = cmul(v[12], cmul(w4, w8));
+            v[13] = cmul(v[13], cmul(w5, w8));
+            v[14] = cmul(v[14], cmul(w6, w8));
+            v[15] = cmul(v[15], cmul(w7, w8));
         }
-#pragma unroll
-        for (int r = 1; r < R; r++) v[r] = cmul(v[r], w[r]);
     }
 }
 
@@ -262,21 +277,23 @@ __device__ __forceinline__ int first_index(int t, int b, int r) {
 }
 
 // For a transform whose input is nonzero only on [L/4, 3L/4) (centred 2x zero
-// padding), stage-0 element (b, r) is statically zero when this returns false.
-// Valid because stage 0 reads j + r L/R0 with j < L/R0.
+// padding): whether stage-0 element (b, r) can be nonzero.  Thread t of
+// butterfly block b reads j + r L/R0 with j = t + T b, t < T, and L/4 is a
+// multiple of T, so the answer does not depend on t.
 template <int L>
-__device__ __forceinline__ constexpr bool first_nonzero(int r) {
-    constexpr int R = Plan<L>::radix(0);
-    return r >= R / 4 && r < 3 * R / 4;
+__device__ __forceinline__ constexpr bool first_nonzero(int b, int r) {
+    constexpr int R0 = Plan<L>::radix(0), T = Plan<L>::T;
+    const int lo = T * b + r * (L / R0);
+    return lo >= L / 4 && lo + T <= 3 * L / 4;
 }
 
-// Whether last-stage element (b, r) can fall in the kept centre [L/4, 3L/4):
-// static for last radix >= 4, otherwise the caller tests the index.
+// Whether last-stage element (b, r) falls in the kept centre [L/4, 3L/4) of a
+// cropped transform (same argument: the output index is j + r L/RL).
 template <int L>
-__device__ __forceinline__ constexpr bool last_maybe_kept(int r) {
-    constexpr int R = Plan<L>::radix(Plan<L>::NST - 1);
-    if constexpr (R >= 4) return r >= R / 4 && r < 3 * R / 4;
-    else return true;
+__device__ __forceinline__ constexpr bool last_kept(int b, int r) {
+    constexpr int RL = Plan<L>::radix(Plan<L>::NST - 1), T = Plan<L>::T;
+    const int lo = T * b + r * (L / RL);
+    return lo >= L / 4 && lo + T <= 3 * L / 4;
 }
 
 }  // namespace olx

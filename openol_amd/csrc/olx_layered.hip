@@ -1,20 +1,384 @@
-// olx_layered.hip -- layered Fresnel CGH as complex GEMMs on MFMA (placeholder until the
-// MFMA kernel lands: the entry point reports EUNSUPPORTED).
+// olx_layered.hip -- layered Fresnel CGH as complex GEMMs on MFMA (gfx950).
+//
+// Points are snapped to the centres z_l of nlayers uniform layers.  For a point
+// j in layer l (c_l = 1 / (lambda z_l)) the Fresnel term factorises:
+//   exp(i pi ((x_w - x_j)^2 + (y_h - y_j)^2) c_l + i phi_j) A_j
+//     = [cy_l(h) e^{-2 pi i h beta_j} a_j] [cx_l(w) e^{-2 pi i w alpha_j}]
+// with the layer chirps cx_l(w) = e^{i pi (px w)^2 c_l}, cy_l(h) likewise, the
+// cycles per pixel alpha_j = px x_j c_l, beta_j = py y_j c_l, and the point
+// factor a_j = A_j e^{i (phi_j + pi (x_j^2 + y_j^2) c_l)}.  Hence
+//   U = Y X^T,  Y[h, j] = cy_l(h) e^{-2 pi i h beta_j} a_j,  X[w, j] = cx_l(w) e^{-2 pi i w alpha_j},
+// one complex GEMM with K = number of points, each layer's chirp folded into its
+// points' factors (so no per-layer epilogue).  The complex product is two real
+// MFMA GEMMs over K' = 2K interleaved (re, im) columns:
+//   Re U = A Br^T,  Im U = A Bi^T,  A = [Re Y, Im Y],  Br = [Re X, -Im X],  Bi = [Im X, Re X].
+// Phases are reduced exactly before quantisation: chirps in FP64 tables, and
+// w alpha_j mod 1 as w alpha_hi (exact in FP32, alpha_hi has few fraction bits)
+// plus w alpha_lo.  Factors are stored FP16, BF16, or split BF16 (hi + lo, three
+// products Yh Xh + Yh Xl + Yl Xh); accumulation is FP32 throughout.
+#include <climits>
+
 #include "olx_common.hpp"
+
+namespace olx {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef float f16v_t __attribute__((ext_vector_type(16)));
+
+struct LayPt {       // 32 bytes
+    int layer;
+    float ahi, alo;  // alpha_j = ahi + alo (cycles per pixel along w), mod 1
+    float bhi, blo;  // beta_j along h
+    float2 a;        // point factor a_j
+    float pad;
+};
+
+constexpr int GT = 128;    // output tile side (pixels)
+constexpr int KS = 64;     // K step (halves) per LDS stage = 32 points
+constexpr int KP = KS + 8; // padded LDS row (halves)
+constexpr int KC_PTS = 32768;  // points per factor chunk
+
+__host__ __device__ inline int layer_of(double z, double zmin, double zmax, int nl) {
+    double f = floor((z - zmin) / (zmax - zmin) * nl);
+    if (!(f >= 0.0)) f = 0.0;
+    if (f > nl - 1) f = nl - 1;
+    return (int)f;
+}
+
+// split v mod 1 into hi (nb fraction bits) + lo
+__device__ inline void split_cycles(double v, int nb, float* hi, float* lo) {
+    v -= floor(v);
+    const double s = ldexp(1.0, nb);
+    const double h = floor(v * s) / s;
+    *hi = (float)h;
+    *lo = (float)(v - h);
+}
+
+__global__ void k_lay_prep(const olx_point* __restrict__ pts, int n, LayPt* __restrict__ out,
+                           double px, double py, double lambda, double zmin, double zmax, int nl,
+                           int nb) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const olx_point p = pts[j];
+    const int l = layer_of(p.z, zmin, zmax, nl);
+    const double zl = zmin + ((double)l + 0.5) * (zmax - zmin) / nl;
+    const double c = 1.0 / (lambda * zl);
+    LayPt q;
+    q.layer = l;
+    split_cycles(px * p.x * c, nb, &q.ahi, &q.alo);
+    split_cycles(py * p.y * c, nb, &q.bhi, &q.blo);
+    double cyc = 0.5 * (p.x * p.x + p.y * p.y) * c + (double)p.phase * (0.5 / M_PI);
+    cyc -= rint(cyc);
+    double s, co;
+    sincospi(2.0 * cyc, &s, &co);
+    q.a = make_float2((float)(p.amp * co), (float)(p.amp * s));
+    q.pad = 0.f;
+    out[j] = q;
+}
+
+// per-layer chirp tables: ch[l * n + i] = exp(i pi (d i)^2 c_l)
+__global__ void k_lay_chirp(float2* __restrict__ ch, int n, int nl, double d, double lambda,
+                            double zmin, double zmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
+    if (i >= n) return;
+    const double zl = zmin + ((double)l + 0.5) * (zmax - zmin) / nl;
+    const double x = d * i;
+    double cyc = 0.5 * x * x / (lambda * zl);
+    cyc -= rint(cyc);
+    double s, c;
+    sincospi(2.0 * cyc, &s, &c);
+    ch[(size_t)l * n + i] = make_float2((float)c, (float)s);
+}
+
+template <typename T>
+__device__ inline uint32_t pack2(float a, float b, float* ra, float* rb);
+template <>
+__device__ inline uint32_t pack2<_Float16>(float a, float b, float* ra, float* rb) {
+    const _Float16 x = (_Float16)a, y = (_Float16)b;
+    *ra = (float)x;
+    *rb = (float)y;
+    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+}
+template <>
+__device__ inline uint32_t pack2<__bf16>(float a, float b, float* ra, float* rb) {
+    const __bf16 x = (__bf16)a, y = (__bf16)b;
+    *ra = (float)x;
+    *rb = (float)y;
+    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+}
+
+constexpr int GEN_RB = 16;  // rows per thread (one exact anchor, then a recurrence)
+
+// factor rows: for rows r (h for Y, w for X) and chunk points j in [0, kc):
+//   Y: A[r][2j] = Re, A[r][2j+1] = Im of cy_l(r) e^{-2 pi i r beta} a
+//   X: B[r][2j] = Re, B[r][2j+1] = -Im of cx_l(r) e^{-2 pi i r alpha}
+// A thread owns one point and GEN_RB consecutive rows: the phase e^{-2 pi i r alpha}
+// is evaluated exactly (reduced) at the first row and advanced by the exact step
+// e^{-2 pi i alpha} (rounding drift ~1e-6 over the run, far below FP16/BF16 ulps).
+// MODE 0: one factor matrix; MODE 1: hi in out, lo = value - hi in out2 (split bf16)
+template <typename T, bool ISY, int MODE>
+__global__ void k_lay_gen(const LayPt* __restrict__ pts, int j0, int n, int kc,
+                          const float2* __restrict__ chirp, int rows, uint32_t* __restrict__ out,
+                          uint32_t* __restrict__ out2) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;  // chunk-local point
+    const int rb = blockIdx.y * GEN_RB;
+    if (j >= kc) return;
+    const bool live = j0 + j < n;
+    LayPt p;
+    if (live) p = pts[j0 + j];
+    else { p.layer = 0; p.ahi = p.alo = p.bhi = p.blo = 0.f; p.a = make_float2(0.f, 0.f); }
+    const float hi = ISY ? p.bhi : p.ahi, lo = ISY ? p.blo : p.alo;
+    // anchor e^{-2 pi i rb alpha} and step e^{-2 pi i alpha}
+    const float fr = (float)rb;
+    float t = fr * hi;  // exact: hi has few fraction bits
+    t = (t - floorf(t)) + fr * lo;
+    t = -(t - rintf(t));
+    float2 e = make_float2(__builtin_amdgcn_cosf(t), __builtin_amdgcn_sinf(t));
+    float ts = hi + lo;
+    ts = -(ts - rintf(ts));
+    const float2 stp = make_float2(__builtin_amdgcn_cosf(ts), __builtin_amdgcn_sinf(ts));
+    const float2* chl = chirp + (size_t)p.layer * rows;
+    for (int q = 0; q < GEN_RB; q++) {
+        const int r = rb + q;
+        if (r >= rows) break;
+        const float2 ch = chl[r];
+        float2 v = make_float2(ch.x * e.x - ch.y * e.y, ch.x * e.y + ch.y * e.x);
+        if (ISY) v = make_float2(v.x * p.a.x - v.y * p.a.y, v.x * p.a.y + v.y * p.a.x);
+        else v.y = -v.y;
+        if (!live) v = make_float2(0.f, 0.f);
+        const size_t o = (size_t)r * kc + j;  // in 32-bit words (2 halves each)
+        float hx, hy;
+        out[o] = pack2<T>(v.x, v.y, &hx, &hy);
+        if (MODE == 1) {
+            float dx_, dy_;
+            out2[o] = pack2<T>(v.x - hx, v.y - hy, &dx_, &dy_);
+        }
+        e = make_float2(e.x * stp.x - e.y * stp.y, e.x * stp.y + e.y * stp.x);
+    }
+}
+
+// Bi operand from Br: per dword (lo = Re X, hi = -Im X) -> (lo = Im X, hi = Re X)
+__device__ inline uint32_t bi_of_br(uint32_t w) {
+    const uint32_t sw = (w >> 16) | (w << 16);  // (lo = -Im X, hi = Re X)
+    return sw ^ 0x00008000u;                    // negate the low half
+}
+
+template <typename T>
+__device__ inline f16v_t mfma(const uint4& a, const uint4& b, f16v_t c);
+template <>
+__device__ inline f16v_t mfma<_Float16>(const uint4& a, const uint4& b, f16v_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8_t, a),
+                                                  __builtin_bit_cast(half8_t, b), c, 0, 0, 0);
+}
+template <>
+__device__ inline f16v_t mfma<__bf16>(const uint4& a, const uint4& b, f16v_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8_t, a),
+                                                   __builtin_bit_cast(bf8_t, b), c, 0, 0, 0);
+}
+
+// U[h][w] (+)= sum_k A[h][k] Br[w][k]  (re),  A[h][k] Bi[w][k]  (im), k < ldk.
+// Work-group: 256 threads, 128 x 128 output tile; wave (wr, wc) owns 64 x 64.
+template <typename T>
+__global__ void __launch_bounds__(256, 2)
+k_lay_gemm(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* __restrict__ U,
+           int nx, int accumulate) {
+    __shared__ __align__(16) T sA[2][GT * KP];
+    __shared__ __align__(16) T sB[2][GT * KP];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int h0 = blockIdx.y * GT, w0 = blockIdx.x * GT;
+    // global -> LDS: 128 rows x 64 halves = 1024 x 16 B per matrix; 4 per thread
+    const T* gA = A + (size_t)h0 * ldk;
+    const T* gB = Br + (size_t)w0 * ldk;
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+    // element (row, seg) of the q-th 16-byte piece of this thread
+#define OLX_GLOAD(k0)                                                                         \
+    do {                                                                                      \
+        const int r0_ = tid >> 3, sg_ = (tid & 7) * 8;                                        \
+        ra0 = *(const uint4*)(gA + (size_t)(r0_)*ldk + (k0) + sg_);                           \
+        ra1 = *(const uint4*)(gA + (size_t)(r0_ + 32) * ldk + (k0) + sg_);                    \
+        ra2 = *(const uint4*)(gA + (size_t)(r0_ + 64) * ldk + (k0) + sg_);                    \
+        ra3 = *(const uint4*)(gA + (size_t)(r0_ + 96) * ldk + (k0) + sg_);                    \
+        rb0 = *(const uint4*)(gB + (size_t)(r0_)*ldk + (k0) + sg_);                           \
+        rb1 = *(const uint4*)(gB + (size_t)(r0_ + 32) * ldk + (k0) + sg_);                    \
+        rb2 = *(const uint4*)(gB + (size_t)(r0_ + 64) * ldk + (k0) + sg_);                    \
+        rb3 = *(const uint4*)(gB + (size_t)(r0_ + 96) * ldk + (k0) + sg_);                    \
+    } while (0)
+#define OLX_SSTORE(buf)                                                                       \
+    do {                                                                                      \
+        const int r0_ = tid >> 3, sg_ = (tid & 7) * 8;                                        \
+        *(uint4*)(&sA[buf][(r0_)*KP + sg_]) = ra0;                                            \
+        *(uint4*)(&sA[buf][(r0_ + 32) * KP + sg_]) = ra1;                                     \
+        *(uint4*)(&sA[buf][(r0_ + 64) * KP + sg_]) = ra2;                                     \
+        *(uint4*)(&sA[buf][(r0_ + 96) * KP + sg_]) = ra3;                                     \
+        *(uint4*)(&sB[buf][(r0_)*KP + sg_]) = rb0;                                            \
+        *(uint4*)(&sB[buf][(r0_ + 32) * KP + sg_]) = rb1;                                     \
+        *(uint4*)(&sB[buf][(r0_ + 64) * KP + sg_]) = rb2;                                     \
+        *(uint4*)(&sB[buf][(r0_ + 96) * KP + sg_]) = rb3;                                     \
+    } while (0)
+    f16v_t accr[2][2], acci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) accr[i][j][v] = acci[i][j][v] = 0.f;
+
+    const int nsteps = ldk / KS;
+    OLX_GLOAD(0);
+    OLX_SSTORE(0);
+    __syncthreads();
+    for (int st = 0; st < nsteps; st++) {
+        const int buf = st & 1;
+        if (st + 1 < nsteps) OLX_GLOAD((st + 1) * KS);
+#pragma unroll
+        for (int kk = 0; kk < KS / 16; kk++) {
+            const int koff = kk * 16 + 8 * (lane >> 5);
+            uint4 av[2], bv[2], biv[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+                av[i] = *(const uint4*)(&sA[buf][(wr * 64 + i * 32 + (lane & 31)) * KP + koff]);
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                bv[j] = *(const uint4*)(&sB[buf][(wc * 64 + j * 32 + (lane & 31)) * KP + koff]);
+                biv[j] = make_uint4(bi_of_br(bv[j].x), bi_of_br(bv[j].y), bi_of_br(bv[j].z),
+                                    bi_of_br(bv[j].w));
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    accr[i][j] = mfma<T>(av[i], bv[j], accr[i][j]);
+                    acci[i][j] = mfma<T>(av[i], biv[j], acci[i][j]);
+                }
+        }
+        if (st + 1 < nsteps) OLX_SSTORE(buf ^ 1);
+        __syncthreads();
+    }
+#undef OLX_GLOAD
+#undef OLX_SSTORE
+    // epilogue: lane holds column w = l % 32 and rows 8 (v/4) + 4 (l/32) + v % 4
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int h = h0 + wr * 64 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int w = w0 + wc * 64 + j * 32 + (lane & 31);
+                float2* p = &U[(size_t)h * nx + w];
+                float2 o = make_float2(accr[i][j][v], acci[i][j][v]);
+                if (accumulate) {
+                    const float2 u = *p;
+                    o.x += u.x;
+                    o.y += u.y;
+                }
+                *p = o;
+            }
+}
+
+static int nbits_for(int64_t P) {
+    int lg = 0;
+    while ((1ll << lg) < P) lg++;
+    return 24 - lg;
+}
+
+}  // namespace olx
 
 using namespace olx;
 
 extern "C" double olx_layer_depth(double z, double zmin, double zmax, int nlayers) {
-    const double dz = (zmax - zmin) / nlayers;
-    long long l = (long long)std::floor((z - zmin) / (zmax - zmin) * nlayers);
-    if (l < 0) l = 0;
-    if (l >= nlayers) l = nlayers - 1;
-    return zmin + ((double)l + 0.5) * dz;
+    const int l = layer_of(z, zmin, zmax, nlayers);
+    return zmin + ((double)l + 0.5) * (zmax - zmin) / nlayers;
 }
 
 extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
                                    int64_t ny, int64_t nx, double py, double px, double lambda,
                                    int nlayers, double zmin, double zmax, int precision,
                                    uint32_t flags) {
-    OLX_FAIL(OLX_EUNSUPPORTED, "layered CGH not built yet");
+    if (!ctx || !d_U) OLX_FAIL(OLX_EINVAL, "layered: null argument");
+    if (n < 0 || (n > 0 && !d_pts)) OLX_FAIL(OLX_EINVAL, "layered: bad point array");
+    if (n > INT32_MAX / 2) OLX_FAIL(OLX_EUNSUPPORTED, "layered: too many points");
+    if (ny <= 0 || nx <= 0 || ny % GT || nx % GT || ny > 8192 || nx > 8192)
+        OLX_FAIL(OLX_EUNSUPPORTED, "layered: hologram sides must be multiples of %d up to 8192", GT);
+    if (!(py > 0) || !(px > 0) || !(lambda > 0) || nlayers < 1 || !(zmax > zmin) || !(zmin > 0))
+        OLX_FAIL(OLX_EINVAL, "layered: need pitch, lambda > 0, nlayers >= 1, 0 < zmin < zmax");
+    if (precision < OLX_LAYERED_F16 || precision > OLX_LAYERED_BF16X3)
+        OLX_FAIL(OLX_EINVAL, "layered: unknown precision %d", precision);
+    hipStream_t s = ctx->stream;
+    const bool acc0 = (flags & OLX_CGH_ACCUMULATE) != 0;
+    if (n == 0) {
+        if (!acc0) OLX_HIP(hipMemsetAsync(d_U, 0, sizeof(float2) * ny * nx, s));
+        return OLX_OK;
+    }
+    // point records and chirp tables
+    const size_t pts_bytes = sizeof(LayPt) * (size_t)n;
+    const size_t cx_off = (pts_bytes + 255) & ~(size_t)255;
+    const size_t cy_off = cx_off + sizeof(float2) * (size_t)nlayers * nx;
+    const size_t meta_bytes = cy_off + sizeof(float2) * (size_t)nlayers * ny;
+    OLX_TRY(ctx->lay_pts.reserve(meta_bytes));
+    char* meta = (char*)ctx->lay_pts.p;
+    LayPt* lp = (LayPt*)meta;
+    float2* cx = (float2*)(meta + cx_off);
+    float2* cy = (float2*)(meta + cy_off);
+    const int nb = nbits_for(std::max(nx, ny));
+    {
+        TimedLaunch tl(ctx, "layered_prep");
+        k_lay_prep<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d_pts, (int)n, lp, px, py, lambda,
+                                                               zmin, zmax, nlayers, nb);
+        k_lay_chirp<<<dim3((unsigned)((nx + 255) / 256), nlayers), 256, 0, s>>>(cx, (int)nx, nlayers,
+                                                                               px, lambda, zmin, zmax);
+        k_lay_chirp<<<dim3((unsigned)((ny + 255) / 256), nlayers), 256, 0, s>>>(cy, (int)ny, nlayers,
+                                                                               py, lambda, zmin, zmax);
+    }
+    OLX_TRY(check_launch("layered prep"));
+    // factor buffers for one chunk of points (KC multiple of KS/2 points)
+    const int kc = (int)std::min<int64_t>(KC_PTS, ((n + KS / 2 - 1) / (KS / 2)) * (KS / 2));
+    const int ldk = 2 * kc;
+    const int nmat = precision == OLX_LAYERED_BF16X3 ? 4 : 2;
+    const size_t ybytes = sizeof(uint16_t) * (size_t)ny * ldk;
+    const size_t xbytes = sizeof(uint16_t) * (size_t)nx * ldk;
+    OLX_TRY(ctx->lay_counts.reserve((nmat / 2) * (ybytes + xbytes)));
+    char* fb = (char*)ctx->lay_counts.p;
+    void* Yh = fb;
+    void* Xh = fb + ybytes;
+    void* Yl = fb + ybytes + xbytes;
+    void* Xl = fb + 2 * ybytes + xbytes;
+    const dim3 ggrid((unsigned)(nx / GT), (unsigned)(ny / GT));
+    bool first = true;
+    for (int64_t j0 = 0; j0 < n; j0 += kc) {
+        const dim3 gy((unsigned)((kc + 255) / 256), (unsigned)((ny + GEN_RB - 1) / GEN_RB)),
+            gx((unsigned)((kc + 255) / 256), (unsigned)((nx + GEN_RB - 1) / GEN_RB));
+        {
+            TimedLaunch tl(ctx, "layered_factors");
+            if (precision == OLX_LAYERED_F16) {
+                k_lay_gen<_Float16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, (int)ny, (uint32_t*)Yh, nullptr);
+                k_lay_gen<_Float16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, (int)nx, (uint32_t*)Xh, nullptr);
+            } else if (precision == OLX_LAYERED_BF16) {
+                k_lay_gen<__bf16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, (int)ny, (uint32_t*)Yh, nullptr);
+                k_lay_gen<__bf16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, (int)nx, (uint32_t*)Xh, nullptr);
+            } else {
+                k_lay_gen<__bf16, true, 1><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, (int)ny, (uint32_t*)Yh, (uint32_t*)Yl);
+                k_lay_gen<__bf16, false, 1><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, (int)nx, (uint32_t*)Xh, (uint32_t*)Xl);
+            }
+        }
+        OLX_TRY(check_launch("layered factors"));
+        {
+            TimedLaunch tl(ctx, "layered_gemm");
+            const int acc = (first && !acc0) ? 0 : 1;
+            if (precision == OLX_LAYERED_F16) {
+                k_lay_gemm<_Float16><<<ggrid, 256, 0, s>>>((const _Float16*)Yh, (const _Float16*)Xh, ldk, (float2*)d_U, (int)nx, acc);
+            } else {
+                k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Yh, (const __bf16*)Xh, ldk, (float2*)d_U, (int)nx, acc);
+                if (precision == OLX_LAYERED_BF16X3) {
+                    k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Yh, (const __bf16*)Xl, ldk, (float2*)d_U, (int)nx, 1);
+                    k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Yl, (const __bf16*)Xh, ldk, (float2*)d_U, (int)nx, 1);
+                }
+            }
+        }
+        OLX_TRY(check_launch("layered gemm"));
+        first = false;
+    }
+    return OLX_OK;
 }

@@ -14,6 +14,7 @@
 // The padded rows that are all zero are never materialised, so HBM traffic is
 // 8 + 16 + 16 + 16 + 16 + 8 = 80 ny nx bytes per propagation.
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 
 #include "fft_core.hpp"
@@ -37,7 +38,15 @@ struct PropArgs {
     const float2* cy;
     int cwl, chl;
     float scale;
+    int cbl;  // column-block width of W (power of two): W[(m / cbl) ny + h][m % cbl]
+    int cbl_log;
 };
+
+// index of (row h, padded column m) in the column-blocked intermediate
+__device__ __forceinline__ size_t widx(const PropArgs& a, int h, int m) {
+    const int g = m >> a.cbl_log, c = m & (a.cbl - 1);
+    return ((size_t)g * a.ny + h) * a.cbl + c;
+}
 
 template <int L>
 constexpr int row_rpw() { return Plan<L>::T >= 256 ? 1 : 256 / Plan<L>::T; }
@@ -49,6 +58,14 @@ template <int L>
 constexpr int col_nt() { return col_cb<L>() * Plan<L>::T; }
 
 // ------------------------------------------------------------------ pass A
+// Work-groups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Neighbouring column groups share 128-byte lines of W, so the block index is
+// remapped to give every XCD a contiguous run of column groups.
+__device__ __forceinline__ int xcd_remap(int b, int nblk) {
+    if (nblk % 8) return b;
+    return (b % 8) * (nblk / 8) + b / 8;
+}
+
 template <int L>
 __global__ void __launch_bounds__(row_nt<L>())
 k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a,
@@ -58,7 +75,7 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
     constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
     __shared__ float2 lds[RPW * P::LP];
     const int tid = threadIdx.x, rr = tid / P::T, t = tid - rr * P::T;
-    const int row = blockIdx.x * RPW + rr;
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + rr;
     const bool valid = row < a.ny;
     const int nx = a.nx;
     float2 ay = make_float2(1.f, 0.f);
@@ -69,7 +86,7 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
 #pragma unroll
         for (int r = 0; r < R0; r++) {
             float2 x = make_float2(0.f, 0.f);
-            if (first_nonzero<L>(r) && valid) {
+            if (first_nonzero<L>(b, r) && valid) {
                 const int col = first_index<L>(t, b, r) - L / 4;
                 x = src[(size_t)row * nx + col];
                 if (a.ax) x = cmul(x, cmul(a.ax[col], ay));
@@ -78,48 +95,66 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
         }
     run_stages<L, false, TWN, 1>(v, t, lds, rr * P::LP, tw);
     if (valid) {
-        float2* out = W + (size_t)row * L;
 #pragma unroll
         for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
-            for (int r = 0; r < RL; r++) out[last_index<L>(t, b, r)] = v[b * RL + r];
+            for (int r = 0; r < RL; r++) W[widx(a, row, last_index<L>(t, b, r))] = v[b * RL + r];
     }
 }
 
-// transfer function value at padded-spectrum column col (natural order) and row i
-__device__ __forceinline__ float2 asm_h(const PropArgs& a, double u2, bool ucut, int i, int My) {
+// transfer function at spectrum row i (natural order) of a column whose
+// 1/lambda^2 - u^2 is A (FP64).  w = sqrt(A - v^2) comes from an FP32 rsqrt seed
+// and one FP64 Newton step (relative error ~1e-14); the phase w z is reduced
+// mod 1 cycle in FP64 and the sine / cosine are taken by the hardware unit in
+// revolutions (absolute error ~1e-7 on [-1/2, 1/2]).
+__device__ __forceinline__ float2 asm_h(const PropArgs& a, double A, bool ucut, int i, int My) {
     const int fy = i < My / 2 ? i : i - My;
     const double v = (double)fy * a.dv;
     if (ucut || (a.bandlimit && !(fabs(v) * 2 < a.vw))) return make_float2(0.f, 0.f);
-    const double arg = a.inv_l2 - u2 - v * v;
-    if (arg < 0.0) return make_float2(0.f, 0.f);  // evanescent: zero (reference: NaN)
-    const double cyc = sqrt(arg) * a.z;
-    const double f = cyc - rint(cyc);  // phase 2 pi w z reduced to [-pi, pi]
-    float s, c;
-    sincospif((float)(2.0 * f), &s, &c);
+    const double w2 = A - v * v;
+    if (w2 < 0.0) return make_float2(0.f, 0.f);  // evanescent: zero (reference: NaN)
+    double w = 0.0;
+    if (w2 > 1e-30) {
+        const double y0 = (double)__builtin_amdgcn_rsqf((float)w2);
+        const double s0 = w2 * y0;
+        w = fma(fma(-s0, s0, w2), 0.5 * y0, s0);
+    }
+    const double cyc = w * a.z;
+    const float t = (float)(cyc - rint(cyc));
+    const float s = __builtin_amdgcn_sinf(t), c = __builtin_amdgcn_cosf(t);
     return make_float2(c, a.adjoint ? -s : s);
 }
 
 // ------------------------------------------------------------------ pass B
-template <int L>
-__global__ void __launch_bounds__(col_nt<L>())
+template <int L, int CB>
+constexpr int col_waves_per_eu() {
+    // two work-groups per CU when one is at most 512 threads and its LDS fits twice
+    return (CB * Plan<L>::T <= 512 && 2 * CB * Plan<L>::LP * 8 <= 160 * 1024) ? 2 * CB * Plan<L>::T / 256 : 1;
+}
+
+template <int L, int CB, int MODE>
+__global__ void __launch_bounds__(CB * Plan<L>::T, (col_waves_per_eu<L, CB>()))
 k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
     using P = Plan<L>;
-    constexpr int CB = col_cb<L>();
     constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
     __shared__ float2 lds[CB * P::LP];
     const int tid = threadIdx.x, c = tid % CB, t = tid / CB;
     const int Mx = 2 * a.nx;
-    const int col = blockIdx.x * CB + c;
+    const int grp = xcd_remap(blockIdx.x, gridDim.x);
+    const int col = grp * CB + c;
     const bool valid = col < Mx;
+    // blocked layout: this group's contiguous block; row-major: stride Mx
+    const bool rowmajor = a.cbl == Mx;
+    float2* Wb = rowmajor ? W + col : W + (size_t)grp * a.ny * CB + c;
+    const size_t wst = rowmajor ? (size_t)Mx : (size_t)CB;
     float2 v[P::E];
 #pragma unroll
     for (int b = 0; b < P::E / R0; b++)
 #pragma unroll
         for (int r = 0; r < R0; r++) {
             float2 x = make_float2(0.f, 0.f);
-            if (first_nonzero<L>(r) && valid)
-                x = W[(size_t)(first_index<L>(t, b, r) - L / 4) * Mx + col];
+            if (first_nonzero<L>(b, r) && valid)
+                x = Wb[(size_t)(first_index<L>(t, b, r) - L / 4) * wst];
             v[b * R0 + r] = x;
         }
     run_stages<L, false, TWN, CB>(v, t, lds, c, tw);
@@ -129,40 +164,41 @@ k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
         for (int r = 0; r < RL; r++) lds[c + lpad(last_index<L>(t, b, r)) * CB] = v[b * RL + r];
     __syncthreads();
     // transfer function, then inverse transform
-    double u2 = 0.0;
+    double Acol = 0.0;  // 1/lambda^2 - u^2 of this column
     bool ucut = false;
-    const float2 sxc = (a.mode == K_TABLE && valid) ? a.sx[col] : make_float2(0.f, 0.f);
-    if (a.mode == K_ASM) {
+    const float2 sxc = (MODE == K_TABLE && valid) ? a.sx[col] : make_float2(0.f, 0.f);
+    if (MODE == K_ASM) {
         const int fx = col < Mx / 2 ? col : col - Mx;
         const double u = (double)fx * a.du;
-        u2 = u * u;
+        Acol = a.inv_l2 - u * u;
         ucut = a.bandlimit && !(fabs(u) * 2 < a.uw);
     }
-#pragma unroll
-    for (int b = 0; b < P::E / R0; b++)
-#pragma unroll
-        for (int r = 0; r < R0; r++) {
-            const int i = first_index<L>(t, b, r);
-            float2 x = lds[c + lpad(i) * CB];
-            float2 h;
-            if (a.mode == K_TABLE) {
-                h = cmul(sxc, a.sy[i]);
-                if (a.adjoint) h = cconj(h);
-            } else {
-                h = asm_h(a, u2, ucut, i, L);
-            }
-            v[b * R0 + r] = cmul(x, h);
+    // multiply the spectrum in LDS; a short loop keeps few FP64 temporaries live
+#pragma unroll 2
+    for (int q = 0; q < P::E; q++) {
+        const int i = t + P::T * q;
+        float2* p = &lds[c + lpad(i) * CB];
+        float2 h;
+        if constexpr (MODE == K_TABLE) {
+            h = cmul(sxc, a.sy[i]);
+            if (a.adjoint) h = cconj(h);
+        } else {
+            h = asm_h(a, Acol, ucut, i, L);
         }
+        *p = cmul(*p, h);
+    }
+    __syncthreads();
+    stage_load_lds<L, 0, CB>(v, t, lds, c);
     __syncthreads();
     run_stages<L, true, TWN, CB>(v, t, lds, c, tw);
 #pragma unroll
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
         for (int r = 0; r < RL; r++) {
-            if (!last_maybe_kept<L>(r)) continue;
+            if (!last_kept<L>(b, r)) continue;
             const int i = last_index<L>(t, b, r);
             if (valid && i >= L / 4 && i < 3 * L / 4)
-                W[(size_t)(i - L / 4) * Mx + col] = v[b * RL + r];
+                Wb[(size_t)(i - L / 4) * wst] = v[b * RL + r];
         }
 }
 
@@ -176,16 +212,15 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
     constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
     __shared__ float2 lds[RPW * P::LP];
     const int tid = threadIdx.x, rr = tid / P::T, t = tid - rr * P::T;
-    const int row = blockIdx.x * RPW + rr;
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + rr;
     const bool valid = row < a.ny;
     const int nx = a.nx;
     float2 v[P::E];
-    const float2* in = W + (size_t)row * L;
 #pragma unroll
     for (int b = 0; b < P::E / R0; b++)
 #pragma unroll
         for (int r = 0; r < R0; r++)
-            v[b * R0 + r] = valid ? in[first_index<L>(t, b, r)] : make_float2(0.f, 0.f);
+            v[b * R0 + r] = valid ? W[widx(a, row, first_index<L>(t, b, r))] : make_float2(0.f, 0.f);
     run_stages<L, true, TWN, 1>(v, t, lds, rr * P::LP, tw);
     if (!valid) return;
     const bool post = a.cx != nullptr && abs(row - a.ny / 2) < a.chl;
@@ -194,7 +229,7 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
         for (int r = 0; r < RL; r++) {
-            if (!last_maybe_kept<L>(r)) continue;
+            if (!last_kept<L>(b, r)) continue;
             const int i = last_index<L>(t, b, r);
             if (i < L / 4 || i >= 3 * L / 4) continue;
             const int col = i - L / 4;
@@ -363,11 +398,61 @@ static const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, dou
 
 static int ilog2(int64_t n) { return 63 - __builtin_clzll((unsigned long long)n); }
 
+// columns per work-group of pass B: CB * L/16 threads and CB padded columns of
+// LDS must fit (1024 threads, 160 KiB); OLX_COLCB overrides the default for tuning
+template <int L>
+constexpr int col_cb_max() { return L >= 16384 ? 1 : (L >= 1024 ? 16384 / L : 16); }
+
+template <int L>
+static int col_batch() {
+    static int env = [] {
+        const char* e = getenv("OLX_COLCB");
+        return e ? atoi(e) : 0;
+    }();
+    int cb = env > 0 ? env : col_cb<L>();
+    if (cb > col_cb_max<L>()) cb = col_cb_max<L>();
+    int p2 = 1;
+    while (p2 * 2 <= cb) p2 *= 2;
+    return p2;
+}
+
+template <int L, int CB>
+static void launch_cols_cb(const PropArgs& a, float2* W, const float2* tw, hipStream_t s) {
+    const int Mx = 2 * a.nx;
+    const int grid = (Mx + CB - 1) / CB;
+    if (a.mode == K_ASM)
+        k_cols_conv<L, CB, K_ASM><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
+    else
+        k_cols_conv<L, CB, K_TABLE><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
+}
+
+template <int L>
+static int launch_cols(int cb, const PropArgs& a, float2* W, const float2* tw, hipStream_t s) {
+    constexpr int CMAX = col_cb_max<L>();
+    if (cb >= 16 && CMAX >= 16) launch_cols_cb<L, (CMAX >= 16 ? 16 : 1)>(a, W, tw, s);
+    else if (cb >= 8 && CMAX >= 8) launch_cols_cb<L, (CMAX >= 8 ? 8 : 1)>(a, W, tw, s);
+    else if (cb >= 4 && CMAX >= 4) launch_cols_cb<L, (CMAX >= 4 ? 4 : 1)>(a, W, tw, s);
+    else if (cb >= 2 && CMAX >= 2) launch_cols_cb<L, (CMAX >= 2 ? 2 : 1)>(a, W, tw, s);
+    else launch_cols_cb<L, 1>(a, W, tw, s);
+    return OLX_OK;
+}
+
 static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a) {
     const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
     OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
     float2* W = (float2*)ctx->work.p;
     a.scale = (float)(1.0 / ((double)Mx * (double)My));
+    // W layout: row-major (the default), or blocked by the column batch when
+    // OLX_WLAYOUT=blocked (contiguous column pass, scattered row passes)
+    static const bool blocked = [] {
+        const char* e = getenv("OLX_WLAYOUT");
+        return e && strcmp(e, "blocked") == 0;
+    }();
+    int cb = 1;
+    OLX_LEN_SWITCH(ilog2(My), { cb = col_batch<L>(); });
+    if (cb > Mx) cb = (int)Mx;
+    a.cbl = blocked ? cb : (int)Mx;
+    a.cbl_log = ilog2(a.cbl);
     hipStream_t s = ctx->stream;
     {
         TimedLaunch tl(ctx, "prop_rows_pad_fwd");
@@ -380,8 +465,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
     {
         TimedLaunch tl(ctx, "prop_cols_conv");
         OLX_LEN_SWITCH(ilog2(My), {
-            const int grid = (int)((Mx + col_cb<L>() - 1) / col_cb<L>());
-            k_cols_conv<L><<<grid, col_nt<L>(), 0, s>>>(W, a, ctx->twiddle);
+            OLX_TRY(launch_cols<L>(cb, a, W, ctx->twiddle, s));
         });
     }
     OLX_TRY(check_launch("prop pass B"));
