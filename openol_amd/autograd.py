@@ -1,0 +1,84 @@
+"""Differentiable angular-spectrum propagation over the C ABI.
+
+The ASM operator A = crop . IFFT . diag(H) . FFT . pad is C-linear; its adjoint
+A^H = crop . IFFT . diag(conj H) . FFT . pad is the same fused three-pass kernel
+with the conjugate transfer function (olx_asm_c64 with OLX_ADJOINT).  With
+PyTorch's complex-gradient convention (grad = dL/dRe + i dL/dIm) the
+vector-Jacobian product of a C-linear map is A^H applied to the output gradient.
+
+For a phase-only hologram u = exp(i phi):  dL/dphi = Im(conj(u) * grad_u).
+"""
+import torch
+
+from . import ops
+
+
+def _asm_copy(u, dy, dx, lam, z, bandlimit, adjoint):
+    out = u.contiguous().clone()
+    ops.AsmProp(out, out.shape[-2], out.shape[-1], dy, dx, lam, z, bandlimit=bandlimit,
+                adjoint=adjoint)
+    return out
+
+
+class AsmPropFn(torch.autograd.Function):
+    """out = AsmProp(u); backward applies the adjoint kernel."""
+
+    @staticmethod
+    def forward(ctx, u, dy, dx, lam, z, bandlimit):
+        ctx.args = (dy, dx, lam, z, bandlimit)
+        return _asm_copy(u, dy, dx, lam, z, bandlimit, False)
+
+    @staticmethod
+    def backward(ctx, g):
+        dy, dx, lam, z, bandlimit = ctx.args
+        return _asm_copy(g, dy, dx, lam, z, bandlimit, True), None, None, None, None, None
+
+
+def asm_prop(u, dy, dx, lam, z, bandlimit=True):
+    """Differentiable ASM propagation of a complex64 CUDA field (not in place)."""
+    return AsmPropFn.apply(u, dy, dx, lam, z, bandlimit)
+
+
+def normalized_intensity_mse(field, target, norm="max"):
+    """MSE between |field|^2 normalised by its max (or mean) and the target."""
+    inten = field.real ** 2 + field.imag ** 2
+    scale = inten.max() if norm == "max" else inten.mean()
+    return torch.mean((inten / scale - target) ** 2)
+
+
+class MultiPlaneLossFn(torch.autograd.Function):
+    """L(phi) = sum_p MSE(I_p / max I_p, T_p),  I_p = |A_p exp(i phi)|^2.
+
+    Memory-lean: forward keeps only phi; backward recomputes each plane's field,
+    takes the intensity-to-loss gradient with torch (max subgradient included),
+    applies the adjoint propagation and accumulates Im(conj(u) A_p^H G_p)."""
+
+    @staticmethod
+    def forward(ctx, phi, targets, zs, dy, dx, lam, bandlimit, norm):
+        ctx.save_for_backward(phi)
+        ctx.cfg = (targets, zs, dy, dx, lam, bandlimit, norm)
+        u = torch.polar(torch.ones_like(phi), phi)
+        loss = phi.new_zeros(())
+        for t, z in zip(targets, zs):
+            f = _asm_copy(u, dy, dx, lam, z, bandlimit, False)
+            loss = loss + normalized_intensity_mse(f, t, norm)
+        return loss
+
+    @staticmethod
+    def backward(ctx, gout):
+        (phi,) = ctx.saved_tensors
+        targets, zs, dy, dx, lam, bandlimit, norm = ctx.cfg
+        u = torch.polar(torch.ones_like(phi), phi)
+        gu = torch.zeros_like(u)
+        for t, z in zip(targets, zs):
+            f = _asm_copy(u, dy, dx, lam, z, bandlimit, False).requires_grad_(True)
+            with torch.enable_grad():
+                lp = normalized_intensity_mse(f, t, norm)
+                (gf,) = torch.autograd.grad(lp, f)
+            gu += _asm_copy(gf, dy, dx, lam, z, bandlimit, True)
+        gphi = torch.imag(torch.conj(u) * gu) * gout
+        return gphi, None, None, None, None, None, None, None
+
+
+def multiplane_loss(phi, targets, zs, dy, dx, lam, bandlimit=True, norm="max"):
+    return MultiPlaneLossFn.apply(phi, targets, zs, dy, dx, lam, bandlimit, norm)
