@@ -1,0 +1,224 @@
+// olprop.h -- drop-in for the reference's propagation API on MI355X.
+//
+// Host overloads keep the signatures of include/olprop.h (namespace ol, same
+// template parameters and argument order): the field stays in the caller's
+// std::unique_ptr<std::complex<COMPLEX_T>[]> allocation, is staged to HBM, runs
+// through the fused HIP kernels of libolx.so, and is copied back in place.  The
+// device computes complex64 fields with FP64 phase arithmetic (PREC_T=double
+// semantics); COMPLEX_T = double fields are rounded to complex64 for the device.
+// Device overloads mirror include/golprop.h on ol::device_ptr<std::complex<float>[]>.
+// Errors throw std::runtime_error; nothing is printed and nothing exits.
+#ifndef OLPROP_H
+#define OLPROP_H
+#include <climits>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "oldefine.h"
+#include "oldevice.h"
+
+namespace ol {
+namespace detail {
+
+// stage a host field to HBM, run fn(device_ptr), copy back into the same allocation
+template <typename C, typename F>
+void on_device(std::unique_ptr<std::complex<C>[]>& u, int64_t count, F&& fn) {
+    static_assert(std::is_same<C, float>::value || std::is_same<C, double>::value,
+                  "COMPLEX_T must be float or double");
+    const size_t bytes = sizeof(std::complex<float>) * (size_t)count;
+    void* d = staging(0).get(bytes);
+    if constexpr (std::is_same<C, float>::value) {
+        check(olx_copy_h2d(ctx(), d, u.get(), bytes), "copy to device");
+        fn(d);
+        check(olx_copy_d2h(ctx(), u.get(), d, bytes), "copy to host");
+    } else {
+        std::vector<std::complex<float>> t((size_t)count);
+        for (int64_t i = 0; i < count; i++) t[i] = std::complex<float>(u[i]);
+        check(olx_copy_h2d(ctx(), d, t.data(), bytes), "copy to device");
+        fn(d);
+        check(olx_copy_d2h(ctx(), t.data(), d, bytes), "copy to host");
+        for (int64_t i = 0; i < count; i++) u[i] = std::complex<C>(t[i]);
+    }
+}
+
+inline int64_t win(double v) {
+    const double f = std::floor(v);
+    if (!(f == f) || f >= (double)INT_MAX) return INT_MAX;
+    if (f <= (double)INT_MIN) return INT_MIN;
+    return (int64_t)f;
+}
+
+}  // namespace detail
+
+// ------------------------------------------------------------------ checks
+// include/olprop.h:209-225
+template <typename PREC_T = float>
+bool AsmPropCheck(int64_t ny, int64_t nx, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    return olx_asm_prop_check(ny, nx, dy, dx, lambda, z) != 0;
+}
+// include/olprop.h:109-123
+template <typename PREC_T = double>
+bool FresnelPropCheck(size_t ny, size_t nx, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    return olx_fresnel_prop_check((int64_t)ny, (int64_t)nx, dy, dx, lambda, z) != 0;
+}
+
+// ------------------------------------------------------------------ host API
+// include/olprop.h:227-248 (net scale 1)
+template <typename PREC_T = double, typename COMPLEX_T>
+void AsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int64_t width,
+             PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    detail::on_device(u, height * width, [&](void* d) {
+        detail::check(olx_asm_c64(detail::ctx(), d, height, width, dy, dx, lambda, z, 0), "AsmProp");
+    });
+}
+
+// include/olprop.h:125-148 (circconv / M^2)
+template <typename PREC_T = double, typename COMPLEX_T>
+void FresnelProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, size_t height, size_t width,
+                 PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    detail::on_device(u, (int64_t)(height * width), [&](void* d) {
+        detail::check(olx_fresnel_c64(detail::ctx(), d, (int64_t)height, (int64_t)width, dy, dx,
+                                      lambda, z, 0), "FresnelProp");
+    });
+}
+
+// include/olprop.h:150-170 (per-axis window)
+template <typename PREC_T = double, typename COMPLEX_T>
+void FresnelPropBandLimit(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, size_t height,
+                          size_t width, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    detail::on_device(u, (int64_t)(height * width), [&](void* d) {
+        detail::check(olx_fresnel_c64(detail::ctx(), d, (int64_t)height, (int64_t)width, dy, dx,
+                                      lambda, z, OLX_FRESNEL_BL_CPU), "FresnelPropBandLimit");
+    });
+}
+
+// include/olprop.h:338-408: CPU ARSS analogue (no windows, circconv / M^4), in place
+template <typename PREC_T = float, typename COMPLEX_T>
+void shiftedFresnelProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height,
+                        int64_t width, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z, PREC_T s,
+                        PREC_T oy, PREC_T ox) {
+    detail::on_device(u, height * width, [&](void* d) {
+        detail::check(olx_arss_c64(detail::ctx(), d, d, height, width, dy, dx, lambda, z, s, oy,
+                                   ox, OLX_ARSS_CPU), "shiftedFresnelProp");
+    });
+}
+
+// include/olprop.h:656-682 (silent; the reference prints the chosen mode)
+template <typename COMPLEX_T, typename PREC_T>
+void Prop(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int ny, int nx, PREC_T dy, PREC_T dx,
+          PREC_T lambda, PREC_T d, PROPMODE propmode = PROPMODE::AUTO) {
+    const int mode = propmode == ASM ? OLX_MODE_ASM : propmode == FRESNEL ? OLX_MODE_FRESNEL
+                                                                          : OLX_MODE_AUTO;
+    detail::on_device(u, (int64_t)ny * nx, [&](void* dd) {
+        detail::check(olx_prop_c64(detail::ctx(), dd, ny, nx, dy, dx, lambda, d, mode, nullptr),
+                      "Prop");
+    });
+}
+
+// include/olprop.h:172-189 (+ band-limit window of :250-305 when bandlimit is set):
+// the centred transfer function; evanescent entries are 0 (the reference: NaN)
+template <typename PREC_T = float, typename COMPLEX_T>
+void AsmTransferF(std::unique_ptr<std::complex<COMPLEX_T>[]>& H, int64_t ny, int64_t nx,
+                  PREC_T dv, PREC_T du, PREC_T lambda, PREC_T z, bool bandlimit = false) {
+    detail::on_device(H, ny * nx, [&](void* d) {
+        detail::check(olx_asm_transfer_c64(detail::ctx(), d, ny, nx, dv, du, lambda, z,
+                                           bandlimit ? OLX_BANDLIMIT : 0u), "AsmTransferF");
+    });
+}
+
+// include/olprop.h:8-24
+template <typename PREC_T = double, typename COMPLEX_T>
+void FresnelResponse(std::unique_ptr<std::complex<COMPLEX_T>[]>& h, size_t height, size_t width,
+                     PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    detail::on_device(h, (int64_t)(height * width), [&](void* d) {
+        detail::check(olx_fresnel_response_c64(detail::ctx(), d, (int64_t)height, (int64_t)width,
+                                               dy, dx, lambda, z, INT_MAX, INT_MAX),
+                      "FresnelResponse");
+    });
+}
+
+// include/olprop.h:26-54 (per-axis limits) and :56-84 (limits from p_limit)
+template <typename PREC_T = double, typename COMPLEX_T>
+void FresnelResponseBandLimit(std::unique_ptr<std::complex<COMPLEX_T>[]>& h, size_t height,
+                              size_t width, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    const int64_t wlim = detail::win(std::fabs(lambda * z * 0.5 / dx) / dx);
+    const int64_t hlim = detail::win(std::fabs(lambda * z * 0.5 / dy) / dy);
+    detail::on_device(h, (int64_t)(height * width), [&](void* d) {
+        detail::check(olx_fresnel_response_c64(detail::ctx(), d, (int64_t)height, (int64_t)width,
+                                               dy, dx, lambda, z, wlim, hlim),
+                      "FresnelResponseBandLimit");
+    });
+}
+template <typename PREC_T = double, typename COMPLEX_T>
+void FresnelResponseBandLimit(std::unique_ptr<std::complex<COMPLEX_T>[]>& h, size_t height,
+                              size_t width, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z,
+                              PREC_T p_limit) {
+    const int64_t wlim = detail::win(std::fabs(lambda * z * 0.5 / p_limit) / dx);
+    const int64_t hlim = detail::win(std::fabs(lambda * z * 0.5 / p_limit) / dy);
+    detail::on_device(h, (int64_t)(height * width), [&](void* d) {
+        detail::check(olx_fresnel_response_c64(detail::ctx(), d, (int64_t)height, (int64_t)width,
+                                               dy, dx, lambda, z, wlim, hlim),
+                      "FresnelResponseBandLimit");
+    });
+}
+
+// ------------------------------------------------------------------ device API
+// (include/golprop.h; fields are ol::device_ptr<std::complex<float>[]>, updated in place)
+using dfield = device_ptr<std::complex<float>[]>;
+
+// include/golprop.h:506-534
+template <typename PREC_T>
+void gAsmProp(dfield& u, int ny, int nx, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T d) {
+    detail::check(olx_asm_c64(detail::ctx(), u.get(), ny, nx, dy, dx, lambda, d, 0), "gAsmProp");
+}
+// include/golprop.h:167-198
+template <typename PREC_T>
+void gFresnelProp(dfield& u, int ny, int nx, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T d) {
+    detail::check(olx_fresnel_c64(detail::ctx(), u.get(), ny, nx, dy, dx, lambda, d, 0),
+                  "gFresnelProp");
+}
+// include/golprop.h:258-302 (x-limit on both axes)
+template <typename PREC_T>
+void gFresnelPropBandLimit(dfield& u, int ny, int nx, PREC_T dy, PREC_T dx, PREC_T lambda,
+                           PREC_T d) {
+    detail::check(olx_fresnel_c64(detail::ctx(), u.get(), ny, nx, dy, dx, lambda, d,
+                                  OLX_FRESNEL_BL_GPU), "gFresnelPropBandLimit");
+}
+// include/golprop.h:578-604
+template <typename PREC_T>
+void Prop(dfield& u, int ny, int nx, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T d,
+          PROPMODE propmode = PROPMODE::AUTO) {
+    const int mode = propmode == ASM ? OLX_MODE_ASM : propmode == FRESNEL ? OLX_MODE_FRESNEL
+                                                                          : OLX_MODE_AUTO;
+    detail::check(olx_prop_c64(detail::ctx(), u.get(), ny, nx, dy, dx, lambda, d, mode, nullptr),
+                  "Prop");
+}
+// include/golprop.h:941-995; usrc and udst may be the same object; udst is
+// allocated when empty
+template <typename PREC_T = float>
+void ARSSFresnelProp(dfield& usrc, dfield& udst, int ny, int nx, PREC_T dy, PREC_T dx,
+                     PREC_T lambda, PREC_T z, PREC_T s, PREC_T oy, PREC_T ox) {
+    if (!udst) udst = make_device<std::complex<float>[]>((size_t)ny * nx);
+    detail::check(olx_arss_c64(detail::ctx(), usrc.get(), udst.get(), ny, nx, dy, dx, lambda, z,
+                               s, oy, ox, 0), "ARSSFresnelProp");
+}
+template <typename PREC_T = float>
+void ARSSFresnelProp(dfield& usrc, dfield& udst, int ny, int nx, PREC_T dy, PREC_T dx,
+                     PREC_T lambda, PREC_T z, PREC_T s) {
+    ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lambda, z, s, (PREC_T)0, (PREC_T)0);
+}
+
+// the adjoint of AsmProp (conjugate transfer function), for gradient-based design
+template <typename PREC_T>
+void gAsmPropAdjoint(dfield& u, int ny, int nx, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T d,
+                     bool bandlimit = false) {
+    detail::check(olx_asm_c64(detail::ctx(), u.get(), ny, nx, dy, dx, lambda, d,
+                              OLX_ADJOINT | (bandlimit ? OLX_BANDLIMIT : 0u)), "gAsmPropAdjoint");
+}
+
+}  // namespace ol
+#endif

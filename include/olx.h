@@ -37,6 +37,8 @@ extern "C" {
 #define OLX_ADJOINT 0x2u      /* apply the adjoint (conjugate transfer function) */
 #define OLX_FRESNEL_BL_CPU 0x10u /* Fresnel window, per-axis limits (include/olprop.h:26-54) */
 #define OLX_FRESNEL_BL_GPU 0x20u /* Fresnel window, x-limit on both axes (include/golprop.h:223-238) */
+#define OLX_ARSS_CPU 0x800u      /* ARSS with the CPU shiftedFresnelProp semantics: no windows and
+                                    net scale circconv / M^4 (include/olprop.h:338-408) */
 
 /* CGH flags */
 #define OLX_CGH_EXACT_R 0x100u    /* exact spherical distance r instead of the Fresnel quadratic */
@@ -105,6 +107,17 @@ int olx_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, dou
 int olx_asm_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda, double z);
 int olx_fresnel_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda,
                            double z);
+/* materialise the centred ASM transfer function H[n][m], u = (m - nx/2) du,
+ * v = (n - ny/2) dv (AsmTransferF, include/olprop.h:172-189; OLX_BANDLIMIT adds the
+ * window of :250-305; evanescent entries are 0) into a complex64 ny x nx array */
+int olx_asm_transfer_c64(olx_ctx* ctx, void* d_H, int64_t ny, int64_t nx, double dv, double du,
+                         double lambda, double z, uint32_t flags);
+/* materialise the centred Fresnel impulse response h = exp(i pi (x^2 + y^2)/(lambda z)),
+ * x = (m - nx/2) dx, zero where |m - nx/2| >= wlim or |n - ny/2| >= hlim
+ * (FresnelResponse / FresnelResponseBandLimit, include/olprop.h:8-84; pass
+ * INT32_MAX limits for no window) */
+int olx_fresnel_response_c64(olx_ctx* ctx, void* d_h, int64_t ny, int64_t nx, double dy,
+                             double dx, double lambda, double z, int64_t wlim, int64_t hlim);
 /* plain unnormalised 2-D FFT in place (sign -1 forward, +1 backward);
  * the role of ol::fft / ol::ifft (include/olutils.h:19-66, include/golutils.h:55-75) */
 int olx_fft2d_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, int sign);

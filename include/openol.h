@@ -1,0 +1,15 @@
+// openol.h -- umbrella header of the MI355X drop-in (role of the reference's
+// include/openol.h:3-22): the propagation and CGH APIs over libolx.so.
+// Link with -L<repo>/openol_amd -lolx (rpath to the same directory).
+#ifndef OPENOL_H
+#define OPENOL_H
+#include <complex>
+#include <memory>
+
+#include "olcgh.h"
+#include "olclass.h"
+#include "oldefine.h"
+#include "oldevice.h"
+#include "olprop.h"
+#include "olx.h"
+#endif

@@ -20,6 +20,7 @@ FRESNEL_BL_GPU = 0x20
 CGH_EXACT_R = 0x100
 CGH_ACCUMULATE = 0x400
 WRP_SQUARE = 0x200
+ARSS_CPU = 0x800
 
 MODE_ASM, MODE_FRESNEL, MODE_AUTO, MODE_FRESNEL_BL = 0, 1, 2, 3
 LAYERED_F16, LAYERED_BF16, LAYERED_BF16X3 = 0, 1, 2
@@ -57,6 +58,8 @@ SIGNATURES = {
     "olx_asm_prop_check": [_i64, _i64, _d, _d, _d, _d],
     "olx_fresnel_prop_check": [_i64, _i64, _d, _d, _d, _d],
     "olx_fft2d_c64": [_p, _p, _i64, _i64, _i],
+    "olx_asm_transfer_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_fresnel_response_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i64, _i64],
     "olx_cgh_c64": [_p, _p, _i64, _p, _p, _p, _i64, _i64, _d, _d, _d, _u32],
     "olx_cgh_c128": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _u32],
     "olx_encode_c64": [_p, _p, _p, _p, _i64],

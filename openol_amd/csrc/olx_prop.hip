@@ -38,6 +38,7 @@ struct PropArgs {
     const float2* cy;
     int cwl, chl;
     float scale;
+    double extra_scale;  // multiplies the 1/(My Mx) output scale
     int cbl;  // column-block width of W (power of two): W[(m / cbl) ny + h][m % cbl]
     int cbl_log;
 };
@@ -335,6 +336,40 @@ __global__ void k_dft_f64(float2* __restrict__ S, const double2* __restrict__ hc
     S[k] = make_float2((float)(re * scale), (float)((adjoint ? -im : im) * scale));
 }
 
+// centred H / h tables (the reference's AsmTransferF / FresnelResponse layouts)
+__global__ void k_asm_transfer(float2* __restrict__ H, int ny, int nx, double dv, double du,
+                               double inv_l2, double z, int bl, double uw, double vw) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y;
+    if (m >= nx) return;
+    const double v = (double)(n - ny / 2) * dv, u = (double)(m - nx / 2) * du;
+    const double arg = inv_l2 - u * u - v * v;
+    float2 h = make_float2(0.f, 0.f);
+    if (arg >= 0.0 && (!bl || (fabs(u) * 2 < uw && fabs(v) * 2 < vw))) {
+        double cyc = sqrt(arg) * z;
+        cyc -= rint(cyc);
+        double s, c;
+        sincospi(2.0 * cyc, &s, &c);
+        h = make_float2((float)c, (float)s);
+    }
+    H[(size_t)n * nx + m] = h;
+}
+
+__global__ void k_fresnel_response(float2* __restrict__ h, int ny, int nx, double dy, double dx,
+                                   double tmp, long long wlim, long long hlim) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y;
+    if (m >= nx) return;
+    float2 o = make_float2(0.f, 0.f);
+    if (llabs((long long)m - nx / 2) < wlim && llabs((long long)n - ny / 2) < hlim) {
+        const double y = (double)(n - ny / 2) * dy, x = (double)(m - nx / 2) * dx;
+        double cyc = 0.5 * (x * x + y * y) * tmp;  // pi (x^2 + y^2) tmp radians
+        cyc -= rint(cyc);
+        double s, c;
+        sincospi(2.0 * cyc, &s, &c);
+        o = make_float2((float)c, (float)s);
+    }
+    h[(size_t)n * nx + m] = o;
+}
+
 static int64_t win_limit(double v) {
     const double f = std::floor(v);
     if (!(f == f) || f >= (double)INT_MAX) return INT_MAX;
@@ -441,7 +476,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
     const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
     OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
     float2* W = (float2*)ctx->work.p;
-    a.scale = (float)(1.0 / ((double)Mx * (double)My));
+    a.scale = (float)(a.extra_scale / ((double)Mx * (double)My));
     // W layout: row-major (the default), or blocked by the column batch when
     // OLX_WLAYOUT=blocked (contiguous column pass, scattered row passes)
     static const bool blocked = [] {
@@ -494,6 +529,7 @@ static PropArgs base_args(int64_t ny, int64_t nx) {
     memset(&a, 0, sizeof a);
     a.ny = (int)ny;
     a.nx = (int)nx;
+    a.extra_scale = 1.0;
     return a;
 }
 
@@ -567,29 +603,63 @@ extern "C" int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_
     a.mode = K_TABLE;
     const double K = M_PI / (lambda * z);
     int rc = OLX_OK;
+    // CPU shiftedFresnelProp (include/olprop.h:338-408): no windows anywhere
+    const bool cpu = (flags & OLX_ARSS_CPU) != 0;
     // stage u window and pre-chirp (include/golprop.h:849-854, 831-840)
-    const int64_t uwl = win_limit(lambda * std::fabs(z) * 0.5 / (s * std::fabs(s - 1) * dx) / dx);
-    const int64_t uhl = win_limit(lambda * std::fabs(z) * 0.5 / (s * std::fabs(s - 1) * dy) / dy);
+    const int64_t uwl = cpu ? INT_MAX : win_limit(lambda * std::fabs(z) * 0.5 / (s * std::fabs(s - 1) * dx) / dx);
+    const int64_t uhl = cpu ? INT_MAX : win_limit(lambda * std::fabs(z) * 0.5 / (s * std::fabs(s - 1) * dy) / dy);
     a.ax = chirp_table(ctx, (int)nx, dx, K, s * s - s, -2 * s * ox, 0.0, uwl, &rc);
     if (!a.ax) return rc;
     a.ay = chirp_table(ctx, (int)ny, dy, K, s * s - s, -2 * s * oy, 0.0, uhl, &rc);
     if (!a.ay) return rc;
     // stage h: kernel chirp, x-limit on both axes (include/golprop.h:890-896)
-    const int64_t hl = win_limit(std::fabs(lambda * z * 0.5 / (dx * s)) / dx);
+    const int64_t hl = cpu ? INT_MAX : win_limit(std::fabs(lambda * z * 0.5 / (dx * s)) / dx);
     a.sx = spectrum_table(ctx, (int)(2 * nx), dx, K, s, hl, 0, &rc);
     if (!a.sx) return rc;
     a.sy = spectrum_table(ctx, (int)(2 * ny), dy, K, s, hl, 0, &rc);
     if (!a.sy) return rc;
     // stage Cz window and post-chirp (include/golprop.h:928-933, 911-920)
-    const int64_t cwl = win_limit(lambda * std::fabs(z) * 0.5 / (std::fabs(1 - s) * dx) / dx);
-    const int64_t chl = win_limit(lambda * std::fabs(z) * 0.5 / (std::fabs(1 - s) * dy) / dy);
+    const int64_t cwl = cpu ? INT_MAX : win_limit(lambda * std::fabs(z) * 0.5 / (std::fabs(1 - s) * dx) / dx);
+    const int64_t chl = cpu ? INT_MAX : win_limit(lambda * std::fabs(z) * 0.5 / (std::fabs(1 - s) * dy) / dy);
     a.cx = chirp_table(ctx, (int)nx, dx, K, 1 - s, 2 * ox, ox * ox, INT_MAX, &rc);
     if (!a.cx) return rc;
     a.cy = chirp_table(ctx, (int)ny, dy, K, 1 - s, 2 * oy, oy * oy, INT_MAX, &rc);
     if (!a.cy) return rc;
     a.cwl = (int)std::max<int64_t>(std::min<int64_t>(cwl, INT_MAX), INT_MIN);
     a.chl = (int)std::max<int64_t>(std::min<int64_t>(chl, INT_MAX), INT_MIN);
+    // the CPU variant scales u by 1/M^2 twice and the output once more (olprop.h:367,
+    // 384, 389) against the GPU variant's 1/M^2 on u and on h: one extra 1/M^2
+    a.extra_scale = cpu ? 1.0 / (4.0 * (double)nx * (double)ny) : 1.0;
     return run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a);
+}
+
+extern "C" int olx_asm_transfer_c64(olx_ctx* ctx, void* d_H, int64_t ny, int64_t nx, double dv,
+                                    double du, double lambda, double z, uint32_t flags) {
+    if (!ctx || !d_H || ny <= 0 || nx <= 0 || ny > 65535 || nx > (1 << 24))
+        OLX_FAIL(OLX_EINVAL, "asm_transfer: bad argument");
+    double uw = 0, vw = 0;
+    if (flags & OLX_BANDLIMIT) {
+        const double ulm = 1.0 / std::sqrt(std::pow(0.0 - 0.5f / du, -2) * z * z + 1.0f) / lambda;
+        const double ulp = 1.0 / std::sqrt(std::pow(0.0 + 0.5f / du, -2) * z * z + 1.0f) / lambda;
+        const double vlm = 1.0 / std::sqrt(std::pow(0.0 - 0.5f / dv, -2) * z * z + 1.0f) / lambda;
+        const double vlp = 1.0 / std::sqrt(std::pow(0.0 + 0.5f / dv, -2) * z * z + 1.0f) / lambda;
+        uw = ulp + ulm;
+        vw = vlp + vlm;
+    }
+    k_asm_transfer<<<dim3((unsigned)((nx + 255) / 256), (unsigned)ny), 256, 0, ctx->stream>>>(
+        (float2*)d_H, (int)ny, (int)nx, dv, du, 1 / (lambda * lambda), z,
+        (flags & OLX_BANDLIMIT) ? 1 : 0, uw, vw);
+    return check_launch("asm_transfer");
+}
+
+extern "C" int olx_fresnel_response_c64(olx_ctx* ctx, void* d_h, int64_t ny, int64_t nx,
+                                        double dy, double dx, double lambda, double z,
+                                        int64_t wlim, int64_t hlim) {
+    if (!ctx || !d_h || ny <= 0 || nx <= 0 || ny > 65535 || nx > (1 << 24) || z == 0)
+        OLX_FAIL(OLX_EINVAL, "fresnel_response: bad argument");
+    k_fresnel_response<<<dim3((unsigned)((nx + 255) / 256), (unsigned)ny), 256, 0, ctx->stream>>>(
+        (float2*)d_h, (int)ny, (int)nx, dy, dx, 1 / (lambda * z), (long long)wlim, (long long)hlim);
+    return check_launch("fresnel_response");
 }
 
 extern "C" int olx_asm_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda,

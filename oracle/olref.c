@@ -377,6 +377,73 @@ DEFINE_ARSS(c64, cf, float)
 DEFINE_ARSS(c128, cd, double)
 
 /* ------------------------------------------------------------------------- */
+/* shiftedFresnelProp, include/olprop.h:338-408 (CPU; no windows)            */
+/* ------------------------------------------------------------------------- */
+#define DEFINE_SHIFTED(SUF, CT, RT)                                                 \
+int olref_shifted_fresnel_##SUF(RT* uin, int64_t height, int64_t width, double dy,  \
+                                double dx, double lambda, double z, double s,       \
+                                double oy, double ox) {                             \
+    if (!uin || height <= 0 || width <= 0 || !(lambda > 0) || z == 0) return -1;     \
+    CT* u = (CT*)uin;                                                               \
+    double tmp = M_PI / (lambda * z);                                               \
+    int64_t hh = height / 2, hw = width / 2;                                        \
+    for (int64_t h = 0; h < height; h++) {                                          \
+        double y1 = (double)(h - hh) * dy;                                          \
+        double phiu_y = (s * s - s) * y1 * y1 - 2 * s * oy * y1;                    \
+        for (int64_t w = 0; w < width; w++) {                                       \
+            int64_t idx = h * width + w;                                            \
+            double x1 = (double)(w - hw) * dx;                                      \
+            double phiu = (s * s - s) * x1 * x1 - 2 * s * ox * x1;                  \
+            phiu += phiu_y;                                                         \
+            phiu *= tmp;                                                            \
+            u[idx] = u[idx] * ((RT)cos(phiu) + I * (RT)sin(phiu));                  \
+        }                                                                           \
+    }                                                                               \
+    int64_t H = height * 2, Wd = width * 2;                                         \
+    CT* buf = pad_##SUF(u, height, width, H, Wd);                                   \
+    hh *= 2; hw *= 2;                                                               \
+    fft2d_##SUF(buf, H, Wd, -1);                                                    \
+    RT sc = (RT)(1.0 / (double)(H * Wd));                                           \
+    for (int64_t i = 0; i < H * Wd; i++) buf[i] *= sc;                              \
+    CT* hk = (CT*)malloc(sizeof(CT) * (size_t)(H * Wd));                            \
+    for (int64_t n = 0; n < H; n++) {                                               \
+        double y1 = (double)(n - hh) * dy;                                          \
+        double phih_y = s * y1 * y1;                                                \
+        for (int64_t m = 0; m < Wd; m++) {                                          \
+            double x1 = (double)(m - hw) * dx;                                      \
+            double phih = s * x1 * x1 + phih_y;                                     \
+            phih *= tmp;                                                            \
+            hk[n * Wd + m] = (RT)cos(phih) + I * (RT)sin(phih);                     \
+        }                                                                           \
+    }                                                                               \
+    fftshift_##SUF(hk, H, Wd);                                                      \
+    fft2d_##SUF(hk, H, Wd, -1);                                                     \
+    for (int64_t i = 0; i < H * Wd; i++) { buf[i] *= sc; buf[i] = buf[i] * hk[i]; }  \
+    free(hk);                                                                       \
+    fft2d_##SUF(buf, H, Wd, +1);                                                    \
+    crop_##SUF(buf, u, H, Wd);                                                      \
+    free(buf);                                                                      \
+    for (int64_t i = 0; i < height * width; i++) u[i] *= sc;                        \
+    hh /= 2; hw /= 2;                                                               \
+    for (int64_t h = 0; h < height; h++) {                                          \
+        double y2 = (double)(h - hh) * dy;                                          \
+        double phic_y = ((1 - s) * y2 * y2 + 2 * oy * y2 + oy * oy);                \
+        for (int64_t w = 0; w < width; w++) {                                       \
+            int64_t idx = h * width + w;                                            \
+            double x2 = (double)(w - hw) * dx;                                      \
+            double phic = ((1 - s) * x2 * x2 + 2 * ox * x2 + ox * ox);              \
+            phic += phic_y;                                                         \
+            phic *= tmp;                                                            \
+            u[idx] = u[idx] * ((RT)cos(phic) + I * (RT)sin(phic));                  \
+        }                                                                           \
+    }                                                                               \
+    return 0;                                                                       \
+}
+
+DEFINE_SHIFTED(c64, cf, float)
+DEFINE_SHIFTED(c128, cd, double)
+
+/* ------------------------------------------------------------------------- */
 /* Dispatcher and checks: olprop.h:109-123, :209-225, :656-682                */
 /* ------------------------------------------------------------------------- */
 int olref_asm_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda,

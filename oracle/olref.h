@@ -64,6 +64,13 @@ int olref_arss_c128(const double* src, double* dst, int64_t ny, int64_t nx, doub
                     double dx, double lambda, double z, double s, double oy,
                     double ox, unsigned flags);
 
+/* shiftedFresnelProp (CPU ARSS analogue), include/olprop.h:338-408: no windows,
+ * u scaled by 1/M^2 twice and the output once more (net circconv / M^4); in place */
+int olref_shifted_fresnel_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,
+                              double lambda, double z, double s, double oy, double ox);
+int olref_shifted_fresnel_c128(double* u, int64_t ny, int64_t nx, double dy, double dx,
+                               double lambda, double z, double s, double oy, double ox);
+
 /* Prop dispatcher, include/olprop.h:656-682; returns chosen mode in *chosen
  * (0 ASM, 1 FRESNEL, 2 FRESNEL band-limited) */
 int olref_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,

@@ -49,6 +49,8 @@ def lib():
             "olref_arss_c64": [p, p, i64, i64, d, d, d, d, d, d, d, u32],
             "olref_arss_c128": [p, p, i64, i64, d, d, d, d, d, d, d, u32],
             "olref_prop_c64": [p, i64, i64, d, d, d, d, ctypes.c_int, p],
+            "olref_shifted_fresnel_c64": [p, i64, i64, d, d, d, d, d, d, d],
+            "olref_shifted_fresnel_c128": [p, i64, i64, d, d, d, d, d, d, d],
             "olref_asm_prop_check": [i64, i64, d, d, d, d],
             "olref_fresnel_prop_check": [i64, i64, d, d, d, d],
             "olref_cgh_c64": [p, p, p, p, p, i64, p, i64, i64, d, d, d, u32],
@@ -117,6 +119,14 @@ def arss_prop(u, dy, dx, lam, z, s, oy=0.0, ox=0.0, flags=0):
     _check(fn(_ptr(src), _ptr(dst), src.shape[0], src.shape[1], dy, dx, lam, z, s, oy, ox,
               flags), "arss_prop")
     return dst
+
+
+def shifted_fresnel_prop(u, dy, dx, lam, z, s, oy=0.0, ox=0.0):
+    """shiftedFresnelProp (CPU ARSS analogue), include/olprop.h:338-408."""
+    u = np.array(u, copy=True)
+    fn = lib().olref_shifted_fresnel_c64 if u.dtype == np.complex64 else lib().olref_shifted_fresnel_c128
+    _check(fn(_ptr(u), u.shape[0], u.shape[1], dy, dx, lam, z, s, oy, ox), "shifted_fresnel_prop")
+    return u
 
 
 def prop(u, dy, dx, lam, z, mode=2):

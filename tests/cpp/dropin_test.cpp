@@ -1,0 +1,127 @@
+// Exercises the drop-in headers (include/openol.h) the way the reference's test
+// mains call ol:: functions (tests/Prop_test.cpp, tests/gProp_test.cu,
+// tests/gObject_test.cu), writing every result to <dir>/<name>.bin for the
+// Python test to compare against the oracle.
+//   dropin_test <dir>   reads <dir>/field.bin (complex64 n x n), <dir>/points.bin
+//                       (m x 3 double), <dir>/params.txt ("n m")
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "openol.h"
+
+template <typename T>
+static void save(const std::string& path, const T* p, size_t count) {
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(p), sizeof(T) * count);
+}
+template <typename T>
+static std::vector<T> load(const std::string& path, size_t count) {
+    std::vector<T> v(count);
+    std::ifstream f(path, std::ios::binary);
+    f.read(reinterpret_cast<char*>(v.data()), sizeof(T) * count);
+    if (!f) throw std::runtime_error("short read: " + path);
+    return v;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    const std::string dir = argv[1];
+    int n = 0, m = 0;
+    {
+        std::ifstream f(dir + "/params.txt");
+        f >> n >> m;
+    }
+    const double p = 8e-6, lambda = 532e-9;
+    const auto field = load<std::complex<float>>(dir + "/field.bin", (size_t)n * n);
+    const auto xyz = load<double>(dir + "/points.bin", (size_t)m * 3);
+    auto fresh = [&] {
+        auto u = std::make_unique<std::complex<float>[]>((size_t)n * n);
+        for (size_t i = 0; i < (size_t)n * n; i++) u[i] = field[i];
+        return u;
+    };
+    try {
+        // host API, complex<float> (fftw3f path), PREC_T = double
+        auto u = fresh();
+        std::complex<float>* before = u.get();
+        ol::AsmProp(u, n, n, p, p, lambda, 0.05);
+        if (u.get() != before) std::cout << "allocation changed\n";
+        save(dir + "/asm.bin", u.get(), (size_t)n * n);
+        // complex<double> (fftw3 path)
+        auto ud = std::make_unique<std::complex<double>[]>((size_t)n * n);
+        for (size_t i = 0; i < (size_t)n * n; i++) ud[i] = std::complex<double>(field[i]);
+        ol::AsmProp(ud, n, n, p, p, lambda, 0.05);
+        save(dir + "/asm_c128.bin", ud.get(), (size_t)n * n);
+        u = fresh();
+        ol::FresnelProp(u, n, n, p, p, lambda, 0.2);
+        save(dir + "/fresnel.bin", u.get(), (size_t)n * n);
+        u = fresh();
+        ol::FresnelPropBandLimit(u, n, n, p, p, lambda, 0.002);
+        save(dir + "/fresnel_bl.bin", u.get(), (size_t)n * n);
+        u = fresh();
+        ol::Prop(u, n, n, p, p, lambda, 0.05, ol::AUTO);
+        save(dir + "/prop.bin", u.get(), (size_t)n * n);
+        u = fresh();
+        ol::shiftedFresnelProp(u, n, n, p, p, lambda, 0.1, 0.5, 1e-4, -2e-4);
+        save(dir + "/shifted.bin", u.get(), (size_t)n * n);
+        std::cout << "AsmPropCheck " << ol::AsmPropCheck(n, n, p, p, lambda, 0.05) << "\n";
+
+        // CGH on a host point cloud (ol::Object<double>)
+        ol::Object<double> obj(m);
+        for (int j = 0; j < m; j++) obj[j] = ol::Vect3<double>(xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2]);
+        auto U = std::make_unique<std::complex<float>[]>((size_t)n * n);
+        ol::Cgh(obj, U, n, n, p, lambda);
+        save(dir + "/cgh.bin", U.get(), (size_t)n * n);
+        auto ph = std::make_unique<float[]>((size_t)n * n);
+        ol::Cgh(obj, ph, n, n, p, p, lambda);
+        save(dir + "/cgh_phase.bin", ph.get(), (size_t)n * n);
+
+        // device API
+        auto d = ol::cpu2dev(fresh(), (size_t)n * n);
+        ol::gAsmProp(d, n, n, p, p, lambda, 0.05);
+        auto hd = ol::dev2cpu(d, (size_t)n * n);
+        save(dir + "/gasm.bin", hd.get(), (size_t)n * n);
+        auto src = ol::cpu2dev(fresh(), (size_t)n * n);
+        ol::dfield dst;
+        ol::ARSSFresnelProp(src, dst, n, n, p, p, lambda, 0.15, 0.5);
+        hd = ol::dev2cpu(dst, (size_t)n * n);
+        save(dir + "/arss.bin", hd.get(), (size_t)n * n);
+        auto gobj = ol::Object2dev(obj);
+        auto dU = ol::make_device<std::complex<float>[]>((size_t)n * n);
+        ol::gCgh(gobj, dU, n, n, p, lambda);
+        hd = ol::dev2cpu(dU, (size_t)n * n);
+        save(dir + "/gcgh.bin", hd.get(), (size_t)n * n);
+        ol::WRPStep1(gobj, dU, n, n, p, lambda, 0.15);
+        hd = ol::dev2cpu(dU, (size_t)n * n);
+        save(dir + "/wrp.bin", hd.get(), (size_t)n * n);
+        double pz = ol::WRPMethod(gobj, dU, n, n, p, lambda, 0.151, 0.99f);
+        hd = ol::dev2cpu(dU, (size_t)n * n);
+        save(dir + "/wrpm.bin", hd.get(), (size_t)n * n);
+        std::printf("plane_z %.17g\n", pz);
+
+        // tables
+        auto H = std::make_unique<std::complex<float>[]>((size_t)2 * n * 2 * n);
+        ol::AsmTransferF(H, 2 * n, 2 * n, 1 / (p * 2 * n), 1 / (p * 2 * n), lambda, 0.05);
+        save(dir + "/H.bin", H.get(), (size_t)4 * n * n);
+        auto h = std::make_unique<std::complex<float>[]>((size_t)2 * n * 2 * n);
+        ol::FresnelResponseBandLimit(h, 2 * n, 2 * n, p, p, lambda, 0.002);
+        save(dir + "/h.bin", h.get(), (size_t)4 * n * n);
+        ol::synchronize();
+    } catch (const std::exception& e) {
+        std::cout << "error: " << e.what() << "\n";
+        return 1;
+    }
+    // errors are exceptions, not exit(1)
+    try {
+        auto bad = std::make_unique<std::complex<float>[]>(12 * 12);
+        ol::AsmProp(bad, 12, 12, p, p, lambda, 0.05);
+        std::cout << "no exception for a 12x12 field\n";
+        return 1;
+    } catch (const std::runtime_error& e) {
+        std::cout << "caught: " << e.what() << "\n";
+    }
+    std::cout << "done\n";
+    return 0;
+}

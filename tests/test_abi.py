@@ -64,3 +64,16 @@ def test_host_logic_matches_oracle():
     for s, sq in [(0.9, 0), (-1.0, 0), (-1.0, 1), (0.5, 1)]:
         assert L.olx_wrp_plane_z(512, 256, 8e-6, 532e-9, 0.15, 8e-6, ctypes.c_float(s), sq) == \
             olref.wrp_plane_z(512, 256, 8e-6, 532e-9, 0.15, s, square=bool(sq))
+
+
+def test_dropin_headers_compile():
+    """The C++ drop-in headers are valid standalone C++17 (no HIP headers needed)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

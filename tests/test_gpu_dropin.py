@@ -1,0 +1,65 @@
+"""The C++ drop-in headers (include/openol.h) on the GPU: the program
+tests/cpp/dropin_test calls ol:: functions with the reference's signatures; every
+output is compared with the oracle on the same seeded inputs."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import olref
+from openol_amd import synth
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+EXE = os.path.join(ROOT, "tests", "cpp", "dropin_test")
+LAM, P = 532e-9, 8e-6
+
+
+@pytest.mark.gpu
+def test_dropin_program_matches_oracle(tmp_path):
+    assert os.path.exists(EXE), "build() compiles tests/cpp/dropin_test"
+    n, m = 128, 300
+    u0 = synth.random_field(n, n, seed=80)
+    pts = synth.random_points(m, seed=81, xmax=n * P, ymax=n * P, zmin=0.152, zmax=0.155)
+    xyz = np.stack([pts["x"], pts["y"], pts["z"]], axis=1)
+    u0.tofile(tmp_path / "field.bin")
+    xyz.astype(np.float64).tofile(tmp_path / "points.bin")
+    (tmp_path / "params.txt").write_text(f"{n} {m}\n")
+    r = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "done" in r.stdout and "caught:" in r.stdout
+    assert "allocation changed" not in r.stdout
+    rd = lambda name, dt=np.complex64, k=1: np.fromfile(tmp_path / f"{name}.bin", dtype=dt).reshape(k * n, k * n)
+    rel = olref.rel_l2
+    assert rel(rd("asm"), olref.asm_prop(u0, P, P, LAM, 0.05)) < 1e-5
+    assert rel(rd("asm_c128", np.complex128), olref.asm_prop(u0.astype(np.complex128), P, P, LAM, 0.05)) < 1e-5
+    assert rel(rd("fresnel"), olref.fresnel_prop(u0, P, P, LAM, 0.2)) < 1e-5
+    assert rel(rd("fresnel_bl"), olref.fresnel_prop(u0, P, P, LAM, 0.002, olref.FRESNEL_BL_CPU)) < 1e-5
+    assert rel(rd("prop"), olref.prop(u0, P, P, LAM, 0.05)[0]) < 1e-5
+    assert rel(rd("shifted"), olref.shifted_fresnel_prop(u0, P, P, LAM, 0.1, 0.5, 1e-4, -2e-4)) < 1e-5
+    assert rel(rd("gasm"), olref.asm_prop(u0, P, P, LAM, 0.05)) < 1e-5
+    assert rel(rd("arss"), olref.arss_prop(u0, P, P, LAM, 0.15, 0.5)) < 1e-5
+    cgh = olref.cgh(pts["x"], pts["y"], pts["z"], n, n, P, P, LAM, dtype=np.complex128)
+    assert rel(rd("cgh"), cgh) < 1e-4 and rel(rd("gcgh"), cgh) < 1e-4
+    ph = rd("cgh_phase", np.float32)
+    assert np.mean(np.abs(np.angle(np.exp(1j * (ph - np.angle(cgh)))))) < 1e-3
+    assert rel(rd("wrp"), olref.wrp(pts["x"], pts["y"], pts["z"], n, n, P, LAM, 0.15,
+                                    dtype=np.complex128)) < 1e-5
+    pz = float([l for l in r.stdout.splitlines() if l.startswith("plane_z")][0].split()[1])
+    assert pz == olref.wrp_plane_z(n, n, P, LAM, 0.151, 0.99)
+    w = olref.wrp(pts["x"], pts["y"], pts["z"], n, n, P, LAM, pz)
+    # WRPStep2 AUTO (include/golcgh.h:494-517): ASM if AsmPropCheck or not FresnelPropCheck
+    use_asm = olref.asm_prop_check(n, n, P, P, LAM, pz) or not olref.fresnel_prop_check(n, n, P, P, LAM, pz)
+    step2 = olref.asm_prop(w, P, P, LAM, pz) if use_asm else olref.fresnel_prop(w, P, P, LAM, pz)
+    assert rel(rd("wrpm"), step2) < 1e-5
+    # tables: AsmTransferF (centred, no band limit) and FresnelResponseBandLimit
+    M = 2 * n
+    du = 1 / (P * M)
+    c = (np.arange(M) - M // 2)
+    wv = np.sqrt(1 / LAM ** 2 - (c[None, :] * du) ** 2 - (c[:, None] * du) ** 2)
+    assert rel(rd("H", k=2), np.exp(2j * np.pi * wv * 0.05)) < 1e-6
+    lim = int(np.floor(abs(LAM * 0.002 * 0.5 / P) / P))
+    x = c * P
+    hh = np.exp(1j * np.pi * (x[None, :] ** 2 + x[:, None] ** 2) / (LAM * 0.002))
+    hh *= (np.abs(c)[None, :] < lim) & (np.abs(c)[:, None] < lim)
+    assert rel(rd("h", k=2), hh) < 1e-6

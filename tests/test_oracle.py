@@ -168,6 +168,22 @@ def test_arss_equals_scaled_fresnel_direct_sum():
     assert rel(out, direct) < 1e-12
 
 
+def test_shifted_fresnel_equals_scaled_direct_sum_over_m4():
+    # CPU shiftedFresnelProp has no windows and net scale circconv / M^4
+    n, dx, z, s, oy, ox = 16, 8e-6, 0.01, 0.5, 1e-5, -2e-5
+    u = synth.random_field(n, n, seed=19, dtype=np.complex128)
+    out = olref.shifted_fresnel_prop(u, dx, dx, LAM, z, s, oy, ox)
+    c = (np.arange(n) - n // 2) * dx
+    ex = np.exp(1j * np.pi * ((c[:, None] + ox) - s * c[None, :]) ** 2 / (LAM * z))
+    ey = np.exp(1j * np.pi * ((c[:, None] + oy) - s * c[None, :]) ** 2 / (LAM * z))
+    direct = ey @ u @ ex.T / (2 * n) ** 4
+    assert rel(out, direct) < 1e-12
+    # and at a distance where the GPU windows would cut, it keeps everything
+    w = olref.shifted_fresnel_prop(u, dx, dx, LAM, 0.0005, s)
+    a = olref.arss_prop(u, dx, dx, LAM, 0.0005, s) / (2 * n) ** 2
+    assert rel(w, a) > 1e-3
+
+
 def test_arss_unit_scale_equals_gpu_bandlimited_fresnel():
     u = synth.random_field(16, 16, seed=12, dtype=np.complex128)
     a = olref.arss_prop(u, 8e-6, 8e-6, LAM, 0.003, 1.0)
