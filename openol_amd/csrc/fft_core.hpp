@@ -1,12 +1,13 @@
 // fft_core.hpp -- LDS-staged Stockham FFT building blocks for gfx950.
 //
 // A length-L transform (L = 2^p, 8 <= L <= 16384) is carried by T = L/E threads,
-// each holding E = min(16, L) complex values in registers.  Stages use radix 16,
-// the remainder radix 2, 4 or 8 first; between stages the data goes through LDS
-// (one float2 of padding every 16 elements against bank conflicts).  The first
-// stage loads straight from HBM and the last stage stores straight to HBM, so the
-// caller can fuse zero-padding (statically skipped loads), pointwise factors and
-// cropping (statically skipped stores) into them.
+// each holding E = min(16, L) complex values in registers.  Stages use radix 16
+// and one remainder radix 2, 4 or 8, placed first or last (the plan's ORDER);
+// between stages the data goes through LDS (one float2 of padding every 16
+// elements against bank conflicts).  The first stage loads straight from HBM and
+// the last stage stores straight to HBM, so the caller can fuse zero-padding
+// (statically skipped loads), pointwise factors and cropping (statically skipped
+// stores) into them.  Every helper takes the plan type P.
 //
 // Stockham step for stage s (radix R, NS = product of earlier radices):
 //   butterfly j in [0, L/R):  v[r] = x[j + r L/R],  k = j mod NS,
@@ -127,8 +128,17 @@ __device__ __forceinline__ void dft(float2* v) {
 }
 
 // ---------------------------------------------------------------- plan
-template <int L>
+// ORDER = REM_FIRST: remainder radix first.  With centred 2x zero padding its
+// butterflies see one nonzero input each, and the final radix-16 stage of a
+// cropped transform has whole outputs that are never stored (dead code).
+// ORDER = REM_LAST: remainder radix last.  The first radix-16 stage then needs no
+// twiddles (NS = 1) and sees half-zero input under padding, and a cropped last
+// radix-2 stage keeps one output per butterfly.
+enum { REM_FIRST = 0, REM_LAST = 1 };
+
+template <int L_, int ORDER = REM_FIRST>
 struct Plan {
+    static constexpr int L = L_;
     static_assert(L >= 8 && L <= 16384 && (L & (L - 1)) == 0, "unsupported length");
     static constexpr int LOG = __builtin_ctz(L);
     static constexpr int E = L < 16 ? L : 16;  // values per thread
@@ -137,11 +147,10 @@ struct Plan {
     static constexpr int REM = L < 16 ? L : (1 << (LOG % 4));
     static constexpr int NST = N16 + (REM > 1 ? 1 : 0);
     static constexpr int LP = L + L / 16;  // padded LDS length
-    // remainder radix first: with centred 2x padding its butterflies see one
-    // nonzero input each, and the final radix-16 stage of a cropped inverse
-    // transform has whole outputs that are never stored (dead code)
     __host__ __device__ static constexpr int radix(int s) {
-        return REM > 1 ? (s == 0 ? REM : 16) : 16;
+        if (REM == 1) return 16;
+        if (L < 16) return L;
+        return ORDER == REM_FIRST ? (s == 0 ? REM : 16) : (s == NST - 1 ? REM : 16);
     }
     __host__ __device__ static constexpr int ns(int s) {
         int p = 1;
@@ -196,10 +205,61 @@ __device__ __forceinline__ void apply_twiddles(float2* v, int base, const float2
     }
 }
 
+// the same twiddles applied to two value sets of one thread (same k, same W)
+template <int R, bool INV>
+__device__ __forceinline__ void apply_twiddles2(float2* a, float2* b, int base,
+                                                const float2* __restrict__ tw) {
+    if constexpr (R == 2) {
+        float2 w1 = tw[base];
+        if (INV) w1 = cconj(w1);
+        a[1] = cmul(a[1], w1);
+        b[1] = cmul(b[1], w1);
+    } else if constexpr (R == 4) {
+        float2 w1 = tw[base], w2 = tw[2 * base];
+        if (INV) { w1 = cconj(w1); w2 = cconj(w2); }
+        const float2 w3 = cmul(w1, w2);
+        a[1] = cmul(a[1], w1); b[1] = cmul(b[1], w1);
+        a[2] = cmul(a[2], w2); b[2] = cmul(b[2], w2);
+        a[3] = cmul(a[3], w3); b[3] = cmul(b[3], w3);
+    } else {
+        float2 w1 = tw[base], w2 = tw[2 * base], w4 = tw[4 * base];
+        if (INV) { w1 = cconj(w1); w2 = cconj(w2); w4 = cconj(w4); }
+        a[1] = cmul(a[1], w1); b[1] = cmul(b[1], w1);
+        a[2] = cmul(a[2], w2); b[2] = cmul(b[2], w2);
+        a[4] = cmul(a[4], w4); b[4] = cmul(b[4], w4);
+        const float2 w3 = cmul(w1, w2);
+        a[3] = cmul(a[3], w3); b[3] = cmul(b[3], w3);
+        const float2 w5 = cmul(w1, w4);
+        a[5] = cmul(a[5], w5); b[5] = cmul(b[5], w5);
+        const float2 w6 = cmul(w2, w4);
+        a[6] = cmul(a[6], w6); b[6] = cmul(b[6], w6);
+        const float2 w7 = cmul(w3, w4);
+        a[7] = cmul(a[7], w7); b[7] = cmul(b[7], w7);
+        if constexpr (R == 16) {
+            float2 w8 = tw[8 * base];
+            if (INV) w8 = cconj(w8);
+            a[8] = cmul(a[8], w8); b[8] = cmul(b[8], w8);
+            float2 w = cmul(w1, w8);
+            a[9] = cmul(a[9], w); b[9] = cmul(b[9], w);
+            w = cmul(w2, w8);
+            a[10] = cmul(a[10], w); b[10] = cmul(b[10], w);
+            w = cmul(w3, w8);
+            a[11] = cmul(a[11], w); b[11] = cmul(b[11], w);
+            w = cmul(w4, w8);
+            a[12] = cmul(a[12], w); b[12] = cmul(b[12], w);
+            w = cmul(w5, w8);
+            a[13] = cmul(a[13], w); b[13] = cmul(b[13], w);
+            w = cmul(w6, w8);
+            a[14] = cmul(a[14], w); b[14] = cmul(b[14], w);
+            w = cmul(w7, w8);
+            a[15] = cmul(a[15], w); b[15] = cmul(b[15], w);
+        }
+    }
+}
+
 // twiddle + DFT for every butterfly of stage S held by thread t
-template <int L, int S, bool INV, int TWLEN>
+template <class P, int S, bool INV, int TWLEN>
 __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw) {
-    using P = Plan<L>;
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::E / R;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
@@ -212,88 +272,172 @@ __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __
     }
 }
 
-// store stage S outputs to LDS (element i of this transform at off + lpad(i)*STRIDE)
-template <int L, int S, int STRIDE>
-__device__ __forceinline__ void stage_store_lds(const float2* v, int t, float2* lds, int off) {
-    using P = Plan<L>;
+// stage S on two value sets of the same thread, twiddles computed once
+template <class P, int S, bool INV, int TWLEN>
+__device__ __forceinline__ void stage_compute2(float2* va, float2* vb, int t,
+                                               const float2* __restrict__ tw) {
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::E / R;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        int j = t + P::T * b;
-        int k = j & (NS - 1);
-        int base = (j - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; r++) lds[off + lpad(base + r * NS) * STRIDE] = v[b * R + r];
+        if constexpr (NS > 1) {
+            int j = t + P::T * b;
+            int k = j & (NS - 1);
+            apply_twiddles2<R, INV>(va + b * R, vb + b * R, k * (TWLEN / (NS * R)), tw);
+        }
+        dft<R, INV>(va + b * R);
+        dft<R, INV>(vb + b * R);
     }
 }
 
-// load stage S inputs from LDS
-template <int L, int S, int STRIDE>
+// LDS positions.  lpad(x + y) = lpad(x) + lpad(y) whenever x % 16 + y % 16 < 16,
+// which holds for every split below when T (threads per transform) is a multiple
+// of 16: each access is then one per-thread base plus a compile-time offset, so
+// the compiler needs a single address register per stage instead of one per value.
+//
+// stage S output (b, r) goes to position (t - k) R + k + r NS + T R b, k = t mod NS
+// (T is a multiple of NS, so k does not depend on b)
+template <class P, int S>
+__device__ __forceinline__ int store_base(int t) {
+    constexpr int R = P::radix(S), NS = P::ns(S);
+    const int k = t & (NS - 1);
+    return lpad((t - k) * R) + lpad(k);
+}
+template <class P, int S>
+__device__ __forceinline__ constexpr int store_off(int b, int r) {
+    constexpr int R = P::radix(S), NS = P::ns(S);
+    return (r * NS + ((r * NS) >> 4)) + (P::T * R * b + ((P::T * R * b) >> 4));
+}
+
+// store stage S outputs to LDS (element i of this transform at off + lpad(i)*STRIDE)
+template <class P, int S, int STRIDE>
+__device__ __forceinline__ void stage_store_lds(const float2* v, int t, float2* lds, int off) {
+    constexpr int R = P::radix(S), NS = P::ns(S), NB = P::E / R;
+    if constexpr (P::T % 16 == 0) {
+        float2* q = lds + off + store_base<P, S>(t) * STRIDE;
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int r = 0; r < R; r++) q[store_off<P, S>(b, r) * STRIDE] = v[b * R + r];
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            int j = t + P::T * b;
+            int k = j & (NS - 1);
+            int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[off + lpad(base + r * NS) * STRIDE] = v[b * R + r];
+        }
+    }
+}
+
+// load stage S inputs from LDS: (b, r) reads element t + T b + r L/R
+template <class P, int S, int STRIDE>
 __device__ __forceinline__ void stage_load_lds(float2* v, int t, const float2* lds, int off) {
-    using P = Plan<L>;
     constexpr int R = P::radix(S), NB = P::E / R;
+    if constexpr (P::T % 16 == 0) {
+        const float2* q = lds + off + lpad(t) * STRIDE;
 #pragma unroll
-    for (int b = 0; b < NB; b++) {
-        int j = t + P::T * b;
+        for (int b = 0; b < NB; b++)
 #pragma unroll
-        for (int r = 0; r < R; r++) v[b * R + r] = lds[off + lpad(j + r * (L / R)) * STRIDE];
+            for (int r = 0; r < R; r++) {
+                const int o = P::T * b + r * (P::L / R);
+                v[b * R + r] = q[(o + (o >> 4)) * STRIDE];
+            }
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            int j = t + P::T * b;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b * R + r] = lds[off + lpad(j + r * (P::L / R)) * STRIDE];
+        }
     }
 }
 
 // Run stages 0..NST-1 on values already loaded for stage 0; intermediate
 // exchanges go through LDS.  On return v holds the LAST stage's outputs:
-// v[b R + r] is element (j + r L/R) of the natural-order result, j = t + T b,
-// R = last radix (the caller stores them, see last_index).
-template <int L, bool INV, int TWLEN, int STRIDE, int S = 0>
+// v[b R + r] is element last_index(t, b, r) of the natural-order result.
+template <class P, bool INV, int TWLEN, int STRIDE, int S = 0>
 __device__ __forceinline__ void run_stages(float2* v, int t, float2* lds, int off,
                                            const float2* __restrict__ tw) {
-    using P = Plan<L>;
-    stage_compute<L, S, INV, TWLEN>(v, t, tw);
+    stage_compute<P, S, INV, TWLEN>(v, t, tw);
     if constexpr (S + 1 < P::NST) {
-        stage_store_lds<L, S, STRIDE>(v, t, lds, off);
+        stage_store_lds<P, S, STRIDE>(v, t, lds, off);
         __syncthreads();
-        stage_load_lds<L, S + 1, STRIDE>(v, t, lds, off);
+        stage_load_lds<P, S + 1, STRIDE>(v, t, lds, off);
         __syncthreads();
-        run_stages<L, INV, TWLEN, STRIDE, S + 1>(v, t, lds, off, tw);
+        run_stages<P, INV, TWLEN, STRIDE, S + 1>(v, t, lds, off, tw);
+    }
+}
+
+// Two transforms per thread (value sets a and b), each laid out exactly as in
+// run_stages.  Both sets are computed together (their twiddles are the same), and
+// LDS holds one set at a time, so every exchange is done in two rounds.  On
+// return both sets hold their last-stage outputs.
+template <class P, bool INV, int TWLEN, int STRIDE, int S = 0>
+__device__ __forceinline__ void run_stages_pair(float2* va, float2* vb, int t, float2* lds,
+                                                int off, const float2* __restrict__ tw) {
+    // an opaque copy of t per call: twiddle offsets are recomputed (one shift each)
+    // rather than kept live, or spilled, between identical uses
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    if constexpr (S == 0) stage_compute2<P, 0, INV, TWLEN>(va, vb, tt, tw);
+    if constexpr (S + 1 < P::NST) {
+        stage_store_lds<P, S, STRIDE>(va, t, lds, off);
+        __syncthreads();
+        stage_load_lds<P, S + 1, STRIDE>(va, t, lds, off);
+        __syncthreads();
+        stage_store_lds<P, S, STRIDE>(vb, t, lds, off);
+        __syncthreads();
+        // set b's reads overlap the next stage's twiddles and set a's arithmetic
+        stage_load_lds<P, S + 1, STRIDE>(vb, t, lds, off);
+        stage_compute2<P, S + 1, INV, TWLEN>(va, vb, tt, tw);
+        __syncthreads();
+        run_stages_pair<P, INV, TWLEN, STRIDE, S + 1>(va, vb, t, lds, off, tw);
     }
 }
 
 // element index held in v[b R + r] after run_stages (last stage: NS R = L, so the
-// output index is j + r NS with j < NS)
-template <int L>
-__device__ __forceinline__ int last_index(int t, int b, int r) {
-    using P = Plan<L>;
+// output index is j + r NS with j = t + T b < NS)
+template <class P>
+__device__ __forceinline__ constexpr int last_index(int t, int b, int r) {
     constexpr int R = P::radix(P::NST - 1);
-    constexpr int NS = L / R;
-    return t + P::T * b + r * NS;
+    return t + P::T * b + r * (P::L / R);
 }
 
 // element index read by v[b R0 + r] in stage 0
-template <int L>
-__device__ __forceinline__ int first_index(int t, int b, int r) {
-    using P = Plan<L>;
+template <class P>
+__device__ __forceinline__ constexpr int first_index(int t, int b, int r) {
     constexpr int R = P::radix(0);
-    return t + P::T * b + r * (L / R);
+    return t + P::T * b + r * (P::L / R);
+}
+
+// Both indices above are t + T q with q = b + r E/R: a spectrum left by a forward
+// transform of plan PF feeds an inverse transform of plan PI without an exchange,
+// stage-0 slot b R0 + r of PI being PF's last-stage slot spec_slot(b, r).
+template <class PF, class PI>
+__device__ __forceinline__ constexpr int spec_slot(int b, int r) {
+    constexpr int R0 = PI::radix(0), RL = PF::radix(PF::NST - 1);
+    const int q = b + r * (PI::E / R0);
+    const int nbl = PF::E / RL;  // butterflies per thread in PF's last stage
+    return (q % nbl) * RL + q / nbl;
 }
 
 // For a transform whose input is nonzero only on [L/4, 3L/4) (centred 2x zero
 // padding): whether stage-0 element (b, r) can be nonzero.  Thread t of
 // butterfly block b reads j + r L/R0 with j = t + T b, t < T, and L/4 is a
 // multiple of T, so the answer does not depend on t.
-template <int L>
+template <class P>
 __device__ __forceinline__ constexpr bool first_nonzero(int b, int r) {
-    constexpr int R0 = Plan<L>::radix(0), T = Plan<L>::T;
-    const int lo = T * b + r * (L / R0);
-    return lo >= L / 4 && lo + T <= 3 * L / 4;
+    const int lo = first_index<P>(0, b, r);
+    return lo >= P::L / 4 && lo + P::T <= 3 * P::L / 4;
 }
 
 // Whether last-stage element (b, r) falls in the kept centre [L/4, 3L/4) of a
-// cropped transform (same argument: the output index is j + r L/RL).
-template <int L>
+// cropped transform (same argument).
+template <class P>
 __device__ __forceinline__ constexpr bool last_kept(int b, int r) {
-    constexpr int RL = Plan<L>::radix(Plan<L>::NST - 1), T = Plan<L>::T;
-    const int lo = T * b + r * (L / RL);
-    return lo >= L / 4 && lo + T <= 3 * L / 4;
+    const int lo = last_index<P>(0, b, r);
+    return lo >= P::L / 4 && lo + P::T <= 3 * P::L / 4;
 }
 
 }  // namespace olx

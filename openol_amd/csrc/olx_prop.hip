@@ -87,19 +87,19 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
 #pragma unroll
         for (int r = 0; r < R0; r++) {
             float2 x = make_float2(0.f, 0.f);
-            if (first_nonzero<L>(b, r) && valid) {
-                const int col = first_index<L>(t, b, r) - L / 4;
+            if (first_nonzero<P>(b, r) && valid) {
+                const int col = first_index<P>(t, b, r) - L / 4;
                 x = src[(size_t)row * nx + col];
                 if (a.ax) x = cmul(x, cmul(a.ax[col], ay));
             }
             v[b * R0 + r] = x;
         }
-    run_stages<L, false, TWN, 1>(v, t, lds, rr * P::LP, tw);
+    run_stages<P, false, TWN, 1>(v, t, lds, rr * P::LP, tw);
     if (valid) {
 #pragma unroll
         for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
-            for (int r = 0; r < RL; r++) W[widx(a, row, last_index<L>(t, b, r))] = v[b * RL + r];
+            for (int r = 0; r < RL; r++) W[widx(a, row, last_index<P>(t, b, r))] = v[b * RL + r];
     }
 }
 
@@ -111,18 +111,18 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
 __device__ __forceinline__ float2 asm_h(const PropArgs& a, double A, bool ucut, int i, int My) {
     const int fy = i < My / 2 ? i : i - My;
     const double v = (double)fy * a.dv;
-    if (ucut || (a.bandlimit && !(fabs(v) * 2 < a.vw))) return make_float2(0.f, 0.f);
     const double w2 = A - v * v;
-    if (w2 < 0.0) return make_float2(0.f, 0.f);  // evanescent: zero (reference: NaN)
-    double w = 0.0;
-    if (w2 > 1e-30) {
-        const double y0 = (double)__builtin_amdgcn_rsqf((float)w2);
-        const double s0 = w2 * y0;
-        w = fma(fma(-s0, s0, w2), 0.5 * y0, s0);
-    }
+    // branch-free: every lane evaluates and a 0/1 factor zeroes the cut and
+    // evanescent ones (bitwise &, so no short-circuit control flow either)
+    const bool keep = (!ucut) & ((!a.bandlimit) | (fabs(v) * 2 < a.vw)) & (w2 >= 0.0);
+    const double w2c = fmax(w2, 1e-30);
+    const double y0 = (double)__builtin_amdgcn_rsqf((float)w2c);
+    const double s0 = w2c * y0;
+    const double w = w2 > 1e-30 ? fma(fma(-s0, s0, w2c), 0.5 * y0, s0) : 0.0;
     const double cyc = w * a.z;
     const float t = (float)(cyc - rint(cyc));
-    const float s = __builtin_amdgcn_sinf(t), c = __builtin_amdgcn_cosf(t);
+    const float m = keep ? 1.f : 0.f;
+    const float s = __builtin_amdgcn_sinf(t) * m, c = __builtin_amdgcn_cosf(t) * m;
     return make_float2(c, a.adjoint ? -s : s);
 }
 
@@ -154,15 +154,15 @@ k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
 #pragma unroll
         for (int r = 0; r < R0; r++) {
             float2 x = make_float2(0.f, 0.f);
-            if (first_nonzero<L>(b, r) && valid)
-                x = Wb[(size_t)(first_index<L>(t, b, r) - L / 4) * wst];
+            if (first_nonzero<P>(b, r) && valid)
+                x = Wb[(size_t)(first_index<P>(t, b, r) - L / 4) * wst];
             v[b * R0 + r] = x;
         }
-    run_stages<L, false, TWN, CB>(v, t, lds, c, tw);
+    run_stages<P, false, TWN, CB>(v, t, lds, c, tw);
 #pragma unroll
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
-        for (int r = 0; r < RL; r++) lds[c + lpad(last_index<L>(t, b, r)) * CB] = v[b * RL + r];
+        for (int r = 0; r < RL; r++) lds[c + lpad(last_index<P>(t, b, r)) * CB] = v[b * RL + r];
     __syncthreads();
     // transfer function, then inverse transform
     double Acol = 0.0;  // 1/lambda^2 - u^2 of this column
@@ -189,15 +189,15 @@ k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
         *p = cmul(*p, h);
     }
     __syncthreads();
-    stage_load_lds<L, 0, CB>(v, t, lds, c);
+    stage_load_lds<P, 0, CB>(v, t, lds, c);
     __syncthreads();
-    run_stages<L, true, TWN, CB>(v, t, lds, c, tw);
+    run_stages<P, true, TWN, CB>(v, t, lds, c, tw);
 #pragma unroll
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
         for (int r = 0; r < RL; r++) {
-            if (!last_kept<L>(b, r)) continue;
-            const int i = last_index<L>(t, b, r);
+            if (!last_kept<P>(b, r)) continue;
+            const int i = last_index<P>(t, b, r);
             if (valid && i >= L / 4 && i < 3 * L / 4)
                 Wb[(size_t)(i - L / 4) * wst] = v[b * RL + r];
         }
@@ -221,8 +221,8 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
     for (int b = 0; b < P::E / R0; b++)
 #pragma unroll
         for (int r = 0; r < R0; r++)
-            v[b * R0 + r] = valid ? W[widx(a, row, first_index<L>(t, b, r))] : make_float2(0.f, 0.f);
-    run_stages<L, true, TWN, 1>(v, t, lds, rr * P::LP, tw);
+            v[b * R0 + r] = valid ? W[widx(a, row, first_index<P>(t, b, r))] : make_float2(0.f, 0.f);
+    run_stages<P, true, TWN, 1>(v, t, lds, rr * P::LP, tw);
     if (!valid) return;
     const bool post = a.cx != nullptr && abs(row - a.ny / 2) < a.chl;
     const float2 cy = post ? a.cy[row] : make_float2(1.f, 0.f);
@@ -230,14 +230,313 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
         for (int r = 0; r < RL; r++) {
-            if (!last_kept<L>(b, r)) continue;
-            const int i = last_index<L>(t, b, r);
+            if (!last_kept<P>(b, r)) continue;
+            const int i = last_index<P>(t, b, r);
             if (i < L / 4 || i >= 3 * L / 4) continue;
             const int col = i - L / 4;
             float2 x = cscale(v[b * RL + r], a.scale);
             if (post && abs(col - nx / 2) < a.cwl) x = cmul(x, cmul(a.cx[col], cy));
             dst[(size_t)row * nx + col] = x;
         }
+}
+
+// ------------------------------------------------------------------ tiled passes
+// For padded lengths >= 2048 the three passes use a tiled intermediate.  A
+// work-group carries 2 KR transforms in KR T threads (T = L/16): thread (c, t),
+// c = tid % KR, holds transform c in value set a and transform KR + c in set b,
+// and LDS holds the KR transforms of one set at a time.
+//   W layout: column tiles of CBL columns (a power of two >= the column pass's
+//   2 KR), each tile a contiguous ny x CBL block (widx with cbl = CBL).
+// Pass B reads and writes whole tile columns; passes A and C handle 2 KR
+// consecutive rows, so each tile receives a 2 KR x CBL block per work-group.  The
+// HBM byte count is the 80 ny nx of the row-major form; what changes is that the
+// column pass no longer reads one 8-byte value per 128-byte line.
+
+// Global accesses of the tiled passes are raw buffer operations: a per-thread
+// 32-bit byte offset in one VGPR, the per-value part (uniform) in an SGPR.  The
+// descriptor carries the buffer's size, so an access past the end reads zero and a
+// store past the end is dropped instead of touching other memory.
+struct Rsrc {
+    __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ Rsrc make_rsrc(const void* p, size_t bytes) {
+    // raw buffer, stride 0, gfx9 data format word
+    return Rsrc{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000)};
+}
+__device__ __forceinline__ float2 bload(Rsrc b, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(Rsrc b, uint32_t voff, uint32_t soff, float2 v) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), b.r, voff, soff, 0);
+}
+// two adjacent complex values in one 16-byte access
+__device__ __forceinline__ void bload2(Rsrc b, uint32_t voff, uint32_t soff, float2& x, float2& y) {
+    const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, voff, soff, 0));
+    x = make_float2(q.x, q.y);
+    y = make_float2(q.z, q.w);
+}
+__device__ __forceinline__ void bstore2(Rsrc b, uint32_t voff, uint32_t soff, float2 x, float2 y) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, make_float4(x.x, x.y, y.x, y.y)),
+                                           b.r, voff, soff, 0);
+}
+
+constexpr int PAIR_MIN = 2048;  // shortest padded length on the tiled path
+// largest KR: 1024 threads and KR padded transforms of LDS (136 KiB)
+template <int L>
+constexpr int kr_max() { return 16384 / L; }
+template <int L>
+constexpr int kr_lo() { return kr_max<L>() > 1 ? kr_max<L>() / 2 : 1; }
+// waves per SIMD the register budget must allow: two work-groups per CU when a
+// work-group is at most 512 threads and its LDS fits twice
+template <int L, int KR>
+constexpr int kr_waves() {
+    return (KR * Plan<L>::T <= 512 && 2 * KR * Plan<L>::LP * 8 <= 160 * 1024) ? 2 * KR * Plan<L>::T / 256 : 1;
+}
+
+template <int L, int KR, int ORD>
+__global__ void __launch_bounds__(KR * Plan<L>::T, (kr_waves<L, KR>()))
+k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a,
+                const float2* __restrict__ tw) {
+    using P = Plan<L, ORD>;
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
+    static_assert(P::T % 16 == 0, "pair plan");
+    __shared__ float2 lds[KR * P::LP];
+    const int tid = threadIdx.x, c = tid % KR, t = tid / KR;
+    const int h0 = xcd_remap(blockIdx.x, gridDim.x) * 2 * KR;
+    const int ha = h0 + c, hb = h0 + KR + c;
+    const int nx = a.nx;
+    const Rsrc sb = make_rsrc(src, (size_t)a.ny * nx * sizeof(float2));
+    const Rsrc wb = make_rsrc(W, (size_t)a.ny * 2 * nx * sizeof(float2));
+    const uint32_t rowb = (uint32_t)(KR * nx * sizeof(float2));  // set b rows past set a's
+    const uint32_t so = (uint32_t)((ha * nx + t) * sizeof(float2));
+    float2 aya = make_float2(1.f, 0.f), ayb = aya;
+    if (a.ay) {
+        aya = a.ay[ha];
+        ayb = a.ay[hb];
+    }
+    float2 va[P::E], vb[P::E];
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            float2 xa = make_float2(0.f, 0.f), xb = xa;
+            if (first_nonzero<P>(b, r)) {
+                // column t + D of rows ha / hb
+                const int D = first_index<P>(0, b, r) - L / 4;
+                xa = bload(sb, so, D * sizeof(float2));
+                xb = bload(sb, so, D * sizeof(float2) + rowb);
+                if (a.ax) {
+                    const float2 ax = a.ax[t + D];
+                    xa = cmul(xa, cmul(ax, aya));
+                    xb = cmul(xb, cmul(ax, ayb));
+                }
+            }
+            va[b * R0 + r] = xa;
+            vb[b * R0 + r] = xb;
+        }
+    run_stages_pair<P, false, TWN, KR>(va, vb, t, lds, c, tw);
+    // element t + D of row h sits at widx(h, t) + D ny (D a multiple of the tile width)
+    const uint32_t wo = (uint32_t)(widx(a, ha, t) * sizeof(float2));
+    const uint32_t setb = (uint32_t)(KR * a.cbl * sizeof(float2));
+#pragma unroll
+    for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const uint32_t so2 = (uint32_t)(last_index<P>(0, b, r) * a.ny * sizeof(float2));
+            bstore(wb, wo, so2, va[b * RL + r]);
+            bstore(wb, wo, so2 + setb, vb[b * RL + r]);
+        }
+}
+
+template <int L, int KR, int MODE, int ORDF, int ORDI>
+__global__ void __launch_bounds__(KR * Plan<L>::T, (kr_waves<L, KR>()))
+k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
+    using PF = Plan<L, ORDF>;  // forward transform
+    using PI = Plan<L, ORDI>;  // inverse transform
+    constexpr int R0 = PF::radix(0), RLF = PF::radix(PF::NST - 1);
+    constexpr int R0I = PI::radix(0), RLI = PI::radix(PI::NST - 1);
+    constexpr int E = PF::E, T = PF::T;
+    static_assert(T % 16 == 0, "pair plan");
+    __shared__ float2 lds[KR * PF::LP];
+    const int tid = threadIdx.x, c = tid % KR, t = tid / KR;
+    const int cbl = a.cbl;
+    // sets a and b are two neighbouring columns: one 16-byte access per row serves both
+    const int cola = xcd_remap(blockIdx.x, gridDim.x) * 2 * KR + 2 * c, colb = cola + 1;
+    const Rsrc wr = make_rsrc(W, (size_t)a.ny * 2 * a.nx * sizeof(float2));
+    // tile base; column within the tile
+    const uint32_t gb = (uint32_t)((size_t)(cola >> a.cbl_log) * a.ny * cbl * sizeof(float2));
+    const uint32_t wo = (uint32_t)((t * cbl + (cola & (cbl - 1))) * sizeof(float2));
+    const uint32_t rowst = (uint32_t)(cbl * sizeof(float2));
+    float2 va[E], vb[E];
+#pragma unroll
+    for (int b = 0; b < E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            float2 xa = make_float2(0.f, 0.f), xb = xa;
+            if (first_nonzero<PF>(b, r))
+                bload2(wr, wo, gb + (first_index<PF>(0, b, r) - L / 4) * rowst, xa, xb);
+            va[b * R0 + r] = xa;
+            vb[b * R0 + r] = xb;
+        }
+    run_stages_pair<PF, false, TWN, KR>(va, vb, t, lds, c, tw);
+    // transfer function on the spectrum held in registers
+    if constexpr (MODE == K_TABLE) {
+        const float2 sxa = a.sx[cola], sxb = a.sx[colb];
+        const Rsrc syb = make_rsrc(a.sy, L * sizeof(float2));
+        const uint32_t to = (uint32_t)(t * sizeof(float2));
+#pragma unroll
+        for (int b = 0; b < E / RLF; b++)
+#pragma unroll
+            for (int r = 0; r < RLF; r++) {
+                const float2 sy = bload(syb, to, last_index<PF>(0, b, r) * sizeof(float2));
+                float2 ha = cmul(sxa, sy), hb = cmul(sxb, sy);
+                if (a.adjoint) {
+                    ha = cconj(ha);
+                    hb = cconj(hb);
+                }
+                va[b * RLF + r] = cmul(va[b * RLF + r], ha);
+                vb[b * RLF + r] = cmul(vb[b * RLF + r], hb);
+            }
+    } else {
+        const int Mx = 2 * a.nx;
+        const int fxa = cola < Mx / 2 ? cola : cola - Mx, fxb = colb < Mx / 2 ? colb : colb - Mx;
+        const double ua = (double)fxa * a.du, ub = (double)fxb * a.du;
+        const double Aa = a.inv_l2 - ua * ua, Ab = a.inv_l2 - ub * ub;
+        const bool cuta = a.bandlimit && !(fabs(ua) * 2 < a.uw);
+        const bool cutb = a.bandlimit && !(fabs(ub) * 2 < a.uw);
+#pragma unroll
+        for (int b = 0; b < E / RLF; b++)
+#pragma unroll
+            for (int r = 0; r < RLF; r++) {
+                const int i = last_index<PF>(t, b, r);
+                va[b * RLF + r] = cmul(va[b * RLF + r], asm_h(a, Aa, cuta, i, L));
+                vb[b * RLF + r] = cmul(vb[b * RLF + r], asm_h(a, Ab, cutb, i, L));
+                // keep the FP64 temporaries of one element pair live at a time
+                __builtin_amdgcn_sched_barrier(0);
+            }
+    }
+    float2 wa[E], wb[E];
+#pragma unroll
+    for (int b = 0; b < E / R0I; b++)
+#pragma unroll
+        for (int r = 0; r < R0I; r++) {
+            wa[b * R0I + r] = va[spec_slot<PF, PI>(b, r)];
+            wb[b * R0I + r] = vb[spec_slot<PF, PI>(b, r)];
+        }
+    // (the forward transform ended with a barrier after its last LDS reads)
+    run_stages_pair<PI, true, TWN, KR>(wa, wb, t, lds, c, tw);
+#pragma unroll
+    for (int b = 0; b < E / RLI; b++)
+#pragma unroll
+        for (int r = 0; r < RLI; r++) {
+            if (!last_kept<PI>(b, r)) continue;
+            bstore2(wr, wo, gb + (last_index<PI>(0, b, r) - L / 4) * rowst, wa[b * RLI + r],
+                    wb[b * RLI + r]);
+        }
+}
+
+template <int L, int KR, int ORD>
+__global__ void __launch_bounds__(KR * Plan<L>::T, (kr_waves<L, KR>()))
+k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a,
+                 const float2* __restrict__ tw) {
+    using P = Plan<L, ORD>;
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
+    static_assert(P::T % 16 == 0, "pair plan");
+    __shared__ float2 lds[KR * P::LP];
+    const int tid = threadIdx.x, c = tid % KR, t = tid / KR;
+    const int h0 = xcd_remap(blockIdx.x, gridDim.x) * 2 * KR;
+    const int ha = h0 + c, hb = h0 + KR + c;
+    const int nx = a.nx;
+    // element t + D of row h sits at widx(h, t) + D ny (D a multiple of the tile width)
+    const Rsrc wb = make_rsrc(W, (size_t)a.ny * 2 * nx * sizeof(float2));
+    const Rsrc db = make_rsrc(dst, (size_t)a.ny * nx * sizeof(float2));
+    const uint32_t wo = (uint32_t)(widx(a, ha, t) * sizeof(float2));
+    const uint32_t setb = (uint32_t)(KR * a.cbl * sizeof(float2));
+    float2 va[P::E], vb[P::E];
+#pragma unroll
+    for (int b = 0; b < P::E / R0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const uint32_t so = (uint32_t)(first_index<P>(0, b, r) * a.ny * sizeof(float2));
+            va[b * R0 + r] = bload(wb, wo, so);
+            vb[b * R0 + r] = bload(wb, wo, so + setb);
+        }
+    run_stages_pair<P, true, TWN, KR>(va, vb, t, lds, c, tw);
+    const uint32_t so = (uint32_t)((ha * nx + t) * sizeof(float2));
+    const uint32_t rowb = (uint32_t)(KR * nx * sizeof(float2));
+    if (a.cx == nullptr) {
+#pragma unroll
+        for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                if (!last_kept<P>(b, r)) continue;
+                // column t + D
+                const uint32_t D = (last_index<P>(0, b, r) - L / 4) * sizeof(float2);
+                bstore(db, so, D, cscale(va[b * RL + r], a.scale));
+                bstore(db, so, D + rowb, cscale(vb[b * RL + r], a.scale));
+            }
+        return;
+    }
+    const bool posta = abs(ha - a.ny / 2) < a.chl;
+    const bool postb = abs(hb - a.ny / 2) < a.chl;
+    const float2 cya = posta ? a.cy[ha] : make_float2(1.f, 0.f);
+    const float2 cyb = postb ? a.cy[hb] : make_float2(1.f, 0.f);
+#pragma unroll
+    for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            if (!last_kept<P>(b, r)) continue;
+            const int D = last_index<P>(0, b, r) - L / 4;  // column t + D
+            const int col = t + D;
+            float2 xa = cscale(va[b * RL + r], a.scale), xb = cscale(vb[b * RL + r], a.scale);
+            if (abs(col - nx / 2) < a.cwl) {
+                const float2 cx = a.cx[col];
+                if (posta) xa = cmul(xa, cmul(cx, cya));
+                if (postb) xb = cmul(xb, cmul(cx, cyb));
+            }
+            bstore(db, so, D * sizeof(float2), xa);
+            bstore(db, so, D * sizeof(float2) + rowb, xb);
+        }
+}
+
+// work-group shapes and radix orders of the tiled path; OLX_TILE="ka,kb,kc,cbl"
+// and OLX_ORDER="oa,of,oi,oc" override them for tuning (each KR is kr_max or
+// kr_max / 2, each order 0 = remainder radix first, 1 = last)
+struct TileCfg {
+    int kra, krb, krc, cbl;
+    int oa, of, oi, oc;
+};
+
+template <int L, int KR>
+static void launch_rows_fwd(const TileCfg& g, const PropArgs& a, const float2* src, float2* W,
+                            const float2* tw, hipStream_t s) {
+    const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
+    if (g.oa) k_rows_pad_fwd2<L, KR, REM_LAST><<<grid, nt, 0, s>>>(src, W, a, tw);
+    else k_rows_pad_fwd2<L, KR, REM_FIRST><<<grid, nt, 0, s>>>(src, W, a, tw);
+}
+template <int L, int KR, int MODE>
+static void launch_cols_mode(const TileCfg& g, const PropArgs& a, float2* W, const float2* tw,
+                             hipStream_t s) {
+    const int grid = 2 * a.nx / (2 * KR), nt = KR * Plan<L>::T;
+    if (g.of && g.oi) k_cols_conv2<L, KR, MODE, REM_LAST, REM_LAST><<<grid, nt, 0, s>>>(W, a, tw);
+    else if (g.of) k_cols_conv2<L, KR, MODE, REM_LAST, REM_FIRST><<<grid, nt, 0, s>>>(W, a, tw);
+    else if (g.oi) k_cols_conv2<L, KR, MODE, REM_FIRST, REM_LAST><<<grid, nt, 0, s>>>(W, a, tw);
+    else k_cols_conv2<L, KR, MODE, REM_FIRST, REM_FIRST><<<grid, nt, 0, s>>>(W, a, tw);
+}
+template <int L, int KR>
+static void launch_cols_conv(const TileCfg& g, const PropArgs& a, float2* W, const float2* tw,
+                             hipStream_t s) {
+    if (a.mode == K_ASM) launch_cols_mode<L, KR, K_ASM>(g, a, W, tw, s);
+    else launch_cols_mode<L, KR, K_TABLE>(g, a, W, tw, s);
+}
+template <int L, int KR>
+static void launch_rows_inv(const TileCfg& g, const PropArgs& a, const float2* W, float2* dst,
+                            const float2* tw, hipStream_t s) {
+    const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
+    if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST><<<grid, nt, 0, s>>>(W, dst, a, tw);
+    else k_rows_crop_inv2<L, KR, REM_FIRST><<<grid, nt, 0, s>>>(W, dst, a, tw);
 }
 
 // ------------------------------------------------------------------ plain 2-D FFT
@@ -257,13 +556,13 @@ k_rows_fft(float2* __restrict__ u, int rows, const float2* __restrict__ tw) {
     for (int b = 0; b < P::E / R0; b++)
 #pragma unroll
         for (int r = 0; r < R0; r++)
-            v[b * R0 + r] = valid ? p[first_index<L>(t, b, r)] : make_float2(0.f, 0.f);
-    run_stages<L, INV, TWN, 1>(v, t, lds, rr * P::LP, tw);
+            v[b * R0 + r] = valid ? p[first_index<P>(t, b, r)] : make_float2(0.f, 0.f);
+    run_stages<P, INV, TWN, 1>(v, t, lds, rr * P::LP, tw);
     if (!valid) return;
 #pragma unroll
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
-        for (int r = 0; r < RL; r++) p[last_index<L>(t, b, r)] = v[b * RL + r];
+        for (int r = 0; r < RL; r++) p[last_index<P>(t, b, r)] = v[b * RL + r];
 }
 
 template <int L, bool INV>
@@ -281,14 +580,14 @@ k_cols_fft(float2* __restrict__ u, int cols, const float2* __restrict__ tw) {
     for (int b = 0; b < P::E / R0; b++)
 #pragma unroll
         for (int r = 0; r < R0; r++)
-            v[b * R0 + r] = valid ? u[(size_t)first_index<L>(t, b, r) * cols + col]
+            v[b * R0 + r] = valid ? u[(size_t)first_index<P>(t, b, r) * cols + col]
                                   : make_float2(0.f, 0.f);
-    run_stages<L, INV, TWN, CB>(v, t, lds, c, tw);
+    run_stages<P, INV, TWN, CB>(v, t, lds, c, tw);
     if (!valid) return;
 #pragma unroll
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
-        for (int r = 0; r < RL; r++) u[(size_t)last_index<L>(t, b, r) * cols + col] = v[b * RL + r];
+        for (int r = 0; r < RL; r++) u[(size_t)last_index<P>(t, b, r) * cols + col] = v[b * RL + r];
 }
 
 // ------------------------------------------------------------------ tables
@@ -472,6 +771,46 @@ static int launch_cols(int cb, const PropArgs& a, float2* W, const float2* tw, h
     return OLX_OK;
 }
 
+static int kr_max_of(int L) { return 16384 / L; }
+
+// Defaults (measured at 4096^2, padded 8192): 512-thread work-groups, so two fit on
+// a CU and one's memory phases overlap the other's arithmetic; tiles 8 columns
+// wide, so a row pass's two rows fill whole 128-byte lines; the column pass with
+// the remainder radix last in both transforms (no twiddles in its first radix-16
+// stage), the row passes with it first.
+static TileCfg tile_cfg(int Mx, int My) {
+    TileCfg c;
+    c.kra = c.krc = std::max(1, kr_max_of(Mx) / 2);
+    c.krb = std::max(1, kr_max_of(My) / 2);
+    c.cbl = std::max(8, 2 * c.krb);
+    c.oa = c.oc = REM_FIRST;
+    c.of = c.oi = REM_LAST;
+    static const char* oenv = getenv("OLX_ORDER");
+    if (oenv) {
+        int o[4];
+        if (sscanf(oenv, "%d,%d,%d,%d", &o[0], &o[1], &o[2], &o[3]) == 4) {
+            c.oa = o[0] ? REM_LAST : REM_FIRST;
+            c.of = o[1] ? REM_LAST : REM_FIRST;
+            c.oi = o[2] ? REM_LAST : REM_FIRST;
+            c.oc = o[3] ? REM_LAST : REM_FIRST;
+        }
+    }
+    static const char* env = getenv("OLX_TILE");
+    if (env) {
+        int v[4];
+        if (sscanf(env, "%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3]) == 4) {
+            auto pick = [](int want, int mx) { return want >= mx ? mx : std::max(1, mx / 2); };
+            c.kra = pick(v[0], kr_max_of(Mx));
+            c.krb = pick(v[1], kr_max_of(My));
+            c.krc = pick(v[2], kr_max_of(Mx));
+            int cb = 2 * c.krb;
+            while (cb < v[3] && cb < 16) cb *= 2;
+            c.cbl = cb;
+        }
+    }
+    return c;
+}
+
 static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a) {
     const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
     OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
@@ -483,12 +822,51 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
         const char* e = getenv("OLX_WLAYOUT");
         return e && strcmp(e, "blocked") == 0;
     }();
+    static const bool pair_off = [] {
+        const char* e = getenv("OLX_PAIR");
+        return e && strcmp(e, "0") == 0;
+    }();
+    hipStream_t s = ctx->stream;
+    if (!pair_off && Mx >= PAIR_MIN && My >= PAIR_MIN) {
+        TileCfg cfg = tile_cfg((int)Mx, (int)My);
+        a.cbl = cfg.cbl;
+        a.cbl_log = ilog2(a.cbl);
+        {
+            TimedLaunch tl(ctx, "prop_rows_pad_fwd");
+            OLX_LEN_SWITCH(ilog2(Mx), {
+                if constexpr (L >= PAIR_MIN) {
+                    if (cfg.kra == kr_max<L>()) launch_rows_fwd<L, kr_max<L>()>(cfg, a, src, W, ctx->twiddle, s);
+                    else launch_rows_fwd<L, kr_lo<L>()>(cfg, a, src, W, ctx->twiddle, s);
+                }
+            });
+        }
+        OLX_TRY(check_launch("prop pass A"));
+        {
+            TimedLaunch tl(ctx, "prop_cols_conv");
+            OLX_LEN_SWITCH(ilog2(My), {
+                if constexpr (L >= PAIR_MIN) {
+                    if (cfg.krb == kr_max<L>()) launch_cols_conv<L, kr_max<L>()>(cfg, a, W, ctx->twiddle, s);
+                    else launch_cols_conv<L, kr_lo<L>()>(cfg, a, W, ctx->twiddle, s);
+                }
+            });
+        }
+        OLX_TRY(check_launch("prop pass B"));
+        {
+            TimedLaunch tl(ctx, "prop_rows_crop_inv");
+            OLX_LEN_SWITCH(ilog2(Mx), {
+                if constexpr (L >= PAIR_MIN) {
+                    if (cfg.krc == kr_max<L>()) launch_rows_inv<L, kr_max<L>()>(cfg, a, W, dst, ctx->twiddle, s);
+                    else launch_rows_inv<L, kr_lo<L>()>(cfg, a, W, dst, ctx->twiddle, s);
+                }
+            });
+        }
+        return check_launch("prop pass C");
+    }
     int cb = 1;
     OLX_LEN_SWITCH(ilog2(My), { cb = col_batch<L>(); });
     if (cb > Mx) cb = (int)Mx;
     a.cbl = blocked ? cb : (int)Mx;
     a.cbl_log = ilog2(a.cbl);
-    hipStream_t s = ctx->stream;
     {
         TimedLaunch tl(ctx, "prop_rows_pad_fwd");
         OLX_LEN_SWITCH(ilog2(Mx), {

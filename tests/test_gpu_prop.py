@@ -124,6 +124,54 @@ def test_asm_matches_oracle_large(n):
     assert rel(u.cpu().numpy(), olref.asm_prop(u0, 8e-6, 8e-6, LAM, 0.12, olref.BANDLIMIT)) < TOL
 
 
+@pytest.mark.parametrize("shape,dy,dx,z", [
+    ((1024, 2048), 8e-6, 6e-6, 0.08), ((2048, 1024), 6e-6, 8e-6, 0.11), ((8192, 1024), 8e-6, 8e-6, 0.1)])
+def test_asm_tiled_rectangular(shape, dy, dx, z):
+    """Padded sides >= 2048 run the tiled passes, whose work-group shapes follow each
+    axis separately (4, 8 or 16 rows / columns per group)."""
+    u0 = synth.random_field(*shape, seed=35)
+    for flags, adj in ((olref.BANDLIMIT, False), (olref.BANDLIMIT | olref.ADJOINT, True)):
+        u = dev(u0)
+        ol().AsmProp(u, *shape, dy, dx, LAM, z, bandlimit=True, adjoint=adj)
+        assert rel(u.cpu().numpy(), olref.asm_prop(u0, dy, dx, LAM, z, flags)) < TOL
+
+
+@pytest.mark.parametrize("variant,flags", [(None, 0), ("cpu", olref.FRESNEL_BL_CPU)])
+def test_fresnel_tiled(variant, flags):
+    shape, d, z = (1024, 2048), 8e-6, 0.3
+    u0 = synth.random_field(*shape, seed=36)
+    u = dev(u0)
+    if variant is None:
+        ol().FresnelProp(u, *shape, d, d, LAM, z)
+    else:
+        ol().FresnelPropBandLimit(u, *shape, d, d, LAM, z, variant=variant)
+    assert rel(u.cpu().numpy(), olref.fresnel_prop(u0, d, d, LAM, z, flags)) < TOL
+
+
+def test_arss_tiled_offsets():
+    shape = (2048, 1024)
+    u0 = synth.random_field(*shape, seed=37)
+    src = dev(u0)
+    dst = torch.empty_like(src)
+    ol().ARSSFresnelProp(src, dst, *shape, 8e-6, 8e-6, LAM, -0.12, 0.25, 1e-4, -2e-4)
+    ref = olref.arss_prop(u0, 8e-6, 8e-6, LAM, -0.12, 0.25, 1e-4, -2e-4)
+    assert rel(dst.cpu().numpy(), ref) < TOL
+
+
+def test_asm_8192_properties():
+    """Largest supported field (8192^2, padded 16384^2): z = 0 identity and linearity."""
+    n = 8192
+    u0 = synth.random_field(n, n, seed=38)
+    u = dev(u0)
+    ol().AsmProp(u, n, n, 8e-6, 8e-6, LAM, 0.0, bandlimit=True)
+    assert rel(u.cpu().numpy(), u0) < 1e-6
+    a = dev(u0)
+    c = dev(u0 * (0.5 - 1j))
+    for t in (a, c):
+        ol().AsmProp(t, n, n, 8e-6, 8e-6, LAM, 0.2, bandlimit=True)
+    assert (torch.linalg.norm(a * (0.5 - 1j) - c) / torch.linalg.norm(c)).item() < 1e-6
+
+
 @pytest.mark.parametrize("variant,flags", [(None, 0), ("cpu", olref.FRESNEL_BL_CPU),
                                            ("gpu", olref.FRESNEL_BL_GPU)])
 @pytest.mark.parametrize("shape,dy,dx,z", [((8, 8), 6e-6, 8e-6, 0.001), ((8, 4), 6e-6, 8e-6, 0.002),
