@@ -77,6 +77,18 @@ def bench_asm(ol, lib, torch, n, steps, warmup):
     return dt, rep
 
 
+# Per-launch counters of the three passes at n = 4096 with the default work-group
+# shapes, measured with rocprofv3 --pmc: HBM bytes = FETCH_SIZE x 2 + WRITE_SIZE x 1
+# (the factors a plain 512 MiB copy kernel reads back), and SQ_INSTS_VALU.
+MEASURED_4096 = {
+    "prop_rows_pad_fwd": {"traffic": 404.4e6, "valu_instr": 24.87e6},
+    "prop_cols_conv": {"traffic": 538.9e6, "valu_instr": 117.31e6},
+    "prop_rows_crop_inv": {"traffic": 404.0e6, "valu_instr": 25.35e6},
+}
+# wave64 VALU instructions per second: 1024 SIMDs x 2.4 GHz / 4 cycles (no packed FP32)
+VALU_PEAK = 1024 * 2.4e9 / 4
+
+
 def roofline_of(rep, n):
     pb = pass_bytes(n)
     kern = {k: v for k, v in rep.items() if k in pb}
@@ -86,10 +98,21 @@ def roofline_of(rep, n):
     achieved = pb[dom] / avg_s / 1e9
     per_kernel = {k: {"avg_us": round(v[1] / v[0] * 1e3, 2),
                       "GBps": round(pb[k] / (v[1] / v[0] / 1e3) / 1e9, 1)} for k, v in kern.items()}
-    return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "algorithmic_bytes_per_launch": pb[dom], "avg_launch_us": round(avg_s * 1e6, 2),
-            "per_kernel": per_kernel}
+    # the counters hold for the default work-group shapes only
+    meas = MEASURED_4096.get(dom) if (n == 4096 and not os.environ.get("OLX_TILE")
+                                     and not os.environ.get("OLX_ORDER")) else None
+    out = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "traffic": round(meas["traffic"]) if meas else None,
+           "algorithmic_bytes_per_launch": pb[dom], "avg_launch_us": round(avg_s * 1e6, 2),
+           "per_kernel": per_kernel}
+    if meas:
+        # the column pass is limited by VALU issue, not by HBM: report that roofline too
+        rate = meas["valu_instr"] / avg_s
+        out["valu"] = {"instr_per_launch": meas["valu_instr"], "achieved": round(rate / 1e9, 1),
+                       "peak": round(VALU_PEAK / 1e9, 1), "unit": "G wave64-instr/s",
+                       "frac": round(rate / VALU_PEAK, 4)}
+    return out
 
 
 def cpu_baseline(n):

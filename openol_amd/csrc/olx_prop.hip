@@ -29,7 +29,8 @@ struct PropArgs {
     int mode;    // K_ASM / K_TABLE
     int adjoint;
     int bandlimit;
-    double du, dv, inv_l2, z, uw, vw;  // ASM transfer function and band limit
+    double du, dv, inv_l2, zs, uw, vw;  // ASM transfer function (zs: z, -z for the
+                                        // adjoint) and band limit
     const float2* sx;                  // K_TABLE: natural-order spectra, length Mx / My
     const float2* sy;
     const float2* ax;  // pre-chirp tables (length nx / ny), zero outside the window
@@ -105,25 +106,25 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
 
 // transfer function at spectrum row i (natural order) of a column whose
 // 1/lambda^2 - u^2 is A (FP64).  w = sqrt(A - v^2) comes from an FP32 rsqrt seed
-// and one FP64 Newton step (relative error ~1e-14); the phase w z is reduced
-// mod 1 cycle in FP64 and the sine / cosine are taken by the hardware unit in
-// revolutions (absolute error ~1e-7 on [-1/2, 1/2]).
+// and one FP64 Newton step (relative error ~1e-14); the phase w z is reduced to
+// [0, 1) cycle in FP64 and the sine / cosine are taken by the hardware unit in
+// revolutions (absolute error ~1e-7).  zs is z, negated for the adjoint (which
+// conjugates H).  Branch-free: every lane evaluates and a 0/1 factor zeroes the
+// cut and evanescent ones (a select of these values is turned into a branch by
+// the compiler); the argument of the root is |w2| + 1e-30, so every lane's
+// value is finite and the product is exact zero where masked.
 __device__ __forceinline__ float2 asm_h(const PropArgs& a, double A, bool ucut, int i, int My) {
     const int fy = i < My / 2 ? i : i - My;
     const double v = (double)fy * a.dv;
     const double w2 = A - v * v;
-    // branch-free: every lane evaluates and a 0/1 factor zeroes the cut and
-    // evanescent ones (bitwise &, so no short-circuit control flow either)
     const bool keep = (!ucut) & ((!a.bandlimit) | (fabs(v) * 2 < a.vw)) & (w2 >= 0.0);
-    const double w2c = fmax(w2, 1e-30);
+    const double w2c = fabs(w2) + 1e-30;
     const double y0 = (double)__builtin_amdgcn_rsqf((float)w2c);
     const double s0 = w2c * y0;
-    const double w = w2 > 1e-30 ? fma(fma(-s0, s0, w2c), 0.5 * y0, s0) : 0.0;
-    const double cyc = w * a.z;
-    const float t = (float)(cyc - rint(cyc));
+    const double w = fma(fma(-s0, s0, w2c), 0.5 * y0, s0);
+    const float t = (float)__builtin_amdgcn_fract(w * a.zs);
     const float m = keep ? 1.f : 0.f;
-    const float s = __builtin_amdgcn_sinf(t) * m, c = __builtin_amdgcn_cosf(t) * m;
-    return make_float2(c, a.adjoint ? -s : s);
+    return make_float2(__builtin_amdgcn_cosf(t) * m, __builtin_amdgcn_sinf(t) * m);
 }
 
 // ------------------------------------------------------------------ pass B
@@ -927,7 +928,7 @@ extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, doub
     a.du = 1 / (dx * nx2);
     a.dv = 1 / (dy * ny2);
     a.inv_l2 = 1 / (lambda * lambda);
-    a.z = z;
+    a.zs = a.adjoint ? -z : z;
     a.bandlimit = (flags & OLX_BANDLIMIT) ? 1 : 0;
     if (a.bandlimit) {
         // shiftedAsmTransferF at ox = oy = 0 (include/olprop.h:258-270): uw = ulp + ulm
