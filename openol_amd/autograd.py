@@ -40,14 +40,24 @@ def asm_prop(u, dy, dx, lam, z, bandlimit=True):
 
 
 def normalized_intensity_mse(field, target, norm="max"):
-    """MSE between |field|^2 normalised by its max (or mean) and the target."""
+    """MSE between the normalised intensity |field|^2 and the target.
+
+    norm="max":  mean((I / max I - T)^2), T in [0, 1];
+    norm="mean": mean((I / mean I - T / mean T)^2), invariant to the scale of
+    either side.  The max form has a degenerate optimum at large fields (one
+    bright pixel makes I / max I ~ 0 everywhere else, loss ~ mean T^2); the mean
+    form does not, and is the default of the optimiser."""
     inten = field.real ** 2 + field.imag ** 2
-    scale = inten.max() if norm == "max" else inten.mean()
-    return torch.mean((inten / scale - target) ** 2)
+    if norm == "max":
+        return torch.mean((inten / inten.max() - target) ** 2)
+    if norm == "mean":
+        return torch.mean((inten / inten.mean() - target / target.mean()) ** 2)
+    raise ValueError(f"unknown normalisation {norm!r}")
 
 
 class MultiPlaneLossFn(torch.autograd.Function):
-    """L(phi) = sum_p MSE(I_p / max I_p, T_p),  I_p = |A_p exp(i phi)|^2.
+    """L(phi) = sum_p MSE(I_p / max I_p, T_p) (or the mean-normalised form),
+    I_p = |A_p exp(i phi)|^2.
 
     Memory-lean: forward keeps only phi; backward recomputes each plane's field,
     takes the intensity-to-loss gradient with torch (max subgradient included),

@@ -2,7 +2,8 @@
 
 A phase map phi (float32, ny x nx) is propagated with the angular-spectrum method
 to several planes; the loss is the sum over planes of the MSE between the
-max-normalised intensity and a target in [0, 1].  Adam updates phi; gradients
+normalised intensity and the normalised target (mean normalisation by default,
+max normalisation on request).  Adam updates phi; gradients
 come from MultiPlaneLossFn, whose backward recomputes each plane's field and
 runs the hand-written adjoint propagation.
 """
@@ -22,7 +23,7 @@ def gaussian_targets(ny, nx, zs, seed=5, device="cuda"):
 
 
 def optimize_phase(targets, zs, dy=synth.PITCH, dx=synth.PITCH, lam=synth.LAMBDA, steps=200,
-                   lr=0.05, seed=0, phi0=None, bandlimit=True, norm="max", callback=None):
+                   lr=0.05, seed=0, phi0=None, bandlimit=True, norm="mean", callback=None):
     """Run `steps` Adam iterations; returns (phi, losses, seconds)."""
     ny, nx = targets[0].shape
     dev = targets[0].device

@@ -4,7 +4,8 @@
 * the multi-plane loss gradient equals an FP64 reference gradient built from the
   oracle's complex128 ASM and its adjoint (max-normalisation subgradient included),
   and central finite differences of the FP64 loss on 64 seeded pixels;
-* 200 Adam steps reduce the loss.
+* the same for the mean-normalised loss;
+* 200 Adam steps reduce the loss (both normalisations).
 """
 import numpy as np
 import pytest
@@ -22,20 +23,27 @@ def ag():
     return autograd
 
 
-def ref_loss_grad(phi, targets, zs, dy=P, dx=P):
-    """FP64 loss and dL/dphi (oracle complex128 ASM + adjoint)."""
+def ref_loss_grad(phi, targets, zs, dy=P, dx=P, norm="max"):
+    """FP64 loss and dL/dphi (oracle complex128 ASM + adjoint), with the
+    normalisation's derivative written out by hand."""
     u = np.exp(1j * phi.astype(np.float64))
     L, gu = 0.0, np.zeros_like(u)
     for t, z in zip(targets, zs):
         f = olref.asm_prop(u, dy, dx, LAM, z, olref.BANDLIMIT)
         inten = np.abs(f) ** 2
-        k = np.unravel_index(np.argmax(inten), inten.shape)
-        m = inten[k]
-        r = inten / m - t
-        n = r.size
+        n = inten.size
+        if norm == "max":
+            k = np.unravel_index(np.argmax(inten), inten.shape)
+            m = inten[k]
+            r = inten / m - t
+            dI = 2 * r / (m * n)
+            dI[k] += -np.sum(2 * r * inten / (m * m * n))
+        else:
+            # r = I/mu - T/tau, mu = mean I: dr_j/dI_k = delta_jk/mu - I_j/(mu^2 n)
+            mu = inten.mean()
+            r = inten / mu - t / t.mean()
+            dI = 2 * r / (mu * n) - 2 * np.sum(r * inten) / (mu * mu * n * n)
         L += np.mean(r ** 2)
-        dI = 2 * r / (m * n)
-        dI[k] += -np.sum(2 * r * inten / (m * m * n))
         gu += olref.asm_prop(2 * dI * f, dy, dx, LAM, z, olref.BANDLIMIT | olref.ADJOINT)
     return L, np.imag(np.conj(u) * gu)
 
@@ -90,13 +98,52 @@ def test_multiplane_gradient_vs_fp64(n):
         assert olref.rel_l2(got, fd) < 1e-3, (got[:4], fd[:4])
 
 
+@pytest.mark.parametrize("n", [64, 512])
+def test_multiplane_gradient_mean_norm_vs_fp64(n):
+    zs = [0.010, 0.012, 0.015] if n == 64 else [0.05, 0.06, 0.075]
+    phi0 = synth.uniform(8, synth.S_PHASE, n * n, 0, 2 * np.pi).reshape(n, n).astype(np.float32)
+    tg = [synth.gaussian_targets(n, n, 21 + i, smin=max(2.0, n / 100), smax=max(4.0, n / 10))
+          for i in range(3)]
+    phi = torch.from_numpy(phi0).cuda().requires_grad_(True)
+    targets = [torch.from_numpy(t).cuda() for t in tg]
+    loss = ag().multiplane_loss(phi, targets, zs, P, P, LAM, norm="mean")
+    loss.backward()
+    Lr, gr = ref_loss_grad(phi0, [t.astype(np.float64) for t in tg], zs, norm="mean")
+    assert abs(loss.item() - Lr) < 1e-5 * Lr
+    assert olref.rel_l2(phi.grad.cpu().numpy(), gr) < 1e-4
+    if n == 64:
+        rng = np.random.default_rng(10)
+        idx = rng.choice(n * n, 32, replace=False)
+        eps = 1e-3
+        fd = []
+        for k in idx:
+            p1 = phi0.astype(np.float64).copy().ravel()
+            p2 = p1.copy()
+            p1[k] += eps
+            p2[k] -= eps
+            l1, _ = ref_loss_grad(p1.reshape(n, n), [t.astype(np.float64) for t in tg], zs, norm="mean")
+            l2, _ = ref_loss_grad(p2.reshape(n, n), [t.astype(np.float64) for t in tg], zs, norm="mean")
+            fd.append((l1 - l2) / (2 * eps))
+        assert olref.rel_l2(phi.grad.cpu().numpy().ravel()[idx], np.array(fd)) < 1e-3
+
+
+def test_adam_mean_norm_2048():
+    """The optimisation workload's shape: the mean-normalised loss falls by half."""
+    from openol_amd import optim
+    n, zs = 2048, [0.10, 0.12, 0.15]
+    targets = optim.gaussian_targets(n, n, zs, seed=5)
+    phi, losses, dt = optim.optimize_phase(targets, zs, steps=60, lr=0.05, norm="mean")
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.6 * losses[0]
+
+
 def test_adam_reduces_loss():
     from openol_amd import optim
     n = 256
     zs = [0.05, 0.06, 0.075]
     targets = [torch.from_numpy(synth.gaussian_targets(n, n, 3 + i, smin=4, smax=30)).cuda()
                for i in range(3)]
-    phi, losses, dt = optim.optimize_phase(targets, zs, steps=200, lr=0.05)
+    phi, losses, dt = optim.optimize_phase(targets, zs, steps=200, lr=0.05, norm="max")
     print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f} in {dt:.2f} s")
     # three competing planes from one phase-only hologram: the loss must fall
     # clearly and keep falling (late average below early average)
