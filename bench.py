@@ -136,10 +136,53 @@ def cpu_baseline(n):
                       f"= {med:.2f} s/prop; own OpenMP radix-2 FFT (FFTW unavailable - substitute FFT)"}
 
 
+def ceilings(lib, torch):
+    """Rates this device reaches on kernels that do nothing else: a plain copy
+    (1 GiB) and the FMA / sine / MFMA issue loops of olx_peak_probe."""
+    import ctypes
+    L = lib.load()
+    c = lib.ctx()
+    out = {}
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        L.olx_stream_copy(c, b.data_ptr(), a.data_ptr(), nbytes)
+    torch.cuda.synchronize()
+    L.olx_ctx_reset_timing(c)
+    L.olx_ctx_enable_timing(c, 1)
+    for _ in range(10):
+        L.olx_stream_copy(c, b.data_ptr(), a.data_ptr(), nbytes)
+    torch.cuda.synchronize()
+    L.olx_ctx_enable_timing(c, 0)
+    cnt, tot = lib.timing_report()["stream_copy"]
+    out["copy_1GiB_TBps"] = round(2 * nbytes / (tot / cnt / 1e3) / 1e12, 3)
+    del a, b
+    r = ctypes.c_double()
+    for name, kind, scale in (("fma_f32_TFLOPs", 0, 1e12), ("fma_f64_TFLOPs", 1, 1e12),
+                              ("sin_f32_Gevals_s", 2, 1e9), ("mfma_f16_TFLOPs", 3, 1e12),
+                              ("mfma_bf16_TFLOPs", 4, 1e12)):
+        lib.check(L.olx_peak_probe(c, kind, ctypes.byref(r)), "peak_probe")
+        out[name] = round(r.value / scale, 1)
+    return out
+
+
+def asm_sizes(ol, lib, torch):
+    """ASM props/s at the smaller metric sizes (1024^2 and 2048^2 fields)."""
+    out = {}
+    for n in (1024, 2048):
+        dt, _ = bench_asm(ol, lib, torch, n, 100, 10)
+        out[f"asm_{n}_props_s"] = round(100 / dt, 1)
+    return out
+
+
 def extras(ol, lib, torch):
-    """Secondary workloads of the metric: direct vs layered CGH, WRP + ARSS."""
+    """Secondary workloads of the metric: direct vs layered CGH, WRP + ARSS,
+    the smaller ASM sizes, the multi-plane optimisation and the device ceilings."""
     from openol_amd import synth
     out = {}
+    out["ceilings"] = ceilings(lib, torch)
+    out.update(asm_sizes(ol, lib, torch))
     # direct CGH, 10,000 points over 2048^2, exact r (config 2)
     N, n = 2048, 10000
     pts = synth.random_points(n, seed=2, xmax=N * PITCH, ymax=N * PITCH, zmin=0.10, zmax=0.20)
@@ -155,6 +198,19 @@ def extras(ol, lib, torch):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         out["cgh_direct_" + ("exact" if exact else "fresnel") + "_Gptpx_s"] = round(n * N * N / dt / 1e9, 1)
+    # the FP64 kernel (exact r), complex128 output
+    U64 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+    ol.CghFP64(obj, U64, N, N, PITCH, PITCH, LAM, exact_r=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ol.CghFP64(obj, U64, N, N, PITCH, PITCH, LAM, exact_r=True)
+    torch.cuda.synchronize()
+    out["cgh_direct_exact_fp64_Gptpx_s"] = round(n * N * N / (time.perf_counter() - t0) / 1e9, 1)
+    del U64
+    # one interaction needs a sine and a cosine: the transcendental ceiling
+    sin_rate = out["ceilings"]["sin_f32_Gevals_s"] * 1e9
+    out["cgh_direct_fresnel_frac_of_sincos_ceiling"] = round(
+        out["cgh_direct_fresnel_Gptpx_s"] * 1e9 / (sin_rate / 2), 3)
     # WRP (200,000 points) + ARSS to the hologram (config 3)
     N, n = 4096, 200000
     pts = synth.random_points(n, seed=3, xmax=N * PITCH, ymax=N * PITCH, zmin=0.151, zmax=0.155)
@@ -179,6 +235,7 @@ def extras(ol, lib, torch):
     arss_s = (t2 - t1) / reps
     out["arss_4096_ms"] = round(arss_s * 1e3, 3)
     out["arss_4096_GBps_80N2"] = round(80 * N * N / arss_s / 1e9, 1)
+    out["wrp_arss_4096_end_to_end_ms"] = round(out["wrp_200k_4096_ms"] + arss_s * 1e3, 3)
     return out
 
 
@@ -211,6 +268,11 @@ def layered(ol, torch):
         dt = (time.perf_counter() - t0) / reps
         got = U[h0:h0 + win, w0:w0 + win]
         err = (torch.linalg.norm(got - D) / torch.linalg.norm(D)).item()
+        # phase error: amplitude-weighted mean and 99.9th percentile of |arg(got/D)|
+        dphi = torch.angle(got * torch.conj(D)).abs()
+        amp = D.abs()
+        out[f"cgh_layered_{prec}_phase_err_wmean_rad"] = float(f"{(dphi * amp).sum().item() / amp.sum().item():.3e}")
+        out[f"cgh_layered_{prec}_phase_err_p999_rad"] = float(f"{torch.quantile(dphi.flatten(), 0.999).item():.3e}")
         out[f"cgh_layered_{prec}_ms"] = round(dt * 1e3, 2)
         out[f"cgh_layered_{prec}_Gptpx_s"] = round(n * N * N / dt / 1e9, 1)
         out[f"cgh_layered_{prec}_relL2_vs_direct_f32"] = float(f"{err:.3e}")

@@ -173,6 +173,16 @@ int olx_ctx_reset_timing(olx_ctx* ctx);
 int olx_ctx_timing_report(olx_ctx* ctx, char* buf, size_t buflen);
 /* plain device copy kernel (calibrates the HBM byte counters) */
 int olx_stream_copy(olx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+/* On-device ceilings (no reference counterpart; measurement only): runs a kernel
+ * that does nothing but the named operation on every CU and returns its rate:
+ * FP32 / FP64 FMA in flop/s (2 per FMA), hardware sine in evaluations/s, FP16 /
+ * BF16 32x32x16 MFMA in flop/s.  Synchronises the context stream. */
+#define OLX_PROBE_FMA_F32 0
+#define OLX_PROBE_FMA_F64 1
+#define OLX_PROBE_SIN_F32 2
+#define OLX_PROBE_MFMA_F16 3
+#define OLX_PROBE_MFMA_BF16 4
+int olx_peak_probe(olx_ctx* ctx, int kind, double* rate);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,7 @@ SIGNATURES = {
     "olx_ctx_reset_timing": [_p],
     "olx_ctx_timing_report": [_p, ctypes.c_char_p, ctypes.c_size_t],
     "olx_stream_copy": [_p, _p, _p, ctypes.c_size_t],
+    "olx_peak_probe": [_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
 }
 DOUBLE_RESULT = {"olx_wrp_plane_z", "olx_layer_depth"}
 
