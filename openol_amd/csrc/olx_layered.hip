@@ -77,12 +77,14 @@ __global__ void k_lay_prep(const olx_point* __restrict__ pts, int n, LayPt* __re
     out[j] = q;
 }
 
-// per-layer chirp tables: ch[l * n + i] = exp(i pi (d i)^2 c_l)
-__global__ void k_lay_chirp(float2* __restrict__ ch, int n, int nl, double d, double lambda,
-                            double zmin, double zmax) {
+// per-layer chirp tables: ch[l * n + i] = exp(i pi (d i)^2 c_l), and the chirp's
+// half-step in cycles per pixel^2, cd2[l] = d^2 c_l / 2
+__global__ void k_lay_chirp(float2* __restrict__ ch, float* __restrict__ cd2, int n, int nl,
+                            double d, double lambda, double zmin, double zmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
     if (i >= n) return;
     const double zl = zmin + ((double)l + 0.5) * (zmax - zmin) / nl;
+    if (i == 0) cd2[l] = (float)(0.5 * d * d / (lambda * zl));
     const double x = d * i;
     double cyc = 0.5 * x * x / (lambda * zl);
     cyc -= rint(cyc);
@@ -119,8 +121,8 @@ constexpr int GEN_RB = 16;  // rows per thread (one exact anchor, then a recurre
 // MODE 0: one factor matrix; MODE 1: hi in out, lo = value - hi in out2 (split bf16)
 template <typename T, bool ISY, int MODE>
 __global__ void k_lay_gen(const LayPt* __restrict__ pts, int j0, int n, int kc,
-                          const float2* __restrict__ chirp, int rows, uint32_t* __restrict__ out,
-                          uint32_t* __restrict__ out2) {
+                          const float2* __restrict__ chirp, const float* __restrict__ chd2,
+                          int rows, uint32_t* __restrict__ out, uint32_t* __restrict__ out2) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;  // chunk-local point
     const int rb = blockIdx.y * GEN_RB;
     if (j >= kc) return;
@@ -138,11 +140,21 @@ __global__ void k_lay_gen(const LayPt* __restrict__ pts, int j0, int n, int kc,
     float ts = hi + lo;
     ts = -(ts - rintf(ts));
     const float2 stp = make_float2(__builtin_amdgcn_cosf(ts), __builtin_amdgcn_sinf(ts));
-    const float2* chl = chirp + (size_t)p.layer * rows;
+    // layer chirp e^{i pi c (d r)^2}: the FP64-built table gives the anchor at rb;
+    // the next rows follow from ch(r+1) = ch(r) g(r), g(r) = e^{i pi c d^2 (2r+1)},
+    // g(r+1) = g(r) e^{2 pi i c d^2}; their phases are at most a few cycles, exact
+    // enough in FP32 (c d^2 = cd2 cycles per pixel^2 over 2)
+    float2 ch = chirp[(size_t)p.layer * rows + rb];
+    const float cd2 = chd2[p.layer];
+    float tg = cd2 * (2.0f * fr + 1.0f);
+    tg -= rintf(tg);
+    float2 gr = make_float2(__builtin_amdgcn_cosf(tg), __builtin_amdgcn_sinf(tg));
+    float tgg = 2.0f * cd2;
+    tgg -= rintf(tgg);
+    const float2 gg = make_float2(__builtin_amdgcn_cosf(tgg), __builtin_amdgcn_sinf(tgg));
     for (int q = 0; q < GEN_RB; q++) {
         const int r = rb + q;
         if (r >= rows) break;
-        const float2 ch = chl[r];
         float2 v = make_float2(ch.x * e.x - ch.y * e.y, ch.x * e.y + ch.y * e.x);
         if (ISY) v = make_float2(v.x * p.a.x - v.y * p.a.y, v.x * p.a.y + v.y * p.a.x);
         else v.y = -v.y;
@@ -155,6 +167,8 @@ __global__ void k_lay_gen(const LayPt* __restrict__ pts, int j0, int n, int kc,
             out2[o] = pack2<T>(v.x - hx, v.y - hy, &dx_, &dy_);
         }
         e = make_float2(e.x * stp.x - e.y * stp.y, e.x * stp.y + e.y * stp.x);
+        ch = make_float2(ch.x * gr.x - ch.y * gr.y, ch.x * gr.y + ch.y * gr.x);
+        gr = make_float2(gr.x * gg.x - gr.y * gg.y, gr.x * gg.y + gr.y * gg.x);
     }
 }
 
@@ -316,21 +330,25 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
     const size_t pts_bytes = sizeof(LayPt) * (size_t)n;
     const size_t cx_off = (pts_bytes + 255) & ~(size_t)255;
     const size_t cy_off = cx_off + sizeof(float2) * (size_t)nlayers * nx;
-    const size_t meta_bytes = cy_off + sizeof(float2) * (size_t)nlayers * ny;
+    const size_t dx_off = cy_off + sizeof(float2) * (size_t)nlayers * ny;
+    const size_t dy_off = dx_off + sizeof(float) * (size_t)nlayers;
+    const size_t meta_bytes = dy_off + sizeof(float) * (size_t)nlayers;
     OLX_TRY(ctx->lay_pts.reserve(meta_bytes));
     char* meta = (char*)ctx->lay_pts.p;
     LayPt* lp = (LayPt*)meta;
     float2* cx = (float2*)(meta + cx_off);
     float2* cy = (float2*)(meta + cy_off);
+    float* cdx = (float*)(meta + dx_off);
+    float* cdy = (float*)(meta + dy_off);
     const int nb = nbits_for(std::max(nx, ny));
     {
         TimedLaunch tl(ctx, "layered_prep");
         k_lay_prep<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d_pts, (int)n, lp, px, py, lambda,
                                                                zmin, zmax, nlayers, nb);
-        k_lay_chirp<<<dim3((unsigned)((nx + 255) / 256), nlayers), 256, 0, s>>>(cx, (int)nx, nlayers,
-                                                                               px, lambda, zmin, zmax);
-        k_lay_chirp<<<dim3((unsigned)((ny + 255) / 256), nlayers), 256, 0, s>>>(cy, (int)ny, nlayers,
-                                                                               py, lambda, zmin, zmax);
+        k_lay_chirp<<<dim3((unsigned)((nx + 255) / 256), nlayers), 256, 0, s>>>(
+            cx, cdx, (int)nx, nlayers, px, lambda, zmin, zmax);
+        k_lay_chirp<<<dim3((unsigned)((ny + 255) / 256), nlayers), 256, 0, s>>>(
+            cy, cdy, (int)ny, nlayers, py, lambda, zmin, zmax);
     }
     OLX_TRY(check_launch("layered prep"));
     // factor buffers for one chunk of points (KC multiple of KS/2 points)
@@ -353,14 +371,14 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
         {
             TimedLaunch tl(ctx, "layered_factors");
             if (precision == OLX_LAYERED_F16) {
-                k_lay_gen<_Float16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, (int)ny, (uint32_t*)Yh, nullptr);
-                k_lay_gen<_Float16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, (int)nx, (uint32_t*)Xh, nullptr);
+                k_lay_gen<_Float16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)Yh, nullptr);
+                k_lay_gen<_Float16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)Xh, nullptr);
             } else if (precision == OLX_LAYERED_BF16) {
-                k_lay_gen<__bf16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, (int)ny, (uint32_t*)Yh, nullptr);
-                k_lay_gen<__bf16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, (int)nx, (uint32_t*)Xh, nullptr);
+                k_lay_gen<__bf16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)Yh, nullptr);
+                k_lay_gen<__bf16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)Xh, nullptr);
             } else {
-                k_lay_gen<__bf16, true, 1><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, (int)ny, (uint32_t*)Yh, (uint32_t*)Yl);
-                k_lay_gen<__bf16, false, 1><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, (int)nx, (uint32_t*)Xh, (uint32_t*)Xl);
+                k_lay_gen<__bf16, true, 1><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)Yh, (uint32_t*)Yl);
+                k_lay_gen<__bf16, false, 1><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)Xh, (uint32_t*)Xl);
             }
         }
         OLX_TRY(check_launch("layered factors"));
