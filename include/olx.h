@@ -31,6 +31,7 @@ extern "C" {
 #define OLX_ENOMEM (-2)       /* device allocation failed */
 #define OLX_EDEVICE (-3)      /* HIP runtime / launch error */
 #define OLX_EUNSUPPORTED (-4) /* valid but unsupported size or mode */
+#define OLX_EIO (-5)          /* file cannot be opened, read or written */
 
 /* propagation flags */
 #define OLX_BANDLIMIT 0x1u    /* ASM band limit |u| < 1/(lambda sqrt((2 du z)^2 + 1)), include/olprop.h:250-305 */
@@ -164,6 +165,83 @@ int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d
 /* the snapping rule, on the host: layer l = clamp(floor((z - zmin)/(zmax - zmin) * L), 0, L-1),
  * z_l = zmin + (l + 0.5) (zmax - zmin)/L */
 double olx_layer_depth(double z, double zmin, double zmax, int nlayers);
+
+/* ---- point-cloud ingest (include/olobject.h) ----------------------------- */
+/* .3d: int32 N, then N x (int32 x, y, z); .3df: int32 N, then N x (float x, y, z),
+ * little-endian.  OLX_OBJ_AUTO picks the format by extension as objread does
+ * (include/olobject.h:88-100).  Each point becomes x = v_x scale + ox (likewise y, z;
+ * objread with offset and scale, include/olobject.h:102-162; scale 1 and a zero
+ * offset give objread_3d / objread_3df, :9-86), amp = 1, phase = 0.  With pts ==
+ * NULL only *n is set (the file's N); otherwise min(N, cap) points are written to
+ * host memory and *n is set to N. */
+#define OLX_OBJ_AUTO 0
+#define OLX_OBJ_3D 1
+#define OLX_OBJ_3DF 2
+int olx_obj_read(const char* path, int format, olx_point* pts, int64_t cap, int64_t* n,
+                 double scale, double ox, double oy, double oz);
+/* device bounding box (objinfo's minmax_element, include/olobject.h:164-175):
+ * mn[3] / mx[3] receive (x, y, z) minima / maxima (host pointers) */
+int olx_obj_bounds(olx_ctx* ctx, const olx_point* d_pts, int64_t n, double* mn, double* mx);
+/* objset on the device (include/olobject.h:195-264): scale the cloud so that its
+ * x/y extent fits the range(s), centre x and y on the offset and put z's minimum
+ * (OLX_OBJSET_ZMIN) or mid-range (OLX_OBJSET_ZMIDDLE) at oz.  y_range <= 0 selects
+ * the single-range overload (scale = xy_range / max(range_x, range_y)), else
+ * scale = min(x_range / range_x, y_range / range_y).  The ranges are rounded to
+ * float first, as the reference's parameters are.  The arithmetic is that of the
+ * reference's Object<double> (FP64, one rounding per operation); with
+ * OLX_OBJSET_F32 or'ed into mode it is that of Object<float> (ranges, scale,
+ * offsets and every point in FP32, coordinates and offsets rounded to float first). */
+#define OLX_OBJSET_ZMIN 0
+#define OLX_OBJSET_ZMIDDLE 1
+#define OLX_OBJSET_F32 0x100
+int olx_obj_set(olx_ctx* ctx, olx_point* d_pts, int64_t n, double x_range, double y_range,
+                double ox, double oy, double oz, int mode);
+/* objsort on the device (include/olobject.h:177-182): descending z.  Stable: points
+ * with equal z (+0 and -0 are equal) keep their input order (the reference's
+ * std::sort leaves that order unspecified). */
+int olx_obj_sort_z(olx_ctx* ctx, olx_point* d_pts, int64_t n);
+
+/* ---- 8-bit hologram encodings and BMP files (include/olimg.h) ------------- */
+/* The device encodings take float (complex<float>) input, or double (complex<double>)
+ * with OLX_IN_F64 or'ed into flags / mode: the reference's templates over REAL_T /
+ * PREC_T.  Arithmetic is FP64 with one rounding per operation, rounding is C round()
+ * (half away from zero), and the conversion of the rounded value to uint8 is the one
+ * the reference's x86-64 build performs (cvttsd2si: modulo 256 inside the int32
+ * range, 0 for NaN and outside it). */
+#define OLX_IN_F64 0x100
+/* phase8bit (include/olimg.h:205-213) on n device values: round((phi - pi) / 2pi
+ * * 255), which is negative for phi in (-pi, pi) and so wraps modulo 256.
+ * OLX_PHASE8_OFFSET maps round((phi + pi) / 2pi * 255) instead (no wrap for phi in
+ * [-pi, pi]; no reference counterpart). */
+#define OLX_PHASE8_OFFSET 0x1u
+int olx_phase8bit(olx_ctx* ctx, const void* d_phase, uint8_t* d_img, int64_t n, uint32_t flags);
+/* quant8bit (include/olimg.h:178-199): round((v - min) / (max - min) * 255) with the
+ * extremes of the n values; all-equal input gives 0 (the reference's 0 / 0 NaN,
+ * converted as above).  flags: 0 or OLX_IN_F64. */
+int olx_quant8bit(olx_ctx* ctx, const void* d_real, uint8_t* d_img, int64_t n, uint32_t flags);
+/* complex2img (include/olimg.h:314-391) on n complex device values, fused (two
+ * reads of u, no intermediate array; one read for PHASE): REAL, IMAG, AMP
+ * (sqrt(re*re + im*im), products and sum in the element type, square root in FP64)
+ * or PHASE (atan2 in FP64, fixed range [-pi, pi]), mapped by
+ * round(255 (v - min) / (max - min)).  OLX_IMG_REAL_LEGACY is the bool overload's
+ * real mode, whose maximum starts at 0 (:314-346); its amplitude mode equals
+ * OLX_IMG_AMP.  mode may carry OLX_IN_F64. */
+#define OLX_IMG_PHASE 0
+#define OLX_IMG_REAL 1
+#define OLX_IMG_IMAG 2
+#define OLX_IMG_AMP 3
+#define OLX_IMG_REAL_LEGACY 4
+int olx_complex2img(olx_ctx* ctx, const void* d_u, uint8_t* d_img, int64_t n, int mode);
+/* complex2real (include/olimg.h:393-420): REAL / IMAG / AMP / PHASE into the element
+ * type (float, or double with OLX_IN_F64 in mode) */
+int olx_complex2real(olx_ctx* ctx, const void* d_u, void* d_real, int64_t n, int mode);
+/* bmpwrite (include/olimg.h:136-165): 8-bit grey BMP with a 256-entry palette from
+ * a host image of ny rows of nx bytes, written in order, each row followed by
+ * nx % 4 zero bytes (the reference's padding) */
+int olx_bmp_write(const char* path, const uint8_t* img, int ny, int nx);
+/* bmpread (include/olimg.h:56-99): skips the 14 + 40 + 1024 header bytes, then reads
+ * ny rows of nx bytes, each followed by nx % 4 skipped bytes */
+int olx_bmp_read(const char* path, uint8_t* img, int ny, int nx);
 
 /* ---- instrumentation ---------------------------------------------------- */
 /* per-kernel device time measured with HIP events on the context stream */

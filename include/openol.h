@@ -1,5 +1,6 @@
 // openol.h -- umbrella header of the MI355X drop-in (role of the reference's
-// include/openol.h:3-22): the propagation and CGH APIs over libolx.so.
+// include/openol.h:3-22): the propagation, CGH, point-cloud and image APIs over
+// libolx.so.
 // Link with -L<repo>/openol_amd -lolx (rpath to the same directory).
 #ifndef OPENOL_H
 #define OPENOL_H
@@ -10,6 +11,8 @@
 #include "olclass.h"
 #include "oldefine.h"
 #include "oldevice.h"
+#include "olimg.h"
+#include "olobject.h"
 #include "olprop.h"
 #include "olx.h"
 #endif

@@ -11,7 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libolx.so")
 
 OK = 0
-EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED = -1, -2, -3, -4
+EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, EIO = -1, -2, -3, -4, -5
 
 BANDLIMIT = 0x1
 ADJOINT = 0x2
@@ -24,6 +24,11 @@ ARSS_CPU = 0x800
 
 MODE_ASM, MODE_FRESNEL, MODE_AUTO, MODE_FRESNEL_BL = 0, 1, 2, 3
 LAYERED_F16, LAYERED_BF16, LAYERED_BF16X3 = 0, 1, 2
+OBJ_AUTO, OBJ_3D, OBJ_3DF = 0, 1, 2
+OBJSET_ZMIN, OBJSET_ZMIDDLE, OBJSET_F32 = 0, 1, 0x100
+IN_F64 = 0x100
+PHASE8_OFFSET = 0x1
+IMG_PHASE, IMG_REAL, IMG_IMAG, IMG_AMP, IMG_REAL_LEGACY = 0, 1, 2, 3, 4
 
 
 class OlxError(RuntimeError):
@@ -74,6 +79,16 @@ SIGNATURES = {
     "olx_ctx_timing_report": [_p, ctypes.c_char_p, ctypes.c_size_t],
     "olx_stream_copy": [_p, _p, _p, ctypes.c_size_t],
     "olx_peak_probe": [_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+    "olx_obj_read": [ctypes.c_char_p, _i, _p, _i64, ctypes.POINTER(_i64), _d, _d, _d, _d],
+    "olx_obj_bounds": [_p, _p, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d)],
+    "olx_obj_set": [_p, _p, _i64, _d, _d, _d, _d, _d, _i],
+    "olx_obj_sort_z": [_p, _p, _i64],
+    "olx_phase8bit": [_p, _p, _p, _i64, _u32],
+    "olx_quant8bit": [_p, _p, _p, _i64, _u32],
+    "olx_complex2img": [_p, _p, _p, _i64, _i],
+    "olx_complex2real": [_p, _p, _p, _i64, _i],
+    "olx_bmp_write": [ctypes.c_char_p, _p, _i, _i],
+    "olx_bmp_read": [ctypes.c_char_p, _p, _i, _i],
 }
 DOUBLE_RESULT = {"olx_wrp_plane_z", "olx_layer_depth"}
 

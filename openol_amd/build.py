@@ -31,10 +31,18 @@ FLAGS = [
     "-fno-signed-zeros",
 ]
 
+# Byte-exact ports of the reference's x86-64 arithmetic: one rounding per operation
+# (HIP contracts a * b + c into an FMA by default, and __dadd_rn / __dmul_rn are
+# plain operators there) and IEEE signed zeros (atan2(-0, x), the +0 / -0 sort key).
+FILE_FLAGS = {
+    "olx_obj.hip": ["-ffp-contract=off", "-fsigned-zeros"],
+    "olx_img.hip": ["-ffp-contract=off", "-fsigned-zeros"],
+}
+
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
                     if "packed-fp32-ops" not in l)

@@ -65,6 +65,8 @@ struct olx_ctx {
     // WRP / CGH scratch
     olx::DevBuf wrp_counts, wrp_lists, wrp_pts;
     olx::DevBuf lay_pts, lay_counts;
+    // point-cloud sort and reduction scratch
+    olx::DevBuf obj_keys, obj_tmp;
     // timing
     bool timing = false;
     std::map<std::string, olx::KernelStat> stats;

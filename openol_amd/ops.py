@@ -1,19 +1,21 @@
 """Python mirror of OpenOL's diffraction / CGH operators on MI355X.
 
 Same names, argument order and meaning as the reference's C++ templates
-(include/olprop.h, include/olcgh.h, include/golprop.h, include/golcgh.h), on
+(include/olprop.h, include/olcgh.h, include/golprop.h, include/golcgh.h,
+include/olobject.h, include/olimg.h), on
 torch tensors that live on the GPU.  Fields are complex64 tensors of shape
 (height, width), row-major (idx = h * width + w), updated in place like the
 reference's unique_ptr buffers.  Every call goes through the C ABI of libolx.so;
 errors raise (OlxInvalid for bad arguments) instead of exiting the process.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check
+from ._lib import OlxInvalid, check
 
 ASM, FRESNEL, AUTO = _lib.MODE_ASM, _lib.MODE_FRESNEL, _lib.MODE_AUTO
 FRESNEL_BL = _lib.MODE_FRESNEL_BL
@@ -137,11 +139,31 @@ class Object:
         rec["z"] = np.asarray(z, dtype=np.float64).ravel()
         rec["amp"] = 1.0 if amp is None else np.asarray(amp, dtype=np.float32).ravel()
         rec["phase"] = 0.0 if phase is None else np.asarray(phase, dtype=np.float32).ravel()
-        self.size = n
-        self.host = rec
+        self._upload(rec, device)
+
+    def _upload(self, rec, device):
+        self.size = rec.size
+        self._host = rec
         dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         buf = torch.from_numpy(rec.view(np.uint8).copy())
-        self.data = buf.to(dev) if n else torch.empty(32, dtype=torch.uint8, device=dev)
+        self.data = buf.to(dev) if rec.size else torch.empty(32, dtype=torch.uint8, device=dev)
+
+    @classmethod
+    def from_records(cls, rec, device=None):
+        """A cloud from a POINT_DTYPE record array (as returned by objread_host)."""
+        obj = cls.__new__(cls)
+        obj._upload(np.ascontiguousarray(rec, dtype=POINT_DTYPE), device)
+        return obj
+
+    @property
+    def host(self):
+        """The records, read back from the device after an in-place device update."""
+        if self._host is None:
+            self._host = self.data[: self.size * POINT_DTYPE.itemsize].cpu().numpy().view(POINT_DTYPE).copy()
+        return self._host
+
+    def _touched(self):
+        self._host = None
 
     @property
     def device(self):
@@ -278,3 +300,151 @@ def CghLayered(obj, U, ny, nx, py, px, lam, nlayers, zmin, zmax, precision="f16"
 def layer_depth(z, zmin, zmax, nlayers):
     """Depth a point is snapped to by CghLayered."""
     return _lib.load().olx_layer_depth(z, zmin, zmax, nlayers)
+
+
+# ------------------------------------------------------------------ point-cloud ingest
+ZMIN, ZMIDDLE = _lib.OBJSET_ZMIN, _lib.OBJSET_ZMIDDLE
+_OBJFMT = {"auto": _lib.OBJ_AUTO, "3d": _lib.OBJ_3D, "3df": _lib.OBJ_3DF}
+
+
+def objread_host(path, scale=1.0, offset=(0.0, 0.0, 0.0), fmt="auto"):
+    """ol::objread (include/olobject.h:9-162) into host POINT_DTYPE records:
+    .3d (int32) or .3df (float) triples, x = v scale + offset, amp 1, phase 0."""
+    L = _lib.load()
+    n = ctypes.c_int64()
+    bpath = os.fsencode(path)
+    check(L.olx_obj_read(bpath, _OBJFMT[fmt], None, 0, ctypes.byref(n), scale, *offset), "objread")
+    rec = np.zeros(n.value, dtype=POINT_DTYPE)
+    if n.value:
+        check(L.olx_obj_read(bpath, _OBJFMT[fmt], rec.ctypes.data, n.value, ctypes.byref(n), scale,
+                             *offset), "objread")
+    return rec
+
+
+def objread(path, scale=1.0, offset=(0.0, 0.0, 0.0), fmt="auto", device=None):
+    """ol::objread followed by Object2cuda: the cloud in device memory."""
+    return Object.from_records(objread_host(path, scale, offset, fmt), device)
+
+
+def objinfo(obj):
+    """Bounding box of a device cloud, ((xmin, ymin, zmin), (xmax, ymax, zmax))
+    (ol::objinfo, include/olobject.h:164-175, without printing)."""
+    mn, mx = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+    check(_lib.load().olx_obj_bounds(_lib.ctx(obj.device.index), obj.ptr(), obj.size, mn, mx),
+          "objinfo")
+    return tuple(mn), tuple(mx)
+
+
+def objset(obj, x_range, y_range=None, offset=(0.0, 0.0, 0.0), mode=ZMIN, f32=False):
+    """ol::objset (include/olobject.h:195-264) in place on the device: one range
+    (the larger extent fits it) or x and y ranges (the tighter scale wins).
+    f32=True reproduces the arithmetic of an Object<float> (float coordinates,
+    scale and offsets); the default that of an Object<double>."""
+    m = mode | (_lib.OBJSET_F32 if f32 else 0)
+    check(_lib.load().olx_obj_set(_lib.ctx(obj.device.index), obj.ptr(), obj.size, x_range,
+                                  -1.0 if y_range is None else y_range, *offset, m), "objset")
+    obj._touched()
+    return obj
+
+
+def objsort(obj):
+    """ol::objsort (include/olobject.h:177-182) in place on the device: descending z,
+    equal depths in input order."""
+    check(_lib.load().olx_obj_sort_z(_lib.ctx(obj.device.index), obj.ptr(), obj.size), "objsort")
+    obj._touched()
+    return obj
+
+
+# ------------------------------------------------------------------ 8-bit encodings
+PHASE, REAL, IMAG, AMP = _lib.IMG_PHASE, _lib.IMG_REAL, _lib.IMG_IMAG, _lib.IMG_AMP
+_IMGMODE = {"phase": _lib.IMG_PHASE, "real": _lib.IMG_REAL, "imag": _lib.IMG_IMAG,
+            "amp": _lib.IMG_AMP, "real_legacy": _lib.IMG_REAL_LEGACY}
+
+
+def _img_like(t, img):
+    if img is None:
+        img = torch.empty(t.shape, dtype=torch.uint8, device=t.device)
+    if img.dtype != torch.uint8 or img.numel() != t.numel() or not img.is_contiguous():
+        raise OlxInvalid(_lib.EINVAL, "img must be a contiguous uint8 tensor of the input's size")
+    return img
+
+
+_REAL_T = {torch.float32: 0, torch.float64: _lib.IN_F64}
+_COMPLEX_T = {torch.complex64: 0, torch.complex128: _lib.IN_F64}
+
+
+def _typed(t, table, name):
+    if t.dtype not in table or not t.is_contiguous() or not t.is_cuda:
+        raise OlxInvalid(_lib.EINVAL, f"{name} must be a contiguous CUDA tensor of type "
+                         + " or ".join(str(d) for d in table))
+    return table[t.dtype]
+
+
+def phase8bit(phase, img=None, offset=False):
+    """ol::phase8bit (include/olimg.h:205-213) on a float32 / float64 device phase:
+    the reference's mapping round((phi - pi) / 2pi * 255) taken modulo 256 as its
+    x86 build converts it; offset=True maps round((phi + pi) / 2pi * 255)."""
+    f = _typed(phase, _REAL_T, "phase")
+    img = _img_like(phase, img)
+    check(_lib.load().olx_phase8bit(_lib.ctx(phase.device.index), _ptr(phase), _ptr(img),
+                                    phase.numel(), f | (_lib.PHASE8_OFFSET if offset else 0)),
+          "phase8bit")
+    return img
+
+
+def quant8bit(real, img=None):
+    """ol::quant8bit (include/olimg.h:178-199): min/max scaled to 0..255."""
+    f = _typed(real, _REAL_T, "real")
+    img = _img_like(real, img)
+    check(_lib.load().olx_quant8bit(_lib.ctx(real.device.index), _ptr(real), _ptr(img), real.numel(),
+                                    f), "quant8bit")
+    return img
+
+
+def complex2img(u, img=None, mode="phase"):
+    """ol::complex2img (include/olimg.h:314-391) of a complex64 / complex128 device
+    field; mode "phase", "real", "imag", "amp" or "real_legacy" (the bool overload)."""
+    f = _typed(u, _COMPLEX_T, "u")
+    img = _img_like(u, img)
+    check(_lib.load().olx_complex2img(_lib.ctx(u.device.index), _ptr(u), _ptr(img), u.numel(),
+                                      _IMGMODE[mode] | f), "complex2img")
+    return img
+
+
+def complex2real(u, out=None, mode="phase"):
+    """ol::complex2real (include/olimg.h:393-420) into the field's real type."""
+    f = _typed(u, _COMPLEX_T, "u")
+    rt = torch.float64 if f else torch.float32
+    if out is None:
+        out = torch.empty(u.shape, dtype=rt, device=u.device)
+    if out.dtype != rt or out.numel() != u.numel() or not out.is_contiguous():
+        raise OlxInvalid(_lib.EINVAL, f"out must be a contiguous {rt} tensor of u's size")
+    check(_lib.load().olx_complex2real(_lib.ctx(u.device.index), _ptr(u), _ptr(out), u.numel(),
+                                       _IMGMODE[mode] | f), "complex2real")
+    return out
+
+
+def Save(path, u, mode="phase"):
+    """ol::Save (include/olimg.h:430-454) for .bmp paths: a complex field through
+    complex2img, or a real array through quant8bit, then bmpwrite.  Other image
+    formats need OpenCV in the reference and are refused."""
+    if not str(path).endswith(".bmp"):
+        raise OlxInvalid(_lib.EINVAL, "Save writes .bmp files only")
+    img = complex2img(u, mode=mode) if u.is_complex() else quant8bit(u)
+    bmpwrite(path, img.reshape(u.shape[-2], u.shape[-1]))
+
+
+def bmpwrite(path, img):
+    """ol::bmpwrite (include/olimg.h:136-165): 8-bit grey BMP from a 2-D uint8 array."""
+    a = np.ascontiguousarray(img.cpu().numpy() if isinstance(img, torch.Tensor) else img, dtype=np.uint8)
+    if a.ndim != 2:
+        raise OlxInvalid(_lib.EINVAL, "bmpwrite needs a 2-D image")
+    check(_lib.load().olx_bmp_write(os.fsencode(path), a.ctypes.data, a.shape[0], a.shape[1]),
+          "bmpwrite")
+
+
+def bmpread(path, ny, nx):
+    """ol::bmpread (include/olimg.h:56-104) of a file bmpwrite made: (ny, nx) uint8."""
+    a = np.zeros((ny, nx), dtype=np.uint8)
+    check(_lib.load().olx_bmp_read(os.fsencode(path), a.ctypes.data, ny, nx), "bmpread")
+    return a

@@ -199,3 +199,184 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.complex128)
     nb = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
+
+
+# ---------------------------------------------------------------------------
+# Point-cloud ingest and 8-bit encodings: numpy restatements (byte and integer
+# work, plus scalar float arithmetic done one IEEE operation at a time).
+# ---------------------------------------------------------------------------
+import struct as _struct
+
+
+def obj_read(path, scale=1.0, offset=(0.0, 0.0, 0.0)):
+    """objread (include/olobject.h:79-91): dispatch on the extension to objread_3d
+    (:9-42, int32 N then N x int32 x, y, z) or objread_3df (:44-77, float triples),
+    read native (little-endian) order.  With scale / offset: x = x_int scale + o in
+    double (:93-126 with SCALE_T = OFFSET_T = double).  Returns an (N, 3) float64
+    array."""
+    ext = os.path.splitext(str(path))[1]
+    if ext not in (".3d", ".3df"):
+        raise ValueError("extension is neither .3d nor .3df")
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[:4], dtype="<i4")[0])
+    vals = np.frombuffer(raw[4:4 + 12 * n], dtype="<i4" if ext == ".3d" else "<f4")
+    if vals.size != 3 * n:
+        raise ValueError("short file")
+    xyz = vals.reshape(n, 3).astype(np.float64)
+    if scale != 1.0 or any(o != 0.0 for o in offset):
+        xyz = xyz * np.float64(scale) + np.asarray(offset, dtype=np.float64)[None, :]
+    return xyz
+
+
+def objset(xyz, x_range, y_range=None, offset=(0.0, 0.0, 0.0), zmiddle=False, dtype=np.float64):
+    """objset in the object's element type (dtype float32 = Object<float>, float64 =
+    Object<double>).  One range: include/olobject.h:196-228 (scale = range /
+    max(range_x, range_y)); two ranges: :232-264 (scale = min(x_range / range_x,
+    y_range / range_y)).  The ranges are float parameters.  Offsets are shifted by
+    the centre of x and y and by z's minimum (ZMIN, :215-216) or mid-range
+    (ZMIDDLE, :218-219); then every point becomes p scale + offset (:223-227)."""
+    T = dtype
+    p = np.asarray(xyz, dtype=T)
+    mn = p.min(axis=0)
+    mx = p.max(axis=0)
+    f32 = np.float32
+    rx, ry = T(mx[0] - mn[0]), T(mx[1] - mn[1])
+    if y_range is None:
+        scale = rx if rx > ry else ry
+        scale = T(T(f32(x_range)) / scale)
+    else:
+        sx = T(T(f32(x_range)) / rx)
+        sy = T(T(f32(y_range)) / ry)
+        scale = sx if sx < sy else sy
+    o = [T(v) for v in offset]
+    o[0] = T(o[0] - T(T(T(mx[0] + mn[0]) / T(2)) * scale))
+    o[1] = T(o[1] - T(T(T(mx[1] + mn[1]) / T(2)) * scale))
+    if zmiddle:
+        o[2] = T(o[2] - T(T(T(mx[2] + mn[2]) / T(2)) * scale))
+    else:
+        o[2] = T(o[2] - T(mn[2] * scale))
+    out = np.empty_like(p)
+    for c in range(3):
+        out[:, c] = (p[:, c] * scale).astype(T) + o[c]
+    return out
+
+
+def objsort_order(z):
+    """objsort (include/olobject.h:176-181): descending z.  std::sort is not stable;
+    the order returned keeps equal depths in input order (one of the orders it may
+    produce).  Returns the permutation."""
+    z = np.asarray(z, dtype=np.float64)
+    return np.argsort(-z, kind="stable")
+
+
+def c_round(t):
+    """C round(): half away from zero (numpy's round is half to even)."""
+    t = np.asarray(t, dtype=np.float64)
+    r = np.trunc(t)
+    with np.errstate(invalid="ignore"):
+        bump = np.abs(t - r) >= 0.5
+    return r + np.where(bump, np.sign(t), 0.0)
+
+
+def x86_u8(v):
+    """The reference's x86-64 conversion of a double to uint8 (cvttsd2si to int32, then
+    the low byte): modulo 256 inside the int32 range, 0 for NaN and outside it."""
+    v = np.asarray(v, dtype=np.float64)
+    out = np.zeros(v.shape, dtype=np.uint8)
+    with np.errstate(invalid="ignore"):
+        ok = (v >= -2147483648.0) & (v < 2147483648.0)
+    out[ok] = (v[ok].astype(np.int64) & 0xFF).astype(np.uint8)
+    return out
+
+
+def phase8bit(phase, offset=False):
+    """phase8bit (include/olimg.h:206-213): round((phi - pi) / (2 pi) * 255) stored
+    to uint8; offset=True uses phi + pi (no reference counterpart)."""
+    t = np.asarray(phase).astype(np.float64)
+    t = t + np.pi if offset else t - np.pi
+    return x86_u8(c_round((t / (2 * np.pi)) * 255.0))
+
+
+def _ref_minmax(v, max_start=None):
+    """The reference's running min / max seeded with element 0, so a NaN there sticks
+    (every output is then 0) and later NaNs are skipped.  max_start replaces the max
+    seed (complex2img's bool overload starts it at 0, include/olimg.h:331)."""
+    v = np.asarray(v, dtype=np.float64).ravel()
+    if np.isnan(v[0]):
+        return np.nan, np.nan
+    mn, mx = np.nanmin(v), np.nanmax(v)
+    if max_start is not None:
+        mx = max(mx, max_start)
+    return mn, mx
+
+
+def quant8bit(real):
+    """quant8bit (include/olimg.h:178-198): round((v - min) / (max - min) * 255)."""
+    v = np.asarray(real).astype(np.float64)
+    mn, mx = _ref_minmax(v)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return x86_u8(c_round(((v - mn) / (mx - mn)) * 255.0))
+
+
+IMG_PHASE, IMG_REAL, IMG_IMAG, IMG_AMP, IMG_REAL_LEGACY = 0, 1, 2, 3, 4
+
+
+def complex_value(u, mode):
+    """complex2img / complex2real per-element value (include/olimg.h:355-366,
+    399-410): real, imag, sqrt(re re + im im) with the products and sum in the
+    element type and the root in double (unqualified sqrt / atan2 on float arguments
+    resolve to the double functions under the reference's includes), atan2(im, re)."""
+    u = np.asarray(u)
+    if mode in (IMG_REAL, IMG_REAL_LEGACY):
+        return u.real.astype(np.float64)
+    if mode == IMG_IMAG:
+        return u.imag.astype(np.float64)
+    if mode == IMG_AMP:
+        re, im = u.real, u.imag
+        return np.sqrt((re * re + im * im).astype(np.float64))
+    return np.arctan2(u.imag.astype(np.float64), u.real.astype(np.float64))
+
+
+def complex2img(u, mode):
+    """complex2img: MODE overload (include/olimg.h:348-391; PHASE range fixed to
+    [-pi, pi], others min / max seeded with element 0) or the bool overload's real
+    mode (IMG_REAL_LEGACY, :314-346: max seeded with 0).  img = round(255 (v - min) /
+    (max - min))."""
+    v = complex_value(u, mode)
+    if mode == IMG_PHASE:
+        mn, mx = -np.pi, np.pi
+    else:
+        mn, mx = _ref_minmax(v, 0.0 if mode == IMG_REAL_LEGACY else None)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return x86_u8(c_round((255.0 * (v - mn)) / (mx - mn)))
+
+
+def complex2real(u, mode):
+    """complex2real (include/olimg.h:393-413) in the field's real type."""
+    u = np.asarray(u)
+    rt = np.float32 if u.dtype == np.complex64 else np.float64
+    return complex_value(u, mode).astype(rt)
+
+
+def bmp_bytes(img):
+    """bmpwrite (include/olimg.h:136-165): 14-byte file header (type 19778, size
+    1078 + nx ny, offset 1078), 40-byte info header (nx, ny, 1 plane, 8 bits), a grey
+    256-entry palette, then each row followed by nx % 4 zero bytes."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    ny, nx = img.shape
+    out = bytearray()
+    out += _struct.pack("<HIHHI", 19778, (14 + 40 + 1024 + nx * ny) & 0xFFFFFFFF, 0, 0, 14 + 40 + 1024)
+    out += _struct.pack("<IiiHHIIiiII", 40, nx, ny, 1, 8, 0, 0, 0, 0, 0, 0)
+    out += bytes(b for i in range(256) for b in (i, i, i, 0))
+    pad = bytes(nx % 4)
+    for r in range(ny):
+        out += img[r].tobytes() + pad
+    return bytes(out)
+
+
+def bmp_pixels(data, ny, nx):
+    """bmpread (include/olimg.h:56-99): skip 14 + 40 + 1024 bytes, then ny rows of nx
+    bytes, each followed by nx % 4 skipped bytes."""
+    off, pad = 14 + 40 + 1024, nx % 4
+    rows = [np.frombuffer(data, dtype=np.uint8, count=nx, offset=off + r * (nx + pad)) for r in range(ny)]
+    return np.stack(rows)

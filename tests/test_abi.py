@@ -73,7 +73,8 @@ def test_dropin_headers_compile():
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("no g++")
-    src = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
-    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
-                        "-I" + os.path.join(ROOT, "include"), src], capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr
+    for prog in ("dropin_test.cpp", "ingest_test.cpp"):
+        src = os.path.join(ROOT, "tests", "cpp", prog)
+        r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                            "-I" + os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
