@@ -12,8 +12,9 @@ JSON line.  `value` is propagations per second.  The line also carries:
   stream the kernels run on, inside the timed region), against 8 TB/s;
 * cpu_baseline: the oracle's CPU restatement of the reference AsmProp (own
   radix-2 FFT with OpenMP; FFTW is not installed) timed on this host;
-* extras (--extras, on by default): direct CGH, layered MFMA CGH, WRP + ARSS
-  and the multi-plane optimisation step, each with its own throughput.
+* extras (on by default): direct CGH, layered MFMA CGH, WRP + ARSS, the
+  multi-plane optimisation step, point-cloud objset / objsort and the 8-bit
+  encodings, each with its own throughput.
 """
 import argparse
 import json
@@ -42,6 +43,8 @@ def parse():
     ap.add_argument("--n", type=int, default=4096, help="field side (metric: 4096)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the ASM metric")
+    ap.add_argument("--only", choices=["ingest"], default=None,
+                    help="run one secondary workload alone and print its figures (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (for PMC byte-counter calibration)")
     return ap.parse_args()
@@ -131,9 +134,23 @@ def cpu_baseline(n):
         olref.asm_prop(u, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
         ts.append(time.perf_counter() - t0)
     med = float(np.median(ts))
+    # the ingest / encoding rows on one core (numpy restatements): objsort of the
+    # 1,000,000-point cloud and complex2img (AMP) of a 4096^2 field
+    pts = synth.random_points(1_000_000, seed=4, xmax=1.0, ymax=1.0, zmin=-1.0, zmax=1.0)
+    t0 = time.perf_counter()
+    olref.objsort_order(pts["z"])
+    t_sort = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    u4 = (rng.standard_normal((4096, 4096)) + 1j * rng.standard_normal((4096, 4096))).astype(np.complex64)
+    t0 = time.perf_counter()
+    olref.complex2img(u4, olref.IMG_AMP)
+    t_img = time.perf_counter() - t0
     return {"value": round(1.0 / med, 4), "unit": "props/s", "cores": cores, "kind": "port",
             "sample": f"{n}x{n} AsmProp (band-limited, pad {2 * n}), 1 warm-up + median of 3 "
-                      f"= {med:.2f} s/prop; own OpenMP radix-2 FFT (FFTW unavailable - substitute FFT)"}
+                      f"= {med:.2f} s/prop; own OpenMP radix-2 FFT (FFTW unavailable - substitute FFT)",
+            "extras": {"objsort_1M_ms": round(t_sort * 1e3, 1),
+                       "complex2img_amp_4096_ms": round(t_img * 1e3, 1),
+                       "cores": 1, "kind": "port (numpy)"}}
 
 
 def ceilings(lib, torch):
@@ -232,10 +249,60 @@ def extras(ol, lib, torch):
     del W
     out.update(layered(ol, torch))
     out.update(multiplane(torch))
+    out.update(ingest_encode(ol, torch))
     arss_s = (t2 - t1) / reps
     out["arss_4096_ms"] = round(arss_s * 1e3, 3)
     out["arss_4096_GBps_80N2"] = round(80 * N * N / arss_s / 1e9, 1)
     out["wrp_arss_4096_end_to_end_ms"] = round(out["wrp_200k_4096_ms"] + arss_s * 1e3, 3)
+    return out
+
+
+def _event_ms(torch, fn, reps):
+    """Average device time of fn() over reps calls (HIP events on torch's current
+    stream, which the library context is bound to), after one warm-up call."""
+    fn()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def ingest_encode(ol, torch):
+    """Point-cloud ingest and 8-bit encodings at full size, each against its HBM
+    roofline (algorithmic bytes / time vs 8 TB/s): objset and objsort of the
+    1,000,000-point layered cloud, complex2img (AMP: two reads of the field; PHASE:
+    one) and phase8bit of a 4096^2 hologram."""
+    from openol_amd import synth
+    out = {}
+    n = 1_000_000
+    pts = synth.random_points(n, seed=4, xmax=1.0, ymax=1.0, zmin=-1.0, zmax=1.0)
+    obj = ol.Object(pts["x"], pts["y"], pts["z"])
+    # bounds (read 32 B / point) then the affine map (read + write 32 B / point)
+    ms = _event_ms(torch, lambda: ol.objset(obj, 1e-3, 1e-3, (0.0, 0.0, 0.1), ol.ZMIDDLE), 20)
+    out["objset_1M_ms"] = round(ms, 4)
+    out["objset_1M_GBps"] = round(96 * n / ms / 1e6, 1)
+    ms = _event_ms(torch, lambda: ol.objsort(obj), 10)
+    out["objsort_1M_ms"] = round(ms, 3)
+    out["objsort_1M_Mpts_s"] = round(n / ms / 1e3, 1)
+    del obj
+    N = 4096
+    rng = np.random.default_rng(7)
+    u = torch.from_numpy((rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+                         .astype(np.complex64)).cuda()
+    img = torch.empty((N, N), dtype=torch.uint8, device="cuda")
+    for mode, nbytes in (("amp", 2 * 8 * N * N + N * N), ("phase", 8 * N * N + N * N)):
+        ms = _event_ms(torch, lambda: ol.complex2img(u, img, mode=mode), 20)
+        out[f"complex2img_{mode}_4096_ms"] = round(ms, 4)
+        out[f"complex2img_{mode}_4096_GBps"] = round(nbytes / ms / 1e6, 1)
+        out[f"complex2img_{mode}_4096_frac"] = round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)
+    ph = torch.angle(u).contiguous()
+    ms = _event_ms(torch, lambda: ol.phase8bit(ph, img), 20)
+    out["phase8bit_4096_ms"] = round(ms, 4)
+    out["phase8bit_4096_GBps"] = round(5 * N * N / ms / 1e6, 1)
     return out
 
 
@@ -317,6 +384,9 @@ def main():
     import openol_amd as ol
     from openol_amd import _lib as lib
 
+    if args.only == "ingest":
+        print(json.dumps({"ingest_encode": ingest_encode(ol, torch)}), flush=True)
+        return
     n = args.n
     dt, rep = bench_asm(ol, lib, torch, n, args.steps, args.warmup)
     ms = dt / args.steps * 1e3

@@ -21,31 +21,50 @@ __device__ inline uint8_t to_u8(double v) {
     return (uint8_t)(uint32_t)(int32_t)v;
 }
 
+// Fast path for an encoding q = f(x) whose C round() is wanted: q_fast is an FP32
+// estimate within err of the FP64 value the reference computes.  When q_fast lies
+// farther than err from every half-integer, rounding it gives the same integer as
+// rounding the FP64 value; otherwise (and for NaN / huge values) the caller takes
+// the FP64 path.
+__device__ inline bool fast_round(float q_fast, float err, int* k) {
+    if (!(fabsf(q_fast) < 65536.f)) return false;
+    const float f = q_fast - floorf(q_fast);
+    if (!(fabsf(f - 0.5f) > err)) return false;
+    *k = (int)rintf(q_fast);
+    return true;
+}
+
 template <typename T>
 __global__ void k_phase8(const T* __restrict__ ph, uint8_t* __restrict__ img, int64_t n,
                          int offset) {
     const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (j >= n) return;
-    const double p = (double)ph[j];
+    const T x = ph[j];
+    int k;
+    if (sizeof(T) == 4) {
+        // |q| < 65536: FP32 error of the three operations below plus the rounding of
+        // pi and 255 / 2pi stays under 2^-12 + 4e-7 |q|
+        const float pf = (float)x;
+        const float qf = (offset ? pf + 3.14159265f : pf - 3.14159265f) * 40.5845105f;
+        if (fast_round(qf, 2.5e-4f + 4e-7f * fabsf(qf), &k)) {
+            img[j] = (uint8_t)(uint32_t)k;
+            return;
+        }
+    }
+    const double p = (double)x;
     const double t = offset ? __dadd_rn(p, M_PI) : __dsub_rn(p, M_PI);
     img[j] = to_u8(round(__dmul_rn(__ddiv_rn(t, 2 * M_PI), 255.0)));
 }
 
 // ------------------------------------------------------------------ extremes
-// Grid-stride min / max of v(j) as doubles: per-block partials part[2b], part[2b+1],
-// then one thread folds them into part[0..1].  fmin / fmax skip NaN, as the
+// min / max of v(j) as doubles: per-block partials part[2b], part[2b+1] from a
+// grid-stride loop, then one work-group folds them.  fmin / fmax skip NaN, as the
 // reference's > / < comparisons do for every element after the first.
-constexpr int QT = 256, QB = 512;
+constexpr int QT = 256, QB = 2048;
 
-template <typename F>
-__device__ inline void block_minmax(F v, int64_t n, double* __restrict__ part) {
+// LDS fold of one (mn, mx) pair per thread; every thread returns the block's pair
+__device__ inline void block_fold(double& mn, double& mx) {
     __shared__ double smin[QT], smax[QT];
-    double mn = INFINITY, mx = -INFINITY;
-    for (int64_t j = blockIdx.x * (int64_t)QT + threadIdx.x; j < n; j += (int64_t)gridDim.x * QT) {
-        const double x = v(j);
-        mn = fmin(mn, x);
-        mx = fmax(mx, x);
-    }
     smin[threadIdx.x] = mn;
     smax[threadIdx.x] = mx;
     __syncthreads();
@@ -56,10 +75,34 @@ __device__ inline void block_minmax(F v, int64_t n, double* __restrict__ part) {
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = smin[0];
-        part[2 * blockIdx.x + 1] = smax[0];
+    mn = smin[0];
+    mx = smax[0];
+}
+
+template <typename F>
+__device__ inline void block_minmax(F v, int64_t n, double* __restrict__ part) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (int64_t j = blockIdx.x * (int64_t)QT + threadIdx.x; j < n; j += (int64_t)gridDim.x * QT) {
+        const double x = v(j);
+        mn = fmin(mn, x);
+        mx = fmax(mx, x);
     }
+    block_fold(mn, mx);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = mn;
+        part[2 * blockIdx.x + 1] = mx;
+    }
+}
+
+// the partials of nb blocks folded by one work-group of QT threads
+__device__ inline void fold_partials(const double* __restrict__ part, int nb, double& mn, double& mx) {
+    mn = INFINITY;
+    mx = -INFINITY;
+    for (int b = threadIdx.x; b < nb; b += QT) {
+        mn = fmin(mn, part[2 * b]);
+        mx = fmax(mx, part[2 * b + 1]);
+    }
+    block_fold(mn, mx);
 }
 
 template <typename T>
@@ -68,16 +111,15 @@ __global__ void __launch_bounds__(QT) k_real_minmax(const T* __restrict__ r, int
     block_minmax([=](int64_t j) { return (double)r[j]; }, n, part);
 }
 
-// folds the partials.  The reference seeds min and max with element 0, so a NaN
-// there sticks (and every output is then 0); a NaN elsewhere is skipped.
+// folds the partials into part[0..1] (written after every thread has read them).
+// The reference seeds min and max with element 0, so a NaN there sticks (and every
+// output is then 0); a NaN elsewhere is skipped.
 template <typename T>
-__global__ void k_minmax_final(double* __restrict__ part, int nb, const T* __restrict__ r) {
+__global__ void __launch_bounds__(QT) k_minmax_final(double* __restrict__ part, int nb,
+                                                     const T* __restrict__ r) {
+    double mn, mx;
+    fold_partials(part, nb, mn, mx);
     if (threadIdx.x != 0) return;
-    double mn = part[0], mx = part[1];
-    for (int b = 1; b < nb; b++) {
-        mn = fmin(mn, part[2 * b]);
-        mx = fmax(mx, part[2 * b + 1]);
-    }
     if (r[0] != r[0]) mn = mx = (double)r[0];
     part[0] = mn;
     part[1] = mx;
@@ -121,23 +163,32 @@ __device__ inline double c2v(V u, int mode) {
     }
 }
 
+// the value whose extremes give complex2img's bounds: for AMP the squared modulus
+// (the correctly rounded square root is monotone, so the root of the extreme is the
+// extreme of the roots, and one root per bound replaces one per element)
+template <typename V>
+__device__ inline double c2v_order(V u, int mode) {
+    return mode == OLX_IMG_AMP ? (double)sq_sum(u) : c2v(u, mode);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(QT) k_c2_minmax(const typename C2<T>::V* __restrict__ u, int64_t n,
                                                   int mode, double* __restrict__ part) {
-    block_minmax([=](int64_t j) { return c2v(u[j], mode); }, n, part);
+    block_minmax([=](int64_t j) { return c2v_order(u[j], mode); }, n, part);
 }
 
-// complex2img's bounds per mode into mm[0..1]: PHASE [-pi, pi]; the legacy real
-// mode's maximum starts at 0; the others are the extremes.  A NaN value at element 0
-// seeds the reference's minimum and sticks there.
+// complex2img's bounds into mm[0..1] (non-PHASE modes): the extremes (square roots
+// taken for AMP); the legacy real mode's maximum starts at 0.  A NaN value at
+// element 0 seeds the reference's minimum and sticks there.
 template <typename T>
-__global__ void k_img_bounds(double* __restrict__ mm, int nb, int mode,
-                             const typename C2<T>::V* __restrict__ u) {
+__global__ void __launch_bounds__(QT) k_img_bounds(double* __restrict__ mm, int nb, int mode,
+                                                   const typename C2<T>::V* __restrict__ u) {
+    double mn, mx;
+    fold_partials(mm, nb, mn, mx);
     if (threadIdx.x != 0) return;
-    double mn = mm[0], mx = mm[1];
-    for (int b = 1; b < nb; b++) {
-        mn = fmin(mn, mm[2 * b]);
-        mx = fmax(mx, mm[2 * b + 1]);
+    if (mode == OLX_IMG_AMP) {
+        mn = sqrt(mn);
+        mx = sqrt(mx);
     }
     if (mode == OLX_IMG_REAL_LEGACY) mx = fmax(mx, 0.0);
     const double v0 = c2v(u[0], mode);
@@ -146,14 +197,25 @@ __global__ void k_img_bounds(double* __restrict__ mm, int nb, int mode,
     mm[1] = mx;
 }
 
-// img = round(255 (v - min) / (max - min)); PHASE takes its fixed bounds in registers
+// img = round(255 (v - min) / (max - min)); PHASE takes its fixed bounds [-pi, pi]
+// in registers and, for float fields, an FP32 atan2 fast path (its error, under
+// 1e-6 rad, moves q = 255 (v + pi) / 2pi by under 1e-4)
 template <typename T, bool PHASE>
 __global__ void k_c2img(const typename C2<T>::V* __restrict__ u, uint8_t* __restrict__ img,
                         int64_t n, int mode, const double* __restrict__ mm) {
     const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (j >= n) return;
+    const typename C2<T>::V z = u[j];
+    if (PHASE && sizeof(T) == 4) {
+        const float qf = (atan2f((float)z.y, (float)z.x) + 3.14159265f) * 40.5845105f;
+        int k;
+        if (fast_round(qf, 1e-3f, &k)) {
+            img[j] = (uint8_t)(uint32_t)k;
+            return;
+        }
+    }
     const double mn = PHASE ? -M_PI : mm[0], mx = PHASE ? M_PI : mm[1];
-    const double v = PHASE ? c2v(u[j], OLX_IMG_PHASE) : c2v(u[j], mode);
+    const double v = PHASE ? c2v(z, OLX_IMG_PHASE) : c2v(z, mode);
     img[j] = to_u8(round(__ddiv_rn(__dmul_rn(255.0, __dsub_rn(v, mn)), __dsub_rn(mx, mn))));
 }
 
@@ -172,7 +234,7 @@ static int quant8(olx_ctx* ctx, const T* d_real, uint8_t* d_img, int64_t n) {
     double* part = (double*)ctx->obj_tmp.p;
     hipStream_t s = ctx->stream;
     k_real_minmax<T><<<nb, QT, 0, s>>>(d_real, n, part);
-    k_minmax_final<T><<<1, 64, 0, s>>>(part, nb, d_real);
+    k_minmax_final<T><<<1, QT, 0, s>>>(part, nb, d_real);
     k_quant8<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d_real, d_img, n, part);
     return check_launch("quant8bit");
 }
@@ -191,7 +253,7 @@ static int c2img(olx_ctx* ctx, const void* d_u, uint8_t* d_img, int64_t n, int m
     OLX_TRY(ctx->obj_tmp.reserve(sizeof(double) * 2 * (size_t)nb));
     double* part = (double*)ctx->obj_tmp.p;
     k_c2_minmax<T><<<nb, QT, 0, s>>>(u, n, mode, part);
-    k_img_bounds<T><<<1, 64, 0, s>>>(part, nb, mode, u);
+    k_img_bounds<T><<<1, QT, 0, s>>>(part, nb, mode, u);
     k_c2img<T, false><<<g, 256, 0, s>>>(u, d_img, n, mode, part);
     return check_launch("complex2img");
 }

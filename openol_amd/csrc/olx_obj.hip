@@ -58,7 +58,7 @@ static int read_cloud(const char* path, int format, olx_point* pts, int64_t cap,
 }
 
 // ------------------------------------------------------------------ bounds
-constexpr int RED_T = 256, RED_B = 512;
+constexpr int RED_T = 256, RED_B = 1024;
 
 // per-block minima and maxima of x, y, z: part[b * 6 + {0,1,2}] = min, {3,4,5} = max
 __global__ void __launch_bounds__(RED_T) k_obj_bounds(const olx_point* __restrict__ p, int64_t n,
@@ -92,12 +92,32 @@ __global__ void __launch_bounds__(RED_T) k_obj_bounds(const olx_point* __restric
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void k_obj_bounds_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
-    const int c = threadIdx.x;  // 6 threads
-    if (c >= 6) return;
-    double r = part[c];
-    for (int b = 1; b < nb; b++) r = c < 3 ? fmin(r, part[b * 6 + c]) : fmax(r, part[b * 6 + c]);
-    out[c] = r;
+// folds the nb per-block partials with one work-group of RED_T threads
+__global__ void __launch_bounds__(RED_T) k_obj_bounds_final(const double* __restrict__ part, int nb,
+                                                            double* __restrict__ out) {
+    __shared__ double s[6][RED_T];
+    double r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nb; b += RED_T) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            r[c] = fmin(r[c], part[b * 6 + c]);
+            r[3 + c] = fmax(r[3 + c], part[b * 6 + 3 + c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) s[c][threadIdx.x] = r[c];
+    __syncthreads();
+    for (int w = RED_T / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                s[c][threadIdx.x] = fmin(s[c][threadIdx.x], s[c][threadIdx.x + w]);
+                s[3 + c][threadIdx.x] = fmax(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 int device_bounds(olx_ctx* ctx, const olx_point* d, int64_t n, double* mn, double* mx) {
@@ -106,7 +126,7 @@ int device_bounds(olx_ctx* ctx, const olx_point* d, int64_t n, double* mn, doubl
     double* part = (double*)ctx->obj_tmp.p;
     double* out = part + 6 * nb;
     k_obj_bounds<<<nb, RED_T, 0, ctx->stream>>>(d, n, part);
-    k_obj_bounds_final<<<1, 64, 0, ctx->stream>>>(part, nb, out);
+    k_obj_bounds_final<<<1, RED_T, 0, ctx->stream>>>(part, nb, out);
     OLX_TRY(check_launch("obj_bounds"));
     double h[6];
     OLX_HIP(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, ctx->stream));

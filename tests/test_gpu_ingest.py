@@ -132,6 +132,13 @@ def test_phase8bit_bit_exact(dtype, offset):
                        np.pi - np.pi / 255, 100.0]
     img = ol.phase8bit(gpu(ph), offset=offset).cpu().numpy()
     assert np.array_equal(img, olref.phase8bit(ph, offset=offset))
+    # phases whose code lies at (or one ulp from) a rounding tie k + 1/2: the FP32
+    # fast path must hand these to the FP64 path
+    k = np.arange(-300, 300) + 0.5
+    ties = (k / 255 * 2 * np.pi + (-np.pi if offset else np.pi)).astype(dtype)
+    near = np.concatenate([ties, np.nextafter(ties, dtype(np.inf)), np.nextafter(ties, dtype(-np.inf))])
+    img = ol.phase8bit(gpu(near), offset=offset).cpu().numpy()
+    assert np.array_equal(img, olref.phase8bit(near, offset=offset))
 
 
 @pytest.mark.gpu
@@ -159,6 +166,24 @@ def test_complex2img_bit_exact(dtype, mode):
     u.ravel()[:6] = [-1 + 0j, 1, complex(-1, -0.0), 1j, 0, 3 + 4j]
     got = ol.complex2img(gpu(u), mode=mode).cpu().numpy()
     assert np.array_equal(got, olref.complex2img(u, MODE_ID[mode]))
+    # angles at the PHASE code's rounding ties (k + 1/2) / 255 * 2pi - pi, and unit
+    # vectors one ulp off them
+    th = (np.arange(256) + 0.5) / 255 * 2 * np.pi - np.pi
+    rt = np.float32 if dtype == np.complex64 else np.float64
+    c, sn = np.cos(th).astype(rt), np.sin(th).astype(rt)
+    ties = np.concatenate([c + 1j * sn, np.nextafter(c, rt(2)) + 1j * sn, c + 1j * np.nextafter(sn, rt(2))])
+    ties = ties.astype(dtype)
+    got = ol.complex2img(gpu(ties), mode=mode).cpu().numpy()
+    ref = olref.complex2img(ties, MODE_ID[mode])
+    if mode == "phase" and dtype == np.complex128:
+        # a double angle within a few ulps of a tie: the device's atan2 and the host
+        # libm's (neither correctly rounded) may round the code either way there
+        q = 255 * (np.arctan2(ties.imag, ties.real) + np.pi) / (2 * np.pi)
+        at_tie = np.abs(q - np.floor(q) - 0.5) < 1e-12
+        assert np.array_equal(got[~at_tie], ref[~at_tie])
+        assert np.all(np.abs(got[at_tie].astype(int) - ref[at_tie].astype(int)) <= 1)
+    else:
+        assert np.array_equal(got, ref)
     # all-negative real parts: the legacy mode's maximum stays at 0
     f = field((33, 17), 4, dtype)
     neg = (-np.abs(f.real) + 1j * f.imag).astype(dtype)
