@@ -15,6 +15,7 @@
 #include <complex>
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
 #include <type_traits>
 #include <vector>
 
@@ -104,6 +105,52 @@ void shiftedFresnelProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t h
     detail::on_device(u, height * width, [&](void* d) {
         detail::check(olx_arss_c64(detail::ctx(), d, d, height, width, dy, dx, lambda, z, s, oy,
                                    ox, OLX_ARSS_CPU), "shiftedFresnelProp");
+    });
+}
+
+// include/olprop.h:307-330: shifted (off-axis) ASM, usrc -> udst (allocated when
+// empty, may be usrc), with the shifted band-limit window of :250-305 and the
+// reference's net scale 1/M^2
+template <typename PREC_T = double, typename COMPLEX_T>
+void shiftedAsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& usrc,
+                    std::unique_ptr<std::complex<COMPLEX_T>[]>& udst, int64_t height,
+                    int64_t width, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z, PREC_T oy,
+                    PREC_T ox) {
+    if (&usrc != &udst) {
+        if (!udst) udst = std::make_unique<std::complex<COMPLEX_T>[]>((size_t)(height * width));
+        for (int64_t i = 0; i < height * width; i++) udst[i] = usrc[i];
+    }
+    detail::on_device(udst, height * width, [&](void* d) {
+        detail::check(olx_shifted_asm_c64(detail::ctx(), d, d, height, width, dy, dx, lambda, z,
+                                          oy, ox, OLX_SHIFTED_CPU), "shiftedAsmProp");
+    });
+}
+
+// include/olprop.h:614-651: blocknum_y x blocknum_x blocks, every block propagated to
+// every block with shiftedAsmProp and summed, in place
+template <typename PREC_T = float, typename COMPLEX_T>
+void splitAsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int64_t width,
+                  PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z, int blocknum_y = 2,
+                  int blocknum_x = 2) {
+    detail::on_device(u, height * width, [&](void* d) {
+        detail::check(olx_split_prop_c64(detail::ctx(), d, height, width, dy, dx, lambda, z,
+                                         OLX_MODE_ASM, blocknum_y, blocknum_x, OLX_SHIFTED_CPU),
+                      "splitAsmProp");
+    });
+}
+
+// include/golprop.h:773-813: the GPU split propagation of a host field (blocks through
+// gshiftedAsmProp, or ARSS with s = 1 for FRESNEL), in place
+template <typename PREC_T = float, typename COMPLEX_T>
+void gsplitProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int64_t width,
+                PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z, PROPMODE propmode,
+                int blocknum_y = 2, int blocknum_x = 2) {
+    if (propmode != ASM && propmode != FRESNEL)
+        throw std::runtime_error("gsplitProp: propmode must be ASM or FRESNEL");
+    detail::on_device(u, height * width, [&](void* d) {
+        detail::check(olx_split_prop_c64(detail::ctx(), d, height, width, dy, dx, lambda, z,
+                                         propmode == ASM ? OLX_MODE_ASM : OLX_MODE_FRESNEL,
+                                         blocknum_y, blocknum_x, 0), "gsplitProp");
     });
 }
 
@@ -210,6 +257,25 @@ template <typename PREC_T = float>
 void ARSSFresnelProp(dfield& usrc, dfield& udst, int ny, int nx, PREC_T dy, PREC_T dx,
                      PREC_T lambda, PREC_T z, PREC_T s) {
     ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lambda, z, s, (PREC_T)0, (PREC_T)0);
+}
+
+// include/golprop.h:536-576: shifted ASM without a window, net scale 1; udst is
+// allocated when empty and may be usrc
+template <typename PREC_T>
+void gshiftedAsmProp(dfield& usrc, dfield& udst, int ny, int nx, PREC_T dy, PREC_T dx,
+                     PREC_T lambda, PREC_T d, PREC_T oy, PREC_T ox) {
+    if (!udst) udst = make_device<std::complex<float>[]>((size_t)ny * nx);
+    detail::check(olx_shifted_asm_c64(detail::ctx(), usrc.get(), udst.get(), ny, nx, dy, dx,
+                                      lambda, d, oy, ox, 0), "gshiftedAsmProp");
+}
+// include/golprop.h:756-770: gshiftedAsmProp (ASM) or ARSS with s = 1 (FRESNEL)
+template <typename PREC_T>
+void shiftedProp(dfield& usrc, dfield& udst, int ny, int nx, PREC_T dy, PREC_T dx,
+                 PREC_T lambda, PREC_T d, PREC_T oy, PREC_T ox, PROPMODE propmode) {
+    if (propmode == ASM) gshiftedAsmProp(usrc, udst, ny, nx, dy, dx, lambda, d, oy, ox);
+    else if (propmode == FRESNEL)
+        ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lambda, d, (PREC_T)1, oy, ox);
+    else throw std::runtime_error("shiftedProp: propmode must be ASM or FRESNEL");
 }
 
 // the adjoint of AsmProp (conjugate transfer function), for gradient-based design

@@ -40,6 +40,8 @@ extern "C" {
 #define OLX_FRESNEL_BL_GPU 0x20u /* Fresnel window, x-limit on both axes (include/golprop.h:223-238) */
 #define OLX_ARSS_CPU 0x800u      /* ARSS with the CPU shiftedFresnelProp semantics: no windows and
                                     net scale circconv / M^4 (include/olprop.h:338-408) */
+#define OLX_SHIFTED_CPU 0x1000u  /* shifted ASM with the CPU shiftedAsmProp semantics: the shifted
+                                    band-limit window and net scale 1/M^2 (include/olprop.h:250-330) */
 
 /* CGH flags */
 #define OLX_CGH_EXACT_R 0x100u    /* exact spherical distance r instead of the Fresnel quadratic */
@@ -99,6 +101,25 @@ int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, 
 int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx,
                  double dy, double dx, double lambda, double z, double s, double oy,
                  double ox, uint32_t flags);
+/* shifted (off-axis) ASM, src -> dst (may alias): H = exp(2 pi i (w z + ox u + oy v))
+ * maps an ny x nx source window to the destination window offset by (oy, ox) (in
+ * length units, source minus destination).  No window and net scale 1 is
+ * gshiftedAsmProp (include/golprop.h:536-576, its window is commented out);
+ * OLX_BANDLIMIT adds the shifted band-limit window of shiftedAsmTransferF
+ * (include/olprop.h:250-305); OLX_SHIFTED_CPU is ol::shiftedAsmProp (include/
+ * olprop.h:307-330: window and net scale 1/M^2).  OLX_ADJOINT conjugates H. */
+int olx_shifted_asm_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx,
+                        double dy, double dx, double lambda, double z, double oy, double ox,
+                        uint32_t flags);
+/* split propagation of an ny x nx field in place as by x bx blocks: every block is
+ * propagated to every block with the shift of their offsets (in float, as the
+ * reference computes them) and the contributions are summed.  mode OLX_MODE_ASM:
+ * gshiftedAsmProp per pair (gsplitProp, include/golprop.h:773-813), or with
+ * OLX_SHIFTED_CPU the CPU shiftedAsmProp (splitAsmProp, include/olprop.h:614-651);
+ * OLX_MODE_FRESNEL: ARSS with s = 1 per pair (gsplitProp's FRESNEL branch through
+ * shiftedProp, include/golprop.h:756-770).  Blocks must be powers of two in [4, 8192]. */
+int olx_split_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
+                       double lambda, double z, int mode, int by, int bx, uint32_t flags);
 /* Prop dispatcher with AUTO selection; *chosen receives OLX_MODE_*.  Replaces
  * ol::Prop (include/olprop.h:656-682, include/golprop.h:578-604). */
 int olx_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,

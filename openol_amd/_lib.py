@@ -21,6 +21,7 @@ CGH_EXACT_R = 0x100
 CGH_ACCUMULATE = 0x400
 WRP_SQUARE = 0x200
 ARSS_CPU = 0x800
+SHIFTED_CPU = 0x1000
 
 MODE_ASM, MODE_FRESNEL, MODE_AUTO, MODE_FRESNEL_BL = 0, 1, 2, 3
 LAYERED_F16, LAYERED_BF16, LAYERED_BF16X3 = 0, 1, 2
@@ -59,6 +60,8 @@ SIGNATURES = {
     "olx_asm_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_fresnel_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_arss_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _u32],
+    "olx_shifted_asm_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _u32],
+    "olx_split_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, _i, _i, _u32],
     "olx_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "olx_asm_prop_check": [_i64, _i64, _d, _d, _d, _d],
     "olx_fresnel_prop_check": [_i64, _i64, _d, _d, _d, _d],

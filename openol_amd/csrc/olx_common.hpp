@@ -67,6 +67,7 @@ struct olx_ctx {
     olx::DevBuf lay_pts, lay_counts;
     // point-cloud sort and reduction scratch
     olx::DevBuf obj_keys, obj_tmp;
+    olx::DevBuf split_acc;  // split propagation: the summed destination field
     // timing
     bool timing = false;
     std::map<std::string, olx::KernelStat> stats;

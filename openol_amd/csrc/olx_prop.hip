@@ -22,15 +22,18 @@
 
 namespace olx {
 
-enum { K_ASM = 0, K_TABLE = 1 };
+enum { K_ASM = 0, K_TABLE = 1, K_ASHIFT = 2 };
 
 struct PropArgs {
     int ny, nx;  // unpadded field
-    int mode;    // K_ASM / K_TABLE
+    int mode;    // K_ASM / K_TABLE / K_ASHIFT
     int adjoint;
     int bandlimit;
     double du, dv, inv_l2, zs, uw, vw;  // ASM transfer function (zs: z, -z for the
                                         // adjoint) and band limit
+    double u0, v0, ox, oy;  // K_ASHIFT: window centres and the shift's linear phase
+    int ss, ds;             // row strides (elements) of the source and destination
+    int accum;              // pass C adds into the destination instead of storing
     const float2* sx;                  // K_TABLE: natural-order spectra, length Mx / My
     const float2* sy;
     const float2* ax;  // pre-chirp tables (length nx / ny), zero outside the window
@@ -90,7 +93,7 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
             float2 x = make_float2(0.f, 0.f);
             if (first_nonzero<P>(b, r) && valid) {
                 const int col = first_index<P>(t, b, r) - L / 4;
-                x = src[(size_t)row * nx + col];
+                x = src[(size_t)row * a.ss + col];
                 if (a.ax) x = cmul(x, cmul(a.ax[col], ay));
             }
             v[b * R0 + r] = x;
@@ -113,16 +116,24 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
 // cut and evanescent ones (a select of these values is turned into a branch by
 // the compiler); the argument of the root is |w2| + 1e-30, so every lane's
 // value is finite and the product is exact zero where masked.
-__device__ __forceinline__ float2 asm_h(const PropArgs& a, double A, bool ucut, int i, int My) {
+// The shifted form (SHIFT, shiftedAsmTransferF include/olprop.h:250-305) adds the
+// linear phase ox u + oy v of a destination window offset by (ox, oy) (pc = ox u
+// of the column, sign-flipped with z for the adjoint) and centres the band-limit
+// window of each axis on u0 / v0.
+template <bool SHIFT = false>
+__device__ __forceinline__ float2 asm_h(const PropArgs& a, double A, bool ucut, int i, int My,
+                                        double pc = 0.0) {
     const int fy = i < My / 2 ? i : i - My;
     const double v = (double)fy * a.dv;
     const double w2 = A - v * v;
-    const bool keep = (!ucut) & ((!a.bandlimit) | (fabs(v) * 2 < a.vw)) & (w2 >= 0.0);
+    const double vc = SHIFT ? v - a.v0 : v;
+    const bool keep = (!ucut) & ((!a.bandlimit) | (fabs(vc) * 2 < a.vw)) & (w2 >= 0.0);
     const double w2c = fabs(w2) + 1e-30;
     const double y0 = (double)__builtin_amdgcn_rsqf((float)w2c);
     const double s0 = w2c * y0;
     const double w = fma(fma(-s0, s0, w2c), 0.5 * y0, s0);
-    const float t = (float)__builtin_amdgcn_fract(w * a.zs);
+    const float t = SHIFT ? (float)__builtin_amdgcn_fract(fma(w, a.zs, fma(a.oy, v, pc)))
+                          : (float)__builtin_amdgcn_fract(w * a.zs);
     const float m = keep ? 1.f : 0.f;
     return make_float2(__builtin_amdgcn_cosf(t) * m, __builtin_amdgcn_sinf(t) * m);
 }
@@ -167,13 +178,15 @@ k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
     __syncthreads();
     // transfer function, then inverse transform
     double Acol = 0.0;  // 1/lambda^2 - u^2 of this column
+    double pc = 0.0;    // K_ASHIFT: ox u
     bool ucut = false;
     const float2 sxc = (MODE == K_TABLE && valid) ? a.sx[col] : make_float2(0.f, 0.f);
-    if (MODE == K_ASM) {
+    if (MODE != K_TABLE) {
         const int fx = col < Mx / 2 ? col : col - Mx;
         const double u = (double)fx * a.du;
         Acol = a.inv_l2 - u * u;
-        ucut = a.bandlimit && !(fabs(u) * 2 < a.uw);
+        ucut = a.bandlimit && !(fabs(MODE == K_ASHIFT ? u - a.u0 : u) * 2 < a.uw);
+        pc = a.ox * u;
     }
     // multiply the spectrum in LDS; a short loop keeps few FP64 temporaries live
 #pragma unroll 2
@@ -184,6 +197,8 @@ k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
         if constexpr (MODE == K_TABLE) {
             h = cmul(sxc, a.sy[i]);
             if (a.adjoint) h = cconj(h);
+        } else if constexpr (MODE == K_ASHIFT) {
+            h = asm_h<true>(a, Acol, ucut, i, L, pc);
         } else {
             h = asm_h(a, Acol, ucut, i, L);
         }
@@ -237,7 +252,9 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
             const int col = i - L / 4;
             float2 x = cscale(v[b * RL + r], a.scale);
             if (post && abs(col - nx / 2) < a.cwl) x = cmul(x, cmul(a.cx[col], cy));
-            dst[(size_t)row * nx + col] = x;
+            float2* o = &dst[(size_t)row * a.ds + col];
+            if (a.accum) x = make_float2(o->x + x.x, o->y + x.y);
+            *o = x;
         }
 }
 
@@ -308,10 +325,10 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
     const int h0 = xcd_remap(blockIdx.x, gridDim.x) * 2 * KR;
     const int ha = h0 + c, hb = h0 + KR + c;
     const int nx = a.nx;
-    const Rsrc sb = make_rsrc(src, (size_t)a.ny * nx * sizeof(float2));
+    const Rsrc sb = make_rsrc(src, ((size_t)(a.ny - 1) * a.ss + nx) * sizeof(float2));
     const Rsrc wb = make_rsrc(W, (size_t)a.ny * 2 * nx * sizeof(float2));
-    const uint32_t rowb = (uint32_t)(KR * nx * sizeof(float2));  // set b rows past set a's
-    const uint32_t so = (uint32_t)((ha * nx + t) * sizeof(float2));
+    const uint32_t rowb = (uint32_t)(KR * a.ss * sizeof(float2));  // set b rows past set a's
+    const uint32_t so = (uint32_t)(((size_t)ha * a.ss + t) * sizeof(float2));
     float2 aya = make_float2(1.f, 0.f), ayb = aya;
     if (a.ay) {
         aya = a.ay[ha];
@@ -401,19 +418,21 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
                 vb[b * RLF + r] = cmul(vb[b * RLF + r], hb);
             }
     } else {
+        constexpr bool SH = MODE == K_ASHIFT;
         const int Mx = 2 * a.nx;
         const int fxa = cola < Mx / 2 ? cola : cola - Mx, fxb = colb < Mx / 2 ? colb : colb - Mx;
         const double ua = (double)fxa * a.du, ub = (double)fxb * a.du;
         const double Aa = a.inv_l2 - ua * ua, Ab = a.inv_l2 - ub * ub;
-        const bool cuta = a.bandlimit && !(fabs(ua) * 2 < a.uw);
-        const bool cutb = a.bandlimit && !(fabs(ub) * 2 < a.uw);
+        const bool cuta = a.bandlimit && !(fabs(SH ? ua - a.u0 : ua) * 2 < a.uw);
+        const bool cutb = a.bandlimit && !(fabs(SH ? ub - a.u0 : ub) * 2 < a.uw);
+        const double pca = SH ? a.ox * ua : 0.0, pcb = SH ? a.ox * ub : 0.0;
 #pragma unroll
         for (int b = 0; b < E / RLF; b++)
 #pragma unroll
             for (int r = 0; r < RLF; r++) {
                 const int i = last_index<PF>(t, b, r);
-                va[b * RLF + r] = cmul(va[b * RLF + r], asm_h(a, Aa, cuta, i, L));
-                vb[b * RLF + r] = cmul(vb[b * RLF + r], asm_h(a, Ab, cutb, i, L));
+                va[b * RLF + r] = cmul(va[b * RLF + r], asm_h<SH>(a, Aa, cuta, i, L, pca));
+                vb[b * RLF + r] = cmul(vb[b * RLF + r], asm_h<SH>(a, Ab, cutb, i, L, pcb));
                 // keep the FP64 temporaries of one element pair live at a time
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -438,7 +457,16 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
         }
 }
 
-template <int L, int KR, int ORD>
+// ACC: add into the destination (split propagation sums the blocks' contributions)
+__device__ __forceinline__ void bstore_acc(Rsrc b, uint32_t voff, uint32_t soff, float2 v, bool acc) {
+    if (acc) {
+        const float2 o = bload(b, voff, soff);
+        v = make_float2(o.x + v.x, o.y + v.y);
+    }
+    bstore(b, voff, soff, v);
+}
+
+template <int L, int KR, int ORD, bool ACC = false>
 __global__ void __launch_bounds__(KR * Plan<L>::T, (kr_waves<L, KR>()))
 k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a,
                  const float2* __restrict__ tw) {
@@ -452,7 +480,7 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
     const int nx = a.nx;
     // element t + D of row h sits at widx(h, t) + D ny (D a multiple of the tile width)
     const Rsrc wb = make_rsrc(W, (size_t)a.ny * 2 * nx * sizeof(float2));
-    const Rsrc db = make_rsrc(dst, (size_t)a.ny * nx * sizeof(float2));
+    const Rsrc db = make_rsrc(dst, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float2));
     const uint32_t wo = (uint32_t)(widx(a, ha, t) * sizeof(float2));
     const uint32_t setb = (uint32_t)(KR * a.cbl * sizeof(float2));
     float2 va[P::E], vb[P::E];
@@ -465,8 +493,8 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
             vb[b * R0 + r] = bload(wb, wo, so + setb);
         }
     run_stages_pair<P, true, TWN, KR>(va, vb, t, lds, c, tw);
-    const uint32_t so = (uint32_t)((ha * nx + t) * sizeof(float2));
-    const uint32_t rowb = (uint32_t)(KR * nx * sizeof(float2));
+    const uint32_t so = (uint32_t)(((size_t)ha * a.ds + t) * sizeof(float2));
+    const uint32_t rowb = (uint32_t)(KR * a.ds * sizeof(float2));
     if (a.cx == nullptr) {
 #pragma unroll
         for (int b = 0; b < P::E / RL; b++)
@@ -475,8 +503,8 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
                 if (!last_kept<P>(b, r)) continue;
                 // column t + D
                 const uint32_t D = (last_index<P>(0, b, r) - L / 4) * sizeof(float2);
-                bstore(db, so, D, cscale(va[b * RL + r], a.scale));
-                bstore(db, so, D + rowb, cscale(vb[b * RL + r], a.scale));
+                bstore_acc(db, so, D, cscale(va[b * RL + r], a.scale), ACC);
+                bstore_acc(db, so, D + rowb, cscale(vb[b * RL + r], a.scale), ACC);
             }
         return;
     }
@@ -497,8 +525,8 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
                 if (posta) xa = cmul(xa, cmul(cx, cya));
                 if (postb) xb = cmul(xb, cmul(cx, cyb));
             }
-            bstore(db, so, D * sizeof(float2), xa);
-            bstore(db, so, D * sizeof(float2) + rowb, xb);
+            bstore_acc(db, so, D * sizeof(float2), xa, ACC);
+            bstore_acc(db, so, D * sizeof(float2) + rowb, xb, ACC);
         }
 }
 
@@ -530,14 +558,20 @@ template <int L, int KR>
 static void launch_cols_conv(const TileCfg& g, const PropArgs& a, float2* W, const float2* tw,
                              hipStream_t s) {
     if (a.mode == K_ASM) launch_cols_mode<L, KR, K_ASM>(g, a, W, tw, s);
+    else if (a.mode == K_ASHIFT) launch_cols_mode<L, KR, K_ASHIFT>(g, a, W, tw, s);
     else launch_cols_mode<L, KR, K_TABLE>(g, a, W, tw, s);
 }
 template <int L, int KR>
 static void launch_rows_inv(const TileCfg& g, const PropArgs& a, const float2* W, float2* dst,
                             const float2* tw, hipStream_t s) {
     const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
-    if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST><<<grid, nt, 0, s>>>(W, dst, a, tw);
-    else k_rows_crop_inv2<L, KR, REM_FIRST><<<grid, nt, 0, s>>>(W, dst, a, tw);
+    if (a.accum) {
+        if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
+        else k_rows_crop_inv2<L, KR, REM_FIRST, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
+    } else {
+        if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST><<<grid, nt, 0, s>>>(W, dst, a, tw);
+        else k_rows_crop_inv2<L, KR, REM_FIRST><<<grid, nt, 0, s>>>(W, dst, a, tw);
+    }
 }
 
 // ------------------------------------------------------------------ plain 2-D FFT
@@ -757,6 +791,8 @@ static void launch_cols_cb(const PropArgs& a, float2* W, const float2* tw, hipSt
     const int grid = (Mx + CB - 1) / CB;
     if (a.mode == K_ASM)
         k_cols_conv<L, CB, K_ASM><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
+    else if (a.mode == K_ASHIFT)
+        k_cols_conv<L, CB, K_ASHIFT><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
     else
         k_cols_conv<L, CB, K_TABLE><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
 }
@@ -908,6 +944,7 @@ static PropArgs base_args(int64_t ny, int64_t nx) {
     memset(&a, 0, sizeof a);
     a.ny = (int)ny;
     a.nx = (int)nx;
+    a.ss = a.ds = (int)nx;
     a.extra_scale = 1.0;
     return a;
 }
@@ -968,17 +1005,15 @@ extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, 
     return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a);
 }
 
-extern "C" int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny,
-                            int64_t nx, double dy, double dx, double lambda, double z, double s,
-                            double oy, double ox, uint32_t flags) {
-    OLX_TRY(check_field(ctx, d_src, ny, nx));
-    if (!d_dst) OLX_FAIL(OLX_EINVAL, "arss: null destination");
+// ARSS arguments (tables built or fetched from the cache); the caller sets strides
+static int arss_args(olx_ctx* ctx, PropArgs& a, int64_t ny, int64_t nx, double dy, double dx,
+                     double lambda, double z, double s, double oy, double ox, uint32_t flags) {
     if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !(z != 0) || !std::isfinite(z) ||
         !(s != 0) || !std::isfinite(s))
         OLX_FAIL(OLX_EINVAL, "arss: lambda, dx, dy > 0, z and s finite and nonzero");
     if (flags & OLX_ADJOINT)
         OLX_FAIL(OLX_EUNSUPPORTED, "arss: adjoint not provided (chirp order reverses)");
-    PropArgs a = base_args(ny, nx);
+    a = base_args(ny, nx);
     a.mode = K_TABLE;
     const double K = M_PI / (lambda * z);
     int rc = OLX_OK;
@@ -1009,8 +1044,117 @@ extern "C" int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_
     // the CPU variant scales u by 1/M^2 twice and the output once more (olprop.h:367,
     // 384, 389) against the GPU variant's 1/M^2 on u and on h: one extra 1/M^2
     a.extra_scale = cpu ? 1.0 / (4.0 * (double)nx * (double)ny) : 1.0;
+    return OLX_OK;
+}
+
+// shiftedAsmTransferF's window on one axis (include/olprop.h:255-285): centre c and
+// width w of the band |f - c| * 2 < w for a destination offset o, spectral pitch d
+static void shift_window(double o, double d, double z, double lambda, double* c, double* w) {
+    const double sx = 0.5f / d;
+    const double lm = 1 / std::sqrt(std::pow(o - 0.5f / d, -2) * z * z + 1.0f) / lambda;
+    const double lp = 1 / std::sqrt(std::pow(o + 0.5f / d, -2) * z * z + 1.0f) / lambda;
+    if (sx < o) {
+        *c = 0.5f * (lp + lm);
+        *w = lp - lm;
+    } else if (-sx <= o && o < sx) {
+        *c = 0.5f * (lp - lm);
+        *w = lp + lm;
+    } else {
+        *c = -0.5f * (lp + lm);
+        *w = lm - lp;
+    }
+}
+
+// shifted ASM arguments: H = exp(2 pi i (w z + ox u + oy v)), window per flags
+static int shifted_asm_args(PropArgs& a, int64_t ny, int64_t nx, double dy, double dx,
+                            double lambda, double z, double oy, double ox, uint32_t flags) {
+    if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !std::isfinite(z) || !std::isfinite(oy) ||
+        !std::isfinite(ox))
+        OLX_FAIL(OLX_EINVAL, "shifted asm: lambda, dx, dy must be > 0 and z, oy, ox finite");
+    a = base_args(ny, nx);
+    a.mode = K_ASHIFT;
+    a.adjoint = (flags & OLX_ADJOINT) ? 1 : 0;
+    const int64_t nx2 = 2 * nx, ny2 = 2 * ny;
+    a.du = 1 / (dx * nx2);
+    a.dv = 1 / (dy * ny2);
+    a.inv_l2 = 1 / (lambda * lambda);
+    // the adjoint conjugates H: every phase term changes sign
+    a.zs = a.adjoint ? -z : z;
+    a.ox = a.adjoint ? -ox : ox;
+    a.oy = a.adjoint ? -oy : oy;
+    a.bandlimit = (flags & (OLX_BANDLIMIT | OLX_SHIFTED_CPU)) ? 1 : 0;
+    if (a.bandlimit) {
+        shift_window(ox, a.du, z, lambda, &a.u0, &a.uw);
+        shift_window(oy, a.dv, z, lambda, &a.v0, &a.vw);
+    }
+    // the CPU shiftedAsmProp scales by 1/M^2 before the inverse FFT and again after
+    // the crop (include/olprop.h:325, 329): one 1/M^2 more than the exact result
+    a.extra_scale = (flags & OLX_SHIFTED_CPU) ? 1.0 / ((double)nx2 * (double)ny2) : 1.0;
+    return OLX_OK;
+}
+
+extern "C" int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny,
+                            int64_t nx, double dy, double dx, double lambda, double z, double s,
+                            double oy, double ox, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_src, ny, nx));
+    if (!d_dst) OLX_FAIL(OLX_EINVAL, "arss: null destination");
+    PropArgs a;
+    OLX_TRY(arss_args(ctx, a, ny, nx, dy, dx, lambda, z, s, oy, ox, flags));
     return run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a);
 }
+
+extern "C" int olx_shifted_asm_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny,
+                                   int64_t nx, double dy, double dx, double lambda, double z,
+                                   double oy, double ox, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_src, ny, nx));
+    if (!d_dst) OLX_FAIL(OLX_EINVAL, "shifted asm: null destination");
+    PropArgs a;
+    OLX_TRY(shifted_asm_args(a, ny, nx, dy, dx, lambda, z, oy, ox, flags));
+    return run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a);
+}
+
+extern "C" int olx_split_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
+                                  double dx, double lambda, double z, int mode, int by, int bx,
+                                  uint32_t flags) {
+    if (!ctx || !d_u) OLX_FAIL(OLX_EINVAL, "split prop: null argument");
+    if (by <= 0 || bx <= 0 || ny % by || nx % bx)
+        OLX_FAIL(OLX_EINVAL, "split prop: %d x %d blocks do not tile %lld x %lld", by, bx,
+                 (long long)ny, (long long)nx);
+    if (mode != OLX_MODE_ASM && mode != OLX_MODE_FRESNEL)
+        OLX_FAIL(OLX_EINVAL, "split prop: mode must be ASM or FRESNEL");
+    if ((flags & OLX_SHIFTED_CPU) && mode != OLX_MODE_ASM)
+        OLX_FAIL(OLX_EINVAL, "split prop: the CPU form exists for ASM only");
+    const int64_t bh = ny / by, bw = nx / bx;
+    OLX_TRY(check_field(ctx, d_u, bh, bw));
+    OLX_TRY(ctx->split_acc.reserve(sizeof(float2) * (size_t)(ny * nx)));
+    float2* u = (float2*)d_u;
+    float2* acc = (float2*)ctx->split_acc.p;
+    OLX_HIP(hipMemsetAsync(acc, 0, sizeof(float2) * (size_t)(ny * nx), ctx->stream));
+    // include/olprop.h:614-651 (splitAsmProp) and include/golprop.h:773-813
+    // (gsplitProp): every source block to every destination block, shifted by
+    // (src - dst) block offsets computed in float as there, summed
+    for (int gn = 0; gn < by; gn++)
+        for (int gm = 0; gm < bx; gm++)
+            for (int dn = 0; dn < by; dn++)
+                for (int dm = 0; dm < bx; dm++) {
+                    const float oy = (float)((gn - dn) * (int)bh) * (float)dy;
+                    const float ox = (float)((gm - dm) * (int)bw) * (float)dx;
+                    PropArgs a;
+                    if (mode == OLX_MODE_ASM)
+                        OLX_TRY(shifted_asm_args(a, bh, bw, dy, dx, lambda, z, oy, ox,
+                                                 flags & OLX_SHIFTED_CPU));
+                    else
+                        OLX_TRY(arss_args(ctx, a, bh, bw, dy, dx, lambda, z, 1.0, oy, ox, 0));
+                    a.ss = a.ds = (int)nx;
+                    a.accum = 1;
+                    OLX_TRY(run_three_pass(ctx, u + gn * bh * nx + gm * bw,
+                                           acc + dn * bh * nx + dm * bw, a));
+                }
+    OLX_HIP(hipMemcpyAsync(u, acc, sizeof(float2) * (size_t)(ny * nx), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    return OLX_OK;
+}
+
 
 extern "C" int olx_asm_transfer_c64(olx_ctx* ctx, void* d_H, int64_t ny, int64_t nx, double dv,
                                     double du, double lambda, double z, uint32_t flags) {

@@ -84,6 +84,59 @@ def ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lam, z, s, oy=0.0, ox=0.0):
     return udst
 
 
+def _shifted(usrc, udst, ny, nx, dy, dx, lam, z, oy, ox, flags, name):
+    _field(usrc, ny, nx, "usrc")
+    _field(udst, ny, nx, "udst")
+    check(_lib.load().olx_shifted_asm_c64(_lib.ctx(usrc.device.index), _ptr(usrc), _ptr(udst), ny,
+                                          nx, dy, dx, lam, z, oy, ox, flags), name)
+    return udst
+
+
+def shiftedAsmProp(usrc, udst, ny, nx, dy, dx, lam, z, oy, ox, adjoint=False):
+    """ol::shiftedAsmProp (include/olprop.h:307-330): off-axis ASM to the window offset
+    by (oy, ox), with the shifted band-limit window of shiftedAsmTransferF (:250-305)
+    and the reference's net scale 1/M^2.  usrc and udst may alias."""
+    return _shifted(usrc, udst, ny, nx, dy, dx, lam, z, oy, ox,
+                    _lib.SHIFTED_CPU | (_lib.ADJOINT if adjoint else 0), "shiftedAsmProp")
+
+
+def gshiftedAsmProp(usrc, udst, ny, nx, dy, dx, lam, z, oy, ox, bandlimit=False, adjoint=False):
+    """ol::gshiftedAsmProp (include/golprop.h:536-576): no window (its window is
+    commented out there; bandlimit=True adds it), net scale 1."""
+    flags = (_lib.BANDLIMIT if bandlimit else 0) | (_lib.ADJOINT if adjoint else 0)
+    return _shifted(usrc, udst, ny, nx, dy, dx, lam, z, oy, ox, flags, "gshiftedAsmProp")
+
+
+def shiftedProp(usrc, udst, ny, nx, dy, dx, lam, d, oy, ox, propmode):
+    """ol::shiftedProp (include/golprop.h:756-770): gshiftedAsmProp for ASM, ARSS with
+    s = 1 for FRESNEL."""
+    if propmode == ASM:
+        return gshiftedAsmProp(usrc, udst, ny, nx, dy, dx, lam, d, oy, ox)
+    if propmode == FRESNEL:
+        return ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lam, d, 1.0, oy, ox)
+    raise OlxInvalid(_lib.EINVAL, "shiftedProp: propmode must be ASM or FRESNEL")
+
+
+def splitAsmProp(u, height, width, dy, dx, lam, z, blocknum_y=2, blocknum_x=2):
+    """ol::splitAsmProp (include/olprop.h:614-651): every block to every block through
+    shiftedAsmProp, summed, in place."""
+    _field(u, height, width)
+    check(_lib.load().olx_split_prop_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx,
+                                         lam, z, _lib.MODE_ASM, blocknum_y, blocknum_x,
+                                         _lib.SHIFTED_CPU), "splitAsmProp")
+    return u
+
+
+def gsplitProp(u, height, width, dy, dx, lam, z, propmode, blocknum_y=2, blocknum_x=2):
+    """ol::gsplitProp (include/golprop.h:773-813): blocks through shiftedProp, in place."""
+    _field(u, height, width)
+    if propmode not in (ASM, FRESNEL):
+        raise OlxInvalid(_lib.EINVAL, "gsplitProp: propmode must be ASM or FRESNEL")
+    check(_lib.load().olx_split_prop_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx,
+                                         lam, z, propmode, blocknum_y, blocknum_x, 0), "gsplitProp")
+    return u
+
+
 def Prop(u, ny, nx, dy, dx, lam, d, propmode=AUTO):
     """ol::Prop (include/olprop.h:656-682).  Returns the mode used (ASM, FRESNEL or
     FRESNEL_BL when AUTO falls back to the band-limited Fresnel)."""

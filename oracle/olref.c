@@ -233,6 +233,88 @@ DEFINE_ASM(c64, cf, float)
 DEFINE_ASM(c128, cd, double)
 
 /* ------------------------------------------------------------------------- */
+/* Shifted ASM: shiftedAsmTransferF olprop.h:250-305 (window; the GPU twin,  */
+/* golprop.h:333-356, has it commented out), shiftedAsmProp :307-330 and      */
+/* gshiftedAsmProp golprop.h:536-576.                                         */
+/* ------------------------------------------------------------------------- */
+/* window centre and width on one axis, olprop.h:255-285 */
+static void shift_window(double o, double d, double z, double lambda, double* c, double* w) {
+    double sx = 0.5f / d;
+    double lm = 1 / sqrt(pow((o - 0.5f / d), -2) * z * z + 1.0f) / lambda;
+    double lp = 1 / sqrt(pow((o + 0.5f / d), -2) * z * z + 1.0f) / lambda;
+    if (sx < o) {
+        *c = 0.5f * (lp + lm);
+        *w = lp - lm;
+    } else if (-sx <= o && o < sx) {
+        *c = 0.5f * (lp - lm);
+        *w = lp + lm;
+    } else {
+        *c = -0.5f * (lp + lm);
+        *w = lm - lp;
+    }
+}
+
+static void shifted_asm_transfer(cd* H, int64_t ny, int64_t nx, double dv, double du,
+                                 double lambda, double z, double oy, double ox, unsigned flags) {
+    int win = (flags & OLREF_SHIFT_WINDOW) != 0;
+    double u0 = 0, uw = 0, v0 = 0, vw = 0;
+    if (win) {
+        shift_window(ox, du, z, lambda, &u0, &uw);
+        shift_window(oy, dv, z, lambda, &v0, &vw);
+    }
+    double tmp = 1 / (lambda * lambda);
+    #pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < ny; n++) {
+        double v = (n - ny / 2) * dv;
+        for (int64_t m = 0; m < nx; m++) {
+            int64_t idx = m + n * nx;
+            double u = (m - nx / 2) * du;
+            double arg = tmp - u * u - v * v;
+            if ((win && !((fabs(u - u0) * 2 < uw) && (fabs(v - v0) * 2 < vw))) || arg < 0) {
+                H[idx] = 0;
+                continue;
+            }
+            double w = sqrt(arg);
+            double phase = 2 * M_PI * (w * z + ox * u + oy * v);
+            double s = sin(phase);
+            if (flags & OLREF_ADJOINT) s = -s;
+            H[idx] = cos(phase) + I * s;
+        }
+    }
+}
+
+#define DEFINE_SHIFTED_ASM(SUF, CT, RT)                                             \
+int olref_shifted_asm_##SUF(const RT* srcin, RT* dstin, int64_t ny, int64_t nx,     \
+                            double dy, double dx, double lambda, double z,          \
+                            double oy, double ox, unsigned flags) {                 \
+    if (!srcin || !dstin || ny <= 0 || nx <= 0 || !(lambda > 0) || !(dx > 0) ||     \
+        !(dy > 0))                                                                  \
+        return -1;                                                                  \
+    int64_t ny2 = ny * 2, nx2 = nx * 2;                                             \
+    double du = 1 / (dx * nx2), dv = 1 / (dy * ny2);                                \
+    CT* buf = pad_##SUF((const CT*)srcin, ny, nx, ny2, nx2);                        \
+    cd* Hd = (cd*)malloc(sizeof(cd) * (size_t)(ny2 * nx2));                         \
+    shifted_asm_transfer(Hd, ny2, nx2, dv, du, lambda, z, oy, ox, flags);           \
+    CT* H = (CT*)malloc(sizeof(CT) * (size_t)(ny2 * nx2));                          \
+    for (int64_t i = 0; i < ny2 * nx2; i++) H[i] = (CT)Hd[i];                       \
+    free(Hd);                                                                       \
+    fftshift_##SUF(H, ny2, nx2);                                                    \
+    fft2d_##SUF(buf, ny2, nx2, -1);                                                 \
+    RT sc = (RT)(1.0 / (double)(ny2 * nx2));                                        \
+    for (int64_t i = 0; i < ny2 * nx2; i++) { buf[i] = buf[i] * H[i]; buf[i] *= sc; }\
+    fft2d_##SUF(buf, ny2, nx2, +1);                                                 \
+    CT* dst = (CT*)dstin;                                                           \
+    crop_##SUF(buf, dst, ny2, nx2);                                                 \
+    if (flags & OLREF_SHIFT_CPU_SCALE)                                              \
+        for (int64_t i = 0; i < ny * nx; i++) dst[i] *= sc;                         \
+    free(H); free(buf);                                                             \
+    return 0;                                                                       \
+}
+
+DEFINE_SHIFTED_ASM(c64, cf, float)
+DEFINE_SHIFTED_ASM(c128, cd, double)
+
+/* ------------------------------------------------------------------------- */
 /* Fresnel: FresnelProp olprop.h:125-148 with FresnelResponse :8-24;         */
 /* FresnelPropBandLimit :150-170 with FresnelResponseBandLimit :26-54;        */
 /* GPU band limit (x-limit on both axes) golprop.h:201-238.                   */
@@ -446,6 +528,44 @@ DEFINE_SHIFTED(c128, cd, double)
 /* ------------------------------------------------------------------------- */
 /* Dispatcher and checks: olprop.h:109-123, :209-225, :656-682                */
 /* ------------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------- */
+/* Split propagation: splitAsmProp olprop.h:614-651, gsplitProp golprop.h:    */
+/* 773-813 (shiftedProp golprop.h:756-770 picks ASM or ARSS with s = 1).      */
+/* ------------------------------------------------------------------------- */
+int olref_split_prop_c64(float* uin, int64_t height, int64_t width, double dy, double dx,
+                         double lambda, double z, int mode, int by, int bx, unsigned flags) {
+    if (!uin || by <= 0 || bx <= 0 || height % by || width % bx) return -1;
+    cf* u = (cf*)uin;
+    int64_t bw = width / bx, bh = height / by;
+    cf* udst = (cf*)calloc((size_t)(height * width), sizeof(cf));
+    cf* src = (cf*)malloc(sizeof(cf) * (size_t)(bh * bw));
+    cf* dst = (cf*)malloc(sizeof(cf) * (size_t)(bh * bw));
+    for (int gn = 0; gn < by; gn++)
+        for (int gm = 0; gm < bx; gm++) {
+            for (int64_t ln = 0; ln < bh; ln++)
+                for (int64_t lm = 0; lm < bw; lm++)
+                    src[ln * bw + lm] = u[(gn * bh + ln) * width + gm * bw + lm];
+            for (int dn = 0; dn < by; dn++)
+                for (int dm = 0; dm < bx; dm++) {
+                    /* float oy = (global_n - dst_n) * blockheight * dy (PREC_T float) */
+                    float oy = (float)((gn - dn) * (int)bh) * (float)dy;
+                    float ox = (float)((gm - dm) * (int)bw) * (float)dx;
+                    if (mode == 0)
+                        olref_shifted_asm_c64((const float*)src, (float*)dst, bh, bw, dy, dx,
+                                              lambda, z, oy, ox, flags);
+                    else
+                        olref_arss_c64((const float*)src, (float*)dst, bh, bw, dy, dx, lambda,
+                                       z, 1.0, oy, ox, 0);
+                    for (int64_t ln = 0; ln < bh; ln++)
+                        for (int64_t lm = 0; lm < bw; lm++)
+                            udst[(dn * bh + ln) * width + dm * bw + lm] += dst[ln * bw + lm];
+                }
+        }
+    memcpy(u, udst, sizeof(cf) * (size_t)(height * width));
+    free(udst); free(src); free(dst);
+    return 0;
+}
+
 int olref_asm_prop_check(int64_t ny, int64_t nx, double dy, double dx, double lambda,
                          double z) {
     int64_t nx2 = 2 * nx, ny2 = 2 * ny;

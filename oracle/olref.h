@@ -38,6 +38,8 @@ extern "C" {
 #define OLREF_CGH_EXACT_R 0x100u  /* exact spherical r instead of the Fresnel quadratic */
 /* WRP flags */
 #define OLREF_WRP_SQUARE  0x200u  /* square window D = sqrt(2) W (golcgh.h:391-468) */
+#define OLREF_SHIFT_WINDOW 0x1000u /* shifted ASM: shiftedAsmTransferF window (olprop.h:250-305) */
+#define OLREF_SHIFT_CPU_SCALE 0x2000u /* shifted ASM: the CPU's second 1/M^2 (olprop.h:325-329) */
 
 /* 2-D FFT, in place, row-major ny x nx, sign -1 forward / +1 backward, unnormalised */
 int olref_fft2d_c64(float* data, int64_t ny, int64_t nx, int sign);
@@ -70,6 +72,22 @@ int olref_shifted_fresnel_c64(float* u, int64_t ny, int64_t nx, double dy, doubl
                               double lambda, double z, double s, double oy, double ox);
 int olref_shifted_fresnel_c128(double* u, int64_t ny, int64_t nx, double dy, double dx,
                                double lambda, double z, double s, double oy, double ox);
+
+/* shifted ASM: shiftedAsmProp include/olprop.h:307-330 with shiftedAsmTransferF
+ * :250-305 (flags SHIFT_WINDOW | SHIFT_CPU_SCALE), or gshiftedAsmProp
+ * include/golprop.h:536-576 (flags 0: no window, net scale 1); ADJOINT conjugates
+ * H.  src and dst may alias. */
+int olref_shifted_asm_c64(const float* src, float* dst, int64_t ny, int64_t nx, double dy,
+                          double dx, double lambda, double z, double oy, double ox,
+                          unsigned flags);
+int olref_shifted_asm_c128(const double* src, double* dst, int64_t ny, int64_t nx, double dy,
+                           double dx, double lambda, double z, double oy, double ox,
+                           unsigned flags);
+/* splitAsmProp include/olprop.h:614-651 / gsplitProp include/golprop.h:773-813, in
+ * place: mode 0 ASM (shifted ASM per block pair with the given flags), 1 FRESNEL
+ * (ARSS with s = 1 per block pair, golprop.h:756-770) */
+int olref_split_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,
+                         double lambda, double z, int mode, int by, int bx, unsigned flags);
 
 /* Prop dispatcher, include/olprop.h:656-682; returns chosen mode in *chosen
  * (0 ASM, 1 FRESNEL, 2 FRESNEL band-limited) */

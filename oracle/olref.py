@@ -22,6 +22,8 @@ FRESNEL_BL_CPU = 0x10
 FRESNEL_BL_GPU = 0x20
 CGH_EXACT_R = 0x100
 WRP_SQUARE = 0x200
+SHIFT_WINDOW = 0x1000
+SHIFT_CPU_SCALE = 0x2000
 
 _lib = None
 
@@ -51,6 +53,10 @@ def lib():
             "olref_prop_c64": [p, i64, i64, d, d, d, d, ctypes.c_int, p],
             "olref_shifted_fresnel_c64": [p, i64, i64, d, d, d, d, d, d, d],
             "olref_shifted_fresnel_c128": [p, i64, i64, d, d, d, d, d, d, d],
+            "olref_shifted_asm_c64": [p, p, i64, i64, d, d, d, d, d, d, u32],
+            "olref_shifted_asm_c128": [p, p, i64, i64, d, d, d, d, d, d, u32],
+            "olref_split_prop_c64": [p, i64, i64, d, d, d, d, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, u32],
             "olref_asm_prop_check": [i64, i64, d, d, d, d],
             "olref_fresnel_prop_check": [i64, i64, d, d, d, d],
             "olref_cgh_c64": [p, p, p, p, p, i64, p, i64, i64, d, d, d, u32],
@@ -127,6 +133,26 @@ def shifted_fresnel_prop(u, dy, dx, lam, z, s, oy=0.0, ox=0.0):
     fn = lib().olref_shifted_fresnel_c64 if u.dtype == np.complex64 else lib().olref_shifted_fresnel_c128
     _check(fn(_ptr(u), u.shape[0], u.shape[1], dy, dx, lam, z, s, oy, ox), "shifted_fresnel_prop")
     return u
+
+
+def shifted_asm(u, dy, dx, lam, z, oy, ox, flags=0):
+    """shiftedAsmProp (include/olprop.h:307-330, flags SHIFT_WINDOW | SHIFT_CPU_SCALE)
+    or gshiftedAsmProp (include/golprop.h:536-576, flags 0)."""
+    src = _cplx(u, u.dtype if u.dtype == np.complex128 else np.complex64)
+    out = np.empty_like(src)
+    fn = lib().olref_shifted_asm_c128 if src.dtype == np.complex128 else lib().olref_shifted_asm_c64
+    _check(fn(_ptr(src), _ptr(out), src.shape[0], src.shape[1], dy, dx, lam, z, oy, ox, flags),
+           "shifted_asm")
+    return out
+
+
+def split_prop(u, dy, dx, lam, z, by=2, bx=2, mode=0, flags=0):
+    """splitAsmProp (include/olprop.h:614-651; mode 0 with SHIFT_WINDOW |
+    SHIFT_CPU_SCALE) or gsplitProp (include/golprop.h:773-813; mode 0 ASM, 1 FRESNEL)."""
+    v = _cplx(u, np.complex64).copy()
+    _check(lib().olref_split_prop_c64(_ptr(v), v.shape[0], v.shape[1], dy, dx, lam, z, mode, by,
+                                      bx, flags), "split_prop")
+    return v
 
 
 def prop(u, dy, dx, lam, z, mode=2):

@@ -67,6 +67,23 @@ int main(int argc, char** argv) {
         ol::shiftedFresnelProp(u, n, n, p, p, lambda, 0.1, 0.5, 1e-4, -2e-4);
         save(dir + "/shifted.bin", u.get(), (size_t)n * n);
         std::cout << "AsmPropCheck " << ol::AsmPropCheck(n, n, p, p, lambda, 0.05) << "\n";
+        // off-axis and split propagation (PREC_T = float, as the reference's tests)
+        {
+            const float pf = (float)p, lf = (float)lambda;
+            auto us = fresh();
+            std::unique_ptr<std::complex<float>[]> ud;
+            ol::shiftedAsmProp(us, ud, n, n, pf, pf, lf, 0.004f, 0.0f, 0.5f * n * pf);
+            save(dir + "/shifted_asm.bin", ud.get(), (size_t)n * n);
+            u = fresh();
+            ol::splitAsmProp(u, n, n, pf, pf, lf, 0.002f);
+            save(dir + "/split_cpu.bin", u.get(), (size_t)n * n);
+            u = fresh();
+            ol::gsplitProp(u, n, n, pf, pf, lf, 0.002f, ol::ASM);
+            save(dir + "/gsplit.bin", u.get(), (size_t)n * n);
+            u = fresh();
+            ol::gsplitProp(u, n, n, pf, pf, lf, 0.05f, ol::FRESNEL, 2, 4);
+            save(dir + "/gsplit_fresnel.bin", u.get(), (size_t)n * n);
+        }
 
         // CGH on a host point cloud (ol::Object<double>)
         ol::Object<double> obj(m);
@@ -88,6 +105,13 @@ int main(int argc, char** argv) {
         ol::ARSSFresnelProp(src, dst, n, n, p, p, lambda, 0.15, 0.5);
         hd = ol::dev2cpu(dst, (size_t)n * n);
         save(dir + "/arss.bin", hd.get(), (size_t)n * n);
+        ol::dfield gs;
+        ol::gshiftedAsmProp(src, gs, n, n, p, p, lambda, 0.004, -0.25 * n * p, 0.5 * n * p);
+        hd = ol::dev2cpu(gs, (size_t)n * n);
+        save(dir + "/gshifted.bin", hd.get(), (size_t)n * n);
+        ol::shiftedProp(src, gs, n, n, p, p, lambda, 0.05, 1e-4, 0.0, ol::FRESNEL);
+        hd = ol::dev2cpu(gs, (size_t)n * n);
+        save(dir + "/shiftedprop_fresnel.bin", hd.get(), (size_t)n * n);
         auto gobj = ol::Object2dev(obj);
         auto dU = ol::make_device<std::complex<float>[]>((size_t)n * n);
         ol::gCgh(gobj, dU, n, n, p, lambda);

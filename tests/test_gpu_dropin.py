@@ -39,6 +39,16 @@ def test_dropin_program_matches_oracle(tmp_path):
     assert rel(rd("shifted"), olref.shifted_fresnel_prop(u0, P, P, LAM, 0.1, 0.5, 1e-4, -2e-4)) < 1e-5
     assert rel(rd("gasm"), olref.asm_prop(u0, P, P, LAM, 0.05)) < 1e-5
     assert rel(rd("arss"), olref.arss_prop(u0, P, P, LAM, 0.15, 0.5)) < 1e-5
+    pf, lf = float(np.float32(P)), float(np.float32(LAM))
+    f32 = lambda v: float(np.float32(v))
+    assert rel(rd("shifted_asm"), olref.shifted_asm(u0, pf, pf, lf, f32(0.004), 0.0, f32(0.5 * n * pf),
+                                                   olref.SHIFT_WINDOW | olref.SHIFT_CPU_SCALE)) < 1e-5
+    assert rel(rd("split_cpu"), olref.split_prop(u0, pf, pf, lf, f32(0.002), 2, 2, 0,
+                                                 olref.SHIFT_WINDOW | olref.SHIFT_CPU_SCALE)) < 1e-5
+    assert rel(rd("gsplit"), olref.split_prop(u0, pf, pf, lf, f32(0.002), 2, 2, 0, 0)) < 1e-5
+    assert rel(rd("gsplit_fresnel"), olref.split_prop(u0, pf, pf, lf, f32(0.05), 2, 4, 1, 0)) < 1e-5
+    assert rel(rd("gshifted"), olref.shifted_asm(u0, P, P, LAM, 0.004, -0.25 * n * P, 0.5 * n * P)) < 1e-5
+    assert rel(rd("shiftedprop_fresnel"), olref.arss_prop(u0, P, P, LAM, 0.05, 1.0, 1e-4, 0.0)) < 1e-5
     cgh = olref.cgh(pts["x"], pts["y"], pts["z"], n, n, P, P, LAM, dtype=np.complex128)
     assert rel(rd("cgh"), cgh) < 1e-4 and rel(rd("gcgh"), cgh) < 1e-4
     ph = rd("cgh_phase", np.float32)
