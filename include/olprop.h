@@ -154,6 +154,40 @@ void gsplitProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, i
     });
 }
 
+// include/olprop.h:410-412
+enum INTERPOLATION { BILINEAR, BICUBIC };
+
+// include/olprop.h:414-526: from a source plane tilted by angle_src2dst to a parallel
+// plane at distance z; in place, net scale 1/M^2 as the reference's
+template <typename PREC_T = float, typename COMPLEX_T>
+void TiltedSourceProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int ny, int nx, PREC_T dy,
+                      PREC_T dx, PREC_T lambda, PREC_T z, PREC_T angle_src2dst,
+                      INTERPOLATION inp = BILINEAR, PROPMODE propmode = ASM) {
+    if (propmode != ASM && propmode != FRESNEL)
+        throw std::runtime_error("TiltedSourceProp: propmode must be ASM or FRESNEL");
+    detail::on_device(u, (int64_t)ny * nx, [&](void* d) {
+        detail::check(olx_tilted_c64(detail::ctx(), d, ny, nx, dy, dx, lambda, z, angle_src2dst,
+                                     inp == BICUBIC ? OLX_INTERP_BICUBIC : OLX_INTERP_BILINEAR,
+                                     propmode == ASM ? OLX_MODE_ASM : OLX_MODE_FRESNEL,
+                                     OLX_TILT_SOURCE), "TiltedSourceProp");
+    });
+}
+
+// include/olprop.h:528-594: propagation by z, then the reference plane rotated by rotate
+template <typename PREC_T = float, typename COMPLEX_T>
+void TiltedReferenceProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t ny, int64_t nx,
+                         PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z, PREC_T rotate,
+                         INTERPOLATION inp = BILINEAR, PROPMODE propmode = ASM) {
+    if (propmode != ASM && propmode != FRESNEL)
+        throw std::runtime_error("TiltedReferenceProp: propmode must be ASM or FRESNEL");
+    detail::on_device(u, ny * nx, [&](void* d) {
+        detail::check(olx_tilted_c64(detail::ctx(), d, ny, nx, dy, dx, lambda, z, rotate,
+                                     inp == BICUBIC ? OLX_INTERP_BICUBIC : OLX_INTERP_BILINEAR,
+                                     propmode == ASM ? OLX_MODE_ASM : OLX_MODE_FRESNEL,
+                                     OLX_TILT_REFERENCE), "TiltedReferenceProp");
+    });
+}
+
 // include/olprop.h:656-682 (silent; the reference prints the chosen mode)
 template <typename COMPLEX_T, typename PREC_T>
 void Prop(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int ny, int nx, PREC_T dy, PREC_T dx,

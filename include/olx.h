@@ -120,6 +120,20 @@ int olx_shifted_asm_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny
  * shiftedProp, include/golprop.h:756-770).  Blocks must be powers of two in [4, 8192]. */
 int olx_split_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
                        double lambda, double z, int mode, int by, int bx, uint32_t flags);
+/* tilted-plane propagation in place: kind OLX_TILT_SOURCE is TiltedSourceProp
+ * (include/olprop.h:414-526), OLX_TILT_REFERENCE TiltedReferenceProp (:528-594).
+ * The padded spectrum is resampled at frequencies rotated by angle about the y
+ * axis (interp OLX_INTERP_BILINEAR / OLX_INTERP_BICUBIC: include/olutils.h:213-234,
+ * 290-320), weighted by the rotation's Jacobian and an ASM (mode OLX_MODE_ASM) or
+ * Fresnel (OLX_MODE_FRESNEL) transfer function.  Net scale 1/M^2, as the
+ * reference's (it scales before and after the inverse transform).  Evanescent
+ * samples are 0 (the reference: NaN).  Sides: powers of two in [4, 8192]. */
+#define OLX_TILT_SOURCE 0
+#define OLX_TILT_REFERENCE 1
+#define OLX_INTERP_BILINEAR 0
+#define OLX_INTERP_BICUBIC 1
+int olx_tilted_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
+                   double lambda, double z, double angle, int interp, int mode, int kind);
 /* Prop dispatcher with AUTO selection; *chosen receives OLX_MODE_*.  Replaces
  * ol::Prop (include/olprop.h:656-682, include/golprop.h:578-604). */
 int olx_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,

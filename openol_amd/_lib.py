@@ -22,6 +22,8 @@ CGH_ACCUMULATE = 0x400
 WRP_SQUARE = 0x200
 ARSS_CPU = 0x800
 SHIFTED_CPU = 0x1000
+TILT_SOURCE, TILT_REFERENCE = 0, 1
+INTERP_BILINEAR, INTERP_BICUBIC = 0, 1
 
 MODE_ASM, MODE_FRESNEL, MODE_AUTO, MODE_FRESNEL_BL = 0, 1, 2, 3
 LAYERED_F16, LAYERED_BF16, LAYERED_BF16X3 = 0, 1, 2
@@ -62,6 +64,7 @@ SIGNATURES = {
     "olx_arss_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _u32],
     "olx_shifted_asm_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _u32],
     "olx_split_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, _i, _i, _u32],
+    "olx_tilted_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _d, _i, _i, _i],
     "olx_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "olx_asm_prop_check": [_i64, _i64, _d, _d, _d, _d],
     "olx_fresnel_prop_check": [_i64, _i64, _d, _d, _d, _d],

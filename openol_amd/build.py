@@ -37,6 +37,8 @@ FLAGS = [
 FILE_FLAGS = {
     "olx_obj.hip": ["-ffp-contract=off", "-fsigned-zeros"],
     "olx_img.hip": ["-ffp-contract=off", "-fsigned-zeros"],
+    # tilted propagation: sample positions must floor() to the reference's cells
+    "olx_tilt.hip": ["-ffp-contract=off", "-fsigned-zeros"],
 }
 
 

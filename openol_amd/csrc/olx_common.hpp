@@ -94,6 +94,10 @@ int check_launch(const char* what);
 // table cache: returns a device buffer for key, creating it with bytes when absent;
 // *fresh is set when the caller must fill it
 float2* table(olx_ctx* ctx, const std::string& key, size_t count, bool* fresh, int* rc);
+// cached natural-order spectrum (scaled by 1/M) of the centred chirp K (qa x^2 + qb x),
+// x = (m - M/2) d, zero where |m - M/2| >= lim (olx_prop.hip)
+const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa, int64_t lim,
+                             int adj, int* rc, double qb = 0.0);
 
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 

@@ -733,15 +733,16 @@ static const float2* chirp_table(olx_ctx* ctx, int n, double d, double K, double
     return t;
 }
 
-// spectrum (length M, natural order, scaled by 1/M) of the centred chirp K qa x^2
-static const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa,
-                                    int64_t lim, int adj, int* rc) {
+// spectrum (length M, natural order, scaled by 1/M) of the centred chirp
+// K (qa x^2 + qb x), zero outside the window lim
+const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa, int64_t lim,
+                             int adj, int* rc, double qb) {
     bool fresh = false;
-    float2* t = table(ctx, key_of("spec", M, d, K, qa, 0, 0, lim, adj), M, &fresh, rc);
+    float2* t = table(ctx, key_of("spec", M, d, K, qa, qb, 0, lim, adj), M, &fresh, rc);
     if (!t || !fresh) return t;
     if ((*rc = ctx->scratch2.reserve(sizeof(double2) * M)) != OLX_OK) return nullptr;
     double2* tmp = (double2*)ctx->scratch2.p;
-    k_chirp_f64<<<(M + 255) / 256, 256, 0, ctx->stream>>>(tmp, M, d, K, qa, 0.0, 0.0, lim);
+    k_chirp_f64<<<(M + 255) / 256, 256, 0, ctx->stream>>>(tmp, M, d, K, qa, qb, 0.0, lim);
     k_dft_f64<<<(M + 127) / 128, 128, 0, ctx->stream>>>(t, tmp, M, 1.0 / M, adj);
     if ((*rc = check_launch("spectrum table")) != OLX_OK) return nullptr;
     return t;

@@ -137,6 +137,32 @@ def gsplitProp(u, height, width, dy, dx, lam, z, propmode, blocknum_y=2, blocknu
     return u
 
 
+BILINEAR, BICUBIC = _lib.INTERP_BILINEAR, _lib.INTERP_BICUBIC
+
+
+def _tilted(u, ny, nx, dy, dx, lam, z, angle, inp, propmode, kind, name):
+    _field(u, ny, nx)
+    if propmode not in (ASM, FRESNEL):
+        raise OlxInvalid(_lib.EINVAL, f"{name}: propmode must be ASM or FRESNEL")
+    check(_lib.load().olx_tilted_c64(_lib.ctx(u.device.index), _ptr(u), ny, nx, dy, dx, lam, z,
+                                     angle, inp, propmode, kind), name)
+    return u
+
+
+def TiltedSourceProp(u, ny, nx, dy, dx, lam, z, angle_src2dst, inp=BILINEAR, propmode=ASM):
+    """ol::TiltedSourceProp (include/olprop.h:414-526): from a source plane tilted by
+    angle_src2dst to a parallel destination plane; in place, net scale 1/M^2."""
+    return _tilted(u, ny, nx, dy, dx, lam, z, angle_src2dst, inp, propmode, _lib.TILT_SOURCE,
+                   "TiltedSourceProp")
+
+
+def TiltedReferenceProp(u, ny, nx, dy, dx, lam, z, rotate, inp=BILINEAR, propmode=ASM):
+    """ol::TiltedReferenceProp (include/olprop.h:528-594): propagation then rotation of
+    the reference plane by rotate; in place, net scale 1/M^2."""
+    return _tilted(u, ny, nx, dy, dx, lam, z, rotate, inp, propmode, _lib.TILT_REFERENCE,
+                   "TiltedReferenceProp")
+
+
 def Prop(u, ny, nx, dy, dx, lam, d, propmode=AUTO):
     """ol::Prop (include/olprop.h:656-682).  Returns the mode used (ASM, FRESNEL or
     FRESNEL_BL when AUTO falls back to the band-limited Fresnel)."""

@@ -529,6 +529,139 @@ DEFINE_SHIFTED(c128, cd, double)
 /* Dispatcher and checks: olprop.h:109-123, :209-225, :656-682                */
 /* ------------------------------------------------------------------------- */
 /* ------------------------------------------------------------------------- */
+/* Tilted-plane propagation: TiltedSourceProp olprop.h:414-526 and            */
+/* TiltedReferenceProp :528-594, with bilinear olutils.h:213-234 and bicubic  */
+/* :290-320 sampling the centred spectrum.  complex<double> data (the         */
+/* reference compiles them with PREC_T = COMPLEX_T); evanescent samples are 0 */
+/* (the reference: NaN) and a NaN sample position reads 0 (the reference: UB). */
+/* ------------------------------------------------------------------------- */
+static cd tilt_bilinear(const cd* src, double dsty, double dstx, int64_t ny, int64_t nx) {
+    if (!(dstx == dstx) || !(dsty == dsty)) return 0;
+    int64_t ms = (int64_t)floor(dstx), ns = (int64_t)floor(dsty);
+    if (0 <= ms && ms < (nx - 1) && 0 <= ns && ns < (ny - 1)) {
+        double s = dstx - ms, t = dsty - ns;
+        cd U1 = (1 - s) * src[ms + ns * nx] + s * src[(ms + 1) + ns * nx];
+        cd U2 = (1 - s) * src[ms + (ns + 1) * nx] + s * src[(ms + 1) + (ns + 1) * nx];
+        return (1 - t) * U1 + t * U2;
+    } else if (0 <= ms && ms < nx && 0 <= ns && ns < ny) {
+        return src[ms + ns * nx];
+    }
+    return 0;
+}
+
+static cd tilt_bicubic(const cd* src, double dsty, double dstx, int64_t ny, int64_t nx) {
+    const double a = -1;
+    if (!(dstx == dstx) || !(dsty == dsty)) return 0;
+    int64_t ms = (int64_t)floor(dstx), ns = (int64_t)floor(dsty);
+    if (!(0 <= ms && ms < nx && 0 <= ns && ns < ny)) return 0;
+    cd avg = 0;
+    for (int64_t j = 0; j < 4; j++)
+        for (int64_t i = 0; i < 4; i++) {
+            int64_t m = ms - 1 + i, n = ns - 1 + j;
+            if (0 <= m && m < nx && 0 <= n && n < ny) {
+                double d = sqrt((m - dstx) * (m - dstx) + (n - dsty) * (n - dsty));
+                double k;
+                if (0 <= d && d < 1) k = 1 - (a + 3) * d * d + (a + 2) * d * d * d;
+                else if (1 <= d && d < 2) k = -4 * a + 8 * a * d - 5 * a * d * d + a * d * d * d;
+                else k = 0;
+                avg += k * src[m + n * nx];
+            }
+        }
+    return avg;
+}
+
+/* centred ASM transfer function with the carrier u shift of the source form (ush = 0
+ * for the reference form's AsmTransferF) */
+static void tilt_asm_h(cd* H, int64_t ny2, int64_t nx2, double dv, double du, double lambda,
+                       double z, double ush) {
+    double tmp = 1 / (lambda * lambda);
+    for (int64_t n = 0; n < ny2; n++) {
+        double v = (n - ny2 / 2) * dv;
+        for (int64_t m = 0; m < nx2; m++) {
+            double u = (m - nx2 / 2) * du + ush;
+            double arg = tmp - u * u - v * v;
+            if (arg < 0) { H[m + n * nx2] = 0; continue; }
+            double w = sqrt(arg);
+            double phase = 2 * M_PI * w * z;
+            H[m + n * nx2] = cos(phase) + I * sin(phase);
+        }
+    }
+}
+
+/* centred spectrum of FresnelResponse (olprop.h:8-24) times the source form's tilt
+ * ramp exp(-2 pi i x a13 f0) (olprop.h:480-489); ramp = 0 for the reference form */
+static void tilt_fresnel_h(cd* H, int64_t ny2, int64_t nx2, double dy, double dx, double lambda,
+                           double z, double ramp) {
+    double tmp = 1 / (lambda * z);
+    int64_t hh = ny2 / 2, hw = nx2 / 2;
+    for (int64_t n = 0; n < ny2; n++) {
+        double y = ((double)(n - hh)) * dy;
+        for (int64_t m = 0; m < nx2; m++) {
+            double x = ((double)(m - hw)) * dx;
+            double phase = M_PI * (x * x + y * y) * tmp;
+            cd h = cos(phase) + I * sin(phase);
+            double w = (m - nx2 / 2) * dx;
+            double ph2 = -2 * M_PI * w * ramp;
+            H[m + n * nx2] = h * (cos(ph2) + I * sin(ph2));
+        }
+    }
+    fftshift_c128(H, ny2, nx2);
+    fft2d_c128(H, ny2, nx2, -1);
+    double sc = 1.0 / (double)(ny2 * nx2);
+    for (int64_t i = 0; i < ny2 * nx2; i++) H[i] *= sc;
+    fftshift_c128(H, ny2, nx2);
+}
+
+int olref_tilted_c128(double* uin, int64_t ny, int64_t nx, double dy, double dx, double lambda,
+                      double z, double angle, int interp, int mode, int kind) {
+    if (!uin || ny <= 0 || nx <= 0 || !(lambda > 0) || !(dx > 0) || !(dy > 0)) return -1;
+    cd* u = (cd*)uin;
+    int64_t ny2 = ny * 2, nx2 = nx * 2;
+    double du = 1 / (dx * nx2), dv = 1 / (dy * ny2);
+    double fsqr = 1 / (lambda * lambda), f0 = 1 / lambda;
+    double sc = 1.0 / (double)(ny2 * nx2);
+    cd* U = pad_c128(u, ny, nx, ny2, nx2);
+    fft2d_c128(U, ny2, nx2, -1);
+    fftshift_c128(U, ny2, nx2);
+    for (int64_t i = 0; i < ny2 * nx2; i++) U[i] *= sc;
+    /* source form: rotation dst -> src by -angle, carrier term -a13 f0;
+     * reference form: rotation by angle, no carrier term */
+    double rot = kind == 0 ? -angle : angle;
+    double a11 = cos(rot), a12 = 0, a13 = -sin(rot), a21 = 0, a22 = 1, a23 = 0;
+    cd* H = (cd*)malloc(sizeof(cd) * (size_t)(ny2 * nx2));
+    if (mode == 0) tilt_asm_h(H, ny2, nx2, dv, du, lambda, z, kind == 0 ? a13 * f0 : 0.0);
+    else tilt_fresnel_h(H, ny2, nx2, dy, dx, lambda, z, kind == 0 ? a13 * f0 : 0.0);
+    if (kind == 1)
+        for (int64_t i = 0; i < ny2 * nx2; i++) U[i] = U[i] * H[i];
+    cd* Up = (cd*)malloc(sizeof(cd) * (size_t)(ny2 * nx2));
+    #pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < ny2; n++)
+        for (int64_t m = 0; m < nx2; m++) {
+            double fpx = (m - nx2 / 2) * du;
+            double fpy = (n - ny2 / 2) * dv;
+            double arg = fsqr - fpx * fpx - fpy * fpy;
+            if (arg < 0) { Up[m + n * nx2] = 0; continue; }
+            double fpz = sqrt(arg);
+            double fsx = kind == 0 ? a11 * fpx + a12 * fpy + a13 * fpz - a13 * f0
+                                   : a11 * fpx + a12 * fpy + a13 * fpz;
+            double fsy = a21 * fpx + a22 * fpy + a23 * fpz;
+            double py = fsy / dv + ny2 / 2, px = fsx / du + nx2 / 2;
+            cd v = interp == 0 ? tilt_bilinear(U, py, px, ny2, nx2) : tilt_bicubic(U, py, px, ny2, nx2);
+            double J = (a11 * a22 - a12 * a21) + (a12 * a23 - a13 * a22) * fpx / fpz +
+                       (a13 * a21 - a11 * a23) * fpy / fpz;
+            Up[m + n * nx2] = v * J;
+        }
+    if (kind == 0)
+        for (int64_t i = 0; i < ny2 * nx2; i++) Up[i] = Up[i] * H[i];
+    fftshift_c128(Up, ny2, nx2);
+    fft2d_c128(Up, ny2, nx2, +1);
+    crop_c128(Up, u, ny2, nx2);
+    for (int64_t i = 0; i < ny * nx; i++) u[i] *= sc;
+    free(U); free(H); free(Up);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Split propagation: splitAsmProp olprop.h:614-651, gsplitProp golprop.h:    */
 /* 773-813 (shiftedProp golprop.h:756-770 picks ASM or ARSS with s = 1).      */
 /* ------------------------------------------------------------------------- */

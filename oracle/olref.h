@@ -89,6 +89,12 @@ int olref_shifted_asm_c128(const double* src, double* dst, int64_t ny, int64_t n
 int olref_split_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,
                          double lambda, double z, int mode, int by, int bx, unsigned flags);
 
+/* tilted-plane propagation in place on complex<double> data: kind 0 TiltedSourceProp
+ * (include/olprop.h:414-526), 1 TiltedReferenceProp (:528-594); interp 0 bilinear,
+ * 1 bicubic (include/olutils.h:213-234, 290-320); mode 0 ASM, 1 FRESNEL */
+int olref_tilted_c128(double* u, int64_t ny, int64_t nx, double dy, double dx, double lambda,
+                      double z, double angle, int interp, int mode, int kind);
+
 /* Prop dispatcher, include/olprop.h:656-682; returns chosen mode in *chosen
  * (0 ASM, 1 FRESNEL, 2 FRESNEL band-limited) */
 int olref_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,

@@ -57,6 +57,8 @@ def lib():
             "olref_shifted_asm_c128": [p, p, i64, i64, d, d, d, d, d, d, u32],
             "olref_split_prop_c64": [p, i64, i64, d, d, d, d, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, u32],
+            "olref_tilted_c128": [p, i64, i64, d, d, d, d, d, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int],
             "olref_asm_prop_check": [i64, i64, d, d, d, d],
             "olref_fresnel_prop_check": [i64, i64, d, d, d, d],
             "olref_cgh_c64": [p, p, p, p, p, i64, p, i64, i64, d, d, d, u32],
@@ -152,6 +154,17 @@ def split_prop(u, dy, dx, lam, z, by=2, bx=2, mode=0, flags=0):
     v = _cplx(u, np.complex64).copy()
     _check(lib().olref_split_prop_c64(_ptr(v), v.shape[0], v.shape[1], dy, dx, lam, z, mode, by,
                                       bx, flags), "split_prop")
+    return v
+
+
+def tilted(u, dy, dx, lam, z, angle, interp="bilinear", mode="asm", kind="source"):
+    """TiltedSourceProp (include/olprop.h:414-526) or TiltedReferenceProp (:528-594)
+    on complex128 data; returns the propagated field."""
+    v = _cplx(u, np.complex128).copy()
+    _check(lib().olref_tilted_c128(_ptr(v), v.shape[0], v.shape[1], dy, dx, lam, z, angle,
+                                   {"bilinear": 0, "bicubic": 1}[interp],
+                                   {"asm": 0, "fresnel": 1}[mode],
+                                   {"source": 0, "reference": 1}[kind]), "tilted")
     return v
 
 
