@@ -77,6 +77,51 @@ void AsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int6
     });
 }
 
+// include/molprop.h:80-101: the angular spectrum propagation the reference stages
+// through host memory in blocks (fft2d_hyblid); net scale 1/M^2 as there
+template <typename PREC_T = double, typename COMPLEX_T>
+void mAsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int64_t width,
+              PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    detail::on_device(u, height * width, [&](void* d) {
+        detail::check(olx_asm_c64(detail::ctx(), d, height, width, dy, dx, lambda, z,
+                                  OLX_EXTRA_INV_M2), "mAsmProp");
+    });
+}
+
+// include/molprop.h:16-78: unnormalised 2-D FFT of a host array (forward / inverse),
+// src -> dst (dst allocated when empty, may be src).  The reference moves blocks of
+// blocknum_y rows / blocknum_x columns through device memory; the whole array fits
+// in HBM here (sides: powers of two in [8, 16384]), so the block counts only keep
+// the signature.
+namespace detail {
+template <typename PREC_T>
+void hybrid_fft(std::unique_ptr<std::complex<PREC_T>[]>& src,
+                std::unique_ptr<std::complex<PREC_T>[]>& dst, int64_t ny, int64_t nx, int sign) {
+    static_assert(std::is_same<PREC_T, float>::value, "fft2d_hyblid: complex<float> data");
+    if (&src != &dst) {
+        if (!dst) dst = std::make_unique<std::complex<PREC_T>[]>((size_t)(ny * nx));
+        for (int64_t i = 0; i < ny * nx; i++) dst[i] = src[i];
+    }
+    on_device(dst, ny * nx, [&](void* d) { check(olx_fft2d_c64(ctx(), d, ny, nx, sign), "fft2d_hyblid"); });
+}
+}  // namespace detail
+template <typename PREC_T>
+void fft2d_hyblid(std::unique_ptr<std::complex<PREC_T>[]>& src,
+                  std::unique_ptr<std::complex<PREC_T>[]>& dst, int64_t ny, int64_t nx,
+                  int blocknum_y = 2, int blocknum_x = 2) {
+    (void)blocknum_y;
+    (void)blocknum_x;
+    detail::hybrid_fft(src, dst, ny, nx, -1);
+}
+template <typename PREC_T>
+void ifft2d_hyblid(std::unique_ptr<std::complex<PREC_T>[]>& src,
+                   std::unique_ptr<std::complex<PREC_T>[]>& dst, int64_t ny, int64_t nx,
+                   int blocknum_y = 2, int blocknum_x = 2) {
+    (void)blocknum_y;
+    (void)blocknum_x;
+    detail::hybrid_fft(src, dst, ny, nx, +1);
+}
+
 // include/olprop.h:125-148 (circconv / M^2)
 template <typename PREC_T = double, typename COMPLEX_T>
 void FresnelProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, size_t height, size_t width,

@@ -40,6 +40,8 @@ extern "C" {
 #define OLX_FRESNEL_BL_GPU 0x20u /* Fresnel window, x-limit on both axes (include/golprop.h:223-238) */
 #define OLX_ARSS_CPU 0x800u      /* ARSS with the CPU shiftedFresnelProp semantics: no windows and
                                     net scale circconv / M^4 (include/olprop.h:338-408) */
+#define OLX_EXTRA_INV_M2 0x2000u /* ASM: multiply the result by one more 1/M^2, the net scale of
+                                    mAsmProp (include/molprop.h:80-101) */
 #define OLX_SHIFTED_CPU 0x1000u  /* shifted ASM with the CPU shiftedAsmProp semantics: the shifted
                                     band-limit window and net scale 1/M^2 (include/olprop.h:250-330) */
 

@@ -968,6 +968,7 @@ extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, doub
     a.inv_l2 = 1 / (lambda * lambda);
     a.zs = a.adjoint ? -z : z;
     a.bandlimit = (flags & OLX_BANDLIMIT) ? 1 : 0;
+    if (flags & OLX_EXTRA_INV_M2) a.extra_scale = 1.0 / ((double)nx2 * (double)ny2);
     if (a.bandlimit) {
         // shiftedAsmTransferF at ox = oy = 0 (include/olprop.h:258-270): uw = ulp + ulm
         const double ulm = 1.0 / std::sqrt(std::pow(0.0 - 0.5f / a.du, -2) * z * z + 1.0f) / lambda;

@@ -50,6 +50,16 @@ def AsmProp(u, height, width, dy, dx, lam, z, bandlimit=False, adjoint=False):
     return u
 
 
+def mAsmProp(u, height, width, dy, dx, lam, z):
+    """ol::mAsmProp (include/molprop.h:80-101): the angular spectrum propagation whose
+    transforms the reference stages through host memory, net scale 1/M^2 (it scales
+    before and after the inverse transform).  The field stays in HBM here."""
+    _field(u, height, width)
+    check(_lib.load().olx_asm_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z,
+                                  _lib.EXTRA_INV_M2), "mAsmProp")
+    return u
+
+
 def FresnelProp(u, height, width, dy, dx, lam, z, adjoint=False):
     """ol::FresnelProp (include/olprop.h:125-148): circconv(pad u, h) / M^2, in place."""
     _field(u, height, width)

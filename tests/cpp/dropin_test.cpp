@@ -83,6 +83,23 @@ int main(int argc, char** argv) {
             u = fresh();
             ol::gsplitProp(u, n, n, pf, pf, lf, 0.05f, ol::FRESNEL, 2, 4);
             save(dir + "/gsplit_fresnel.bin", u.get(), (size_t)n * n);
+            u = fresh();
+            ol::mAsmProp(u, n, n, p, p, lambda, 0.05);
+            save(dir + "/masm.bin", u.get(), (size_t)n * n);
+            u = fresh();
+            std::unique_ptr<std::complex<float>[]> F;
+            ol::fft2d_hyblid(u, F, n, n);
+            save(dir + "/hfft.bin", F.get(), (size_t)n * n);
+            ol::ifft2d_hyblid(F, F, n, n);
+            save(dir + "/hifft.bin", F.get(), (size_t)n * n);
+            // tilted planes (the reference's tests/tiltedProp_test.cpp call)
+            u = fresh();
+            ol::TiltedReferenceProp<float>(u, n, n, pf, pf, lf, 0.004f, (float)(M_PI / 180 * 2),
+                                           ol::BICUBIC, ol::ASM);
+            save(dir + "/tilted_ref.bin", u.get(), (size_t)n * n);
+            u = fresh();
+            ol::TiltedSourceProp<float>(u, n, n, pf, pf, lf, 0.004f, (float)(M_PI / 180 * -3));
+            save(dir + "/tilted_src.bin", u.get(), (size_t)n * n);
         }
 
         // CGH on a host point cloud (ol::Object<double>)

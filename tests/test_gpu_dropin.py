@@ -47,6 +47,16 @@ def test_dropin_program_matches_oracle(tmp_path):
                                                  olref.SHIFT_WINDOW | olref.SHIFT_CPU_SCALE)) < 1e-5
     assert rel(rd("gsplit"), olref.split_prop(u0, pf, pf, lf, f32(0.002), 2, 2, 0, 0)) < 1e-5
     assert rel(rd("gsplit_fresnel"), olref.split_prop(u0, pf, pf, lf, f32(0.05), 2, 4, 1, 0)) < 1e-5
+    assert rel(rd("masm"), olref.asm_prop(u0, P, P, LAM, 0.05) / (4 * n * n)) < 1e-5
+    F = np.fft.fft2(u0.astype(np.complex128))
+    assert rel(rd("hfft"), F) < 1e-5
+    assert rel(rd("hifft"), u0 * (n * n)) < 1e-5
+    ang = float(np.float32(np.pi / 180 * 2))
+    assert rel(rd("tilted_ref"), olref.tilted(u0, pf, pf, lf, f32(0.004), ang, "bicubic", "asm",
+                                              "reference")) < 1e-5
+    ang = float(np.float32(np.pi / 180 * -3))
+    assert rel(rd("tilted_src"), olref.tilted(u0, pf, pf, lf, f32(0.004), ang, "bilinear", "asm",
+                                              "source")) < 1e-5
     assert rel(rd("gshifted"), olref.shifted_asm(u0, P, P, LAM, 0.004, -0.25 * n * P, 0.5 * n * P)) < 1e-5
     assert rel(rd("shiftedprop_fresnel"), olref.arss_prop(u0, P, P, LAM, 0.05, 1.0, 1e-4, 0.0)) < 1e-5
     cgh = olref.cgh(pts["x"], pts["y"], pts["z"], n, n, P, P, LAM, dtype=np.complex128)
