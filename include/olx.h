@@ -280,6 +280,48 @@ int olx_bmp_write(const char* path, const uint8_t* img, int ny, int nx);
  * ny rows of nx bytes, each followed by nx % 4 skipped bytes */
 int olx_bmp_read(const char* path, uint8_t* img, int ny, int nx);
 
+/* ---- optical elements (include/Diffuser.h, include/Lensarry.h) ---------- */
+/* hologram step (ol::HOLOGRAMSTEP, include/oldefine.h:11-13) */
+#define OLX_STEP_RECORD 0
+#define OLX_STEP_PLAY 1
+/* Diffuser::random_phase, CPU form (include/Diffuser.h:191-215): pixel (h, w), in
+ * row-major order, takes two std::mt19937(seed) draws (a 53-bit value v) and
+ * multiplies u at ((h + oy) mod ny, (w + ox) mod nx) by exp(i phase), phase =
+ * range (v - 0.5) (PLAY) or its negative (RECORD), rounded to float.  The stream is
+ * generated on the host (it is sequential) and applied on the device; synchronises
+ * the context stream. */
+int olx_diffuser_random_phase_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, int step,
+                                  uint32_t seed, float range, int oy, int ox);
+/* the GPU diffuser kernels (include/Diffuser.h:14-129) on a field, given the uniform
+ * array d_b the reference takes from cuRAND: exp(i (b - 0.5) scalar) per pixel
+ * (PHASE, KernelMultPhase), per column (XPHASE) or row (YPHASE); pi where b > 0.5
+ * (BITRANDOM); per blocksize^2 block (BLOCK) plus a (x + y) / blocksize ramp
+ * (BLOCKSTEP); exp(i F32_PI) on the even checkerboard (BITPHASE, b unused). */
+#define OLX_DIFFUSER_PHASE 0
+#define OLX_DIFFUSER_XPHASE 1
+#define OLX_DIFFUSER_YPHASE 2
+#define OLX_DIFFUSER_BITRANDOM 3
+#define OLX_DIFFUSER_BLOCK 4
+#define OLX_DIFFUSER_BLOCKSTEP 5
+#define OLX_DIFFUSER_BITPHASE 6
+int olx_diffuser_apply_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, const float* d_b,
+                           float scalar, int kind, int blocksize);
+/* n uniform floats in (0, 1] from a Philox4x32-10 counter stream keyed by seed (no
+ * reference counterpart: cuRAND's MT19937 ordering is its own) */
+int olx_uniform_f32(olx_ctx* ctx, float* d_out, int64_t n, uint64_t seed);
+/* lens array of Lensarry (include/Lensarry.h:84-311): square lenslets of
+ * round(blockwidth / pitch) pixels tiling the field, each the Fresnel response at
+ * z = -f (PLAY) or +f (RECORD) computed in float, shifted by (oy, ox) pixels;
+ * p_limit > 0 windows each lenslet (FresnelResponseBandLimit, include/olprop.h:56-84);
+ * OLX_LENS_CIRCLE: LensarryCircle's hexagonal rows of circular lenslets on a field
+ * of ones (:383-447).  The field is multiplied by the lens array (mult_complex), or
+ * overwritten by it with OLX_LENS_WRITE (gen_complex). */
+#define OLX_LENS_CIRCLE 0x1u
+#define OLX_LENS_WRITE 0x2u
+int olx_lens_array_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, float pitch,
+                       float blockwidth, float f, float lambda, int step, int oy, int ox,
+                       float p_limit, uint32_t flags);
+
 /* ---- instrumentation ---------------------------------------------------- */
 /* per-kernel device time measured with HIP events on the context stream */
 int olx_ctx_enable_timing(olx_ctx* ctx, int on);

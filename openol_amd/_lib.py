@@ -24,6 +24,8 @@ ARSS_CPU = 0x800
 SHIFTED_CPU = 0x1000
 EXTRA_INV_M2 = 0x2000
 TILT_SOURCE, TILT_REFERENCE = 0, 1
+STEP_RECORD, STEP_PLAY = 0, 1
+LENS_CIRCLE, LENS_WRITE = 0x1, 0x2
 INTERP_BILINEAR, INTERP_BICUBIC = 0, 1
 
 MODE_ASM, MODE_FRESNEL, MODE_AUTO, MODE_FRESNEL_BL = 0, 1, 2, 3
@@ -66,6 +68,11 @@ SIGNATURES = {
     "olx_shifted_asm_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _u32],
     "olx_split_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, _i, _i, _u32],
     "olx_tilted_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _d, _i, _i, _i],
+    "olx_diffuser_random_phase_c64": [_p, _p, _i64, _i64, _i, _u32, ctypes.c_float, _i, _i],
+    "olx_diffuser_apply_c64": [_p, _p, _i64, _i64, _p, ctypes.c_float, _i, _i],
+    "olx_uniform_f32": [_p, _p, _i64, ctypes.c_uint64],
+    "olx_lens_array_c64": [_p, _p, _i64, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_float, _i, _i, _i, ctypes.c_float, _u32],
     "olx_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "olx_asm_prop_check": [_i64, _i64, _d, _d, _d, _d],
     "olx_fresnel_prop_check": [_i64, _i64, _d, _d, _d, _d],

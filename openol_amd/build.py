@@ -39,6 +39,8 @@ FILE_FLAGS = {
     "olx_img.hip": ["-ffp-contract=off", "-fsigned-zeros"],
     # tilted propagation: sample positions must floor() to the reference's cells
     "olx_tilt.hip": ["-ffp-contract=off", "-fsigned-zeros"],
+    # optical elements: float phases and products as the reference rounds them
+    "olx_optics.hip": ["-ffp-contract=off", "-fsigned-zeros"],
 }
 
 

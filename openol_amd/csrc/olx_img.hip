@@ -144,10 +144,20 @@ template <> struct C2<double> { using V = double2; };
 __device__ inline float sq_sum(float2 u) { return __fadd_rn(__fmul_rn(u.x, u.x), __fmul_rn(u.y, u.y)); }
 __device__ inline double sq_sum(double2 u) { return __dadd_rn(__dmul_rn(u.x, u.x), __dmul_rn(u.y, u.y)); }
 
-// the per-element value of complex2img / complex2real: the products and their sum
-// are taken in the element type as the reference's complex<T> arithmetic does, the
-// square root and atan2 in FP64 (unqualified sqrt / atan2 on float arguments name
-// the double functions under the reference's includes)
+// the per-element value of complex2img / complex2real.  The reference's headers
+// include <math.h> before these templates, so unqualified sqrt / atan2 on float
+// arguments are the float overloads: for complex<float> the modulus is the float
+// square root of the float sum of float products, the phase the float angle; for
+// complex<double> both are FP64.  The float angle here is the FP64 angle rounded to
+// float (the correctly rounded atan2f, up to an FP64 ulp at a float tie).
+// the correctly rounded float root: the FP64 root rounded to float (for a square
+// root the double rounding is exact; the hardware FP32 root is not correctly rounded)
+__device__ inline float sqrt_f32_rn(float s) { return (float)sqrt((double)s); }
+__device__ inline double c2v_amp(float2 u) { return (double)sqrt_f32_rn(sq_sum(u)); }
+__device__ inline double c2v_amp(double2 u) { return sqrt(sq_sum(u)); }
+__device__ inline double c2v_phase(float2 u) { return (double)(float)atan2((double)u.y, (double)u.x); }
+__device__ inline double c2v_phase(double2 u) { return atan2(u.y, u.x); }
+
 template <typename V>
 __device__ inline double c2v(V u, int mode) {
     switch (mode) {
@@ -157,9 +167,9 @@ __device__ inline double c2v(V u, int mode) {
         case OLX_IMG_IMAG:
             return (double)u.y;
         case OLX_IMG_AMP:
-            return sqrt((double)sq_sum(u));
+            return c2v_amp(u);
         default:
-            return atan2((double)u.y, (double)u.x);
+            return c2v_phase(u);
     }
 }
 
@@ -187,8 +197,13 @@ __global__ void __launch_bounds__(QT) k_img_bounds(double* __restrict__ mm, int 
     fold_partials(mm, nb, mn, mx);
     if (threadIdx.x != 0) return;
     if (mode == OLX_IMG_AMP) {
-        mn = sqrt(mn);
-        mx = sqrt(mx);
+        if (sizeof(T) == 4) {  // the float root of the (float) squared modulus
+            mn = (double)sqrt_f32_rn((float)mn);
+            mx = (double)sqrt_f32_rn((float)mx);
+        } else {
+            mn = sqrt(mn);
+            mx = sqrt(mx);
+        }
     }
     if (mode == OLX_IMG_REAL_LEGACY) mx = fmax(mx, 0.0);
     const double v0 = c2v(u[0], mode);

@@ -537,3 +537,119 @@ def bmpread(path, ny, nx):
     a = np.zeros((ny, nx), dtype=np.uint8)
     check(_lib.load().olx_bmp_read(os.fsencode(path), a.ctypes.data, ny, nx), "bmpread")
     return a
+
+
+# ------------------------------------------------------------------ optical elements
+RECORD, PLAY = _lib.STEP_RECORD, _lib.STEP_PLAY
+_DIFFUSER_KIND = {"phase": 0, "xphase": 1, "yphase": 2, "bitrandom": 3, "block": 4, "blockstep": 5,
+                  "bitphase": 6}
+
+
+def uniform(n, seed=1, device=None):
+    """n uniform floats in (0, 1] on the device (Philox4x32-10 stream keyed by seed)."""
+    dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    check(_lib.load().olx_uniform_f32(_lib.ctx(dev.index), _ptr(out), n, seed), "uniform")
+    return out
+
+
+class Diffuser:
+    """ol::Diffuser (include/Diffuser.h:172-326) on complex64 device fields."""
+
+    def __init__(self, height, width, pitch=None):
+        self.height, self.width, self.pitch = height, width, pitch
+
+    def random_phase(self, u, step, seed=1, range=2 * np.pi, oy=0, ox=0):
+        """The CPU form (include/Diffuser.h:191-215): the std::mt19937(seed) stream in
+        pixel order, phases range (v - 0.5) (PLAY) or their negative (RECORD)."""
+        _field(u, self.height, self.width)
+        check(_lib.load().olx_diffuser_random_phase_c64(_lib.ctx(u.device.index), _ptr(u), self.height,
+                                                        self.width, step, seed, range, oy, ox),
+              "random_phase")
+        return u
+
+    def _apply(self, u, kind, b, scalar, blocksize=1):
+        _field(u, self.height, self.width)
+        if b.dtype != torch.float32 or not b.is_contiguous():
+            raise OlxInvalid(_lib.EINVAL, "uniforms must be a contiguous float32 tensor")
+        check(_lib.load().olx_diffuser_apply_c64(_lib.ctx(u.device.index), _ptr(u), self.height,
+                                                 self.width, _ptr(b), scalar, _DIFFUSER_KIND[kind],
+                                                 blocksize), "diffuser")
+        return u
+
+    def _uniforms(self, n, seed, u):
+        return uniform(n, seed, u.device.index)
+
+    def grandom_phase(self, u, step, seed=1, range=2 * np.pi, b=None):
+        """The GPU form (include/Diffuser.h:218-238, KernelMultPhase) with uniforms b
+        (default: the Philox stream for seed).  The reference's shift of the uniforms
+        is applied only to a quarter of the array (its grid is halved on both axes,
+        include/Diffuser.h:151-158); no shift is offered here."""
+        b = self._uniforms(self.height * self.width, seed, u) if b is None else b
+        return self._apply(u, "phase", b, range if step == PLAY else -range)
+
+    def random_xphase(self, u, step, seed=1, range=2 * np.pi, b=None):
+        """include/Diffuser.h:240-259 (one value per column)."""
+        b = self._uniforms(self.width, seed, u) if b is None else b
+        return self._apply(u, "xphase", b, range if step == PLAY else -range)
+
+    def random_blockphase(self, u, step, seed=1, range=2 * np.pi, blocksize=1, b=None):
+        """include/Diffuser.h:261-279 (one value per blocksize^2 block)."""
+        b = self._uniforms(self.height * self.width // (blocksize * blocksize), seed, u) if b is None else b
+        return self._apply(u, "block", b, range if step == PLAY else -range, blocksize)
+
+    def random_blockstepphase(self, u, step, seed=1, range=2 * np.pi, blocksize=4, b=None):
+        """include/Diffuser.h:281-299 (per block, plus a (x + y) / blocksize ramp)."""
+        b = self._uniforms(self.height * self.width // (blocksize * blocksize), seed, u) if b is None else b
+        return self._apply(u, "blockstep", b, range if step == PLAY else -range, blocksize)
+
+    def bitphase(self, u):
+        """include/Diffuser.h:303-308: exp(i F32_PI) on the even checkerboard."""
+        dummy = torch.empty(1, dtype=torch.float32, device=u.device)
+        return self._apply(u, "bitphase", dummy, 0.0)
+
+
+class Lensarry:
+    """ol::Lensarry (include/Lensarry.h:84-311): square lenslets of blockwidth over a
+    height x width field of pitch; the lens field is computed per pixel on the device."""
+    circle = False
+
+    def __init__(self, height, width, pitch, blockwidth, f, lam):
+        self.height, self.width = height, width
+        self.pitch, self.blockwidth, self.f, self.lam = pitch, blockwidth, f, lam
+        self.blockpx = int(np.round(np.float32(blockwidth) / np.float32(pitch)))
+
+    def _run(self, u, step, oy, ox, p_limit, write):
+        flags = (_lib.LENS_CIRCLE if self.circle else 0) | (_lib.LENS_WRITE if write else 0)
+        check(_lib.load().olx_lens_array_c64(_lib.ctx(u.device.index), _ptr(u), self.height, self.width,
+                                             self.pitch, self.blockwidth, self.f, self.lam, step, oy,
+                                             ox, p_limit, flags), "Lensarry")
+        return u
+
+    def gen_complex(self, step=PLAY, oy=0, ox=0, device=None):
+        """The lens field (gen_complex, include/Lensarry.h:254-265) as a new tensor."""
+        dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        u = torch.empty((self.height, self.width), dtype=torch.complex64, device=dev)
+        return self._run(u, step, oy, ox, 0.0, True)
+
+    def gen_complex_limit(self, p_limit, step=PLAY, oy=0, ox=0, device=None):
+        """include/Lensarry.h:300-311: band-limited lenslets."""
+        dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        u = torch.empty((self.height, self.width), dtype=torch.complex64, device=dev)
+        return self._run(u, step, oy, ox, p_limit, True)
+
+    def mult_complex(self, u, step=PLAY, oy=0, ox=0):
+        """include/Lensarry.h:266-271: u times the lens field, in place."""
+        _field(u, self.height, self.width)
+        return self._run(u, step, oy, ox, 0.0, False)
+
+    def mult_complex_limit(self, u, p_limit, step=PLAY, oy=0, ox=0):
+        """include/Lensarry.h:312-317."""
+        _field(u, self.height, self.width)
+        return self._run(u, step, oy, ox, p_limit, False)
+
+
+class LensarryCircle(Lensarry):
+    """ol::LensarryCircle (include/Lensarry.h:383-447): hexagonal rows of circular
+    lenslets, ones between them."""
+    circle = True

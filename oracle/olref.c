@@ -662,6 +662,176 @@ int olref_tilted_c128(double* uin, int64_t ny, int64_t nx, double dy, double dx,
 }
 
 /* ------------------------------------------------------------------------- */
+/* Optical elements: Diffuser include/Diffuser.h, Lensarry include/Lensarry.h */
+/* ------------------------------------------------------------------------- */
+/* MT19937 (Matsumoto & Nishimura 1998), the engine std::mt19937 specifies */
+typedef struct { uint32_t mt[624]; int i; } mt19937;
+static void mt_seed(mt19937* g, uint32_t s) {
+    g->mt[0] = s;
+    for (int i = 1; i < 624; i++)
+        g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+    g->i = 624;
+}
+static uint32_t mt_next(mt19937* g) {
+    if (g->i >= 624) {
+        for (int k = 0; k < 624; k++) {
+            uint32_t y = (g->mt[k] & 0x80000000u) | (g->mt[(k + 1) % 624] & 0x7fffffffu);
+            g->mt[k] = g->mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->i = 0;
+    }
+    uint32_t y = g->mt[g->i++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+uint32_t olref_mt19937_nth(uint32_t seed, int64_t n) {
+    mt19937 g;
+    mt_seed(&g, seed);
+    uint32_t v = 0;
+    for (int64_t k = 0; k < n; k++) v = mt_next(&g);
+    return v;
+}
+/* two draws -> a double in [0, 1) with 53 random bits (Diffuser.h:205-207) */
+static double mt_res53(mt19937* g) {
+    int a = (int)(mt_next(g) >> 5);
+    int b = (int)(mt_next(g) >> 6);
+    return (a * 67108864.0 + b) / 9007199254740992.0;
+}
+
+/* Diffuser::random_phase (CPU), include/Diffuser.h:191-215: pixel (h, w) in row-major
+ * order draws a value and multiplies u at ((h + oy) mod H, (w + ox) mod W) by
+ * exp(i phase), phase = +-range (value - 0.5) rounded to float, cos / sin in float;
+ * step 1 = PLAY (+), 0 = RECORD (-).  complex<float> data. */
+int olref_diffuser_random_phase(float* uin, int64_t height, int64_t width, int step,
+                                uint32_t seed, float range, int oy, int ox) {
+    cf* u = (cf*)uin;
+    mt19937 g;
+    mt_seed(&g, seed);
+    for (int h = 0; h < height; h++)
+        for (int w = 0; w < width; w++) {
+            int _h = (int)((h + oy) % height), _w = (int)((w + ox) % width);
+            _h = (_h >= 0) ? _h : _h + (int)height;
+            _w = (_w >= 0) ? _w : _w + (int)width;
+            double value = mt_res53(&g);
+            float phase = step == 1 ? (float)(range * (value - 0.5f))
+                                    : (float)(-range * (value - 0.5f));
+            /* cos / sin of a float: the float overloads (<math.h> comes first in the
+             * reference's includes) */
+            cf c = cosf(phase) + I * sinf(phase);
+            cf a = u[_h * width + _w];
+            float re = crealf(a) * crealf(c) - cimagf(a) * cimagf(c);
+            float im = crealf(a) * cimagf(c) + cimagf(a) * crealf(c);
+            u[_h * width + _w] = re + I * im;
+        }
+    return 0;
+}
+
+/* the GPU diffuser kernels (include/Diffuser.h:14-129) given the uniform array b:
+ * kind 0 KernelMultPhase (b[adr]), 1 KernelMultLinePhase (b[x]), 2 KernelMultLineYPhase
+ * (b[y]), 3 KernelbitRandomPhase (pi if b > 0.5), 4 KernelBlockPhase, 5
+ * KernelBlockStepPhase, 6 Kernelbitphase (checkerboard of exp(i F32_PI)); float math */
+int olref_diffuser_apply(float* uin, int64_t height, int64_t width, const float* b, float scalar,
+                         int kind, int blocksize) {
+    cf* u = (cf*)uin;
+    const float F32PI = 3.14159265f;
+    for (int64_t y = 0; y < height; y++)
+        for (int64_t x = 0; x < width; x++) {
+            float phase;
+            if (kind == 0) phase = (b[x + y * width] - 0.5f) * scalar;
+            else if (kind == 1) phase = (b[x] - 0.5f) * scalar;
+            else if (kind == 2) phase = (b[y] - 0.5f) * scalar;
+            else if (kind == 3) phase = (b[x + y * width] > 0.5f) ? F32PI : 0;
+            else if (kind == 4 || kind == 5) {
+                int64_t bw = x / blocksize, bh = y / blocksize;
+                int64_t root = bw + bh * width / blocksize;
+                float stp = kind == 5 ? ((float)(x % blocksize) + (float)(y % blocksize)) / (float)blocksize : 0.0f;
+                phase = kind == 5 ? (b[root] + stp - 0.5f) * scalar : (b[root] - 0.5f) * scalar;
+            } else {
+                if ((x % 2) != (y % 2)) continue;
+                phase = F32PI;
+            }
+            cf c = cosf(phase) + I * sinf(phase);
+            cf a = u[x + y * width];
+            float re = crealf(a) * crealf(c) - cimagf(a) * cimagf(c);
+            float im = crealf(a) * cimagf(c) + cimagf(a) * crealf(c);
+            u[x + y * width] = re + I * im;
+        }
+    return 0;
+}
+
+/* FresnelResponse on a blockpx^2 lens (include/olprop.h:8-24 with PREC_T = float:
+ * float coordinates, the phase M_PI (x^2 + y^2) tmp formed in double and stored to
+ * float, cos / sin in float) or its p_limit window (:56-84, p_limit > 0) */
+static void lens_block(cf* ub, int bp, float pitch, float lambda, float z, float p_limit) {
+    float tmp = 1 / (lambda * z);
+    int64_t hh = bp / 2, hw = bp / 2;
+    int64_t wlim = INT_MAX, hlim = INT_MAX;
+    if (p_limit > 0) {
+        float xlim = fabsf((float)(lambda * z * 0.5 / p_limit));
+        wlim = (int64_t)floorf(xlim / pitch);
+        hlim = wlim;
+    }
+    for (int64_t n = 0; n < bp; n++) {
+        float y = ((float)(n - hh)) * pitch;
+        for (int64_t m = 0; m < bp; m++) {
+            float x = ((float)(m - hw)) * pitch;
+            if (llabs(m - hw) < wlim && llabs(n - hh) < hlim) {
+                float phase = (float)(M_PI * (x * x + y * y) * tmp);
+                ub[n * bp + m] = cosf(phase) + I * sinf(phase);
+            } else ub[n * bp + m] = 0;
+        }
+    }
+}
+
+/* Lensarry::gen_complex (include/Lensarry.h:254-265) / gen_complex_limit (:300-311)
+ * with duplicate (:137-160), or LensarryCircle::duplicate (:406-447) when circle;
+ * step 1 = PLAY, 0 = RECORD (focal length negated).  Writes the lens field. */
+int olref_lens_array(float* out, int64_t height, int64_t width, float pitch, float blockwidth,
+                     float f, float lambda, int step, int on, int om, float p_limit, int circle) {
+    cf* u = (cf*)out;
+    int bp = (int)roundf(blockwidth / pitch);
+    float tmpf = step == 0 ? -f : f;
+    cf* ub = (cf*)malloc(sizeof(cf) * (size_t)bp * bp);
+    lens_block(ub, bp, pitch, lambda, -tmpf, p_limit);
+    if (!circle) {
+        for (int64_t i = 0; i < height * width; i++) u[i] = 0;
+        for (int bn = 0; bn < height; bn += bp)
+            for (int bm = 0; bm < width; bm += bp)
+                for (int ln = 0; ln < bp; ln++)
+                    for (int lm = 0; lm < bp; lm++) {
+                        int ms = (lm - om) % bp, ns = (ln - on) % bp;
+                        ms = ms >= 0 ? ms : ms + bp;
+                        ns = ns >= 0 ? ns : ns + bp;
+                        int m = bm + lm, n = bn + ln;
+                        if (m >= width || n >= height) break;
+                        u[m + n * width] = ub[ms + ns * bp];
+                    }
+    } else {
+        float stepy = blockwidth * sqrtf(3.0f) * 0.5f, stepx = blockwidth, r = blockwidth * 0.5f;
+        int stepn = (int)roundf(stepy / pitch), stepm = (int)roundf(stepx / pitch);
+        for (int64_t i = 0; i < height * width; i++) u[i] = 1.0f;
+        for (int bn = 0; bn < height; bn += stepn)
+            for (int bm = 0; bm < width; bm += stepm)
+                for (int ln = 0; ln < bp; ln++)
+                    for (int lm = 0; lm < bp; lm++) {
+                        int ms = (lm - om + (bn / stepn % 2) * bp / 2) % bp;
+                        int ns = (ln - on) % bp;
+                        ms = ms >= 0 ? ms : ms + bp;
+                        ns = ns >= 0 ? ns : ns + bp;
+                        int m = bm + lm, n = bn + ln;
+                        if (m >= width || n >= height) break;
+                        float dxr = ms * pitch - blockwidth * 0.5f, dyr = ns * pitch - blockwidth * 0.5f;
+                        if (powf(dxr, 2) + powf(dyr, 2) <= powf(r, 2.0f)) u[m + n * width] = ub[ms + ns * bp];
+                    }
+    }
+    free(ub);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Split propagation: splitAsmProp olprop.h:614-651, gsplitProp golprop.h:    */
 /* 773-813 (shiftedProp golprop.h:756-770 picks ASM or ARSS with s = 1).      */
 /* ------------------------------------------------------------------------- */

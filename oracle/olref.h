@@ -95,6 +95,18 @@ int olref_split_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,
 int olref_tilted_c128(double* u, int64_t ny, int64_t nx, double dy, double dx, double lambda,
                       double z, double angle, int interp, int mode, int kind);
 
+/* MT19937 (the std::mt19937 engine): the n-th output for a seed (n >= 1) */
+uint32_t olref_mt19937_nth(uint32_t seed, int64_t n);
+/* Diffuser::random_phase, CPU form (include/Diffuser.h:191-215), complex<float> in place */
+int olref_diffuser_random_phase(float* u, int64_t height, int64_t width, int step,
+                                uint32_t seed, float range, int oy, int ox);
+/* the GPU diffuser kernels (include/Diffuser.h:14-129) given the uniform array b */
+int olref_diffuser_apply(float* u, int64_t height, int64_t width, const float* b, float scalar,
+                         int kind, int blocksize);
+/* Lensarry / LensarryCircle lens field (include/Lensarry.h:137-311, 406-447) */
+int olref_lens_array(float* out, int64_t height, int64_t width, float pitch, float blockwidth,
+                     float f, float lambda, int step, int on, int om, float p_limit, int circle);
+
 /* Prop dispatcher, include/olprop.h:656-682; returns chosen mode in *chosen
  * (0 ASM, 1 FRESNEL, 2 FRESNEL band-limited) */
 int olref_prop_c64(float* u, int64_t ny, int64_t nx, double dy, double dx,

@@ -59,6 +59,12 @@ def lib():
                                      ctypes.c_int, u32],
             "olref_tilted_c128": [p, i64, i64, d, d, d, d, d, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int],
+            "olref_diffuser_random_phase": [p, i64, i64, ctypes.c_int, u32, ctypes.c_float,
+                                            ctypes.c_int, ctypes.c_int],
+            "olref_diffuser_apply": [p, i64, i64, p, ctypes.c_float, ctypes.c_int, ctypes.c_int],
+            "olref_lens_array": [p, i64, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_float, ctypes.c_int],
             "olref_asm_prop_check": [i64, i64, d, d, d, d],
             "olref_fresnel_prop_check": [i64, i64, d, d, d, d],
             "olref_cgh_c64": [p, p, p, p, p, i64, p, i64, i64, d, d, d, u32],
@@ -73,6 +79,8 @@ def lib():
             f.argtypes = args
             f.restype = ctypes.c_int
         L.olref_wrp_plane_z.restype = ctypes.c_double
+        L.olref_mt19937_nth.argtypes = [u32, i64]
+        L.olref_mt19937_nth.restype = ctypes.c_uint32
         _lib = L
     return _lib
 
@@ -166,6 +174,42 @@ def tilted(u, dy, dx, lam, z, angle, interp="bilinear", mode="asm", kind="source
                                    {"asm": 0, "fresnel": 1}[mode],
                                    {"source": 0, "reference": 1}[kind]), "tilted")
     return v
+
+
+PLAY, RECORD = 1, 0
+DIFFUSER_KINDS = {"phase": 0, "xphase": 1, "yphase": 2, "bitrandom": 3, "block": 4, "blockstep": 5,
+                  "bitphase": 6}
+
+
+def mt19937_nth(seed, n):
+    """n-th output of MT19937 seeded with seed (std::mt19937)."""
+    return int(lib().olref_mt19937_nth(seed, n))
+
+
+def diffuser_random_phase(u, step, seed=1, rng=2 * np.pi, oy=0, ox=0):
+    """Diffuser::random_phase, CPU form (include/Diffuser.h:191-215)."""
+    v = _cplx(u, np.complex64).copy()
+    _check(lib().olref_diffuser_random_phase(_ptr(v), v.shape[0], v.shape[1], step, seed, rng, oy,
+                                             ox), "diffuser_random_phase")
+    return v
+
+
+def diffuser_apply(u, b, scalar, kind, blocksize=1):
+    """The GPU diffuser kernels (include/Diffuser.h:14-129) given uniforms b."""
+    v = _cplx(u, np.complex64).copy()
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    _check(lib().olref_diffuser_apply(_ptr(v), v.shape[0], v.shape[1], _ptr(b), scalar,
+                                      DIFFUSER_KINDS[kind], blocksize), "diffuser_apply")
+    return v
+
+
+def lens_array(height, width, pitch, blockwidth, f, lam, step=PLAY, on=0, om=0, p_limit=0.0,
+               circle=False):
+    """Lensarry::gen_complex(_limit) / LensarryCircle lens field (include/Lensarry.h)."""
+    out = np.empty((height, width), dtype=np.complex64)
+    _check(lib().olref_lens_array(_ptr(out), height, width, pitch, blockwidth, f, lam, step, on, om,
+                                  p_limit, 1 if circle else 0), "lens_array")
+    return out
 
 
 def prop(u, dy, dx, lam, z, mode=2):
@@ -362,18 +406,25 @@ IMG_PHASE, IMG_REAL, IMG_IMAG, IMG_AMP, IMG_REAL_LEGACY = 0, 1, 2, 3, 4
 
 def complex_value(u, mode):
     """complex2img / complex2real per-element value (include/olimg.h:355-366,
-    399-410): real, imag, sqrt(re re + im im) with the products and sum in the
-    element type and the root in double (unqualified sqrt / atan2 on float arguments
-    resolve to the double functions under the reference's includes), atan2(im, re)."""
+    399-410): real, imag, sqrt(re re + im im), atan2(im, re).  The reference's headers
+    include <math.h> (include/olutils.h:5) before these templates, so for
+    complex<float> the unqualified sqrt and atan2 are the float overloads: the
+    products, sum and root are float and the angle is a float (taken here as the
+    correctly rounded one: the FP64 angle rounded to float); complex<double> is FP64
+    throughout."""
     u = np.asarray(u)
     if mode in (IMG_REAL, IMG_REAL_LEGACY):
         return u.real.astype(np.float64)
     if mode == IMG_IMAG:
         return u.imag.astype(np.float64)
-    if mode == IMG_AMP:
+    if u.dtype == np.complex64:
         re, im = u.real, u.imag
-        return np.sqrt((re * re + im * im).astype(np.float64))
-    return np.arctan2(u.imag.astype(np.float64), u.real.astype(np.float64))
+        if mode == IMG_AMP:
+            return np.sqrt(re * re + im * im).astype(np.float64)   # float32 throughout
+        return np.arctan2(im.astype(np.float64), re.astype(np.float64)).astype(np.float32).astype(np.float64)
+    if mode == IMG_AMP:
+        return np.sqrt(u.real * u.real + u.imag * u.imag)
+    return np.arctan2(u.imag, u.real)
 
 
 def complex2img(u, mode):

@@ -193,10 +193,17 @@ def test_oracle_complex2img_known_answers():
     assert olref.complex2real(np.array([3 + 4j], np.complex64), olref.IMG_AMP).tolist() == [5.0]
     assert olref.complex2real(np.array([1j], np.complex64), olref.IMG_PHASE)[0] == np.float32(np.pi / 2)
     assert olref.complex2real(np.array([1j], np.complex128), olref.IMG_PHASE).dtype == np.float64
-    # complex<float> products and sum are float: |(1 + 2^-12) + 0j|^2 rounds in float
+    # complex<float>: the products, their sum and the root are float (the reference's
+    # include/olutils.h:5 includes <math.h>, so unqualified sqrt / atan2 on floats are
+    # the float overloads); the angle is a float
     e = np.float32(1 + 2.0 ** -12)
     assert olref.complex_value(np.array([complex(e, 0)], np.complex64), olref.IMG_AMP)[0] == \
-        np.sqrt(np.float64(np.float32(e * e)))
+        np.float64(np.sqrt(np.float32(e * e)))
+    assert olref.complex_value(np.array([1 + 1j], np.complex64), olref.IMG_AMP)[0] == \
+        np.float64(np.float32(np.sqrt(2.0)))
+    assert olref.complex_value(np.array([1 + 1j], np.complex64), olref.IMG_PHASE)[0] == \
+        np.float64(np.float32(np.pi / 4))
+    assert olref.complex_value(np.array([1 + 1j], np.complex128), olref.IMG_AMP)[0] == np.sqrt(2.0)
 
 
 # --------------------------------------------------------------------------- BMP
