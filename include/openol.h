@@ -7,6 +7,8 @@
 #include <complex>
 #include <memory>
 
+#include "Diffuser.h"
+#include "Lensarry.h"
 #include "olcgh.h"
 #include "olclass.h"
 #include "oldefine.h"

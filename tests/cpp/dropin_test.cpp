@@ -100,6 +100,18 @@ int main(int argc, char** argv) {
             u = fresh();
             ol::TiltedSourceProp<float>(u, n, n, pf, pf, lf, 0.004f, (float)(M_PI / 180 * -3));
             save(dir + "/tilted_src.bin", u.get(), (size_t)n * n);
+            // optical elements (include/Diffuser.h, include/Lensarry.h)
+            u = fresh();
+            ol::Diffuser<float> dif(n, n);
+            dif.random_phase(u, ol::PLAY, 3, (float)(2 * M_PI), 1, -2);
+            save(dir + "/diffuser.bin", u.get(), (size_t)n * n);
+            u = fresh();
+            ol::Lensarry<float> lens(n, n, pf, 16 * pf, 0.02f, lf);
+            lens.mult_complex(u, ol::RECORD, 2, 3);
+            save(dir + "/lens_mult.bin", u.get(), (size_t)n * n);
+            ol::LensarryCircle<float> circ(n, n, pf, 16 * pf, 0.02f, lf);
+            circ.gen_complex_limit(40e-6f);
+            save(dir + "/lens_circle.bin", circ.u.get(), (size_t)n * n);
         }
 
         // CGH on a host point cloud (ol::Object<double>)

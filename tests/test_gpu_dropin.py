@@ -57,6 +57,12 @@ def test_dropin_program_matches_oracle(tmp_path):
     ang = float(np.float32(np.pi / 180 * -3))
     assert rel(rd("tilted_src"), olref.tilted(u0, pf, pf, lf, f32(0.004), ang, "bilinear", "asm",
                                               "source")) < 1e-5
+    assert rel(rd("diffuser"), olref.diffuser_random_phase(u0, olref.PLAY, 3, float(np.float32(2 * np.pi)),
+                                                        1, -2)) < 1e-6
+    assert rel(rd("lens_mult"), u0 * olref.lens_array(n, n, pf, 16 * pf, f32(0.02), lf, olref.RECORD, 2,
+                                                     3)) < 1e-6
+    assert rel(rd("lens_circle"), olref.lens_array(n, n, pf, 16 * pf, f32(0.02), lf, olref.PLAY, 0, 0,
+                                                   f32(40e-6), True)) < 1e-6
     assert rel(rd("gshifted"), olref.shifted_asm(u0, P, P, LAM, 0.004, -0.25 * n * P, 0.5 * n * P)) < 1e-5
     assert rel(rd("shiftedprop_fresnel"), olref.arss_prop(u0, P, P, LAM, 0.05, 1.0, 1e-4, 0.0)) < 1e-5
     cgh = olref.cgh(pts["x"], pts["y"], pts["z"], n, n, P, P, LAM, dtype=np.complex128)
