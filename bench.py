@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--n", type=int, default=4096, help="field side (metric: 4096)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the ASM metric")
-    ap.add_argument("--only", choices=["ingest"], default=None,
+    ap.add_argument("--only", choices=["ingest", "next"], default=None,
                     help="run one secondary workload alone and print its figures (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (for PMC byte-counter calibration)")
@@ -250,6 +250,7 @@ def extras(ol, lib, torch):
     out.update(layered(ol, torch))
     out.update(multiplane(torch))
     out.update(ingest_encode(ol, torch))
+    out.update(next_rows(ol, torch))
     arss_s = (t2 - t1) / reps
     out["arss_4096_ms"] = round(arss_s * 1e3, 3)
     out["arss_4096_GBps_80N2"] = round(80 * N * N / arss_s / 1e9, 1)
@@ -303,6 +304,42 @@ def ingest_encode(ol, torch):
     ms = _event_ms(torch, lambda: ol.phase8bit(ph, img), 20)
     out["phase8bit_4096_ms"] = round(ms, 4)
     out["phase8bit_4096_GBps"] = round(5 * N * N / ms / 1e6, 1)
+    return out
+
+
+def next_rows(ol, torch):
+    """Off-axis, split and tilted propagation and the optical elements at full size:
+    device time per call (HIP events) and, for the elementwise passes, GB/s."""
+    out = {}
+    N = 4096
+    rng = np.random.default_rng(8)
+    u0 = torch.from_numpy((rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+                          .astype(np.complex64)).cuda()
+    u = u0.clone()
+    ms = _event_ms(torch, lambda: ol.gshiftedAsmProp(u, u, N, N, PITCH, PITCH, LAM, 0.01, 1e-3, -2e-3,
+                                                     bandlimit=True), 10)
+    out["shifted_asm_4096_ms"] = round(ms, 3)
+    ms = _event_ms(torch, lambda: ol.gsplitProp(u, N, N, PITCH, PITCH, LAM, 0.01, ol.ASM, 2, 2), 3)
+    out["split_asm_4096_2x2_ms"] = round(ms, 3)
+    n2 = 2048
+    v = u0[:n2, :n2].contiguous()
+    for interp, name in ((ol.BILINEAR, "bilinear"), (ol.BICUBIC, "bicubic")):
+        ms = _event_ms(torch, lambda: ol.TiltedReferenceProp(v, n2, n2, PITCH, PITCH, LAM, 0.01, 0.05,
+                                                              interp), 5)
+        out[f"tilted_ref_2048_{name}_ms"] = round(ms, 3)
+    lens = ol.Lensarry(N, N, PITCH, 32 * PITCH, 0.05, LAM)
+    ms = _event_ms(torch, lambda: lens.mult_complex(u), 20)
+    out["lens_mult_4096_ms"] = round(ms, 4)
+    out["lens_mult_4096_GBps"] = round(16 * N * N / ms / 1e6, 1)
+    d = ol.Diffuser(N, N)
+    b = ol.uniform(N * N, seed=1, device=u.device.index)
+    ms = _event_ms(torch, lambda: d._apply(u, "phase", b, 6.283), 20)
+    out["diffuser_apply_4096_GBps"] = round(20 * N * N / ms / 1e6, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.random_phase(u, ol.PLAY, seed=1)
+    torch.cuda.synchronize()
+    out["diffuser_random_phase_4096_ms_host_stream"] = round((time.perf_counter() - t0) * 1e3, 1)
     return out
 
 
@@ -386,6 +423,9 @@ def main():
 
     if args.only == "ingest":
         print(json.dumps({"ingest_encode": ingest_encode(ol, torch)}), flush=True)
+        return
+    if args.only == "next":
+        print(json.dumps({"next_rows": next_rows(ol, torch)}), flush=True)
         return
     n = args.n
     dt, rep = bench_asm(ol, lib, torch, n, args.steps, args.warmup)
