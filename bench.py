@@ -145,12 +145,45 @@ def cpu_baseline(n):
     t0 = time.perf_counter()
     olref.complex2img(u4, olref.IMG_AMP)
     t_img = time.perf_counter() - t0
+    # direct CGH (10,000 points over 2048^2, Fresnel, complex64) on a 16-row band of
+    # the hologram, extrapolated linearly in rows
+    N, P = 2048, 10000
+    pts = synth.random_points(P, seed=2, xmax=N * PITCH, ymax=N * PITCH, zmin=0.10, zmax=0.20)
+    rows = 16
+    t0 = time.perf_counter()
+    olref.cgh(pts["x"], pts["y"], pts["z"], rows, N, PITCH, PITCH, LAM, pts["amp"], pts["phase"])
+    t_cgh = time.perf_counter() - t0
+    # one thread: AsmProp at 1024^2 (OpenMP pinned to one thread through libgomp)
+    one = None
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        u1 = synth.random_field(1024, 1024, seed=0)
+        t0 = time.perf_counter()
+        olref.asm_prop(u1, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
+        one = round(1.0 / (time.perf_counter() - t0), 3)
+        gomp.omp_set_num_threads(cores)
+    except OSError:
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": round(1.0 / med, 4), "unit": "props/s", "cores": cores, "kind": "port",
             "sample": f"{n}x{n} AsmProp (band-limited, pad {2 * n}), 1 warm-up + median of 3 "
                       f"= {med:.2f} s/prop; own OpenMP radix-2 FFT (FFTW unavailable - substitute FFT)",
-            "extras": {"objsort_1M_ms": round(t_sort * 1e3, 1),
+            "cpu_model": model, "nproc": os.cpu_count(),
+            "extras": {"asm_1024_props_s_one_thread": one,
+                       "cgh_direct_fresnel_Gptpx_s": round(P * rows * N / t_cgh / 1e9, 3),
+                       "cgh_sample": f"{rows} rows x {N} px x {P} points (extrapolate linearly)",
+                       "objsort_1M_ms": round(t_sort * 1e3, 1),
                        "complex2img_amp_4096_ms": round(t_img * 1e3, 1),
-                       "cores": 1, "kind": "port (numpy)"}}
+                       "numpy_rows": "objsort and complex2img: numpy, 1 core"}}
 
 
 def ceilings(lib, torch):

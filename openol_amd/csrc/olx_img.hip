@@ -34,26 +34,40 @@ __device__ inline bool fast_round(float q_fast, float err, int* k) {
     return true;
 }
 
+// one phase value: the FP32 fast path, else the reference's FP64 expression
 template <typename T>
-__global__ void k_phase8(const T* __restrict__ ph, uint8_t* __restrict__ img, int64_t n,
-                         int offset) {
-    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const T x = ph[j];
+__device__ inline uint8_t phase8_one(T x, int offset) {
     int k;
     if (sizeof(T) == 4) {
         // |q| < 65536: FP32 error of the three operations below plus the rounding of
         // pi and 255 / 2pi stays under 2^-12 + 4e-7 |q|
         const float pf = (float)x;
         const float qf = (offset ? pf + 3.14159265f : pf - 3.14159265f) * 40.5845105f;
-        if (fast_round(qf, 2.5e-4f + 4e-7f * fabsf(qf), &k)) {
-            img[j] = (uint8_t)(uint32_t)k;
-            return;
-        }
+        if (fast_round(qf, 2.5e-4f + 4e-7f * fabsf(qf), &k)) return (uint8_t)(uint32_t)k;
     }
     const double p = (double)x;
     const double t = offset ? __dadd_rn(p, M_PI) : __dsub_rn(p, M_PI);
-    img[j] = to_u8(round(__dmul_rn(__ddiv_rn(t, 2 * M_PI), 255.0)));
+    return to_u8(round(__dmul_rn(__ddiv_rn(t, 2 * M_PI), 255.0)));
+}
+
+// four values per thread (16 bytes of float in, 4 bytes out); the last thread
+// takes the tail one value at a time
+template <typename T>
+__global__ void k_phase8(const T* __restrict__ ph, uint8_t* __restrict__ img, int64_t n,
+                         int offset) {
+    const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
+    if (j >= n) return;
+    if (sizeof(T) == 4 && j + 4 <= n && ((uintptr_t)(ph + j) & 15) == 0 && ((uintptr_t)(img + j) & 3) == 0) {
+        const float4 x = *(const float4*)(ph + j);
+        uchar4 o;
+        o.x = phase8_one<T>((T)x.x, offset);
+        o.y = phase8_one<T>((T)x.y, offset);
+        o.z = phase8_one<T>((T)x.z, offset);
+        o.w = phase8_one<T>((T)x.w, offset);
+        *(uchar4*)(img + j) = o;
+        return;
+    }
+    for (int64_t i = j; i < j + 4 && i < n; i++) img[i] = phase8_one<T>(ph[i], offset);
 }
 
 // ------------------------------------------------------------------ extremes
@@ -301,7 +315,7 @@ extern "C" int olx_phase8bit(olx_ctx* ctx, const void* d_phase, uint8_t* d_img, 
     if (!ctx || (n > 0 && (!d_phase || !d_img))) OLX_FAIL(OLX_EINVAL, "phase8bit: null argument");
     if (flags & ~(OLX_PHASE8_OFFSET | OLX_IN_F64)) OLX_FAIL(OLX_EINVAL, "phase8bit: unknown flags");
     if (n <= 0) return OLX_OK;
-    const unsigned g = (unsigned)((n + 255) / 256);
+    const unsigned g = (unsigned)((n + 1023) / 1024);  // 4 values per thread
     const int off = (flags & OLX_PHASE8_OFFSET) ? 1 : 0;
     if (flags & OLX_IN_F64)
         k_phase8<double><<<g, 256, 0, ctx->stream>>>((const double*)d_phase, d_img, n, off);
