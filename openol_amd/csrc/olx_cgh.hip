@@ -19,19 +19,10 @@ namespace olx {
 
 constexpr int CGH_TX = 32, CGH_TY = 32, CGH_NT = 256, CGH_RPT = CGH_TY / (CGH_NT / CGH_TX);
 
+// sin / cos of t revolutions: the hardware units take revolutions; t is first
+// reduced to [0, 1) with one v_fract (exact for a float)
 __device__ __forceinline__ void sincos_rev(float t, float* s, float* c) {
-    t = t - rintf(t);  // [-0.5, 0.5] revolutions
-    *s = __builtin_amdgcn_sinf(t);
-    *c = __builtin_amdgcn_cosf(t);
-}
-
-// The hardware sine and cosine take revolutions and are accurate for |t| < 256;
-// the tile offsets keep the Fresnel phase relative to the anchor far inside that
-// unless a point's slopes are extreme (RED_LIMIT), so the explicit reduction to
-// [-1/2, 1/2] is done only for such points (it adds no accuracy: t already carries
-// its FP32 rounding).
-constexpr float RED_LIMIT = 128.f;
-__device__ __forceinline__ void sincos_rev_near(float t, float* s, float* c) {
+    t = __builtin_amdgcn_fractf(t);
     *s = __builtin_amdgcn_sinf(t);
     *c = __builtin_amdgcn_cosf(t);
 }
@@ -41,8 +32,6 @@ struct PtAnchor {  // per-(tile, point) constants, FP32
     float gx, gy;  // Fresnel: 2 c dxa px, 2 c dya py (cycles per pixel); exact: 2 dxa px, 2 dya py (m^2 per pixel)
     float qx, qy;  // Fresnel: c px^2, c py^2; exact: ra, ra^2
     float amp;
-    int far;       // Fresnel: |t| may exceed RED_LIMIT on this tile (reduce before sin / cos)
-    int pad;
 };
 
 template <bool EXACT>
@@ -91,24 +80,10 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
             th += (double)p.phase * (0.5 / M_PI);
             pa.th = (float)(th - rint(th));
             pa.amp = p.amp;
-            // bound of |t| over the tile's offsets (|a| <= TX/2, |b| <= TY/2)
-            const double ax = CGH_TX / 2, by = CGH_TY / 2;
-            double bound;
-            if (!EXACT) {
-                bound = 0.5 + ax * (fabs((double)pa.gx) + ax * fabs((double)pa.qx)) +
-                        by * (fabs((double)pa.gy) + by * fabs((double)pa.qy));
-            } else {  // |r - ra| <= |dq| / ra with dq the change of r^2
-                const double dq = ax * (fabs((double)pa.gx) + ax * px * px) +
-                                  by * (fabs((double)pa.gy) + by * py * py);
-                bound = 0.5 + dq / ((double)pa.qx * lambda);
-            }
-            pa.far = !(bound < RED_LIMIT);
         } else {
             pa.th = pa.gx = pa.gy = pa.qx = pa.qy = 0.f;
             pa.amp = 0.f;
-            pa.far = 0;
         }
-        pa.pad = 0;
         sp[tid] = pa;
         __syncthreads();
         const int cnt = min(CGH_NT, n - base);
@@ -116,24 +91,13 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
             const PtAnchor P = sp[q];
             if (!EXACT) {
                 const float thx = P.th + a * (P.gx + a * P.qx);
-                if (!P.far) {  // uniform across the work-group: every thread reads point q
 #pragma unroll
-                    for (int k = 0; k < CGH_RPT; k++) {
-                        const float t = thx + bk[k] * (P.gy + bk[k] * P.qy);
-                        float s, c;
-                        sincos_rev_near(t, &s, &c);
-                        re[k] += P.amp * c;
-                        im[k] += P.amp * s;
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < CGH_RPT; k++) {
-                        const float t = thx + bk[k] * (P.gy + bk[k] * P.qy);
-                        float s, c;
-                        sincos_rev(t, &s, &c);
-                        re[k] += P.amp * c;
-                        im[k] += P.amp * s;
-                    }
+                for (int k = 0; k < CGH_RPT; k++) {
+                    const float t = thx + bk[k] * (P.gy + bk[k] * P.qy);
+                    float s, c;
+                    sincos_rev(t, &s, &c);
+                    re[k] += P.amp * c;
+                    im[k] += P.amp * s;
                 }
             } else {
                 const float dqx = a * (P.gx + a * px2);
@@ -143,8 +107,7 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
                     const float dr = dq / (P.qx + sqrtf(P.qy + dq));  // r - ra
                     const float t = P.th + dr * inv_lambda;
                     float s, c;
-                    if (!P.far) sincos_rev_near(t, &s, &c);
-                    else sincos_rev(t, &s, &c);
+                    sincos_rev(t, &s, &c);
                     re[k] += P.amp * c;
                     im[k] += P.amp * s;
                 }
