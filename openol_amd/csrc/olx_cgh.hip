@@ -13,6 +13,15 @@
 // then evaluates every (pixel, point) pair in FP32 from the anchor with the
 // pixel's small offset, so the 1e4..1e5-cycle phases never meet FP32 rounding.
 // sin/cos use the hardware transcendental unit in revolutions.
+//
+// Exact r, per (pixel, point): with dq = r^2 - ra^2 and e = dq / ra^2,
+// r - ra = ra (sqrt(1 + e) - 1).  When e stays small over the whole tile, the
+// series ra (e/2 - e^2/8 + e^3/16) is used (3 FMA); its truncation error is below
+// (5/128) e^4 ra, kept under 4e-8 cycles by the per-point test.  Other points
+// (near field, wide tiles) use dq / (ra + sqrt(ra^2 + dq)).  Each 256-point chunk
+// is partitioned so both groups run as uniform loops.
+#include <type_traits>
+
 #include "olx_common.hpp"
 
 namespace olx {
@@ -29,17 +38,27 @@ __device__ __forceinline__ void sincos_rev(float t, float* s, float* c) {
 
 struct PtAnchor {  // per-(tile, point) constants, FP32
     float th;      // phase at the anchor pixel, cycles mod 1 (phi folded in)
-    float gx, gy;  // Fresnel: 2 c dxa px, 2 c dya py (cycles per pixel); exact: 2 dxa px, 2 dya py (m^2 per pixel)
-    float qx, qy;  // Fresnel: c px^2, c py^2; exact: ra, ra^2
+    float gx, gy;  // Fresnel: 2 c dxa px, 2 c dya py (cycles per pixel)
+                   // exact, series: 2 dxa px / ra^2, 2 dya py / ra^2; exact, sqrt: 2 dxa px, 2 dya py
+    float qx, qy;  // Fresnel: c px^2, c py^2; exact, series: px^2 / ra^2, py^2 / ra^2; exact, sqrt: ra, ra^2
     float amp;
 };
+
+struct PtAnchorR : PtAnchor {  // exact r: series coefficients (ra / lambda) (1/2, -1/8, 1/16)
+    float c1, c2, c3;
+};
+
+// series bound: (5/128) e^4 ra / lambda <= 4e-8 cycles
+constexpr double CGH_SERIES_E4K = 1.0e-6;
 
 template <bool EXACT>
 __global__ void __launch_bounds__(CGH_NT)
 k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
           float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx, double py,
           double px, double lambda, int accumulate) {
-    __shared__ PtAnchor sp[CGH_NT];
+    typedef typename std::conditional<EXACT, PtAnchorR, PtAnchor>::type Anchor;
+    __shared__ Anchor sp[CGH_NT];
+    __shared__ int wcount[2][CGH_NT / 64];
     const int tid = threadIdx.x, tx = tid % CGH_TX, ty = tid / CGH_TX;
     const int w0 = blockIdx.x * CGH_TX, h0 = blockIdx.y * CGH_TY;
     const int wa = w0 + CGH_TX / 2, ha = h0 + CGH_TY / 2;  // anchor pixel
@@ -47,21 +66,23 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
     float bk[CGH_RPT];
 #pragma unroll
     for (int k = 0; k < CGH_RPT; k++) bk[k] = (float)(ty + k * (CGH_NT / CGH_TX) - CGH_TY / 2);
-    const float fpx = (float)px, fpy = (float)py;
     const float px2 = (float)(px * px), py2 = (float)(py * py);
     const float inv_lambda = (float)(1.0 / lambda);
+    // largest pixel offsets from the anchor, metres
+    const double hx = (CGH_TX / 2) * px, hy = (CGH_TY / 2) * py;
     float re[CGH_RPT], im[CGH_RPT];
 #pragma unroll
     for (int k = 0; k < CGH_RPT; k++) re[k] = im[k] = 0.f;
 
     for (int base = 0; base < n; base += CGH_NT) {
         const int j = base + tid;
-        PtAnchor pa;
+        Anchor pa;
+        bool series = false;
         if (j < n) {
             const olx_point p = pts[j];
             const double dxa = px * wa - p.x, dya = py * ha - p.y;
             double th;
-            if (!EXACT) {
+            if constexpr (!EXACT) {
                 const double c = 1.0 / (2.0 * lambda * p.z);
                 th = c * (dxa * dxa + dya * dya);
                 pa.gx = (float)(2.0 * c * dxa * px);
@@ -72,24 +93,60 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
                 const double ra2 = dxa * dxa + dya * dya + p.z * p.z;
                 const double ra = sqrt(ra2);
                 th = ra / lambda;
-                pa.gx = (float)(2.0 * dxa * px);
-                pa.gy = (float)(2.0 * dya * py);
-                pa.qx = (float)ra;
-                pa.qy = (float)ra2;
+                // |e| over the tile
+                const double em = (2.0 * (fabs(dxa) * hx + fabs(dya) * hy) + hx * hx + hy * hy) / ra2;
+                series = em <= 0.01 && em * em * em * em * th <= CGH_SERIES_E4K;
+                if (series) {
+                    const double ir2 = 1.0 / ra2;
+                    pa.gx = (float)(2.0 * dxa * px * ir2);
+                    pa.gy = (float)(2.0 * dya * py * ir2);
+                    pa.qx = (float)(px * px * ir2);
+                    pa.qy = (float)(py * py * ir2);
+                    pa.c1 = (float)(0.5 * th);
+                    pa.c2 = (float)(-0.125 * th);
+                    pa.c3 = (float)(0.0625 * th);
+                } else {
+                    pa.gx = (float)(2.0 * dxa * px);
+                    pa.gy = (float)(2.0 * dya * py);
+                    pa.qx = (float)ra;
+                    pa.qy = (float)ra2;
+                    pa.c1 = pa.c2 = pa.c3 = 0.f;
+                }
             }
             th += (double)p.phase * (0.5 / M_PI);
             pa.th = (float)(th - rint(th));
             pa.amp = p.amp;
-        } else {
-            pa.th = pa.gx = pa.gy = pa.qx = pa.qy = 0.f;
-            pa.amp = 0.f;
         }
-        sp[tid] = pa;
-        __syncthreads();
         const int cnt = min(CGH_NT, n - base);
-        for (int q = 0; q < cnt; q++) {
-            const PtAnchor P = sp[q];
-            if (!EXACT) {
+        int nser = 0;  // exact r: slots [0, nser) hold series points, [nser, cnt) the others
+        if constexpr (!EXACT) {
+            if (j < n) sp[tid] = pa;
+        } else {
+            const int lane = tid & 63, wv = tid >> 6;
+            const unsigned long long ms = __ballot(j < n && series);
+            const unsigned long long mo = __ballot(j < n && !series);
+            if (lane == 0) {
+                wcount[0][wv] = __popcll(ms);
+                wcount[1][wv] = __popcll(mo);
+            }
+            __syncthreads();
+            int sb = 0, ob = 0;
+#pragma unroll
+            for (int v = 0; v < CGH_NT / 64; v++) {
+                nser += wcount[0][v];
+                if (v < wv) {
+                    sb += wcount[0][v];
+                    ob += wcount[1][v];
+                }
+            }
+            const unsigned long long below = (1ull << lane) - 1ull;
+            if (j < n)
+                sp[series ? sb + __popcll(ms & below) : nser + ob + __popcll(mo & below)] = pa;
+        }
+        __syncthreads();
+        if constexpr (!EXACT) {
+            for (int q = 0; q < cnt; q++) {
+                const PtAnchor P = sp[q];
                 const float thx = P.th + a * (P.gx + a * P.qx);
 #pragma unroll
                 for (int k = 0; k < CGH_RPT; k++) {
@@ -99,7 +156,24 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
                     re[k] += P.amp * c;
                     im[k] += P.amp * s;
                 }
-            } else {
+            }
+        } else {
+            for (int q = 0; q < nser; q++) {
+                const PtAnchorR P = sp[q];
+                const float ex = a * (P.gx + a * P.qx);
+#pragma unroll
+                for (int k = 0; k < CGH_RPT; k++) {
+                    const float e = ex + bk[k] * (P.gy + bk[k] * P.qy);  // dq / ra^2
+                    const float poly = fmaf(e, fmaf(e, P.c3, P.c2), P.c1);
+                    const float t = fmaf(e, poly, P.th);  // + (r - ra) / lambda
+                    float s, c;
+                    sincos_rev(t, &s, &c);
+                    re[k] += P.amp * c;
+                    im[k] += P.amp * s;
+                }
+            }
+            for (int q = nser; q < cnt; q++) {
+                const PtAnchorR P = sp[q];
                 const float dqx = a * (P.gx + a * px2);
 #pragma unroll
                 for (int k = 0; k < CGH_RPT; k++) {

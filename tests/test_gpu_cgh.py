@@ -67,6 +67,21 @@ def test_cgh_reference_semantics_and_golden():
     assert rel(U.cpu().numpy(), ref32) < 1e-5
 
 
+@pytest.mark.parametrize("zmin,zmax", [(2e-3, 5e-3), (2e-3, 0.2)])
+def test_cgh_exact_near_field(zmin, zmax):
+    """Exact r with points a few millimetres from a 2 mm hologram: r^2 changes by a
+    large fraction of ra^2 across a 32 x 32 tile, so the kernel's series for r - ra
+    does not apply and the sqrt form runs; the wide z range mixes both forms inside
+    every 256-point chunk."""
+    ny, nx = 200, 256
+    pts = cloud(700, 91, nx, ny, zmin=zmin, zmax=zmax)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ol().Cgh(obj_of(pts), U, ny, nx, P, P, LAM, exact_r=True)
+    ref = olref.cgh(pts["x"], pts["y"], pts["z"], ny, nx, P, P, LAM, pts["amp"], pts["phase"],
+                    flags=olref.CGH_EXACT_R, dtype=np.complex128)
+    assert rel(U.cpu().numpy(), ref) < 1e-4
+
+
 def test_cgh_outputs_phase_amp_accumulate():
     ny, nx, n = 48, 80, 200
     pts = cloud(n, 41, nx, ny)
