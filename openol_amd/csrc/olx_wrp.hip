@@ -6,8 +6,9 @@
 //      (square: D_j = sqrt(2) W_j) and the 16 x 16 pixel tiles its window touches;
 //   2. counting sort of (tile, point) pairs: count, exclusive scan, fill;
 //   3. one work-group per tile: load the tile's point list, sort it by point index
-//      (fixed summation order), and accumulate exp(i pi rho^2/(lambda dz))/|dz|
-//      for the pixels inside each window.
+//      (fixed summation order), stage each point's constants (position, window
+//      limit, 1/(2 dz lambda), 1/|dz|) in LDS 256 at a time, and accumulate
+//      exp(i pi rho^2/(lambda dz))/|dz| for the pixels inside each window.
 // Window tests and rho^2 use FP64 in the reference's expression order, so pixel
 // membership is decided exactly as the reference does with PREC_T = double.
 #include <climits>
@@ -76,38 +77,118 @@ __global__ void k_wrp_fill(const olx_point* __restrict__ pts, int n, const int* 
         }
 }
 
-// single work-group exclusive scan; offs[ntiles] = total
-__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int* __restrict__ offs,
-                                               int ntiles) {
-    __shared__ long long part[1024];
-    const int tid = threadIdx.x;
-    const int per = (ntiles + 1023) / 1024;
-    const int b = tid * per, e = min(ntiles, b + per);
-    long long s = 0;
-    for (int i = b; i < e; i++) s += counts[i];
-    part[tid] = s;
+// Exclusive scan of the per-tile counts in three kernels:
+//   k_scan_blocks: each 1024-thread block scans SCAN_B = 4096 counts (4 per thread,
+//                  loaded contiguously) into block-local offsets and a block total;
+//   k_scan_top:    one block scans the block totals (64-bit);
+//   k_scan_add:    adds each block's base; offs[ntiles] = total, or -1 beyond 2^31-1.
+constexpr int SCAN_T = 1024, SCAN_V = 4, SCAN_B = SCAN_T * SCAN_V;
+
+// exclusive block-wide scan of one 64-bit value per thread (1024 threads)
+__device__ __forceinline__ long long block_excl_scan(long long v, long long* part, long long* total) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    long long x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) part[wv] = x;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        long long v = tid >= off ? part[tid - off] : 0;
+    if (tid < 64) {
+        long long t = tid < SCAN_T / 64 ? part[tid] : 0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long y = __shfl_up(t, off, 64);
+            if (lane >= off) t += y;
+        }
+        if (tid < SCAN_T / 64) part[SCAN_T / 64 + tid] = t;  // inclusive wave bases
+    }
+    __syncthreads();
+    const long long wbase = wv ? part[SCAN_T / 64 + wv - 1] : 0;
+    *total = part[SCAN_T / 64 + SCAN_T / 64 - 1];
+    return wbase + x - v;
+}
+
+__global__ void __launch_bounds__(SCAN_T)
+k_scan_blocks(const int* __restrict__ counts, int* __restrict__ offs, long long* __restrict__ bsum,
+              int ntiles) {
+    __shared__ long long part[2 * SCAN_T / 64];
+    const int i0 = blockIdx.x * SCAN_B + threadIdx.x * SCAN_V;
+    int c[SCAN_V];
+#pragma unroll
+    for (int k = 0; k < SCAN_V; k++) c[k] = i0 + k < ntiles ? counts[i0 + k] : 0;
+    long long v = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_V; k++) v += c[k];
+    long long tot;
+    long long run = block_excl_scan(v, part, &tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_V; k++) {
+        if (i0 + k < ntiles) offs[i0 + k] = (int)run;  // block-local; the base is added later
+        run += c[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_T)
+k_scan_top(long long* __restrict__ bsum, int nblk, int* __restrict__ total_out) {
+    __shared__ long long part[2 * SCAN_T / 64];
+    __shared__ long long carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nblk; b0 += SCAN_T) {
+        const int i = b0 + threadIdx.x;
+        const long long v = i < nblk ? bsum[i] : 0;
+        long long tot;
+        const long long e = block_excl_scan(v, part, &tot);
+        const long long c = carry;
+        if (i < nblk) bsum[i] = c + e;  // exclusive block base, in place
         __syncthreads();
-        part[tid] += v;
+        if (threadIdx.x == 0) carry = c + tot;
         __syncthreads();
     }
-    long long run = tid ? part[tid - 1] : 0;
-    for (int i = b; i < e; i++) {
-        offs[i] = (int)run;
-        run += counts[i];
-    }
-    if (tid == 1023) offs[ntiles] = part[1023] > INT_MAX ? -1 : (int)part[1023];
+    if (threadIdx.x == 0) *total_out = carry > INT_MAX ? -1 : (int)carry;
+}
+
+__global__ void __launch_bounds__(256)
+k_scan_add(int* __restrict__ offs, const long long* __restrict__ bbase, int ntiles) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < ntiles) offs[i] = (int)(offs[i] + bbase[i / SCAN_B]);
 }
 
 constexpr int WRP_SORT_CAP = 4096;
+constexpr int WRP_CHUNK = 256;  // points staged in LDS at a time
 
+struct WrpPt {     // per-point constants of one tile's list, staged in LDS
+    double x, y;
+    double lim;    // circle: W^2 (tmp < W^2); square: W (2 |dx| <= W)
+    double rc;     // 1 / (2 dz lambda): phase in cycles = tmp * rc
+    float inv;     // 1 / |dz|
+    float pad;
+};
+
+__device__ __forceinline__ WrpPt wrp_stage(const olx_point& q, double lambda, double plane_z,
+                                           double plim, int square) {
+    WrpPt s;
+    const double Wd = wrp_radius(q.z, plane_z, plim, lambda, square);
+    const double dz = q.z - plane_z;
+    s.x = q.x;
+    s.y = q.y;
+    s.lim = square ? Wd : Wd * Wd;
+    s.rc = 1.0 / (2.0 * dz * lambda);
+    s.inv = (float)(1.0 / fabs(dz));
+    s.pad = 0.f;
+    return s;
+}
+
+template <bool SQUARE>
 __global__ void __launch_bounds__(256)
 k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
             const int* __restrict__ list, float2* __restrict__ U, int ny, int nx, double p,
-            double lambda, double plane_z, double plim, int square) {
+            double lambda, double plane_z, double plim) {
     __shared__ int ids[WRP_SORT_CAP];
+    __shared__ WrpPt sp[WRP_CHUNK];
     const int tiles_x = (nx + WT - 1) / WT;
     const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
@@ -121,7 +202,7 @@ k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
         while (np2 < cnt) np2 <<= 1;
         for (int i = threadIdx.x; i < np2; i += 256) ids[i] = i < cnt ? list[beg + i] : INT_MAX;
         __syncthreads();
-        // bitonic sort, ascending point index
+        // bitonic sort, ascending point index: a fixed summation order
         for (int k = 2; k <= np2; k <<= 1)
             for (int jj = k >> 1; jj > 0; jj >>= 1) {
                 for (int i = threadIdx.x; i < np2; i += 256) {
@@ -138,21 +219,28 @@ k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
                 __syncthreads();
             }
     }
-    for (int q = 0; q < cnt; q++) {
-        const int j = sorted ? ids[q] : list[beg + q];
-        const olx_point P = pts[j];
-        const double Wd = wrp_radius(P.z, plane_z, plim, lambda, square);
-        const double dx = xw - P.x, dy = yh - P.y, dz = P.z - plane_z;
-        const double tmp = dx * dx + dy * dy;
-        const bool inside = square ? (fabs(dx) * 2 <= Wd && fabs(dy) * 2 <= Wd) : (tmp < Wd * Wd);
-        if (inside) {
-            double cyc = tmp / (2.0 * dz * lambda);  // 2 pi tmp/(2 dz lambda) rad
-            cyc -= rint(cyc);
-            const float t = (float)cyc;
-            const float inv = (float)(1.0 / fabs(dz));
-            re += __builtin_amdgcn_cosf(t) * inv;
-            im += __builtin_amdgcn_sinf(t) * inv;
+    for (int base = 0; base < cnt; base += WRP_CHUNK) {
+        const int m = min(WRP_CHUNK, cnt - base);
+        if ((int)threadIdx.x < m) {
+            const int j = sorted ? ids[base + threadIdx.x] : list[beg + base + threadIdx.x];
+            sp[threadIdx.x] = wrp_stage(pts[j], lambda, plane_z, plim, SQUARE);
         }
+        __syncthreads();
+        for (int q = 0; q < m; q++) {
+            const WrpPt P = sp[q];
+            const double dx = xw - P.x, dy = yh - P.y;
+            const double tmp = dx * dx + dy * dy;
+            const bool inside = SQUARE ? (fabs(dx) * 2 <= P.lim && fabs(dy) * 2 <= P.lim)
+                                       : (tmp < P.lim);
+            if (inside) {
+                double cyc = tmp * P.rc;
+                cyc -= rint(cyc);
+                const float t = (float)cyc;
+                re += __builtin_amdgcn_cosf(t) * P.inv;
+                im += __builtin_amdgcn_sinf(t) * P.inv;
+            }
+        }
+        __syncthreads();
     }
     if (w < nx && h < ny) U[(size_t)h * nx + w] = make_float2(re, im);
 }
@@ -193,10 +281,13 @@ extern "C" int olx_wrp_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     const int square = (flags & OLX_WRP_SQUARE) ? 1 : 0;
     const int tiles_x = (int)((nx + WT - 1) / WT), tiles_y = (int)((ny + WT - 1) / WT);
     const int ntiles = tiles_x * tiles_y;
-    OLX_TRY(ctx->wrp_counts.reserve(sizeof(int) * (size_t)(3 * ntiles + 2)));
+    const int nblk = (ntiles + SCAN_B - 1) / SCAN_B;
+    const size_t ints = ((size_t)3 * ntiles + 2 + 1) & ~(size_t)1;  // keep the 64-bit part aligned
+    OLX_TRY(ctx->wrp_counts.reserve(sizeof(int) * ints + sizeof(long long) * (size_t)nblk));
     int* counts = (int*)ctx->wrp_counts.p;
     int* offs = counts + ntiles;       // ntiles + 1
     int* cursor = offs + ntiles + 1;   // ntiles
+    long long* bsum = (long long*)(counts + ints);  // nblk
     hipStream_t s = ctx->stream;
     OLX_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (size_t)ntiles, s));
     OLX_HIP(hipMemsetAsync(cursor, 0, sizeof(int) * (size_t)ntiles, s));
@@ -206,7 +297,9 @@ extern "C" int olx_wrp_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
         if (n > 0)
             k_wrp_count<<<nb, 256, 0, s>>>(d_pts, (int)n, counts, (int)ny, (int)nx, p, lambda,
                                            plane_z, p_limit, square);
-        k_scan<<<1, 1024, 0, s>>>(counts, offs, ntiles);
+        k_scan_blocks<<<nblk, SCAN_T, 0, s>>>(counts, offs, bsum, ntiles);
+        k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, offs + ntiles);
+        k_scan_add<<<(ntiles + 255) / 256, 256, 0, s>>>(offs, bsum, ntiles);
     }
     OLX_TRY(check_launch("wrp count/scan"));
     int total = 0;
@@ -224,8 +317,12 @@ extern "C" int olx_wrp_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     OLX_TRY(check_launch("wrp fill"));
     {
         TimedLaunch tl(ctx, "wrp_accum");
-        k_wrp_accum<<<ntiles, 256, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny, (int)nx, p,
-                                           lambda, plane_z, p_limit, square);
+        if (square)
+            k_wrp_accum<true><<<ntiles, 256, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
+                                                     (int)nx, p, lambda, plane_z, p_limit);
+        else
+            k_wrp_accum<false><<<ntiles, 256, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
+                                                      (int)nx, p, lambda, plane_z, p_limit);
     }
     return check_launch("wrp accumulate");
 }

@@ -166,6 +166,41 @@ def test_wrp_matches_oracle(square):
     assert torch.equal(U, U2)
 
 
+def test_wrp_large_plane_matches_windows():
+    """A 33024^2 plane (4.26 M tiles of 16 x 16: the tile-count scan spans 1040 blocks
+    and two rounds of its top level) with clusters of points in three far-apart
+    windows.  With a power-of-two pitch, translating a cluster by a multiple of 16
+    pixels is exact (p w, p (w - w0) and x - w0 p carry no rounding), so each window of
+    the large plane must equal, bit for bit, a 64 x 64 plane holding the translated
+    cluster; that plane is checked against the oracle."""
+    n_side, win = 33024, 64
+    p = 2.0 ** -17
+    rng = np.random.default_rng(12)
+    corners = [(1024, 2048), (16384, 20480), (32960, 32960)]
+    xs, ys, zs = [], [], []
+    for h0, w0 in corners:
+        # windows reach 23 px at dz = 5 mm: keep them inside the 64 x 64 corner
+        xs.append((w0 + rng.uniform(24, win - 24, 15)) * p)
+        ys.append((h0 + rng.uniform(24, win - 24, 15)) * p)
+        zs.append(rng.uniform(0.151, 0.155, 15))
+    x, y, z = np.concatenate(xs), np.concatenate(ys), np.concatenate(zs)
+    big = torch.empty((n_side, n_side), dtype=torch.complex64, device="cuda")
+    ol().WRPStep1(ol().Object(x, y, z), big, n_side, n_side, p, LAM, 0.15)
+    for k, (h0, w0) in enumerate(corners):
+        sl = slice(15 * k, 15 * (k + 1))
+        xk, yk, zk = x[sl] - w0 * p, y[sl] - h0 * p, z[sl]
+        small = torch.empty((win, win), dtype=torch.complex64, device="cuda")
+        ol().WRPStep1(ol().Object(xk, yk, zk), small, win, win, p, LAM, 0.15)
+        assert torch.equal(big[h0:h0 + win, w0:w0 + win], small), (h0, w0)
+        ref = olref.wrp(xk, yk, zk, win, win, p, LAM, 0.15, dtype=np.complex128)
+        assert rel(small.cpu().numpy(), ref) < 1e-5
+    # nothing outside the windows
+    for h0, w0 in corners:
+        big[h0:h0 + win, w0:w0 + win] = 0
+    assert int(torch.count_nonzero(torch.view_as_real(big))) == 0
+    del big
+
+
 def test_wrp_plimit_and_edges():
     # points near and beyond the plane edges, and a p_limit overload (golcgh.h:381-389)
     ny, nx = 64, 48
