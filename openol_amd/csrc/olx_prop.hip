@@ -608,7 +608,8 @@ k_cols_fft(float2* __restrict__ u, int cols, const float2* __restrict__ tw) {
     constexpr int R0 = P::radix(0), RL = P::radix(P::NST - 1);
     __shared__ float2 lds[CB * P::LP];
     const int tid = threadIdx.x, c = tid % CB, t = tid / CB;
-    const int col = blockIdx.x * CB + c;
+    // neighbouring column groups share 128-byte lines: keep them on one XCD's L2
+    const int col = xcd_remap(blockIdx.x, gridDim.x) * CB + c;
     const bool valid = col < cols;
     float2 v[P::E];
 #pragma unroll

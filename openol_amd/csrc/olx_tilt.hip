@@ -11,7 +11,8 @@
 //   pass; source form: x H / M^2 here) -> 2-D IFFT -> crop x 1/M^2.
 // Sample positions are FP64 with one rounding per operation (this file is built
 // without FMA contraction) so that floor() picks the same cells as the reference's
-// double arithmetic; data are complex64, weights and sums FP64.
+// double arithmetic; data are complex64, bilinear weights and sums FP64, bicubic
+// weights and sums FP32.
 #include <climits>
 #include <cmath>
 
@@ -57,30 +58,37 @@ __device__ double2 bilinear(const float2* P, double dsty, double dstx, const Til
 }
 
 // include/olutils.h:290-320 (a = -1): a 4 x 4 window weighted by the cubic kernel of
-// the radial distance
+// the radial distance, k(d) = (1 - d)(1 + d - d^2) for d < 1, (2 - d)^2 (1 - d) for
+// 1 <= d < 2, 0 beyond (taps at d >= 2 are skipped).  The cell (ms, ns) comes from
+// floor() of the FP64 position, as in the reference; the tap offsets are formed from
+// the exact FP64 fraction dst - floor(dst) rounded once to FP32, and the weights and
+// the sum are FP32 (relative error ~1e-7).
 __device__ double2 bicubic(const float2* P, double dsty, double dstx, const TiltArgs& a) {
-    const double ca = -1;
     if (!(dstx == dstx) || !(dsty == dsty)) return make_double2(0.0, 0.0);
     const long long ms = (long long)floor(dstx), ns = (long long)floor(dsty);
     const int nx = a.nx2, ny = a.ny2;
-    double2 avg = make_double2(0.0, 0.0);
-    if (!(0 <= ms && ms < nx && 0 <= ns && ns < ny)) return avg;
+    if (!(0 <= ms && ms < nx && 0 <= ns && ns < ny)) return make_double2(0.0, 0.0);
+    const float fx = (float)(dstx - (double)ms), fy = (float)(dsty - (double)ns);
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
     for (int j = 0; j < 4; j++)
+#pragma unroll
         for (int i = 0; i < 4; i++) {
             const long long m = ms - 1 + i, n = ns - 1 + j;
             if (0 <= m && m < nx && 0 <= n && n < ny) {
-                const double d = sqrt(((double)m - dstx) * ((double)m - dstx) +
-                                      ((double)n - dsty) * ((double)n - dsty));
-                double k;
-                if (0 <= d && d < 1) k = 1 - (ca + 3) * d * d + (ca + 2) * d * d * d;
-                else if (1 <= d && d < 2) k = -4 * ca + 8 * ca * d - 5 * ca * d * d + ca * d * d * d;
-                else k = 0;
-                const double2 p = ld(P, (int)n, (int)m, a);
-                avg.x += k * p.x;
-                avg.y += k * p.y;
+                const float dx = (float)(i - 1) - fx, dy = (float)(j - 1) - fy;
+                const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+                if (d2 >= 4.f) continue;
+                const float d = __builtin_amdgcn_sqrtf(d2);
+                const float e = 1.f - d;
+                const float k = d < 1.f ? e * __builtin_fmaf(-d, d, 1.f + d)
+                                        : (2.f - d) * (2.f - d) * e;
+                const float2 p = P[(size_t)nat((int)n, ny) * nx + nat((int)m, nx)];
+                sx = __builtin_fmaf(k, p.x, sx);
+                sy = __builtin_fmaf(k, p.y, sy);
             }
         }
-    return avg;
+    return make_double2((double)sx, (double)sy);
 }
 
 // centred ASM transfer function exp(2 pi i w z), w = sqrt(1/lambda^2 - u^2 - v^2), the
