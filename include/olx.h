@@ -61,6 +61,8 @@ extern "C" {
 #define OLX_LAYERED_F16 0
 #define OLX_LAYERED_BF16 1
 #define OLX_LAYERED_BF16X3 2      /* split bf16 hi+lo, three products */
+#define OLX_LAYERED_TILE128 0x4000u /* layered: use the 128 x 128 work-group tile even where the
+                                      sides allow 256 x 128 (same sums, same results) */
 
 typedef struct olx_ctx olx_ctx;
 

@@ -30,6 +30,7 @@ INTERP_BILINEAR, INTERP_BICUBIC = 0, 1
 
 MODE_ASM, MODE_FRESNEL, MODE_AUTO, MODE_FRESNEL_BL = 0, 1, 2, 3
 LAYERED_F16, LAYERED_BF16, LAYERED_BF16X3 = 0, 1, 2
+LAYERED_TILE128 = 0x4000
 OBJ_AUTO, OBJ_3D, OBJ_3DF = 0, 1, 2
 OBJSET_ZMIN, OBJSET_ZMIDDLE, OBJSET_F32 = 0, 1, 0x100
 IN_F64 = 0x100

@@ -292,6 +292,161 @@ k_lay_gemm(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* _
             }
 }
 
+// The same product with a 256 x 128 work-group tile for hologram sides that are
+// multiples of 256 (rows) and 128 (columns).  Wave (wr, wc) owns 128 x 64 outputs:
+// per 16-wide K step it reads 4 A and 2 B fragments from LDS and issues 16 MFMAs
+// (the 64 x 64 wave tile above reads 4 fragments per 8).  256 accumulator registers
+// leave one wave per SIMD, so the fragments of the next K step are read into a
+// second register set while the current step's MFMAs run.
+// Stages go from global memory straight into LDS (global_load_lds_dwordx4: lane l of
+// a wave writes 16 bytes at the wave's LDS base + 16 l), so no registers hold them.
+// LDS rows are 64 halves (128 B) without padding; the 16-byte chunk c of row r sits
+// in slot c ^ (r & 7), which spreads the 8 rows a ds_read_b128 group touches over
+// all banks.  Three stages of 256 + 128 rows (144 KiB): loads run two stages ahead
+// of the MFMAs.  The K order of the sums is that of k_lay_gemm, so both kernels give
+// identical results.
+constexpr int GT2M = 256, GT2N = 128;
+constexpr int ROWB = KS * 2;  // bytes per LDS row
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// Work-group barrier without the release fence of __syncthreads(), which would wait
+// for every outstanding global load (the direct loads of the next stage included).
+// Callers wait for the loads that must have landed with s_waitcnt first; the asm
+// statements keep the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void stage_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256, 1)
+k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* __restrict__ U,
+            int nx, int accumulate) {
+    // one LDS object per stage and matrix: reads of one stage are then known not to
+    // alias the direct loads in flight into another, and need not wait for them
+    __shared__ __align__(16) char sA0[GT2M * ROWB], sA1[GT2M * ROWB], sA2[GT2M * ROWB];
+    __shared__ __align__(16) char sB0[GT2N * ROWB], sB1[GT2N * ROWB], sB2[GT2N * ROWB];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 1, wc = wave & 1;
+    const int h0 = blockIdx.y * GT2M, w0 = blockIdx.x * GT2N;
+    const T* gA = A + (size_t)h0 * ldk;
+    const T* gB = Br + (size_t)w0 * ldk;
+    // direct loads: instruction for 8-row group g writes LDS bytes g*1024 .. +1023;
+    // lane l fills row 8g + l/8, slot l%8 with global chunk (l%8) ^ (l/8)
+    const size_t loff = (size_t)(lane >> 3) * ldk + (size_t)(((lane & 7) ^ (lane >> 3)) * 8);
+#define OLX_DLOAD(SA, SB, k0)                                                                    \
+    do {                                                                                         \
+        _Pragma("unroll") for (int q = 0; q < GT2M / 32; q++) {                                  \
+            const int g_ = wave + 4 * q;                                                         \
+            __builtin_amdgcn_global_load_lds(                                                    \
+                (const void*)(gA + (size_t)(8 * g_) * ldk + (k0) + loff),                        \
+                (lds_ptr_t)(&SA[g_ * 8 * ROWB]), 16, 0, 0);                                      \
+        }                                                                                        \
+        _Pragma("unroll") for (int q = 0; q < GT2N / 32; q++) {                                  \
+            const int g_ = wave + 4 * q;                                                         \
+            __builtin_amdgcn_global_load_lds(                                                    \
+                (const void*)(gB + (size_t)(8 * g_) * ldk + (k0) + loff),                        \
+                (lds_ptr_t)(&SB[g_ * 8 * ROWB]), 16, 0, 0);                                      \
+        }                                                                                        \
+    } while (0)
+    // fragment of K step kk: lane holds row (lane % 32), chunk 2 kk + lane / 32, found in
+    // slot chunk ^ (row & 7) = chunk ^ (lane & 7)
+    int soff[KS / 16];
+#pragma unroll
+    for (int kk = 0; kk < KS / 16; kk++)
+        soff[kk] = (lane & 31) * ROWB + (((2 * kk + (lane >> 5)) ^ (lane & 7)) * 16);
+    uint4 fa0[4], fb0[2], fa1[4], fb1[2];
+#define OLX_LFRAG(SA, SB, kk, FA, FB)                                                            \
+    do {                                                                                         \
+        _Pragma("unroll") for (int i = 0; i < 4; i++) FA[i] =                                    \
+            *(const uint4*)(&SA[(wr * 128 + i * 32) * ROWB + soff[kk]]);                        \
+        _Pragma("unroll") for (int j = 0; j < 2; j++) FB[j] =                                    \
+            *(const uint4*)(&SB[(wc * 64 + j * 32) * ROWB + soff[kk]]);                         \
+    } while (0)
+#define OLX_STEP(FA, FB)                                                                         \
+    do {                                                                                         \
+        uint4 bi_[2];                                                                            \
+        _Pragma("unroll") for (int j = 0; j < 2; j++) bi_[j] =                                   \
+            make_uint4(bi_of_br(FB[j].x), bi_of_br(FB[j].y), bi_of_br(FB[j].z), bi_of_br(FB[j].w)); \
+        _Pragma("unroll") for (int i = 0; i < 4; i++)                                            \
+            _Pragma("unroll") for (int j = 0; j < 2; j++) {                                      \
+                accr[i][j] = mfma<T>(FA[i], FB[j], accr[i][j]);                                  \
+                acci[i][j] = mfma<T>(FA[i], bi_[j], acci[i][j]);                                 \
+            }                                                                                    \
+    } while (0)
+    f16v_t accr[4][2], acci[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) accr[i][j][v] = acci[i][j][v] = 0.f;
+
+    static_assert(KS == 64, "four 16-wide K steps of 8 16-byte chunks per stage");
+    // s_waitcnt immediates (expcnt and lgkmcnt left open): all of this wave's loads
+    // landed, or all but the newest stage's 12
+    constexpr int WAIT_VM0 = 0x0F70, WAIT_VM12 = 0x0F7C;
+    static_assert(GT2M / 32 + GT2N / 32 == 12, "direct loads per wave and stage");
+    const int nsteps = ldk / KS;
+    // stage st reads (SA, SB); stage st + 1 is in (SA1_, SB1_); stage st + 2 is loaded
+    // into (SA2_, SB2_), the buffers read in stage st - 1, which ended with a barrier
+#define OLX_STAGE(SA, SB, SA1_, SB1_, SA2_, SB2_, st)                                            \
+    do {                                                                                         \
+        const bool ahead_ = (st) + 2 < nsteps;                                                   \
+        if (ahead_) OLX_DLOAD(SA2_, SB2_, ((st) + 2) * KS);                                      \
+        OLX_LFRAG(SA, SB, 1, fa1, fb1);                                                          \
+        OLX_STEP(fa0, fb0);                                                                      \
+        OLX_LFRAG(SA, SB, 2, fa0, fb0);                                                          \
+        OLX_STEP(fa1, fb1);                                                                      \
+        OLX_LFRAG(SA, SB, 3, fa1, fb1);                                                          \
+        OLX_STEP(fa0, fb0);                                                                      \
+        OLX_STEP(fa1, fb1);                                                                      \
+        if (ahead_) __builtin_amdgcn_s_waitcnt(WAIT_VM12);                                       \
+        else __builtin_amdgcn_s_waitcnt(WAIT_VM0);                                               \
+        stage_barrier();                                                                         \
+        if ((st) + 1 < nsteps) OLX_LFRAG(SA1_, SB1_, 0, fa0, fb0);                               \
+    } while (0)
+    OLX_DLOAD(sA0, sB0, 0);
+    if (nsteps > 1) {
+        OLX_DLOAD(sA1, sB1, KS);
+        __builtin_amdgcn_s_waitcnt(WAIT_VM12);
+    } else {
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+    }
+    stage_barrier();
+    OLX_LFRAG(sA0, sB0, 0, fa0, fb0);
+    for (int st = 0; st < nsteps; st += 3) {
+        OLX_STAGE(sA0, sB0, sA1, sB1, sA2, sB2, st);
+        if (st + 1 < nsteps) OLX_STAGE(sA1, sB1, sA2, sB2, sA0, sB0, st + 1);
+        if (st + 2 < nsteps) OLX_STAGE(sA2, sB2, sA0, sB0, sA1, sB1, st + 2);
+    }
+#undef OLX_STAGE
+#undef OLX_DLOAD
+#undef OLX_LFRAG
+#undef OLX_STEP
+    // epilogue: lane holds column w = l % 32 and rows 8 (v/4) + 4 (l/32) + v % 4
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int h = h0 + wr * 128 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int w = w0 + wc * 64 + j * 32 + (lane & 31);
+                float2* p = &U[(size_t)h * nx + w];
+                float2 o = make_float2(accr[i][j][v], acci[i][j][v]);
+                if (accumulate) {
+                    const float2 u = *p;
+                    o.x += u.x;
+                    o.y += u.y;
+                }
+                *p = o;
+            }
+}
+
 static int nbits_for(int64_t P) {
     int lg = 0;
     while ((1ll << lg) < P) lg++;
@@ -364,6 +519,22 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
     void* Yl = fb + ybytes + xbytes;
     void* Xl = fb + 2 * ybytes + xbytes;
     const dim3 ggrid((unsigned)(nx / GT), (unsigned)(ny / GT));
+    // 256 x 128 tiles where the sides allow, unless the 128 x 128 kernel is asked for
+    const bool big = ny % GT2M == 0 && nx % GT2N == 0 && !(flags & OLX_LAYERED_TILE128);
+    const dim3 ggrid2((unsigned)(nx / GT2N), (unsigned)(ny / GT2M));
+    auto gemm = [&](const void* Ya, const void* Xb, int acc) {
+        if (precision == OLX_LAYERED_F16) {
+            if (big)
+                k_lay_gemm2<_Float16><<<ggrid2, 256, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+            else
+                k_lay_gemm<_Float16><<<ggrid, 256, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+        } else {
+            if (big)
+                k_lay_gemm2<__bf16><<<ggrid2, 256, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+            else
+                k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+        }
+    };
     bool first = true;
     for (int64_t j0 = 0; j0 < n; j0 += kc) {
         const dim3 gy((unsigned)((kc + 255) / 256), (unsigned)((ny + GEN_RB - 1) / GEN_RB)),
@@ -385,14 +556,10 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
         {
             TimedLaunch tl(ctx, "layered_gemm");
             const int acc = (first && !acc0) ? 0 : 1;
-            if (precision == OLX_LAYERED_F16) {
-                k_lay_gemm<_Float16><<<ggrid, 256, 0, s>>>((const _Float16*)Yh, (const _Float16*)Xh, ldk, (float2*)d_U, (int)nx, acc);
-            } else {
-                k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Yh, (const __bf16*)Xh, ldk, (float2*)d_U, (int)nx, acc);
-                if (precision == OLX_LAYERED_BF16X3) {
-                    k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Yh, (const __bf16*)Xl, ldk, (float2*)d_U, (int)nx, 1);
-                    k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Yl, (const __bf16*)Xh, ldk, (float2*)d_U, (int)nx, 1);
-                }
+            gemm(Yh, Xh, acc);
+            if (precision == OLX_LAYERED_BF16X3) {
+                gemm(Yh, Xl, 1);
+                gemm(Yl, Xh, 1);
             }
         }
         OLX_TRY(check_launch("layered gemm"));

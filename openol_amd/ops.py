@@ -375,14 +375,15 @@ _PREC = {"f16": _lib.LAYERED_F16, "bf16": _lib.LAYERED_BF16, "bf16x3": _lib.LAYE
 
 
 def CghLayered(obj, U, ny, nx, py, px, lam, nlayers, zmin, zmax, precision="f16",
-               accumulate=False):
+               accumulate=False, tile128=False):
     """Layered Fresnel CGH as complex GEMMs on MFMA (no reference counterpart).
-    Points are snapped to the centres of nlayers uniform layers over [zmin, zmax)."""
+    Points are snapped to the centres of nlayers uniform layers over [zmin, zmax).
+    tile128 forces the 128 x 128 work-group tile where 256 x 128 would be used."""
     _out2d(U, ny, nx, torch.complex64, "U")
+    flags = (_lib.CGH_ACCUMULATE if accumulate else 0) | (_lib.LAYERED_TILE128 if tile128 else 0)
     check(_lib.load().olx_cgh_layered_c64(_lib.ctx(U.device.index), obj.ptr(), obj.size, _ptr(U),
                                           ny, nx, py, px, lam, nlayers, zmin, zmax,
-                                          _PREC[precision],
-                                          _lib.CGH_ACCUMULATE if accumulate else 0), "CghLayered")
+                                          _PREC[precision], flags), "CghLayered")
     return U
 
 

@@ -81,6 +81,28 @@ def test_layered_chunking_accumulate_and_empty():
     assert torch.count_nonzero(W).item() == 0
 
 
+@pytest.mark.parametrize("prec", ["f16", "bf16x3"])
+@pytest.mark.parametrize("shape", [(256, 128), (512, 384)])
+def test_layered_wide_tile(prec, shape):
+    """Sides that are multiples of 256 x 128 run the 256 x 128 work-group tile.  Its
+    sums run in the 128 x 128 kernel's K order, so the two agree bit for bit (more
+    points than one factor chunk, accumulate on); the oracle bounds the error."""
+    ny, nx = shape
+    n, nl, zmin, zmax = 33000, 8, 0.10, 0.20
+    pts = cloud(n, 62, ny, nx, zmin, zmax)
+    obj = ol().Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ol().CghLayered(obj, U, ny, nx, P, P, LAM, nl, zmin, zmax, precision=prec)
+    V = torch.empty_like(U)
+    ol().CghLayered(obj, V, ny, nx, P, P, LAM, nl, zmin, zmax, precision=prec, tile128=True)
+    assert torch.equal(U, V)
+    zl = snapped(pts, zmin, zmax, nl)
+    rng = np.random.default_rng(4)
+    ph, pw = rng.integers(0, ny, 256), rng.integers(0, nx, 256)
+    ref = olref.cgh_pixels(pts["x"], pts["y"], zl, ph, pw, P, P, LAM, pts["amp"], pts["phase"])
+    assert olref.rel_l2(U.cpu().numpy()[ph, pw], ref) < 1e-3
+
+
 def test_layered_rejects_unsupported_shapes():
     from openol_amd._lib import OlxError
     obj = ol().Object([1e-4], [1e-4], [0.15])
