@@ -301,8 +301,10 @@ k_lay_gemm(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* _
 // Stages go from global memory straight into LDS (global_load_lds_dwordx4: lane l of
 // a wave writes 16 bytes at the wave's LDS base + 16 l), so no registers hold them.
 // LDS rows are 64 halves (128 B) without padding; the 16-byte chunk c of row r sits
-// in slot c ^ (r & 7), which spreads the 8 rows a ds_read_b128 group touches over
-// all banks.  Three stages of 256 + 128 rows (144 KiB): loads run two stages ahead
+// in slot c ^ ((r >> 1) & 7).  A ds_read_b128 serves 16 lanes (16 rows, one chunk)
+// from the 64 banks at a time: rows of one parity take 8 distinct slots in one half
+// of a 256-byte bank line and rows of the other parity the other half, so no two
+// lanes of a group share a bank.  Three stages of 256 + 128 rows (144 KiB): loads run two stages ahead
 // of the MFMAs.  The K order of the sums is that of k_lay_gemm, so both kernels give
 // identical results.
 constexpr int GT2M = 256, GT2N = 128;
@@ -320,63 +322,134 @@ __device__ __forceinline__ void stage_barrier() {
     asm volatile("" ::: "memory");
 }
 
+// fragment of K step kk from the swizzled stage (sa, sb): lane holds row (lane % 32),
+// chunk 2 kk + lane / 32, found at byte soff = (lane % 32) * ROWB + 16 (chunk ^ ((lane >> 1) & 7))
+// (row bases are multiples of 32, so (row >> 1) & 7 = ((lane % 32) >> 1) & 7)
+__device__ __forceinline__ void lay_lfrag(const char* sa, const char* sb, int soff, int wr, int wc,
+                                          uint4 (&fa)[4], uint4 (&fb)[2]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) fa[i] = *(const uint4*)(sa + (wr * 128 + i * 32) * ROWB + soff);
+#pragma unroll
+    for (int j = 0; j < 2; j++) fb[j] = *(const uint4*)(sb + (wc * 64 + j * 32) * ROWB + soff);
+}
+
+__device__ __forceinline__ void lay_bi(const uint4 (&fb)[2], uint4 (&bi)[2]) {
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+        bi[j] = make_uint4(bi_of_br(fb[j].x), bi_of_br(fb[j].y), bi_of_br(fb[j].z), bi_of_br(fb[j].w));
+}
+
+// 16 MFMAs of one K step from (fa, fb, bi).  With NEXT, the Bi operand of the next step
+// (whose fragments were read at the start of this one) is formed during the second
+// half: one MFMA, then two VALU operations, so the VALU work issues while the matrix
+// unit is busy instead of in front of the next step's MFMAs.
+template <typename T, bool NEXT>
+__device__ __forceinline__ void lay_step(const uint4 (&fa)[4], const uint4 (&fb)[2],
+                                         const uint4 (&bi)[2], const uint4 (&fbn)[2],
+                                         uint4 (&bin)[2], f16v_t (&accr)[4][2],
+                                         f16v_t (&acci)[4][2]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            accr[i][j] = mfma<T>(fa[i], fb[j], accr[i][j]);
+            acci[i][j] = mfma<T>(fa[i], bi[j], acci[i][j]);
+        }
+    if constexpr (NEXT) {
+        lay_bi(fbn, bin);
+        // 8 MFMAs alone (the next fragments are still arriving), then 8 x (MFMA, 2 VALU)
+        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+        }
+    }
+}
+
+// direct loads of one stage (K columns k0 .. k0 + 63) into (da, db): the instruction for
+// 8-row group g writes bytes g*1024 .. +1023; lane l fills row r = 8g + l/8, slot l%8
+// with global chunk (l%8) ^ ((r >> 1) & 7) = (l%8) ^ (4 (g & 1) + l/16) -- g = wave + 4q
+// keeps the parity of the wave index (loff is that lane's element offset)
+template <typename T>
+__device__ __forceinline__ void lay_dload(const T* gA, const T* gB, size_t ldk, size_t k0,
+                                          size_t loff, int wave, char* da, char* db) {
+#pragma unroll
+    for (int q = 0; q < GT2M / 32; q++) {
+        const int g = wave + 4 * q;
+        __builtin_amdgcn_global_load_lds((const void*)(gA + (size_t)(8 * g) * ldk + k0 + loff),
+                                         (lds_ptr_t)(da + g * 8 * ROWB), 16, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < GT2N / 32; q++) {
+        const int g = wave + 4 * q;
+        __builtin_amdgcn_global_load_lds((const void*)(gB + (size_t)(8 * g) * ldk + k0 + loff),
+                                         (lds_ptr_t)(db + g * 8 * ROWB), 16, 0, 0);
+    }
+}
+
+// s_waitcnt immediates (expcnt and lgkmcnt left open): all of this wave's loads
+// landed, or all but the newest stage's 12 (loads complete in issue order)
+constexpr int WAIT_VM0 = 0x0F70, WAIT_VM12 = 0x0F7C;
+static_assert(GT2M / 32 + GT2N / 32 == 12, "direct loads per wave and stage");
+
+// One stage: read (sa, sb), load stage st + 2 into (da, db) -- the buffers read in
+// stage st - 1, which ended with a barrier -- and finish with stage st + 1 landed in
+// (na, nb) and its first fragments in (fa0, fb0).  The six LDS pointers are restrict:
+// after inlining, the reads carry alias scopes that tell them apart from the direct
+// loads in flight, so the compiler waits for those loads only where they are read.
+// Nothing is conditional (branches would merge the compiler's wait bookkeeping into
+// full waits): past the end, the last stage is loaded again into a buffer that is not
+// read any more, and the fragments read are not used.
+template <typename T>
+__device__ __forceinline__ void lay_stage(
+    const T* gA, const T* gB, size_t ldk, size_t loff, int wave, int wr, int wc, const int (&soff)[4],
+    int st, int nsteps, const char* __restrict__ sa, const char* __restrict__ sb,
+    const char* __restrict__ na, const char* __restrict__ nb, char* __restrict__ da,
+    char* __restrict__ db, uint4 (&fa0)[4], uint4 (&fb0)[2], uint4 (&fa1)[4], uint4 (&fb1)[2],
+    uint4 (&bi0)[2], uint4 (&bi1)[2], f16v_t (&accr)[4][2], f16v_t (&acci)[4][2]) {
+    lay_dload<T>(gA, gB, ldk, (size_t)min(st + 2, nsteps - 1) * KS, loff, wave, da, db);
+    // sched_barrier(0) fences: each step's fragment reads for the next step are issued
+    // before its 16 MFMAs (about 512 cycles), which cover their latency
+    lay_lfrag(sa, sb, soff[1], wr, wc, fa1, fb1);
+    __builtin_amdgcn_sched_barrier(0);
+    lay_step<T, true>(fa0, fb0, bi0, fb1, bi1, accr, acci);
+    __builtin_amdgcn_sched_barrier(0);
+    lay_lfrag(sa, sb, soff[2], wr, wc, fa0, fb0);
+    __builtin_amdgcn_sched_barrier(0);
+    lay_step<T, true>(fa1, fb1, bi1, fb0, bi0, accr, acci);
+    __builtin_amdgcn_sched_barrier(0);
+    lay_lfrag(sa, sb, soff[3], wr, wc, fa1, fb1);
+    __builtin_amdgcn_sched_barrier(0);
+    lay_step<T, true>(fa0, fb0, bi0, fb1, bi1, accr, acci);
+    __builtin_amdgcn_sched_barrier(0);
+    lay_step<T, false>(fa1, fb1, bi1, fb1, bi1, accr, acci);
+    __builtin_amdgcn_s_waitcnt(WAIT_VM12);
+    stage_barrier();
+    lay_lfrag(na, nb, soff[0], wr, wc, fa0, fb0);
+    lay_bi(fb0, bi0);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256, 1)
 k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* __restrict__ U,
             int nx, int accumulate) {
-    // one LDS object per stage and matrix: reads of one stage are then known not to
-    // alias the direct loads in flight into another, and need not wait for them
     __shared__ __align__(16) char sA0[GT2M * ROWB], sA1[GT2M * ROWB], sA2[GT2M * ROWB];
     __shared__ __align__(16) char sB0[GT2N * ROWB], sB1[GT2N * ROWB], sB2[GT2N * ROWB];
+    char *a0 = sA0, *a1 = sA1, *a2 = sA2, *b0 = sB0, *b1 = sB1, *b2 = sB2;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave >> 1, wc = wave & 1;
     const int h0 = blockIdx.y * GT2M, w0 = blockIdx.x * GT2N;
     const T* gA = A + (size_t)h0 * ldk;
     const T* gB = Br + (size_t)w0 * ldk;
-    // direct loads: instruction for 8-row group g writes LDS bytes g*1024 .. +1023;
-    // lane l fills row 8g + l/8, slot l%8 with global chunk (l%8) ^ (l/8)
-    const size_t loff = (size_t)(lane >> 3) * ldk + (size_t)(((lane & 7) ^ (lane >> 3)) * 8);
-#define OLX_DLOAD(SA, SB, k0)                                                                    \
-    do {                                                                                         \
-        _Pragma("unroll") for (int q = 0; q < GT2M / 32; q++) {                                  \
-            const int g_ = wave + 4 * q;                                                         \
-            __builtin_amdgcn_global_load_lds(                                                    \
-                (const void*)(gA + (size_t)(8 * g_) * ldk + (k0) + loff),                        \
-                (lds_ptr_t)(&SA[g_ * 8 * ROWB]), 16, 0, 0);                                      \
-        }                                                                                        \
-        _Pragma("unroll") for (int q = 0; q < GT2N / 32; q++) {                                  \
-            const int g_ = wave + 4 * q;                                                         \
-            __builtin_amdgcn_global_load_lds(                                                    \
-                (const void*)(gB + (size_t)(8 * g_) * ldk + (k0) + loff),                        \
-                (lds_ptr_t)(&SB[g_ * 8 * ROWB]), 16, 0, 0);                                      \
-        }                                                                                        \
-    } while (0)
-    // fragment of K step kk: lane holds row (lane % 32), chunk 2 kk + lane / 32, found in
-    // slot chunk ^ (row & 7) = chunk ^ (lane & 7)
+    const size_t loff = (size_t)(lane >> 3) * ldk +
+                        (size_t)(((lane & 7) ^ (4 * (wave & 1) + (lane >> 4))) * 8);
     int soff[KS / 16];
 #pragma unroll
     for (int kk = 0; kk < KS / 16; kk++)
-        soff[kk] = (lane & 31) * ROWB + (((2 * kk + (lane >> 5)) ^ (lane & 7)) * 16);
-    uint4 fa0[4], fb0[2], fa1[4], fb1[2];
-#define OLX_LFRAG(SA, SB, kk, FA, FB)                                                            \
-    do {                                                                                         \
-        _Pragma("unroll") for (int i = 0; i < 4; i++) FA[i] =                                    \
-            *(const uint4*)(&SA[(wr * 128 + i * 32) * ROWB + soff[kk]]);                        \
-        _Pragma("unroll") for (int j = 0; j < 2; j++) FB[j] =                                    \
-            *(const uint4*)(&SB[(wc * 64 + j * 32) * ROWB + soff[kk]]);                         \
-    } while (0)
-#define OLX_STEP(FA, FB)                                                                         \
-    do {                                                                                         \
-        uint4 bi_[2];                                                                            \
-        _Pragma("unroll") for (int j = 0; j < 2; j++) bi_[j] =                                   \
-            make_uint4(bi_of_br(FB[j].x), bi_of_br(FB[j].y), bi_of_br(FB[j].z), bi_of_br(FB[j].w)); \
-        _Pragma("unroll") for (int i = 0; i < 4; i++)                                            \
-            _Pragma("unroll") for (int j = 0; j < 2; j++) {                                      \
-                accr[i][j] = mfma<T>(FA[i], FB[j], accr[i][j]);                                  \
-                acci[i][j] = mfma<T>(FA[i], bi_[j], acci[i][j]);                                 \
-            }                                                                                    \
-    } while (0)
+        soff[kk] = (lane & 31) * ROWB + (((2 * kk + (lane >> 5)) ^ ((lane >> 1) & 7)) * 16);
+    uint4 fa0[4], fb0[2], fa1[4], fb1[2], bi0[2], bi1[2];
     f16v_t accr[4][2], acci[4][2];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -386,47 +459,30 @@ k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* 
             for (int v = 0; v < 16; v++) accr[i][j][v] = acci[i][j][v] = 0.f;
 
     static_assert(KS == 64, "four 16-wide K steps of 8 16-byte chunks per stage");
-    // s_waitcnt immediates (expcnt and lgkmcnt left open): all of this wave's loads
-    // landed, or all but the newest stage's 12
-    constexpr int WAIT_VM0 = 0x0F70, WAIT_VM12 = 0x0F7C;
-    static_assert(GT2M / 32 + GT2N / 32 == 12, "direct loads per wave and stage");
     const int nsteps = ldk / KS;
-    // stage st reads (SA, SB); stage st + 1 is in (SA1_, SB1_); stage st + 2 is loaded
-    // into (SA2_, SB2_), the buffers read in stage st - 1, which ended with a barrier
-#define OLX_STAGE(SA, SB, SA1_, SB1_, SA2_, SB2_, st)                                            \
-    do {                                                                                         \
-        const bool ahead_ = (st) + 2 < nsteps;                                                   \
-        if (ahead_) OLX_DLOAD(SA2_, SB2_, ((st) + 2) * KS);                                      \
-        OLX_LFRAG(SA, SB, 1, fa1, fb1);                                                          \
-        OLX_STEP(fa0, fb0);                                                                      \
-        OLX_LFRAG(SA, SB, 2, fa0, fb0);                                                          \
-        OLX_STEP(fa1, fb1);                                                                      \
-        OLX_LFRAG(SA, SB, 3, fa1, fb1);                                                          \
-        OLX_STEP(fa0, fb0);                                                                      \
-        OLX_STEP(fa1, fb1);                                                                      \
-        if (ahead_) __builtin_amdgcn_s_waitcnt(WAIT_VM12);                                       \
-        else __builtin_amdgcn_s_waitcnt(WAIT_VM0);                                               \
-        stage_barrier();                                                                         \
-        if ((st) + 1 < nsteps) OLX_LFRAG(SA1_, SB1_, 0, fa0, fb0);                               \
-    } while (0)
-    OLX_DLOAD(sA0, sB0, 0);
-    if (nsteps > 1) {
-        OLX_DLOAD(sA1, sB1, KS);
-        __builtin_amdgcn_s_waitcnt(WAIT_VM12);
-    } else {
-        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
-    }
+    lay_dload<T>(gA, gB, (size_t)ldk, 0, loff, wave, a0, b0);
+    lay_dload<T>(gA, gB, (size_t)ldk, (size_t)min(1, nsteps - 1) * KS, loff, wave, a1, b1);
+    __builtin_amdgcn_s_waitcnt(WAIT_VM12);
     stage_barrier();
-    OLX_LFRAG(sA0, sB0, 0, fa0, fb0);
-    for (int st = 0; st < nsteps; st += 3) {
-        OLX_STAGE(sA0, sB0, sA1, sB1, sA2, sB2, st);
-        if (st + 1 < nsteps) OLX_STAGE(sA1, sB1, sA2, sB2, sA0, sB0, st + 1);
-        if (st + 2 < nsteps) OLX_STAGE(sA2, sB2, sA0, sB0, sA1, sB1, st + 2);
+    lay_lfrag(a0, b0, soff[0], wr, wc, fa0, fb0);
+    lay_bi(fb0, bi0);
+    int st = 0;
+    for (; st + 3 <= nsteps; st += 3) {
+        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st, nsteps, a0, b0, a1, b1, a2, b2,
+                     fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
+        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 1, nsteps, a1, b1, a2, b2, a0, b0,
+                     fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
+        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 2, nsteps, a2, b2, a0, b0, a1, b1,
+                     fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
     }
-#undef OLX_STAGE
-#undef OLX_DLOAD
-#undef OLX_LFRAG
-#undef OLX_STEP
+    if (st < nsteps)
+        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st, nsteps, a0, b0, a1, b1, a2, b2,
+                     fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
+    if (st + 1 < nsteps)
+        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 1, nsteps, a1, b1, a2, b2, a0, b0,
+                     fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
+    // no direct load may still be writing LDS when the work-group ends
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
     // epilogue: lane holds column w = l % 32 and rows 8 (v/4) + 4 (l/32) + v % 4
 #pragma unroll
     for (int i = 0; i < 4; i++)
