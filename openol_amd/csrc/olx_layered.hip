@@ -293,21 +293,20 @@ k_lay_gemm(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* _
 }
 
 // The same product with a 256 x 128 work-group tile for hologram sides that are
-// multiples of 256 (rows) and 128 (columns).  Wave (wr, wc) owns 128 x 64 outputs:
-// per 16-wide K step it reads 4 A and 2 B fragments from LDS and issues 16 MFMAs
-// (the 64 x 64 wave tile above reads 4 fragments per 8).  256 accumulator registers
-// leave one wave per SIMD, so the fragments of the next K step are read into a
-// second register set while the current step's MFMAs run.
+// multiples of 256 (rows) and 128 (columns): 8 waves in a 4 x 2 grid, each owning
+// 64 x 64 outputs (8 MFMAs per 16-wide K step), two waves per SIMD.
 // Stages go from global memory straight into LDS (global_load_lds_dwordx4: lane l of
 // a wave writes 16 bytes at the wave's LDS base + 16 l), so no registers hold them.
 // LDS rows are 64 halves (128 B) without padding; the 16-byte chunk c of row r sits
 // in slot c ^ ((r >> 1) & 7).  A ds_read_b128 serves 16 lanes (16 rows, one chunk)
 // from the 64 banks at a time: rows of one parity take 8 distinct slots in one half
 // of a 256-byte bank line and rows of the other parity the other half, so no two
-// lanes of a group share a bank.  Three stages of 256 + 128 rows (144 KiB): loads run two stages ahead
-// of the MFMAs.  The K order of the sums is that of k_lay_gemm, so both kernels give
-// identical results.
-constexpr int GT2M = 256, GT2N = 128;
+// lanes of a group share a bank.  Three stages of 256 + 128 rows (144 KiB): loads
+// run two stages ahead of the MFMAs.  Each K step's fragments are read while the
+// previous step's MFMAs run, and its Bi operand is formed between them.  The K order
+// of the sums is that of k_lay_gemm, so both kernels give identical results.
+// (A 4-wave form with 128 x 64 wave tiles, one wave per SIMD, measured 2% slower.)
+constexpr int GT2M = 256, GT2N = 128, GT2_WAVES = 8;
 constexpr int ROWB = KS * 2;  // bytes per LDS row
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
@@ -325,10 +324,11 @@ __device__ __forceinline__ void stage_barrier() {
 // fragment of K step kk from the swizzled stage (sa, sb): lane holds row (lane % 32),
 // chunk 2 kk + lane / 32, found at byte soff = (lane % 32) * ROWB + 16 (chunk ^ ((lane >> 1) & 7))
 // (row bases are multiples of 32, so (row >> 1) & 7 = ((lane % 32) >> 1) & 7)
+template <int MI>
 __device__ __forceinline__ void lay_lfrag(const char* sa, const char* sb, int soff, int wr, int wc,
-                                          uint4 (&fa)[4], uint4 (&fb)[2]) {
+                                          uint4 (&fa)[MI], uint4 (&fb)[2]) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) fa[i] = *(const uint4*)(sa + (wr * 128 + i * 32) * ROWB + soff);
+    for (int i = 0; i < MI; i++) fa[i] = *(const uint4*)(sa + (wr * 32 * MI + i * 32) * ROWB + soff);
 #pragma unroll
     for (int j = 0; j < 2; j++) fb[j] = *(const uint4*)(sb + (wc * 64 + j * 32) * ROWB + soff);
 }
@@ -339,17 +339,17 @@ __device__ __forceinline__ void lay_bi(const uint4 (&fb)[2], uint4 (&bi)[2]) {
         bi[j] = make_uint4(bi_of_br(fb[j].x), bi_of_br(fb[j].y), bi_of_br(fb[j].z), bi_of_br(fb[j].w));
 }
 
-// 16 MFMAs of one K step from (fa, fb, bi).  With NEXT, the Bi operand of the next step
+// The 4 MI MFMAs of one K step from (fa, fb, bi).  With NEXT, the Bi operand of the next step
 // (whose fragments were read at the start of this one) is formed during the second
-// half: one MFMA, then two VALU operations, so the VALU work issues while the matrix
-// unit is busy instead of in front of the next step's MFMAs.
-template <typename T, bool NEXT>
-__device__ __forceinline__ void lay_step(const uint4 (&fa)[4], const uint4 (&fb)[2],
+// half, each MFMA followed by an equal share of its 16 VALU operations, so the VALU
+// work issues while the matrix unit is busy instead of in front of the next step.
+template <typename T, bool NEXT, int MI>
+__device__ __forceinline__ void lay_step(const uint4 (&fa)[MI], const uint4 (&fb)[2],
                                          const uint4 (&bi)[2], const uint4 (&fbn)[2],
-                                         uint4 (&bin)[2], f16v_t (&accr)[4][2],
-                                         f16v_t (&acci)[4][2]) {
+                                         uint4 (&bin)[2], f16v_t (&accr)[MI][2],
+                                         f16v_t (&acci)[MI][2]) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             accr[i][j] = mfma<T>(fa[i], fb[j], accr[i][j]);
@@ -357,12 +357,14 @@ __device__ __forceinline__ void lay_step(const uint4 (&fa)[4], const uint4 (&fb)
         }
     if constexpr (NEXT) {
         lay_bi(fbn, bin);
-        // 8 MFMAs alone (the next fragments are still arriving), then 8 x (MFMA, 2 VALU)
-        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+        // half of the MFMAs alone (the next fragments are still arriving), then
+        // (MFMA, VALU ...) pairs carrying the 16 VALU operations
+        constexpr int H = 2 * MI;
+        __builtin_amdgcn_sched_group_barrier(0x008, H, 0);
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
+        for (int q = 0; q < H; q++) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 16 / H, 0);
         }
     }
 }
@@ -371,67 +373,77 @@ __device__ __forceinline__ void lay_step(const uint4 (&fa)[4], const uint4 (&fb)
 // 8-row group g writes bytes g*1024 .. +1023; lane l fills row r = 8g + l/8, slot l%8
 // with global chunk (l%8) ^ ((r >> 1) & 7) = (l%8) ^ (4 (g & 1) + l/16) -- g = wave + 4q
 // keeps the parity of the wave index (loff is that lane's element offset)
-template <typename T>
+template <typename T, int WAVES>
 __device__ __forceinline__ void lay_dload(const T* gA, const T* gB, size_t ldk, size_t k0,
                                           size_t loff, int wave, char* da, char* db) {
 #pragma unroll
-    for (int q = 0; q < GT2M / 32; q++) {
-        const int g = wave + 4 * q;
+    for (int q = 0; q < GT2M / 8 / WAVES; q++) {
+        const int g = wave + WAVES * q;
         __builtin_amdgcn_global_load_lds((const void*)(gA + (size_t)(8 * g) * ldk + k0 + loff),
                                          (lds_ptr_t)(da + g * 8 * ROWB), 16, 0, 0);
     }
 #pragma unroll
-    for (int q = 0; q < GT2N / 32; q++) {
-        const int g = wave + 4 * q;
+    for (int q = 0; q < GT2N / 8 / WAVES; q++) {
+        const int g = wave + WAVES * q;
         __builtin_amdgcn_global_load_lds((const void*)(gB + (size_t)(8 * g) * ldk + k0 + loff),
                                          (lds_ptr_t)(db + g * 8 * ROWB), 16, 0, 0);
     }
 }
 
-// s_waitcnt immediates (expcnt and lgkmcnt left open): all of this wave's loads
-// landed, or all but the newest stage's 12 (loads complete in issue order)
-constexpr int WAIT_VM0 = 0x0F70, WAIT_VM12 = 0x0F7C;
-static_assert(GT2M / 32 + GT2N / 32 == 12, "direct loads per wave and stage");
+// s_waitcnt immediates (expcnt and, except in the last, lgkmcnt left open): all of this
+// wave's loads landed, or all but the newest stage's (loads complete in issue order);
+// the last also waits for all of this wave's LDS reads
+constexpr int WAIT_VM0 = 0x0F70;
+// direct loads per wave and stage: 12 with 4 waves, 6 with 8
+template <int WAVES> constexpr int wait_vm_stage() { return 0x0F70 | ((GT2M + GT2N) / 8 / WAVES); }
+template <int WAVES> constexpr int wait_vm_stage_lgkm0() { return 0x0070 | ((GT2M + GT2N) / 8 / WAVES); }
+static_assert((GT2M + GT2N) / 8 / 4 == 12 && (GT2M + GT2N) / 8 / 8 == 6, "vmcnt fits 4 bits");
 
-// One stage: read (sa, sb), load stage st + 2 into (da, db) -- the buffers read in
-// stage st - 1, which ended with a barrier -- and finish with stage st + 1 landed in
-// (na, nb) and its first fragments in (fa0, fb0).  The six LDS pointers are restrict:
+// One stage: read (sa, sb), load stage st + 2 into (da, db) -- the buffers of stage
+// st - 1, whose reads all completed before that stage's barrier -- and finish with
+// stage st + 1 landed in (na, nb) and its first fragments and Bi operand in (fa0, fb0,
+// bi0).  The barrier sits before the last step's MFMAs.  The six LDS pointers are restrict:
 // after inlining, the reads carry alias scopes that tell them apart from the direct
 // loads in flight, so the compiler waits for those loads only where they are read.
 // Nothing is conditional (branches would merge the compiler's wait bookkeeping into
 // full waits): past the end, the last stage is loaded again into a buffer that is not
 // read any more, and the fragments read are not used.
-template <typename T>
+template <typename T, int WAVES, int MI>
 __device__ __forceinline__ void lay_stage(
     const T* gA, const T* gB, size_t ldk, size_t loff, int wave, int wr, int wc, const int (&soff)[4],
     int st, int nsteps, const char* __restrict__ sa, const char* __restrict__ sb,
     const char* __restrict__ na, const char* __restrict__ nb, char* __restrict__ da,
-    char* __restrict__ db, uint4 (&fa0)[4], uint4 (&fb0)[2], uint4 (&fa1)[4], uint4 (&fb1)[2],
-    uint4 (&bi0)[2], uint4 (&bi1)[2], f16v_t (&accr)[4][2], f16v_t (&acci)[4][2]) {
-    lay_dload<T>(gA, gB, ldk, (size_t)min(st + 2, nsteps - 1) * KS, loff, wave, da, db);
+    char* __restrict__ db, uint4 (&fa0)[MI], uint4 (&fb0)[2], uint4 (&fa1)[MI], uint4 (&fb1)[2],
+    uint4 (&bi0)[2], uint4 (&bi1)[2], f16v_t (&accr)[MI][2], f16v_t (&acci)[MI][2]) {
+    lay_dload<T, WAVES>(gA, gB, ldk, (size_t)min(st + 2, nsteps - 1) * KS, loff, wave, da, db);
     // sched_barrier(0) fences: each step's fragment reads for the next step are issued
-    // before its 16 MFMAs (about 512 cycles), which cover their latency
+    // before its MFMAs (8 per wave, 16 per SIMD), which cover their latency
     lay_lfrag(sa, sb, soff[1], wr, wc, fa1, fb1);
     __builtin_amdgcn_sched_barrier(0);
-    lay_step<T, true>(fa0, fb0, bi0, fb1, bi1, accr, acci);
+    lay_step<T, true, MI>(fa0, fb0, bi0, fb1, bi1, accr, acci);
     __builtin_amdgcn_sched_barrier(0);
     lay_lfrag(sa, sb, soff[2], wr, wc, fa0, fb0);
     __builtin_amdgcn_sched_barrier(0);
-    lay_step<T, true>(fa1, fb1, bi1, fb0, bi0, accr, acci);
+    lay_step<T, true, MI>(fa1, fb1, bi1, fb0, bi0, accr, acci);
     __builtin_amdgcn_sched_barrier(0);
     lay_lfrag(sa, sb, soff[3], wr, wc, fa1, fb1);
     __builtin_amdgcn_sched_barrier(0);
-    lay_step<T, true>(fa0, fb0, bi0, fb1, bi1, accr, acci);
+    lay_step<T, true, MI>(fa0, fb0, bi0, fb1, bi1, accr, acci);
     __builtin_amdgcn_sched_barrier(0);
-    lay_step<T, false>(fa1, fb1, bi1, fb1, bi1, accr, acci);
-    __builtin_amdgcn_s_waitcnt(WAIT_VM12);
+    // The last step's operands are in registers: once this wave's reads of the stage
+    // are complete and stage st + 1 has landed, the barrier can come before the last
+    // step's MFMAs, which then cover the next stage's first reads (after the barrier
+    // the other waves may start loading into this stage's buffers).
+    __builtin_amdgcn_s_waitcnt(wait_vm_stage_lgkm0<WAVES>());
     stage_barrier();
     lay_lfrag(na, nb, soff[0], wr, wc, fa0, fb0);
-    lay_bi(fb0, bi0);
+    __builtin_amdgcn_sched_barrier(0);
+    lay_step<T, true, MI>(fa1, fb1, bi1, fb0, bi0, accr, acci);
+    __builtin_amdgcn_sched_barrier(0);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256, 1)
+template <typename T, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES, 1)
 k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* __restrict__ U,
             int nx, int accumulate) {
     __shared__ __align__(16) char sA0[GT2M * ROWB], sA1[GT2M * ROWB], sA2[GT2M * ROWB];
@@ -439,20 +451,22 @@ k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* 
     char *a0 = sA0, *a1 = sA1, *a2 = sA2, *b0 = sB0, *b1 = sB1, *b2 = sB2;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int MI = 16 / WAVES;  // A fragments (32 rows each) per wave
     const int wr = wave >> 1, wc = wave & 1;
     const int h0 = blockIdx.y * GT2M, w0 = blockIdx.x * GT2N;
     const T* gA = A + (size_t)h0 * ldk;
     const T* gB = Br + (size_t)w0 * ldk;
     const size_t loff = (size_t)(lane >> 3) * ldk +
                         (size_t)(((lane & 7) ^ (4 * (wave & 1) + (lane >> 4))) * 8);
+    static_assert(WAVES % 2 == 0, "8-row groups g = wave + WAVES q keep the wave's parity");
     int soff[KS / 16];
 #pragma unroll
     for (int kk = 0; kk < KS / 16; kk++)
         soff[kk] = (lane & 31) * ROWB + (((2 * kk + (lane >> 5)) ^ ((lane >> 1) & 7)) * 16);
-    uint4 fa0[4], fb0[2], fa1[4], fb1[2], bi0[2], bi1[2];
-    f16v_t accr[4][2], acci[4][2];
+    uint4 fa0[MI], fb0[2], fa1[MI], fb1[2], bi0[2], bi1[2];
+    f16v_t accr[MI][2], acci[MI][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++)
 #pragma unroll
@@ -460,37 +474,37 @@ k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* 
 
     static_assert(KS == 64, "four 16-wide K steps of 8 16-byte chunks per stage");
     const int nsteps = ldk / KS;
-    lay_dload<T>(gA, gB, (size_t)ldk, 0, loff, wave, a0, b0);
-    lay_dload<T>(gA, gB, (size_t)ldk, (size_t)min(1, nsteps - 1) * KS, loff, wave, a1, b1);
-    __builtin_amdgcn_s_waitcnt(WAIT_VM12);
+    lay_dload<T, WAVES>(gA, gB, (size_t)ldk, 0, loff, wave, a0, b0);
+    lay_dload<T, WAVES>(gA, gB, (size_t)ldk, (size_t)min(1, nsteps - 1) * KS, loff, wave, a1, b1);
+    __builtin_amdgcn_s_waitcnt(wait_vm_stage<WAVES>());
     stage_barrier();
     lay_lfrag(a0, b0, soff[0], wr, wc, fa0, fb0);
     lay_bi(fb0, bi0);
     int st = 0;
     for (; st + 3 <= nsteps; st += 3) {
-        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st, nsteps, a0, b0, a1, b1, a2, b2,
+        lay_stage<T, WAVES, MI>(gA, gB, ldk, loff, wave, wr, wc, soff, st, nsteps, a0, b0, a1, b1, a2, b2,
                      fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
-        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 1, nsteps, a1, b1, a2, b2, a0, b0,
+        lay_stage<T, WAVES, MI>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 1, nsteps, a1, b1, a2, b2, a0, b0,
                      fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
-        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 2, nsteps, a2, b2, a0, b0, a1, b1,
+        lay_stage<T, WAVES, MI>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 2, nsteps, a2, b2, a0, b0, a1, b1,
                      fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
     }
     if (st < nsteps)
-        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st, nsteps, a0, b0, a1, b1, a2, b2,
+        lay_stage<T, WAVES, MI>(gA, gB, ldk, loff, wave, wr, wc, soff, st, nsteps, a0, b0, a1, b1, a2, b2,
                      fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
     if (st + 1 < nsteps)
-        lay_stage<T>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 1, nsteps, a1, b1, a2, b2, a0, b0,
+        lay_stage<T, WAVES, MI>(gA, gB, ldk, loff, wave, wr, wc, soff, st + 1, nsteps, a1, b1, a2, b2, a0, b0,
                      fa0, fb0, fa1, fb1, bi0, bi1, accr, acci);
     // no direct load may still be writing LDS when the work-group ends
     __builtin_amdgcn_s_waitcnt(WAIT_VM0);
     // epilogue: lane holds column w = l % 32 and rows 8 (v/4) + 4 (l/32) + v % 4
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                const int h = h0 + wr * 128 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int h = h0 + wr * 32 * MI + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int w = w0 + wc * 64 + j * 32 + (lane & 31);
                 float2* p = &U[(size_t)h * nx + w];
                 float2 o = make_float2(accr[i][j][v], acci[i][j][v]);
@@ -578,15 +592,16 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
     // 256 x 128 tiles where the sides allow, unless the 128 x 128 kernel is asked for
     const bool big = ny % GT2M == 0 && nx % GT2N == 0 && !(flags & OLX_LAYERED_TILE128);
     const dim3 ggrid2((unsigned)(nx / GT2N), (unsigned)(ny / GT2M));
+
     auto gemm = [&](const void* Ya, const void* Xb, int acc) {
         if (precision == OLX_LAYERED_F16) {
             if (big)
-                k_lay_gemm2<_Float16><<<ggrid2, 256, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+                k_lay_gemm2<_Float16, GT2_WAVES><<<ggrid2, 64 * GT2_WAVES, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
             else
                 k_lay_gemm<_Float16><<<ggrid, 256, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
         } else {
             if (big)
-                k_lay_gemm2<__bf16><<<ggrid2, 256, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+                k_lay_gemm2<__bf16, GT2_WAVES><<<ggrid2, 64 * GT2_WAVES, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
             else
                 k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
         }
