@@ -432,8 +432,9 @@ def layered(ol, torch):
 def multiplane(torch):
     """Config: 2048^2 phase-only hologram, ASM to z = 0.10 / 0.12 / 0.15 m (2x pad),
     seeded 32-Gaussian targets, loss = MSE of mean-normalised intensity against the
-    mean-normalised target, 200 Adam steps; gradients from the adjoint kernel with
-    fields recomputed in backward."""
+    mean-normalised target, 200 Adam steps; loss and gradient from the native
+    one-pass multi-plane kernel (per plane: propagate, reduce, dL/df, adjoint
+    propagation; no plane field kept)."""
     from openol_amd import optim
     n, zs = 2048, [0.10, 0.12, 0.15]
     targets = optim.gaussian_targets(n, n, zs, seed=5)

@@ -205,6 +205,17 @@ int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d
  * z_l = zmin + (l + 0.5) (zmax - zmin)/L */
 double olx_layer_depth(double z, double zmin, double zmax, int nlayers);
 
+/* ---- multi-plane phase-only hologram loss (no reference counterpart) ------ */
+/* u = exp(i phi) (phi: float, ny x nx); f_p = AsmProp_{zs[p]}(u) (2x pad, OLX_BANDLIMIT
+ * honoured); L = sum_p mean((I_p / mean I_p - T_p / mean T_p)^2), I_p = |f_p|^2.  Writes L
+ * to *d_loss (one device double) and, when d_grad is not NULL, dL/dphi to d_grad (float,
+ * ny x nx), using the adjoint ASM.  zs and d_targets are host arrays of nplanes entries;
+ * the targets are device float arrays of ny x nx.  Each plane's field is formed,
+ * reduced, turned into its gradient and propagated back before the next plane. */
+int olx_multiplane_mse_c64(olx_ctx* ctx, const float* d_phi, int64_t ny, int64_t nx, int nplanes,
+                           const double* zs, const float* const* d_targets, double dy, double dx,
+                           double lambda, uint32_t flags, double* d_loss, float* d_grad);
+
 /* ---- point-cloud ingest (include/olobject.h) ----------------------------- */
 /* .3d: int32 N, then N x (int32 x, y, z); .3df: int32 N, then N x (float x, y, z),
  * little-endian.  OLX_OBJ_AUTO picks the format by extension as objread does

@@ -104,6 +104,7 @@ SIGNATURES = {
     "olx_complex2real": [_p, _p, _p, _i64, _i],
     "olx_bmp_write": [ctypes.c_char_p, _p, _i, _i],
     "olx_bmp_read": [ctypes.c_char_p, _p, _i, _i],
+    "olx_multiplane_mse_c64": [_p, _p, _i64, _i64, _i, _p, _p, _d, _d, _d, _u32, _p, _p],
 }
 DOUBLE_RESULT = {"olx_wrp_plane_z", "olx_layer_depth"}
 

@@ -59,14 +59,22 @@ class MultiPlaneLossFn(torch.autograd.Function):
     """L(phi) = sum_p MSE(I_p / max I_p, T_p) (or the mean-normalised form),
     I_p = |A_p exp(i phi)|^2.
 
-    Memory-lean: forward keeps only phi; backward recomputes each plane's field,
-    takes the intensity-to-loss gradient with torch (max subgradient included),
-    applies the adjoint propagation and accumulates Im(conj(u) A_p^H G_p)."""
+    Memory-lean: no plane field is kept.  The mean-normalised form runs natively
+    (ops.multiplane_mse): when phi needs a gradient, forward computes it along with
+    the loss, plane by plane (propagate, reduce, form dL/df, adjoint-propagate), and
+    backward scales it.  The max form keeps phi; backward recomputes each plane's
+    field, takes the intensity-to-loss gradient with torch (max subgradient
+    included), applies the adjoint propagation and accumulates Im(conj(u) A_p^H G_p)."""
 
     @staticmethod
-    def forward(ctx, phi, targets, zs, dy, dx, lam, bandlimit, norm):
-        ctx.save_for_backward(phi)
+    def forward(ctx, phi, targets, zs, dy, dx, lam, bandlimit, norm, need_grad):
         ctx.cfg = (targets, zs, dy, dx, lam, bandlimit, norm)
+        if norm == "mean":
+            loss, g = ops.multiplane_mse(phi.detach().contiguous(), list(targets), list(zs), dy,
+                                         dx, lam, bandlimit, grad=need_grad)
+            ctx.save_for_backward(g)
+            return loss.to(phi.dtype)
+        ctx.save_for_backward(phi)
         u = torch.polar(torch.ones_like(phi), phi)
         loss = phi.new_zeros(())
         for t, z in zip(targets, zs):
@@ -76,8 +84,11 @@ class MultiPlaneLossFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gout):
-        (phi,) = ctx.saved_tensors
         targets, zs, dy, dx, lam, bandlimit, norm = ctx.cfg
+        if norm == "mean":
+            (g,) = ctx.saved_tensors
+            return g * gout, None, None, None, None, None, None, None, None
+        (phi,) = ctx.saved_tensors
         u = torch.polar(torch.ones_like(phi), phi)
         gu = torch.zeros_like(u)
         for t, z in zip(targets, zs):
@@ -87,8 +98,19 @@ class MultiPlaneLossFn(torch.autograd.Function):
                 (gf,) = torch.autograd.grad(lp, f)
             gu += _asm_copy(gf, dy, dx, lam, z, bandlimit, True)
         gphi = torch.imag(torch.conj(u) * gu) * gout
-        return gphi, None, None, None, None, None, None, None
+        return gphi, None, None, None, None, None, None, None, None
 
 
 def multiplane_loss(phi, targets, zs, dy, dx, lam, bandlimit=True, norm="max"):
-    return MultiPlaneLossFn.apply(phi, targets, zs, dy, dx, lam, bandlimit, norm)
+    need_grad = torch.is_grad_enabled() and phi.requires_grad
+    return MultiPlaneLossFn.apply(phi, targets, zs, dy, dx, lam, bandlimit, norm, need_grad)
+
+
+def multiplane_loss_composed(phi, targets, zs, dy, dx, lam, bandlimit=True, norm="max"):
+    """The same loss composed from differentiable ASM propagations and torch
+    operations (autograd through AsmPropFn): a cross-check of the native path."""
+    u = torch.polar(torch.ones_like(phi), phi)
+    loss = phi.new_zeros(())
+    for t, z in zip(targets, zs):
+        loss = loss + normalized_intensity_mse(asm_prop(u, dy, dx, lam, z, bandlimit), t, norm)
+    return loss

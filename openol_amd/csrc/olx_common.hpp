@@ -68,6 +68,7 @@ struct olx_ctx {
     // point-cloud sort and reduction scratch
     olx::DevBuf obj_keys, obj_tmp;
     olx::DevBuf split_acc;  // split propagation: the summed destination field
+    olx::DevBuf mp_work;    // multi-plane loss: exp(i phi), one plane's field, partial sums
     // timing
     bool timing = false;
     std::map<std::string, olx::KernelStat> stats;

@@ -1,0 +1,168 @@
+// olx_optim.hip -- multi-plane phase-only hologram loss and gradient in one pass.
+//
+// For u = exp(i phi) and planes p with f_p = AsmProp_{z_p}(u), I_p = |f_p|^2,
+//   L = sum_p mean((I_p / mean I_p - T_p / mean T_p)^2)          (N pixels per plane)
+//   dL/dI_k = (2/N) (r_k / mI - (sum_j r_j I_j) / (N mI^2)),      r = I / mI - T / mT
+//   dL/df_k = 2 f_k dL/dI_k  (gradient w.r.t. Re f plus i times w.r.t. Im f)
+//   dL/dphi = sum_p Im(conj(u) A_p^H dL/df_p)                    (A^H: the adjoint ASM)
+// Each plane is propagated, reduced, turned into its gradient and propagated back
+// before the next one, so no plane field is kept.  Sums are FP64; every block
+// re-reduces the block partials in the same order, so results are deterministic.
+#include "olx_common.hpp"
+
+extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
+                           double lambda, double z, uint32_t flags);
+
+namespace olx {
+
+constexpr int MP_T = 256;   // threads per block
+constexpr int MP_B = 1024;  // blocks of the reduction kernels (grid-stride)
+
+__global__ void __launch_bounds__(MP_T) k_mp_expi(const float* __restrict__ phi,
+                                                  float2* __restrict__ u, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)gridDim.x * MP_T) {
+        float s, c;
+        sincosf(phi[i], &s, &c);
+        u[i] = make_float2(c, s);
+    }
+}
+
+// sum over the block of two FP64 values per thread, in a fixed order
+__device__ __forceinline__ double2 block_sum2(double a, double b) {
+    __shared__ double sa[MP_T], sb[MP_T];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = MP_T / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sa[threadIdx.x] += sa[threadIdx.x + off];
+            sb[threadIdx.x] += sb[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    const double2 r = make_double2(sa[0], sb[0]);
+    __syncthreads();
+    return r;
+}
+
+// totals of the MP_B block partials (pairs), identical in every block
+__device__ __forceinline__ double2 total2(const double2* __restrict__ part) {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < MP_B; i += MP_T) {
+        a += part[i].x;
+        b += part[i].y;
+    }
+    return block_sum2(a, b);
+}
+
+// pass 1: sum I, sum T
+__global__ void __launch_bounds__(MP_T) k_mp_sums(const float2* __restrict__ f,
+                                                  const float* __restrict__ t, int64_t n,
+                                                  double2* __restrict__ part) {
+    double si = 0.0, st = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)MP_B * MP_T) {
+        const float2 v = f[i];
+        si += (double)v.x * v.x + (double)v.y * v.y;
+        st += (double)t[i];
+    }
+    const double2 s = block_sum2(si, st);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// pass 2: sum r^2, sum r I
+__global__ void __launch_bounds__(MP_T) k_mp_rstats(const float2* __restrict__ f,
+                                                    const float* __restrict__ t, int64_t n,
+                                                    const double2* __restrict__ part,
+                                                    double2* __restrict__ part2) {
+    const double2 tot = total2(part);
+    const double inv_mi = (double)n / tot.x, inv_mt = (double)n / tot.y;
+    double s2 = 0.0, sri = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)MP_B * MP_T) {
+        const float2 v = f[i];
+        const double I = (double)v.x * v.x + (double)v.y * v.y;
+        const double r = I * inv_mi - (double)t[i] * inv_mt;
+        s2 += r * r;
+        sri += r * I;
+    }
+    const double2 s = block_sum2(s2, sri);
+    if (threadIdx.x == 0) part2[blockIdx.x] = s;
+}
+
+// pass 3: dL/df in place; block 0 adds this plane's loss
+__global__ void __launch_bounds__(MP_T) k_mp_grad(float2* __restrict__ f, const float* __restrict__ t,
+                                                  int64_t n, const double2* __restrict__ part,
+                                                  const double2* __restrict__ part2,
+                                                  double* __restrict__ loss) {
+    const double2 tot = total2(part);
+    const double2 rs = total2(part2);
+    const double N = (double)n;
+    const double inv_mi = N / tot.x, inv_mt = N / tot.y;
+    const double c1 = 4.0 / N * inv_mi;                       // 2 * (2/N) / mI
+    const double c2 = 4.0 / N * rs.y / N * inv_mi * inv_mi;   // 2 * (2/N) * sum rI / (N mI^2)
+    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)MP_B * MP_T) {
+        const float2 v = f[i];
+        const double I = (double)v.x * v.x + (double)v.y * v.y;
+        const double r = I * inv_mi - (double)t[i] * inv_mt;
+        const double g = c1 * r - c2;
+        f[i] = make_float2((float)(g * v.x), (float)(g * v.y));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *loss += rs.x / N;
+}
+
+__global__ void k_mp_zero(double* loss) { *loss = 0.0; }
+
+// dL/dphi (+)= Im(conj(u) a)
+__global__ void __launch_bounds__(MP_T) k_mp_phigrad(const float2* __restrict__ u,
+                                                     const float2* __restrict__ a, int64_t n,
+                                                     float* __restrict__ grad, int accumulate) {
+    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)gridDim.x * MP_T) {
+        const float2 x = u[i], y = a[i];
+        const float g = x.x * y.y - x.y * y.x;
+        grad[i] = accumulate ? grad[i] + g : g;
+    }
+}
+
+}  // namespace olx
+
+using namespace olx;
+
+extern "C" int olx_multiplane_mse_c64(olx_ctx* ctx, const float* d_phi, int64_t ny, int64_t nx,
+                                      int nplanes, const double* zs, const float* const* d_targets,
+                                      double dy, double dx, double lambda, uint32_t flags,
+                                      double* d_loss, float* d_grad) {
+    if (!ctx || !d_phi || !zs || !d_targets || !d_loss) OLX_FAIL(OLX_EINVAL, "multiplane: null argument");
+    if (nplanes < 1) OLX_FAIL(OLX_EINVAL, "multiplane: need at least one plane");
+    if (ny <= 0 || nx <= 0) OLX_FAIL(OLX_EINVAL, "multiplane: bad field size");
+    if (flags & ~(uint32_t)OLX_BANDLIMIT) OLX_FAIL(OLX_EINVAL, "multiplane: only OLX_BANDLIMIT is accepted");
+    for (int p = 0; p < nplanes; p++)
+        if (!d_targets[p]) OLX_FAIL(OLX_EINVAL, "multiplane: null target %d", p);
+    const int64_t n = ny * nx;
+    const size_t fb = sizeof(float2) * (size_t)n;
+    OLX_TRY(ctx->mp_work.reserve(2 * fb + 2 * sizeof(double2) * MP_B));
+    float2* u = (float2*)ctx->mp_work.p;
+    float2* f = (float2*)((char*)ctx->mp_work.p + fb);
+    double2* part = (double2*)((char*)ctx->mp_work.p + 2 * fb);
+    double2* part2 = part + MP_B;
+    hipStream_t s = ctx->stream;
+    const unsigned eb = (unsigned)std::min<int64_t>((n + MP_T - 1) / MP_T, 65536);
+    k_mp_expi<<<eb, MP_T, 0, s>>>(d_phi, u, n);
+    k_mp_zero<<<1, 1, 0, s>>>(d_loss);
+    OLX_TRY(check_launch("multiplane setup"));
+    for (int p = 0; p < nplanes; p++) {
+        OLX_HIP(hipMemcpyAsync(f, u, fb, hipMemcpyDeviceToDevice, s));
+        OLX_TRY(olx_asm_c64(ctx, f, ny, nx, dy, dx, lambda, zs[p], flags & OLX_BANDLIMIT));
+        {
+            TimedLaunch tl(ctx, "multiplane_loss");
+            k_mp_sums<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part);
+            k_mp_rstats<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part, part2);
+            k_mp_grad<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part, part2, d_loss);
+        }
+        OLX_TRY(check_launch("multiplane loss"));
+        if (!d_grad) continue;
+        OLX_TRY(olx_asm_c64(ctx, f, ny, nx, dy, dx, lambda, zs[p],
+                            (flags & OLX_BANDLIMIT) | OLX_ADJOINT));
+        k_mp_phigrad<<<eb, MP_T, 0, s>>>(u, f, n, d_grad, p > 0 ? 1 : 0);
+        OLX_TRY(check_launch("multiplane gradient"));
+    }
+    return OLX_OK;
+}

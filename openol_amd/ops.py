@@ -50,6 +50,32 @@ def AsmProp(u, height, width, dy, dx, lam, z, bandlimit=False, adjoint=False):
     return u
 
 
+def multiplane_mse(phi, targets, zs, dy, dx, lam, bandlimit=True, grad=True):
+    """Mean-normalised multi-plane intensity loss of the phase-only field exp(i phi) and,
+    with grad, its gradient dL/dphi, in one native pass (no reference counterpart):
+    L = sum_p mean((I_p / mean I_p - T_p / mean T_p)^2), I_p = |AsmProp_{z_p}(exp(i phi))|^2.
+    Returns (loss as a float64 0-d tensor on the device, gradient or None)."""
+    if phi.dtype != torch.float32 or phi.dim() != 2 or not phi.is_cuda or not phi.is_contiguous():
+        raise OlxInvalid(_lib.EINVAL, "multiplane_mse: phi must be a contiguous 2-D float32 CUDA tensor")
+    if len(targets) != len(zs) or not zs:
+        raise OlxInvalid(_lib.EINVAL, "multiplane_mse: one target per plane")
+    for t in targets:
+        if t.dtype != torch.float32 or t.shape != phi.shape or t.device != phi.device \
+                or not t.is_contiguous():
+            raise OlxInvalid(_lib.EINVAL, "multiplane_mse: targets must be contiguous float32 "
+                             "tensors shaped like phi on its device")
+    ny, nx = phi.shape
+    loss = torch.empty((), dtype=torch.float64, device=phi.device)
+    g = torch.empty_like(phi) if grad else None
+    zarr = (ctypes.c_double * len(zs))(*[float(z) for z in zs])
+    tarr = (ctypes.c_void_p * len(targets))(*[t.data_ptr() for t in targets])
+    check(_lib.load().olx_multiplane_mse_c64(
+        _lib.ctx(phi.device.index), _ptr(phi), ny, nx, len(zs), ctypes.cast(zarr, ctypes.c_void_p),
+        ctypes.cast(tarr, ctypes.c_void_p), dy, dx, lam, _lib.BANDLIMIT if bandlimit else 0,
+        _ptr(loss), _ptr(g) if grad else None), "multiplane_mse")
+    return loss, g
+
+
 def mAsmProp(u, height, width, dy, dx, lam, z):
     """ol::mAsmProp (include/molprop.h:80-101): the angular spectrum propagation whose
     transforms the reference stages through host memory, net scale 1/M^2 (it scales

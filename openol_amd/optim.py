@@ -3,9 +3,10 @@
 A phase map phi (float32, ny x nx) is propagated with the angular-spectrum method
 to several planes; the loss is the sum over planes of the MSE between the
 normalised intensity and the normalised target (mean normalisation by default,
-max normalisation on request).  Adam updates phi; gradients
-come from MultiPlaneLossFn, whose backward recomputes each plane's field and
-runs the hand-written adjoint propagation.
+max normalisation on request).  Adam updates phi; gradients come from
+MultiPlaneLossFn: for the mean normalisation one native pass per plane
+(propagate, reduce, dL/df, hand-written adjoint propagation), for the max
+normalisation a backward that recomputes each plane's field.
 """
 import time
 
@@ -32,7 +33,8 @@ def optimize_phase(targets, zs, dy=synth.PITCH, dx=synth.PITCH, lam=synth.LAMBDA
             synth.uniform(seed, synth.S_PHASE, ny * nx, 0.0, 2 * np.pi).astype(np.float32)
             .reshape(ny, nx))
     phi = phi0.to(dev).clone().requires_grad_(True)
-    opt = torch.optim.Adam([phi], lr=lr)
+    # fused: the whole Adam update in one kernel on the device
+    opt = torch.optim.Adam([phi], lr=lr, fused=phi.is_cuda)
     losses = []
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()

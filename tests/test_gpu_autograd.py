@@ -127,6 +127,46 @@ def test_multiplane_gradient_mean_norm_vs_fp64(n):
         assert olref.rel_l2(phi.grad.cpu().numpy().ravel()[idx], np.array(fd)) < 1e-3
 
 
+@pytest.mark.parametrize("shape", [(128, 128), (64, 256)])
+def test_native_multiplane_matches_composed(shape):
+    """The native mean-normalised loss and gradient (one pass per plane) against the
+    same loss composed from differentiable ASM propagations and torch operations;
+    gout scales the gradient; without grad mode the loss alone is computed."""
+    ny, nx = shape
+    zs = [0.010, 0.013]
+    phi0 = synth.uniform(9, synth.S_PHASE, ny * nx, 0, 2 * np.pi).reshape(ny, nx).astype(np.float32)
+    targets = [torch.from_numpy(synth.gaussian_targets(ny, nx, 31 + i, smin=3.0, smax=15.0)).cuda()
+               for i in range(len(zs))]
+    phi = torch.from_numpy(phi0).cuda().requires_grad_(True)
+    loss = ag().multiplane_loss(phi, targets, zs, P, P, LAM, norm="mean")
+    (2.5 * loss).backward()
+    phi2 = torch.from_numpy(phi0).cuda().requires_grad_(True)
+    loss2 = ag().multiplane_loss_composed(phi2, targets, zs, P, P, LAM, norm="mean")
+    (2.5 * loss2).backward()
+    assert abs(loss.item() - loss2.item()) < 1e-5 * loss2.item()
+    assert olref.rel_l2(phi.grad.cpu().numpy(), phi2.grad.cpu().numpy()) < 1e-5
+    with torch.no_grad():
+        l3 = ag().multiplane_loss(phi, targets, zs, P, P, LAM, norm="mean")
+    assert l3.item() == loss.item()
+    # the composed form with the max normalisation still runs through the same ops
+    l4 = ag().multiplane_loss(phi.detach(), targets, zs, P, P, LAM, norm="max")
+    l5 = ag().multiplane_loss_composed(phi.detach(), targets, zs, P, P, LAM, norm="max")
+    assert abs(l4.item() - l5.item()) < 1e-5 * l5.item()
+
+
+def test_native_multiplane_rejects_bad_inputs():
+    from openol_amd import ops
+    from openol_amd._lib import OlxInvalid
+    phi = torch.zeros((32, 32), device="cuda")
+    t = torch.ones((32, 32), device="cuda")
+    with pytest.raises(OlxInvalid):
+        ops.multiplane_mse(phi.double(), [t], [0.01], P, P, LAM)
+    with pytest.raises(OlxInvalid):
+        ops.multiplane_mse(phi, [t, t], [0.01], P, P, LAM)
+    with pytest.raises(OlxInvalid):
+        ops.multiplane_mse(phi, [t[:16]], [0.01], P, P, LAM)
+
+
 def test_adam_mean_norm_2048():
     """The optimisation workload's shape: the mean-normalised loss falls by half."""
     from openol_amd import optim
