@@ -118,9 +118,29 @@ def roofline_of(rep, n):
     return out
 
 
-def cpu_baseline(n):
+def gpu_outputs(ol, torch, n):
+    """GPU results the CPU leg checks against the oracle: one AsmProp of the metric's
+    seeded field, and the first 16 rows of the direct CGH of config 2 (Fresnel and
+    exact r)."""
+    from openol_amd import synth
+    u = torch.from_numpy(synth.random_field(n, n, seed=0)).cuda()
+    ol.AsmProp(u, n, n, PITCH, PITCH, LAM, 0.10, bandlimit=True)
+    N, P = 2048, 10000
+    pts = synth.random_points(P, seed=2, xmax=N * PITCH, ymax=N * PITCH, zmin=0.10, zmax=0.20)
+    obj = ol.Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    U = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    out = {"asm": u.cpu().numpy()}
+    for exact in (False, True):
+        ol.Cgh(obj, U, N, N, PITCH, PITCH, LAM, exact_r=exact)
+        out["cgh_exact" if exact else "cgh_fresnel"] = U[:16].cpu().numpy()
+    return out
+
+
+def cpu_baseline(n, gpu=None):
     """The oracle's AsmProp (complex64 field, PREC_T = double) on the host cores:
-    1 warm-up, then the median of 3 runs, at the metric size."""
+    1 warm-up, then the median of 3 runs, at the metric size.  With the GPU results
+    of gpu_outputs, also their relative L2 error against the oracle: its complex64
+    restatement (PREC_T = double) and its FP64 (complex128) form."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import olref
     from openol_amd import synth
@@ -131,9 +151,15 @@ def cpu_baseline(n):
     ts = []
     for _ in range(3):
         t0 = time.perf_counter()
-        olref.asm_prop(u, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
+        ref_asm = olref.asm_prop(u, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
         ts.append(time.perf_counter() - t0)
     med = float(np.median(ts))
+    errors = None
+    if gpu is not None:
+        ref64 = olref.asm_prop(u.astype(np.complex128), PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
+        errors = {"asm_metric_vs_c64_restatement": float(f"{olref.rel_l2(gpu['asm'], ref_asm):.3e}"),
+                  "asm_metric_vs_fp64": float(f"{olref.rel_l2(gpu['asm'], ref64):.3e}")}
+        del ref64
     # the ingest / encoding rows on one core (numpy restatements): objsort of the
     # 1,000,000-point cloud and complex2img (AMP) of a 4096^2 field
     pts = synth.random_points(1_000_000, seed=4, xmax=1.0, ymax=1.0, zmin=-1.0, zmax=1.0)
@@ -151,8 +177,18 @@ def cpu_baseline(n):
     pts = synth.random_points(P, seed=2, xmax=N * PITCH, ymax=N * PITCH, zmin=0.10, zmax=0.20)
     rows = 16
     t0 = time.perf_counter()
-    olref.cgh(pts["x"], pts["y"], pts["z"], rows, N, PITCH, PITCH, LAM, pts["amp"], pts["phase"])
+    band = olref.cgh(pts["x"], pts["y"], pts["z"], rows, N, PITCH, PITCH, LAM, pts["amp"], pts["phase"])
     t_cgh = time.perf_counter() - t0
+    if gpu is not None:
+        # the device stores amplitude and phase as float32
+        amp = pts["amp"].astype(np.float32).astype(np.float64)
+        ph = pts["phase"].astype(np.float32).astype(np.float64)
+        errors["cgh_config2_fresnel_rows_vs_c64_restatement"] = float(
+            f"{olref.rel_l2(gpu['cgh_fresnel'], band):.3e}")
+        for name, fl in (("fresnel", 0), ("exact", olref.CGH_EXACT_R)):
+            b64 = olref.cgh(pts["x"], pts["y"], pts["z"], rows, N, PITCH, PITCH, LAM, amp, ph,
+                            flags=fl, dtype=np.complex128)
+            errors[f"cgh_config2_{name}_rows_vs_fp64"] = float(f"{olref.rel_l2(gpu['cgh_' + name], b64):.3e}")
     # one thread: AsmProp at 1024^2 (OpenMP pinned to one thread through libgomp)
     one = None
     try:
@@ -183,7 +219,9 @@ def cpu_baseline(n):
                        "cgh_sample": f"{rows} rows x {N} px x {P} points (extrapolate linearly)",
                        "objsort_1M_ms": round(t_sort * 1e3, 1),
                        "complex2img_amp_4096_ms": round(t_img * 1e3, 1),
-                       "numpy_rows": "objsort and complex2img: numpy, 1 core"}}
+                       "numpy_rows": "objsort and complex2img: numpy, 1 core"},
+            "rel_l2_vs_oracle": errors,
+            "rel_l2_sample": "AsmProp at the metric size; direct CGH of config 2, rows 0-15"}
 
 
 def ceilings(lib, torch):
@@ -495,7 +533,7 @@ def main():
         except Exception as e:  # report, do not hide the metric line
             line["extras"] = {"error": repr(e)}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(n)
+        line["cpu_baseline"] = cpu_baseline(n, gpu_outputs(ol, torch, n))
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
     print(json.dumps(line), flush=True)
 
