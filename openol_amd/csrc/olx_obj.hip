@@ -221,6 +221,28 @@ __global__ void k_sort_init(const olx_point* __restrict__ p, int64_t n, int64_t 
     key[j] = s;
 }
 
+// Bitonic sort of the (key, index) pairs: a total order, so the result is the
+// stable order.
+// * SORT_CHUNK = 2048 pairs are sorted in LDS by one 256-thread work-group (a 1 M-point
+//   cloud gives 512 of them, several per CU).  Steps j >= 8 go through LDS; steps
+//   j = 4, 2, 1 run in registers on 8 consecutive pairs per thread, with LDS rows of
+//   8 pairs padded by one so those reads are free of bank conflicts.
+// * Distances j >= SORT_CHUNK run as global passes of three steps (j, j/2, j/4) on
+//   8-pair groups, or of one or two steps for the remainder.
+constexpr int SORT_CHUNK = 2048, SORT_NT = 256;
+constexpr int SORT_LDS = SORT_CHUNK + SORT_CHUNK / 8;
+
+template <bool PAD>
+__device__ __forceinline__ int sort_lp(int i) { return PAD ? i + (i >> 3) : i; }
+
+__device__ __forceinline__ void sort_cas(SortKey& a, SortKey& b, bool up) {
+    if (key_less(b, a) == up) {
+        const SortKey t = a;
+        a = b;
+        b = t;
+    }
+}
+
 // one compare-exchange step (k: bitonic sequence length, j: partner distance)
 __global__ void k_sort_step(SortKey* __restrict__ key, int64_t n2, int64_t k, int64_t j) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -235,29 +257,107 @@ __global__ void k_sort_step(SortKey* __restrict__ key, int64_t n2, int64_t k, in
     }
 }
 
-// all steps j <= 512 of one k, on 1024-entry chunks in LDS (512 threads)
-__global__ void __launch_bounds__(512) k_sort_local(SortKey* __restrict__ key, int64_t k,
-                                                    int64_t jmax) {
-    __shared__ SortKey s[1024];
-    const int64_t base = blockIdx.x * 1024ll;
-    s[threadIdx.x] = key[base + threadIdx.x];
-    s[threadIdx.x + 512] = key[base + threadIdx.x + 512];
-    __syncthreads();
-    for (int64_t j = jmax; j > 0; j >>= 1) {
-        // thread t handles the pair (i, i + j) with i having bit j clear
-        const int t = threadIdx.x;
-        const int i = (int)((t / j) * 2 * j + (t % j));
-        const int l = i + (int)j;
-        const bool up = ((base + i) & k) == 0;
-        const SortKey a = s[i], b = s[l];
-        if (key_less(b, a) == up) {
-            s[i] = b;
-            s[l] = a;
+// S steps j, j/2, .. in one pass: thread t owns the 2^S pairs whose indices differ
+// from b (t with zero bits inserted at j/2^(S-1) .. j) only in those bits; all
+// share bit k (k >= 2j), so one direction
+template <int S>
+__global__ void k_sort_steps(SortKey* __restrict__ key, int64_t n2, int64_t k, int64_t j) {
+    constexpr int G = 1 << S;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n2 / G) return;
+    const int64_t lo = j >> (S - 1);  // smallest distance of the pass
+    const int64_t b = (t & (lo - 1)) | ((t & ~(lo - 1)) << S);
+    const bool up = (b & k) == 0;
+    SortKey r[G];
+#pragma unroll
+    for (int e = 0; e < G; e++) r[e] = key[b + e * lo];
+#pragma unroll
+    for (int d = G / 2; d >= 1; d >>= 1)
+#pragma unroll
+        for (int e = 0; e < G; e++)
+            if ((e & d) == 0) sort_cas(r[e], r[e + d], up);
+#pragma unroll
+    for (int e = 0; e < G; e++) key[b + e * lo] = r[e];
+}
+
+// steps J, J/2, .., 1 on the thread's 8 consecutive pairs (index base + 8 t + e)
+template <int J>
+__device__ __forceinline__ void sort_reg_steps(SortKey (&r)[8], int64_t first, int64_t k) {
+#pragma unroll
+    for (int d = J; d >= 1; d >>= 1)
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+            if ((e & d) == 0) sort_cas(r[e], r[e + d], ((first + e) & k) == 0);
+}
+
+// steps jmax, jmax/2, .., 1 of sequence length k on one SORT_CHUNK block in LDS
+// (REG: the steps j <= 4 in registers, on the padded layout)
+template <bool REG>
+__device__ __forceinline__ void sort_local_steps(SortKey* s, int64_t base, int64_t k, int jmax) {
+    int j = jmax;
+    for (; j >= (REG ? 8 : 1); j >>= 1) {
+#pragma unroll
+        for (int q = 0; q < SORT_CHUNK / 2 / SORT_NT; q++) {
+            const int p = threadIdx.x + q * SORT_NT;               // pair index
+            const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));   // bit j clear
+            const int ia = sort_lp<REG>(i), ib = sort_lp<REG>(i + j);
+            const SortKey a = s[ia], b = s[ib];
+            if (key_less(b, a) == (((base + i) & k) == 0)) {
+                s[ia] = b;
+                s[ib] = a;
+            }
         }
         __syncthreads();
     }
-    key[base + threadIdx.x] = s[threadIdx.x];
-    key[base + threadIdx.x + 512] = s[threadIdx.x + 512];
+    if constexpr (REG) {
+        SortKey r[8];
+        const int o = 8 * threadIdx.x;
+#pragma unroll
+        for (int e = 0; e < 8; e++) r[e] = s[sort_lp<true>(o + e)];
+        if (j == 4) sort_reg_steps<4>(r, base + o, k);
+        else if (j == 2) sort_reg_steps<2>(r, base + o, k);
+        else sort_reg_steps<1>(r, base + o, k);
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[sort_lp<true>(o + e)] = r[e];
+        __syncthreads();
+    }
+}
+
+template <bool PAD>
+__device__ __forceinline__ void sort_chunk_load(SortKey* s, const SortKey* key, int64_t base) {
+#pragma unroll
+    for (int q = 0; q < SORT_CHUNK / SORT_NT; q++) {
+        const int i = threadIdx.x + q * SORT_NT;
+        s[sort_lp<PAD>(i)] = key[base + i];
+    }
+    __syncthreads();
+}
+
+template <bool PAD>
+__device__ __forceinline__ void sort_chunk_store(const SortKey* s, SortKey* key, int64_t base) {
+#pragma unroll
+    for (int q = 0; q < SORT_CHUNK / SORT_NT; q++) {
+        const int i = threadIdx.x + q * SORT_NT;
+        key[base + i] = s[sort_lp<PAD>(i)];
+    }
+}
+
+// every sequence length k <= SORT_CHUNK, on each block
+__global__ void __launch_bounds__(SORT_NT) k_sort_chunks(SortKey* __restrict__ key) {
+    __shared__ SortKey s[SORT_LDS];
+    const int64_t base = blockIdx.x * (int64_t)SORT_CHUNK;
+    sort_chunk_load<true>(s, key, base);
+    for (int k = 2; k <= SORT_CHUNK; k <<= 1) sort_local_steps<true>(s, base, k, k >> 1);
+    sort_chunk_store<true>(s, key, base);
+}
+
+// the steps j < SORT_CHUNK of sequence length k > SORT_CHUNK
+__global__ void __launch_bounds__(SORT_NT) k_sort_local(SortKey* __restrict__ key, int64_t k) {
+    __shared__ SortKey s[SORT_LDS];
+    const int64_t base = blockIdx.x * (int64_t)SORT_CHUNK;
+    sort_chunk_load<true>(s, key, base);
+    sort_local_steps<true>(s, base, k, SORT_CHUNK / 2);
+    sort_chunk_store<true>(s, key, base);
 }
 
 __global__ void k_sort_gather(const olx_point* __restrict__ src, const SortKey* __restrict__ key,
@@ -318,7 +418,7 @@ extern "C" int olx_obj_sort_z(olx_ctx* ctx, olx_point* d_pts, int64_t n) {
     if (!ctx || (n > 0 && !d_pts)) OLX_FAIL(OLX_EINVAL, "obj_sort_z: null argument");
     if (n < 0 || n > (int64_t)UINT32_MAX) OLX_FAIL(OLX_EUNSUPPORTED, "obj_sort_z: bad count");
     if (n < 2) return OLX_OK;
-    int64_t n2 = 1024;
+    int64_t n2 = SORT_CHUNK;
     while (n2 < n) n2 <<= 1;
     OLX_TRY(ctx->obj_keys.reserve(sizeof(SortKey) * (size_t)n2 + sizeof(olx_point) * (size_t)n));
     SortKey* key = (SortKey*)ctx->obj_keys.p;
@@ -327,10 +427,15 @@ extern "C" int olx_obj_sort_z(olx_ctx* ctx, olx_point* d_pts, int64_t n) {
     OLX_HIP(hipMemcpyAsync(copy, d_pts, sizeof(olx_point) * (size_t)n, hipMemcpyDeviceToDevice, s));
     const unsigned g = (unsigned)((n2 + 255) / 256);
     k_sort_init<<<g, 256, 0, s>>>(copy, n, n2, key);
-    for (int64_t k = 2; k <= n2; k <<= 1) {
+    const unsigned nchunk = (unsigned)(n2 / SORT_CHUNK), g4 = (unsigned)((n2 / 4 + 255) / 256),
+                   g8 = (unsigned)((n2 / 8 + 255) / 256);
+    k_sort_chunks<<<nchunk, SORT_NT, 0, s>>>(key);
+    for (int64_t k = 2 * SORT_CHUNK; k <= n2; k <<= 1) {
         int64_t j = k / 2;
-        for (; j > 512; j >>= 1) k_sort_step<<<g, 256, 0, s>>>(key, n2, k, j);
-        k_sort_local<<<(unsigned)(n2 / 1024), 512, 0, s>>>(key, k, j);
+        for (; j >= 4 * SORT_CHUNK; j >>= 3) k_sort_steps<3><<<g8, 256, 0, s>>>(key, n2, k, j);
+        if (j == 2 * SORT_CHUNK) k_sort_steps<2><<<g4, 256, 0, s>>>(key, n2, k, j);
+        else if (j == SORT_CHUNK) k_sort_step<<<g, 256, 0, s>>>(key, n2, k, j);
+        k_sort_local<<<nchunk, SORT_NT, 0, s>>>(key, k);
     }
     k_sort_gather<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(copy, key, n, d_pts);
     return check_launch("obj_sort_z");
