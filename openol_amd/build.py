@@ -41,6 +41,9 @@ FILE_FLAGS = {
     "olx_tilt.hip": ["-ffp-contract=off", "-fsigned-zeros"],
     # optical elements: float phases and products as the reference rounds them
     "olx_optics.hip": ["-ffp-contract=off", "-fsigned-zeros"],
+    # WRP: window tests dx*dx + dy*dy < W^2 with each product rounded, as the reference's
+    # host arithmetic (and the oracle) evaluate them
+    "olx_wrp.hip": ["-ffp-contract=off"],
     # the 256 x 128 GEMM keeps its register-indexed fragment arrays in registers: with
     # one work-group per CU the compiler would otherwise move them into spare LDS
     "olx_layered.hip": ["-mllvm", "-disable-promote-alloca-to-lds"],

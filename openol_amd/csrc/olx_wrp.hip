@@ -7,8 +7,11 @@
 //   2. counting sort of (tile, point) pairs: count, exclusive scan, fill;
 //   3. one work-group per tile: load the tile's point list, sort it by point index
 //      (fixed summation order), stage each point's constants (position, window
-//      limit, 1/(2 dz lambda), 1/|dz|) in LDS 256 at a time, and accumulate
-//      exp(i pi rho^2/(lambda dz))/|dz| for the pixels inside each window.
+//      limit, 1/(2 dz lambda), 1/|dz|) in LDS 128 at a time, and accumulate
+//      exp(i pi rho^2/(lambda dz))/|dz| for the pixels inside each window (two
+//      pixels of one column per thread).
+// This file is built without FMA contraction: dx*dx + dy*dy rounds each product,
+// as the reference's host arithmetic does, so window membership is decided alike.
 // Window tests and rho^2 use FP64 in the reference's expression order, so pixel
 // membership is decided exactly as the reference does with PREC_T = double.
 #include <climits>
@@ -158,7 +161,6 @@ k_scan_add(int* __restrict__ offs, const long long* __restrict__ bbase, int ntil
 }
 
 constexpr int WRP_SORT_CAP = 4096;
-constexpr int WRP_CHUNK = 256;  // points staged in LDS at a time
 
 struct WrpPt {     // per-point constants of one tile's list, staged in LDS
     double x, y;
@@ -182,30 +184,48 @@ __device__ __forceinline__ WrpPt wrp_stage(const olx_point& q, double lambda, do
     return s;
 }
 
+// One work-group of WRP_NT = 128 threads per 16 x 16 tile: thread t owns column t % 16
+// and rows t / 16 and t / 16 + 8, which share dx and each point read from LDS.
+constexpr int WRP_NT = 128;
+
 template <bool SQUARE>
-__global__ void __launch_bounds__(256)
+__device__ __forceinline__ void wrp_pixel(const WrpPt& P, double dx, double dy, float& re,
+                                          float& im) {
+    const double tmp = dx * dx + dy * dy;
+    const bool inside = SQUARE ? (fabs(dx) * 2 <= P.lim && fabs(dy) * 2 <= P.lim) : (tmp < P.lim);
+    if (inside) {
+        double cyc = tmp * P.rc;
+        cyc -= rint(cyc);
+        const float t = (float)cyc;
+        re += __builtin_amdgcn_cosf(t) * P.inv;
+        im += __builtin_amdgcn_sinf(t) * P.inv;
+    }
+}
+
+template <bool SQUARE>
+__global__ void __launch_bounds__(WRP_NT)
 k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
             const int* __restrict__ list, float2* __restrict__ U, int ny, int nx, double p,
             double lambda, double plane_z, double plim) {
     __shared__ int ids[WRP_SORT_CAP];
-    __shared__ WrpPt sp[WRP_CHUNK];
+    __shared__ WrpPt sp[WRP_NT];
     const int tiles_x = (nx + WT - 1) / WT;
     const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
-    const int w = tx * WT + threadIdx.x % WT, h = ty * WT + threadIdx.x / WT;
+    const int w = tx * WT + threadIdx.x % WT, h0 = ty * WT + threadIdx.x / WT, h1 = h0 + WT / 2;
     const int beg = offs[tile], cnt = offs[tile + 1] - beg;
-    const double xw = p * w, yh = p * h;
-    float re = 0.f, im = 0.f;
+    const double xw = p * w, yh0 = p * h0, yh1 = p * h1;
+    float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
     const bool sorted = cnt <= WRP_SORT_CAP;
     if (sorted) {
         int np2 = 1;
         while (np2 < cnt) np2 <<= 1;
-        for (int i = threadIdx.x; i < np2; i += 256) ids[i] = i < cnt ? list[beg + i] : INT_MAX;
+        for (int i = threadIdx.x; i < np2; i += WRP_NT) ids[i] = i < cnt ? list[beg + i] : INT_MAX;
         __syncthreads();
         // bitonic sort, ascending point index: a fixed summation order
         for (int k = 2; k <= np2; k <<= 1)
             for (int jj = k >> 1; jj > 0; jj >>= 1) {
-                for (int i = threadIdx.x; i < np2; i += 256) {
+                for (int i = threadIdx.x; i < np2; i += WRP_NT) {
                     const int l = i ^ jj;
                     if (l > i) {
                         const int a = ids[i], b = ids[l];
@@ -219,8 +239,8 @@ k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
                 __syncthreads();
             }
     }
-    for (int base = 0; base < cnt; base += WRP_CHUNK) {
-        const int m = min(WRP_CHUNK, cnt - base);
+    for (int base = 0; base < cnt; base += WRP_NT) {
+        const int m = min(WRP_NT, cnt - base);
         if ((int)threadIdx.x < m) {
             const int j = sorted ? ids[base + threadIdx.x] : list[beg + base + threadIdx.x];
             sp[threadIdx.x] = wrp_stage(pts[j], lambda, plane_z, plim, SQUARE);
@@ -228,21 +248,14 @@ k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
         __syncthreads();
         for (int q = 0; q < m; q++) {
             const WrpPt P = sp[q];
-            const double dx = xw - P.x, dy = yh - P.y;
-            const double tmp = dx * dx + dy * dy;
-            const bool inside = SQUARE ? (fabs(dx) * 2 <= P.lim && fabs(dy) * 2 <= P.lim)
-                                       : (tmp < P.lim);
-            if (inside) {
-                double cyc = tmp * P.rc;
-                cyc -= rint(cyc);
-                const float t = (float)cyc;
-                re += __builtin_amdgcn_cosf(t) * P.inv;
-                im += __builtin_amdgcn_sinf(t) * P.inv;
-            }
+            const double dx = xw - P.x;
+            wrp_pixel<SQUARE>(P, dx, yh0 - P.y, re0, im0);
+            wrp_pixel<SQUARE>(P, dx, yh1 - P.y, re1, im1);
         }
         __syncthreads();
     }
-    if (w < nx && h < ny) U[(size_t)h * nx + w] = make_float2(re, im);
+    if (w < nx && h0 < ny) U[(size_t)h0 * nx + w] = make_float2(re0, im0);
+    if (w < nx && h1 < ny) U[(size_t)h1 * nx + w] = make_float2(re1, im1);
 }
 
 }  // namespace olx
@@ -318,10 +331,10 @@ extern "C" int olx_wrp_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     {
         TimedLaunch tl(ctx, "wrp_accum");
         if (square)
-            k_wrp_accum<true><<<ntiles, 256, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
+            k_wrp_accum<true><<<ntiles, WRP_NT, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
                                                      (int)nx, p, lambda, plane_z, p_limit);
         else
-            k_wrp_accum<false><<<ntiles, 256, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
+            k_wrp_accum<false><<<ntiles, WRP_NT, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
                                                       (int)nx, p, lambda, plane_z, p_limit);
     }
     return check_launch("wrp accumulate");
