@@ -600,17 +600,9 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
     hipStream_t sg = s;  // generation stream
     hipEvent_t* ev = ctx->aux_events;  // 0: prep done, 1-2: factors of set k, 3-4: GEMM on set k
     if (overlap) {
-        if (!ctx->aux_stream || !ev[4]) {  // created once, on the context's device
-            int cur = 0;
-            OLX_HIP(hipGetDevice(&cur));
-            OLX_HIP(hipSetDevice(ctx->device));
-            hipError_t e = hipSuccess;
-            if (!ctx->aux_stream) e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
-            for (int k = 0; k < 5 && e == hipSuccess; k++)
-                if (!ev[k]) e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
-            (void)hipSetDevice(cur);
-            OLX_HIP(e);
-        }
+        if (!ctx->aux_stream) OLX_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+        for (int k = 0; k < 5; k++)
+            if (!ev[k]) OLX_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
         sg = ctx->aux_stream;
         OLX_HIP(hipEventRecord(ev[0], s));
         OLX_HIP(hipStreamWaitEvent(sg, ev[0], 0));
