@@ -69,10 +69,6 @@ struct olx_ctx {
     olx::DevBuf obj_keys, obj_tmp;
     olx::DevBuf split_acc;  // split propagation: the summed destination field
     olx::DevBuf mp_work;    // multi-plane loss: exp(i phi), one plane's field, partial sums
-    // second stream (default priority) and events: layered CGH generates the next
-    // chunk's factors there while the current chunk's GEMM runs
-    hipStream_t aux_stream = nullptr;
-    hipEvent_t aux_events[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // timing
     bool timing = false;
     std::map<std::string, olx::KernelStat> stats;

@@ -142,9 +142,6 @@ extern "C" int olx_ctx_destroy(olx_ctx* ctx) {
     (void)hipDeviceSynchronize();
     ctx->pending.clear();
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
-    for (hipEvent_t e : ctx->aux_events)
-        if (e) (void)hipEventDestroy(e);
-    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
     if (ctx->twiddle) (void)hipFree(ctx->twiddle);
     delete ctx;
     return OLX_OK;

@@ -576,41 +576,23 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
             cy, cdy, (int)ny, nlayers, py, lambda, zmin, zmax);
     }
     OLX_TRY(check_launch("layered prep"));
-    // factor buffers for one chunk of points (KC multiple of KS/2 points), two sets:
-    // the factors of chunk c + 1 are generated on a second stream while chunk c's GEMM
-    // runs (the generation kernels use few registers and no LDS, so they share the
-    // CUs with the MFMA-bound GEMM).  With kernel timing on, everything runs in order
-    // on the context's stream.
+    // factor buffers for one chunk of points (KC multiple of KS/2 points)
     const int kc = (int)std::min<int64_t>(KC_PTS, ((n + KS / 2 - 1) / (KS / 2)) * (KS / 2));
     const int ldk = 2 * kc;
     const int nmat = precision == OLX_LAYERED_BF16X3 ? 4 : 2;
     const size_t ybytes = sizeof(uint16_t) * (size_t)ny * ldk;
     const size_t xbytes = sizeof(uint16_t) * (size_t)nx * ldk;
-    const size_t setbytes = (nmat / 2) * (ybytes + xbytes);
-    const int64_t nchunks = (n + kc - 1) / kc;
-    const bool overlap = nchunks > 1 && !ctx->timing;
-    OLX_TRY(ctx->lay_counts.reserve((overlap ? 2 : 1) * setbytes));
-    struct FactorSet {
-        void *Yh, *Xh, *Yl, *Xl;
-    } fs[2];
-    for (int k = 0; k < (overlap ? 2 : 1); k++) {
-        char* fb = (char*)ctx->lay_counts.p + k * setbytes;
-        fs[k] = {fb, fb + ybytes, fb + ybytes + xbytes, fb + 2 * ybytes + xbytes};
-    }
-    hipStream_t sg = s;  // generation stream
-    hipEvent_t* ev = ctx->aux_events;  // 0: prep done, 1-2: factors of set k, 3-4: GEMM on set k
-    if (overlap) {
-        if (!ctx->aux_stream) OLX_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
-        for (int k = 0; k < 5; k++)
-            if (!ev[k]) OLX_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
-        sg = ctx->aux_stream;
-        OLX_HIP(hipEventRecord(ev[0], s));
-        OLX_HIP(hipStreamWaitEvent(sg, ev[0], 0));
-    }
+    OLX_TRY(ctx->lay_counts.reserve((nmat / 2) * (ybytes + xbytes)));
+    char* fb = (char*)ctx->lay_counts.p;
+    void* Yh = fb;
+    void* Xh = fb + ybytes;
+    void* Yl = fb + ybytes + xbytes;
+    void* Xl = fb + 2 * ybytes + xbytes;
     const dim3 ggrid((unsigned)(nx / GT), (unsigned)(ny / GT));
     // 256 x 128 tiles where the sides allow, unless the 128 x 128 kernel is asked for
     const bool big = ny % GT2M == 0 && nx % GT2N == 0 && !(flags & OLX_LAYERED_TILE128);
     const dim3 ggrid2((unsigned)(nx / GT2N), (unsigned)(ny / GT2M));
+
     auto gemm = [&](const void* Ya, const void* Xb, int acc) {
         if (precision == OLX_LAYERED_F16) {
             if (big)
@@ -624,47 +606,35 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
                 k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
         }
     };
-    auto gen = [&](int64_t j0, const FactorSet& f) {
+    bool first = true;
+    for (int64_t j0 = 0; j0 < n; j0 += kc) {
         const dim3 gy((unsigned)((kc + 255) / 256), (unsigned)((ny + GEN_RB - 1) / GEN_RB)),
             gx((unsigned)((kc + 255) / 256), (unsigned)((nx + GEN_RB - 1) / GEN_RB));
-        TimedLaunch tl(ctx, "layered_factors");
-        if (precision == OLX_LAYERED_F16) {
-            k_lay_gen<_Float16, true, 0><<<gy, 256, 0, sg>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)f.Yh, nullptr);
-            k_lay_gen<_Float16, false, 0><<<gx, 256, 0, sg>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)f.Xh, nullptr);
-        } else if (precision == OLX_LAYERED_BF16) {
-            k_lay_gen<__bf16, true, 0><<<gy, 256, 0, sg>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)f.Yh, nullptr);
-            k_lay_gen<__bf16, false, 0><<<gx, 256, 0, sg>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)f.Xh, nullptr);
-        } else {
-            k_lay_gen<__bf16, true, 1><<<gy, 256, 0, sg>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)f.Yh, (uint32_t*)f.Yl);
-            k_lay_gen<__bf16, false, 1><<<gx, 256, 0, sg>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)f.Xh, (uint32_t*)f.Xl);
+        {
+            TimedLaunch tl(ctx, "layered_factors");
+            if (precision == OLX_LAYERED_F16) {
+                k_lay_gen<_Float16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)Yh, nullptr);
+                k_lay_gen<_Float16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)Xh, nullptr);
+            } else if (precision == OLX_LAYERED_BF16) {
+                k_lay_gen<__bf16, true, 0><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)Yh, nullptr);
+                k_lay_gen<__bf16, false, 0><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)Xh, nullptr);
+            } else {
+                k_lay_gen<__bf16, true, 1><<<gy, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cy, cdy, (int)ny, (uint32_t*)Yh, (uint32_t*)Yl);
+                k_lay_gen<__bf16, false, 1><<<gx, 256, 0, s>>>(lp, (int)j0, (int)n, kc, cx, cdx, (int)nx, (uint32_t*)Xh, (uint32_t*)Xl);
+            }
         }
-    };
-    gen(0, fs[0]);
-    OLX_TRY(check_launch("layered factors"));
-    if (overlap) OLX_HIP(hipEventRecord(ev[1], sg));
-    for (int64_t c = 0; c < nchunks; c++) {
-        const int k = overlap ? (int)(c & 1) : 0;
-        if (overlap) OLX_HIP(hipStreamWaitEvent(s, ev[1 + k], 0));
+        OLX_TRY(check_launch("layered factors"));
         {
             TimedLaunch tl(ctx, "layered_gemm");
-            const int acc = (c == 0 && !acc0) ? 0 : 1;
-            gemm(fs[k].Yh, fs[k].Xh, acc);
+            const int acc = (first && !acc0) ? 0 : 1;
+            gemm(Yh, Xh, acc);
             if (precision == OLX_LAYERED_BF16X3) {
-                gemm(fs[k].Yh, fs[k].Xl, 1);
-                gemm(fs[k].Yl, fs[k].Xh, 1);
+                gemm(Yh, Xl, 1);
+                gemm(Yl, Xh, 1);
             }
         }
         OLX_TRY(check_launch("layered gemm"));
-        if (c + 1 == nchunks) break;
-        const int kn = overlap ? (int)((c + 1) & 1) : 0;
-        if (overlap) {
-            OLX_HIP(hipEventRecord(ev[3 + k], s));
-            // set kn was last read by the GEMM of chunk c - 1
-            if (c >= 1) OLX_HIP(hipStreamWaitEvent(sg, ev[3 + kn], 0));
-        }
-        gen((c + 1) * kc, fs[kn]);
-        OLX_TRY(check_launch("layered factors"));
-        if (overlap) OLX_HIP(hipEventRecord(ev[1 + kn], sg));
+        first = false;
     }
     return OLX_OK;
 }

@@ -103,35 +103,6 @@ def test_layered_wide_tile(prec, shape):
     assert olref.rel_l2(U.cpu().numpy()[ph, pw], ref) < 1e-3
 
 
-@pytest.mark.parametrize("prec", ["f16", "bf16x3"])
-def test_layered_overlapped_factor_generation(prec):
-    """Four factor chunks: the next chunk's factors are generated on a second stream
-    while the current chunk's GEMM runs, alternating two factor buffers.  The result
-    equals, bit for bit, the in-order run (kernel timing on keeps everything on one
-    stream), and agrees with the oracle on sampled pixels."""
-    from openol_amd import _lib
-    ny, nx, n, nl = 256, 128, 100013, 8
-    zmin, zmax = 0.10, 0.20
-    pts = cloud(n, 63, ny, nx, zmin, zmax)
-    obj = ol().Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
-    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
-    ol().CghLayered(obj, U, ny, nx, P, P, LAM, nl, zmin, zmax, precision=prec)
-    L, c = _lib.load(), _lib.ctx(0)
-    V = torch.empty_like(U)
-    L.olx_ctx_enable_timing(c, 1)
-    try:
-        ol().CghLayered(obj, V, ny, nx, P, P, LAM, nl, zmin, zmax, precision=prec)
-        torch.cuda.synchronize()
-    finally:
-        L.olx_ctx_enable_timing(c, 0)
-    assert torch.equal(U, V)
-    zl = snapped(pts, zmin, zmax, nl)
-    rng = np.random.default_rng(6)
-    ph, pw = rng.integers(0, ny, 128), rng.integers(0, nx, 128)
-    ref = olref.cgh_pixels(pts["x"], pts["y"], zl, ph, pw, P, P, LAM, pts["amp"], pts["phase"])
-    assert olref.rel_l2(U.cpu().numpy()[ph, pw], ref) < 1e-3
-
-
 def test_layered_rejects_unsupported_shapes():
     from openol_amd._lib import OlxError
     obj = ol().Object([1e-4], [1e-4], [0.15])
