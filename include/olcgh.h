@@ -42,9 +42,7 @@ void cgh_complex(const Object<_Tp>& obj, std::unique_ptr<std::complex<C>[]>& u, 
 
 }  // namespace detail
 
-// phase-only output (atan2 of the sum), include/olcgh.h:351-365.  The
-// normalized variant (_CghFresnelNormalized, include/olcgh.h:215-234) reads x for
-// both coordinates in the reference and is not provided.
+// phase-only output (atan2 of the sum), include/olcgh.h:351-365.
 template <typename _Tp, typename PHASE_T, typename PREC_T>
 void Cgh(Object<_Tp>& obj, std::unique_ptr<PHASE_T[]>& phase, int ny, int nx, PREC_T py,
          PREC_T px, PREC_T lambda) {
@@ -62,11 +60,22 @@ void Cgh(Object<_Tp>& obj, std::unique_ptr<PHASE_T[]>& phase, int ny, int nx, PR
         for (size_t i = 0; i < n; i++) phase[i] = (PHASE_T)t[i];
     }
 }
+// normalized = true runs _CghFresnelNormalized (include/olcgh.h:213-231): point
+// coordinates in pixels, dx = p (w - x_j), dy = p (h - x_j) (the reference reads x_j
+// for both axes), dz = p z_j, unit amplitude.  That is the plain Fresnel CGH of the
+// cloud (p x_j, p x_j, p z_j), which is what runs here.
 template <typename _Tp, typename PHASE_T, typename PREC_T>
 void Cgh(Object<_Tp>& obj, std::unique_ptr<PHASE_T[]>& phase, int ny, int nx, PREC_T p,
          PREC_T lambda, bool normalized = false) {
-    if (normalized) throw std::runtime_error("Cgh: the normalized variant is not provided");
-    Cgh(obj, phase, ny, nx, p, p, lambda);
+    if (!normalized) {
+        Cgh(obj, phase, ny, nx, p, p, lambda);
+        return;
+    }
+    const double pd = (double)p;
+    Object<double> scaled(obj.size);
+    for (int j = 0; j < obj.size; j++)
+        scaled[j] = Vect3<double>(pd * (double)obj[j].x, pd * (double)obj[j].x, pd * (double)obj[j].z);
+    Cgh(scaled, phase, ny, nx, p, p, lambda);
 }
 
 // complex output, include/olcgh.h:367-379 (_CghFresnel :296-313)

@@ -248,6 +248,25 @@ def cgh(x, y, z, ny, nx, py, px, lam, amp=None, phase=None, flags=0, dtype=np.co
     return U
 
 
+def cgh_normalized(x, y, z, ny, nx, p, lam):
+    """_CghFresnelNormalized, include/olcgh.h:213-231 (what Cgh(obj, phase, ny, nx, p,
+    lambda, true) runs): point coordinates in pixels, dx = p (w - x_j), dy = p (h - x_j)
+    (the reference reads x_j for both), dz = p z_j, theta = 2 pi (dx^2 + dy^2) / (2 dz
+    lambda), unit amplitude; returns the phase atan2(Im U, Re U).  FP64, numpy, for
+    small sizes."""
+    x = np.asarray(x, np.float64)
+    z = np.asarray(z, np.float64)
+    w = np.arange(nx, dtype=np.float64)[None, :]
+    h = np.arange(ny, dtype=np.float64)[:, None]
+    U = np.zeros((ny, nx), np.complex128)
+    for xj, zj in zip(x, z):
+        dx = p * (w - xj)
+        dy = p * (h - xj)
+        dz = p * zj
+        U += np.exp(1j * (2 * np.pi * (dx * dx + dy * dy) / (2 * dz * lam)))
+    return np.arctan2(U.imag, U.real)
+
+
 def cgh_pixels(x, y, z, ph, pw, py, px, lam, amp=None, phase=None, flags=0):
     """FP64 CGH sum at the listed pixels only (sampled check at configuration size)."""
     x, y, z, amp, phase = _pts(x, y, z, amp, phase)

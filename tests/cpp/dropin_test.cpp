@@ -123,6 +123,12 @@ int main(int argc, char** argv) {
         auto ph = std::make_unique<float[]>((size_t)n * n);
         ol::Cgh(obj, ph, n, n, p, p, lambda);
         save(dir + "/cgh_phase.bin", ph.get(), (size_t)n * n);
+        // normalized variant: the same cloud in pixel units
+        ol::Object<double> pix(m);
+        for (int j = 0; j < m; j++)
+            pix[j] = ol::Vect3<double>(xyz[3 * j] / p, xyz[3 * j + 1] / p, xyz[3 * j + 2] / p);
+        ol::Cgh(pix, ph, n, n, p, lambda, true);
+        save(dir + "/cgh_norm.bin", ph.get(), (size_t)n * n);
 
         // device API
         auto d = ol::cpu2dev(fresh(), (size_t)n * n);

@@ -69,6 +69,9 @@ def test_dropin_program_matches_oracle(tmp_path):
     assert rel(rd("cgh"), cgh) < 1e-4 and rel(rd("gcgh"), cgh) < 1e-4
     ph = rd("cgh_phase", np.float32)
     assert np.mean(np.abs(np.angle(np.exp(1j * (ph - np.angle(cgh)))))) < 1e-3
+    # Cgh(..., normalized=true): pixel units, x read for both axes
+    pn = olref.cgh_normalized(pts["x"] / P, pts["y"] / P, pts["z"] / P, n, n, P, LAM)
+    assert np.mean(np.abs(np.angle(np.exp(1j * (rd("cgh_norm", np.float32) - pn))))) < 1e-3
     assert rel(rd("wrp"), olref.wrp(pts["x"], pts["y"], pts["z"], n, n, P, LAM, 0.15,
                                     dtype=np.complex128)) < 1e-5
     pz = float([l for l in r.stdout.splitlines() if l.startswith("plane_z")][0].split()[1])

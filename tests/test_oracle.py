@@ -215,6 +215,25 @@ def test_cgh_exact_r_pythagorean():
     assert abs(out[1, 3] - want) > 1e-3
 
 
+def test_cgh_normalized_is_the_cgh_of_the_scaled_cloud():
+    """Cgh(..., normalized=true): pixel coordinates, x_j read for both axes.  A single
+    point one pixel pitch squared over lambda away (theta = pi/2 per pixel^2 offset,
+    as in the closed form above) gives the i^k pattern around (x_j, x_j) whatever y_j
+    is; a random cloud equals the plain CGH of (p x_j, p x_j, p z_j)."""
+    p = 8e-6
+    zpix = 2 * p / LAM  # dz = p z = 2 p^2 / lambda
+    ph = olref.cgh_normalized([5.0], [99.0], [zpix], 10, 12, p, LAM)
+    hh, ww = np.meshgrid(np.arange(10), np.arange(12), indexing="ij")
+    expect = np.angle(1j ** (((ww - 5) ** 2 + (hh - 5) ** 2) % 4))
+    assert np.max(np.abs(np.angle(np.exp(1j * (ph - expect))))) < 1e-9
+    rng = np.random.default_rng(21)
+    x, y = rng.uniform(0, 16, 30), rng.uniform(0, 16, 30)
+    z = rng.uniform(0.1, 0.2, 30) / p
+    ph = olref.cgh_normalized(x, y, z, 16, 16, p, LAM)
+    u = olref.cgh(p * x, p * x, p * z, 16, 16, p, p, LAM, dtype=np.complex128)
+    assert np.max(np.abs(np.angle(np.exp(1j * (ph - np.angle(u)))))) < 1e-7
+
+
 def test_cgh_linearity_and_permutation():
     pts = synth.random_points(20, seed=13, xmax=32 * 8e-6, ymax=32 * 8e-6, zmin=0.1, zmax=0.2)
     args = (16, 16, 8e-6, 8e-6, LAM)
