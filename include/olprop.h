@@ -14,6 +14,8 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <iomanip>
+#include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <type_traits>
@@ -292,9 +294,93 @@ void FresnelResponseBandLimit(std::unique_ptr<std::complex<COMPLEX_T>[]>& h, siz
     });
 }
 
+// include/olprop.h:250-305: the shifted transfer function exp(2 pi i (w z + ox u + oy v)),
+// centred, with the off-axis window (computed in FP64; evanescent entries are 0)
+template <typename PREC_T = float, typename COMPLEX_T>
+void shiftedAsmTransferF(std::unique_ptr<std::complex<COMPLEX_T>[]>& H, int64_t ny, int64_t nx,
+                         PREC_T dv, PREC_T du, PREC_T lambda, PREC_T z, PREC_T oy, PREC_T ox) {
+    detail::on_device(H, ny * nx, [&](void* d) {
+        detail::check(olx_shifted_asm_transfer_c64(detail::ctx(), d, ny, nx, dv, du, lambda, z, oy,
+                                                   ox, OLX_BANDLIMIT),
+                      "shiftedAsmTransferF");
+    });
+}
+
+// include/olprop.h:87-105: prints the Fresnel regime's limits for the 2x padded field
+// (shortest distance from each axis, the band-limit half widths and their pixel counts)
+template <typename PREC_T = double>
+void FresnelResponseCheck(size_t height, size_t width, PREC_T dy, PREC_T dx, PREC_T lambda,
+                          PREC_T z) {
+    const PREC_T half_w = dx * (PREC_T)(2 * width) / 2, half_h = dy * (PREC_T)(2 * height) / 2;
+    const PREC_T xl = lambda * z * 0.5 / dx, yl = lambda * z * 0.5 / dy;
+    std::cout << "zmin (from x) = " << std::setprecision(4) << 2 * half_w * dx / lambda << std::endl;
+    std::cout << "zmin (from y) = " << std::setprecision(4) << 2 * half_h * dy / lambda << std::endl;
+    std::cout << "xlim = " << std::setprecision(4) << xl << std::endl;
+    std::cout << "ylim = " << std::setprecision(4) << yl << std::endl;
+    std::cout << "widthlim[px] = " << (int)std::floor(xl / dx) * 2 << std::endl;
+    std::cout << "heightlim[px] = " << (int)std::floor(yl / dy) * 2 << std::endl;
+}
+
+// include/olprop.h:191-208: prints the largest distance the ASM transfer function
+// samples without aliasing, from each axis of the 2x padded field
+template <typename PREC_T = double>
+void AsmTransferFCheck(int64_t height, int64_t width, PREC_T dy, PREC_T dx, PREC_T lambda,
+                       PREC_T z) {
+    (void)z;
+    const PREC_T du = 1 / (dx * (PREC_T)(2 * width)), dv = 1 / (dy * (PREC_T)(2 * height));
+    const PREC_T umax = du * (PREC_T)width, vmax = dv * (PREC_T)height;
+    const PREC_T root = std::sqrt(1 / (lambda * lambda) - umax * umax - vmax * vmax);
+    std::cout << "zmax (from x) = " << std::setprecision(4) << root / (2 * umax * du) << std::endl;
+    std::cout << "zmax (from y) = " << std::setprecision(4) << root / (2 * vmax * dv) << std::endl;
+}
+
+// include/olprop.h:332-336: an empty function in the reference
+template <typename PREC_T = float, typename COMPLEX_T>
+void shiftedFresnelPropInfo(std::unique_ptr<std::complex<COMPLEX_T>[]>&, int, int, PREC_T, PREC_T,
+                            PREC_T, PREC_T, PREC_T, PREC_T, PREC_T) {}
+
 // ------------------------------------------------------------------ device API
 // (include/golprop.h; fields are ol::device_ptr<std::complex<float>[]>, updated in place)
 using dfield = device_ptr<std::complex<float>[]>;
+
+// include/golprop.h:325-332: the centred transfer function table in HBM
+template <typename PREC_T>
+void gAsmTransferF(dfield& H, int ny, int nx, PREC_T dv, PREC_T du, PREC_T lambda, PREC_T z) {
+    detail::check(olx_asm_transfer_c64(detail::ctx(), H.get(), ny, nx, dv, du, lambda, z, 0),
+                  "gAsmTransferF");
+}
+// include/golprop.h:361-419 (the reference's window is commented out: none here either)
+template <typename PREC_T>
+void gshiftedAsmTransferF(dfield& H, int ny, int nx, PREC_T dv, PREC_T du, PREC_T lambda,
+                          PREC_T z, PREC_T oy, PREC_T ox) {
+    detail::check(olx_shifted_asm_transfer_c64(detail::ctx(), H.get(), ny, nx, dv, du, lambda, z,
+                                               oy, ox, 0),
+                  "gshiftedAsmTransferF");
+}
+// include/golprop.h:58-66: the centred Fresnel impulse response in HBM
+template <typename PREC_T>
+void gFresnelResponse(dfield& H, int ny, int nx, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    detail::check(olx_fresnel_response_c64(detail::ctx(), H.get(), ny, nx, dy, dx, lambda, z,
+                                           INT_MAX, INT_MAX),
+                  "gFresnelResponse");
+}
+// include/golprop.h:223-256: windowed by the x-limit on both axes
+template <typename PREC_T>
+void gFresnelResponseBandLimit(dfield& h, size_t ny, size_t nx, PREC_T dy, PREC_T dx,
+                               PREC_T lambda, PREC_T z) {
+    const int64_t lim = detail::win(std::fabs(lambda * z * 0.5 / dx) / dx);
+    detail::check(olx_fresnel_response_c64(detail::ctx(), h.get(), (int64_t)ny, (int64_t)nx, dy,
+                                           dx, lambda, z, lim, lim),
+                  "gFresnelResponseBandLimit");
+}
+template <typename PREC_T>
+void gFresnelResponseBandLimit(dfield& h, size_t ny, size_t nx, PREC_T dy, PREC_T dx,
+                               PREC_T lambda, PREC_T z, PREC_T p_limit) {
+    const int64_t lim = detail::win(std::fabs(lambda * z * 0.5 / p_limit) / dx);
+    detail::check(olx_fresnel_response_c64(detail::ctx(), h.get(), (int64_t)ny, (int64_t)nx, dy,
+                                           dx, lambda, z, lim, lim),
+                  "gFresnelResponseBandLimit");
+}
 
 // include/golprop.h:506-534
 template <typename PREC_T>

@@ -152,6 +152,14 @@ int olx_fresnel_prop_check(int64_t ny, int64_t nx, double dy, double dx, double 
  * window of :250-305; evanescent entries are 0) into a complex64 ny x nx array */
 int olx_asm_transfer_c64(olx_ctx* ctx, void* d_H, int64_t ny, int64_t nx, double dv, double du,
                          double lambda, double z, uint32_t flags);
+/* materialise the centred shifted transfer function H = exp(2 pi i (w z + ox u + oy v)),
+ * u = (m - nx/2) du, v = (n - ny/2) dv, evanescent entries 0.  OLX_BANDLIMIT adds the
+ * off-axis window of the CPU form.  Replaces shiftedAsmTransferF (include/olprop.h:250-305,
+ * with OLX_BANDLIMIT) and gshiftedAsmTransferF (include/golprop.h:336-419, window
+ * commented out there: flags 0). */
+int olx_shifted_asm_transfer_c64(olx_ctx* ctx, void* d_H, int64_t ny, int64_t nx, double dv,
+                                 double du, double lambda, double z, double oy, double ox,
+                                 uint32_t flags);
 /* materialise the centred Fresnel impulse response h = exp(i pi (x^2 + y^2)/(lambda z)),
  * x = (m - nx/2) dx, zero where |m - nx/2| >= wlim or |n - ny/2| >= hlim
  * (FresnelResponse / FresnelResponseBandLimit, include/olprop.h:8-84; pass

@@ -105,6 +105,7 @@ SIGNATURES = {
     "olx_bmp_write": [ctypes.c_char_p, _p, _i, _i],
     "olx_bmp_read": [ctypes.c_char_p, _p, _i, _i],
     "olx_multiplane_mse_c64": [_p, _p, _i64, _i64, _i, _p, _p, _d, _d, _d, _u32, _p, _p],
+    "olx_shifted_asm_transfer_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _u32],
 }
 DOUBLE_RESULT = {"olx_wrp_plane_z", "olx_layer_depth"}
 

@@ -689,6 +689,27 @@ __global__ void k_asm_transfer(float2* __restrict__ H, int ny, int nx, double dv
     H[(size_t)n * nx + m] = h;
 }
 
+// shiftedAsmTransferF (include/olprop.h:250-305) / gshiftedAsmTransferF (include/golprop.h:
+// 336-419): H = exp(2 pi i (w z + ox u + oy v)), centred; bl: the window
+// |u - u0| * 2 < uw, |v - v0| * 2 < vw of the CPU form; evanescent entries are 0
+__global__ void k_shifted_transfer(float2* __restrict__ H, int ny, int nx, double dv, double du,
+                                   double inv_l2, double z, double oy, double ox, int bl,
+                                   double u0, double uw, double v0, double vw) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y;
+    if (m >= nx) return;
+    const double v = (double)(n - ny / 2) * dv, u = (double)(m - nx / 2) * du;
+    const double arg = inv_l2 - u * u - v * v;
+    float2 h = make_float2(0.f, 0.f);
+    if (arg >= 0.0 && (!bl || (fabs(u - u0) * 2 < uw && fabs(v - v0) * 2 < vw))) {
+        double cyc = sqrt(arg) * z + ox * u + oy * v;
+        cyc -= rint(cyc);
+        double s, c;
+        sincospi(2.0 * cyc, &s, &c);
+        h = make_float2((float)c, (float)s);
+    }
+    H[(size_t)n * nx + m] = h;
+}
+
 __global__ void k_fresnel_response(float2* __restrict__ h, int ny, int nx, double dy, double dx,
                                    double tmp, long long wlim, long long hlim) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y;
@@ -1176,6 +1197,25 @@ extern "C" int olx_asm_transfer_c64(olx_ctx* ctx, void* d_H, int64_t ny, int64_t
         (float2*)d_H, (int)ny, (int)nx, dv, du, 1 / (lambda * lambda), z,
         (flags & OLX_BANDLIMIT) ? 1 : 0, uw, vw);
     return check_launch("asm_transfer");
+}
+
+extern "C" int olx_shifted_asm_transfer_c64(olx_ctx* ctx, void* d_H, int64_t ny, int64_t nx,
+                                            double dv, double du, double lambda, double z,
+                                            double oy, double ox, uint32_t flags) {
+    if (!ctx || !d_H || ny <= 0 || nx <= 0 || ny > 65535 || nx > (1 << 24))
+        OLX_FAIL(OLX_EINVAL, "shifted_asm_transfer: bad argument");
+    if (!(lambda > 0) || !(du > 0) || !(dv > 0) || !std::isfinite(z) || !std::isfinite(oy) ||
+        !std::isfinite(ox))
+        OLX_FAIL(OLX_EINVAL, "shifted_asm_transfer: lambda, du, dv must be > 0 and z, oy, ox finite");
+    double u0 = 0, uw = 0, v0 = 0, vw = 0;
+    const int bl = (flags & OLX_BANDLIMIT) ? 1 : 0;
+    if (bl) {
+        shift_window(ox, du, z, lambda, &u0, &uw);
+        shift_window(oy, dv, z, lambda, &v0, &vw);
+    }
+    k_shifted_transfer<<<dim3((unsigned)((nx + 255) / 256), (unsigned)ny), 256, 0, ctx->stream>>>(
+        (float2*)d_H, (int)ny, (int)nx, dv, du, 1 / (lambda * lambda), z, oy, ox, bl, u0, uw, v0, vw);
+    return check_launch("shifted_asm_transfer");
 }
 
 extern "C" int olx_fresnel_response_c64(olx_ctx* ctx, void* d_h, int64_t ny, int64_t nx,

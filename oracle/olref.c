@@ -283,6 +283,16 @@ static void shifted_asm_transfer(cd* H, int64_t ny, int64_t nx, double dv, doubl
     }
 }
 
+/* the table itself, complex128 (shiftedAsmTransferF with OLREF_SHIFT_WINDOW,
+ * gshiftedAsmTransferF without) */
+int olref_shifted_asm_transfer_c128(double* Hout, int64_t ny, int64_t nx, double dv, double du,
+                                    double lambda, double z, double oy, double ox,
+                                    unsigned flags) {
+    if (!Hout || ny <= 0 || nx <= 0) return -1;
+    shifted_asm_transfer((cd*)Hout, ny, nx, dv, du, lambda, z, oy, ox, flags);
+    return 0;
+}
+
 #define DEFINE_SHIFTED_ASM(SUF, CT, RT)                                             \
 int olref_shifted_asm_##SUF(const RT* srcin, RT* dstin, int64_t ny, int64_t nx,     \
                             double dy, double dx, double lambda, double z,          \

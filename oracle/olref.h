@@ -150,4 +150,7 @@ double olref_wrp_plane_z(int64_t ny, int64_t nx, double p, double lambda,
 #ifdef __cplusplus
 }
 #endif
+int olref_shifted_asm_transfer_c128(double* Hout, int64_t ny, int64_t nx, double dv, double du,
+                                    double lambda, double z, double oy, double ox,
+                                    unsigned flags);
 #endif

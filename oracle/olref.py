@@ -55,6 +55,7 @@ def lib():
             "olref_shifted_fresnel_c128": [p, i64, i64, d, d, d, d, d, d, d],
             "olref_shifted_asm_c64": [p, p, i64, i64, d, d, d, d, d, d, u32],
             "olref_shifted_asm_c128": [p, p, i64, i64, d, d, d, d, d, d, u32],
+            "olref_shifted_asm_transfer_c128": [p, i64, i64, d, d, d, d, d, d, u32],
             "olref_split_prop_c64": [p, i64, i64, d, d, d, d, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, u32],
             "olref_tilted_c128": [p, i64, i64, d, d, d, d, d, ctypes.c_int, ctypes.c_int,
@@ -154,6 +155,16 @@ def shifted_asm(u, dy, dx, lam, z, oy, ox, flags=0):
     _check(fn(_ptr(src), _ptr(out), src.shape[0], src.shape[1], dy, dx, lam, z, oy, ox, flags),
            "shifted_asm")
     return out
+
+
+def shifted_asm_transfer(ny, nx, dv, du, lam, z, oy, ox, flags=0):
+    """The centred table of shiftedAsmTransferF (include/olprop.h:250-305, flags
+    SHIFT_WINDOW) or gshiftedAsmTransferF (include/golprop.h:336-419, flags 0),
+    complex128; evanescent entries 0."""
+    H = np.empty((ny, nx), np.complex128)
+    _check(lib().olref_shifted_asm_transfer_c128(_ptr(H), ny, nx, dv, du, lam, z, oy, ox, flags),
+           "shifted_asm_transfer")
+    return H
 
 
 def split_prop(u, dy, dx, lam, z, by=2, bx=2, mode=0, flags=0):

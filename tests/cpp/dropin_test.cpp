@@ -167,6 +167,21 @@ int main(int argc, char** argv) {
         auto h = std::make_unique<std::complex<float>[]>((size_t)2 * n * 2 * n);
         ol::FresnelResponseBandLimit(h, 2 * n, 2 * n, p, p, lambda, 0.002);
         save(dir + "/h.bin", h.get(), (size_t)4 * n * n);
+        const double dq = 1 / (p * 2 * n);
+        ol::shiftedAsmTransferF(H, 2 * n, 2 * n, dq, dq, lambda, 0.05, 2e-4, -5e-4);
+        save(dir + "/Hs.bin", H.get(), (size_t)4 * n * n);
+        auto gH = ol::make_device<std::complex<float>[]>((size_t)4 * n * n);
+        ol::gAsmTransferF(gH, 2 * n, 2 * n, dq, dq, lambda, 0.05);
+        save(dir + "/gH.bin", ol::dev2cpu(gH, (size_t)4 * n * n).get(), (size_t)4 * n * n);
+        ol::gshiftedAsmTransferF(gH, 2 * n, 2 * n, dq, dq, lambda, 0.05, 2e-4, -5e-4);
+        save(dir + "/gHs.bin", ol::dev2cpu(gH, (size_t)4 * n * n).get(), (size_t)4 * n * n);
+        ol::gFresnelResponse(gH, 2 * n, 2 * n, p, p, lambda, 0.002);
+        save(dir + "/gh.bin", ol::dev2cpu(gH, (size_t)4 * n * n).get(), (size_t)4 * n * n);
+        ol::gFresnelResponseBandLimit(gH, 2 * n, 2 * n, p, p, lambda, 0.002);
+        save(dir + "/ghbl.bin", ol::dev2cpu(gH, (size_t)4 * n * n).get(), (size_t)4 * n * n);
+        ol::AsmTransferFCheck(n, n, p, p, lambda, 0.05);
+        ol::FresnelResponseCheck(n, n, p, p, lambda, 0.05);
+        ol::shiftedFresnelPropInfo(u, n, n, p, p, lambda, 0.05, 0.5, 0.0, 0.0);
         ol::synchronize();
     } catch (const std::exception& e) {
         std::cout << "error: " << e.what() << "\n";

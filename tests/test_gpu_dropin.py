@@ -92,3 +92,17 @@ def test_dropin_program_matches_oracle(tmp_path):
     hh = np.exp(1j * np.pi * (x[None, :] ** 2 + x[:, None] ** 2) / (LAM * 0.002))
     hh *= (np.abs(c)[None, :] < lim) & (np.abs(c)[:, None] < lim)
     assert rel(rd("h", k=2), hh) < 1e-6
+    # the same tables in HBM (gAsmTransferF, gFresnelResponse(BandLimit)), and the shifted
+    # transfer function: host form with the off-axis window, device form without
+    assert rel(rd("gH", k=2), np.exp(2j * np.pi * wv * 0.05)) < 1e-6
+    hf = np.exp(1j * np.pi * (x[None, :] ** 2 + x[:, None] ** 2) / (LAM * 0.002))
+    assert rel(rd("gh", k=2), hf) < 1e-6
+    assert rel(rd("ghbl", k=2), hh) < 1e-6
+    hs = olref.shifted_asm_transfer(M, M, du, du, LAM, 0.05, 2e-4, -5e-4, olref.SHIFT_WINDOW)
+    assert rel(rd("Hs", k=2), hs) < 1e-6
+    assert rel(rd("gHs", k=2), olref.shifted_asm_transfer(M, M, du, du, LAM, 0.05, 2e-4, -5e-4)) < 1e-6
+    # the check functions print their limits
+    zmax = float([l for l in r.stdout.splitlines() if l.startswith("zmax (from x)")][0].split("=")[1])
+    umax = du * n
+    assert abs(zmax - np.sqrt(1 / LAM ** 2 - 2 * umax ** 2) / (2 * umax * du)) < 1e-3 * zmax
+    assert any(l.startswith("widthlim[px] = ") for l in r.stdout.splitlines())

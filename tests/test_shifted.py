@@ -82,6 +82,30 @@ def test_far_offset_window_branches(sign):
     assert olref.rel_l2(wrong, truth) > 0.9
 
 
+def test_shifted_transfer_table():
+    """The shifted table: at zero offset without the window, the plain transfer function
+    exp(2 pi i w z); a shift multiplies it by exp(2 pi i (ox u + oy v)) (sign and axes);
+    the window keeps exactly the band |u - u0| < uw / 2 (here centred, u0 = 0, at zero
+    offset: the band limit |u| < 1 / (lambda sqrt((2 du z)^2 + 1)))."""
+    ny, nx, du, dv, z = 32, 48, 1 / (64 * P), 1 / (96 * 6e-6), 0.01
+    c_u = (np.arange(nx) - nx // 2) * du
+    c_v = (np.arange(ny) - ny // 2) * dv
+    w = np.sqrt(1 / LAM ** 2 - c_u[None, :] ** 2 - c_v[:, None] ** 2)
+    H0 = olref.shifted_asm_transfer(ny, nx, dv, du, LAM, z, 0.0, 0.0)
+    assert np.max(np.abs(H0 - np.exp(2j * np.pi * w * z))) < 1e-9
+    oy, ox = 3e-4, -7e-4
+    H1 = olref.shifted_asm_transfer(ny, nx, dv, du, LAM, z, oy, ox)
+    ramp = np.exp(2j * np.pi * (ox * c_u[None, :] + oy * c_v[:, None]))
+    assert np.max(np.abs(H1 - H0 * ramp)) < 1e-9
+    zl = 0.2  # long enough for the band limit to cut
+    Hw = olref.shifted_asm_transfer(ny, nx, dv, du, LAM, zl, 0.0, 0.0, olref.SHIFT_WINDOW)
+    ulim = 1 / (LAM * np.sqrt((2 * du * zl) ** 2 + 1))
+    vlim = 1 / (LAM * np.sqrt((2 * dv * zl) ** 2 + 1))
+    keep = (np.abs(c_u)[None, :] < ulim) & (np.abs(c_v)[:, None] < vlim)
+    assert 0 < keep.sum() < keep.size
+    assert np.array_equal(Hw != 0, keep)
+
+
 def test_shifted_adjoint_dot_product():
     u = synth.random_field(16, 24, seed=5, dtype=np.complex128)
     v = synth.random_field(16, 24, seed=6, dtype=np.complex128)
