@@ -85,7 +85,7 @@ def bench_asm(ol, lib, torch, n, steps, warmup):
 # (the factors a plain 512 MiB copy kernel reads back), and SQ_INSTS_VALU.
 MEASURED_4096 = {
     "prop_rows_pad_fwd": {"traffic": 404.4e6, "valu_instr": 24.87e6},
-    "prop_cols_conv": {"traffic": 538.9e6, "valu_instr": 117.31e6},
+    "prop_cols_conv": {"traffic": 539.8e6, "valu_instr": 108.40e6},
     "prop_rows_crop_inv": {"traffic": 404.0e6, "valu_instr": 25.35e6},
 }
 # wave64 VALU instructions per second: 1024 SIMDs x 2.4 GHz / 4 cycles (no packed FP32)

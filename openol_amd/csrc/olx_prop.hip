@@ -377,7 +377,7 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
     constexpr int R0I = PI::radix(0), RLI = PI::radix(PI::NST - 1);
     constexpr int E = PF::E, T = PF::T;
     static_assert(T % 16 == 0, "pair plan");
-    __shared__ float2 lds[KR * PF::LP];
+    __shared__ __align__(16) float2 lds[KR * PF::LP];
     const int tid = threadIdx.x, c = tid % KR, t = tid / KR;
     const int cbl = a.cbl;
     // sets a and b are two neighbouring columns: one 16-byte access per row serves both
@@ -426,16 +426,54 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
         const bool cuta = a.bandlimit && !(fabs(SH ? ua - a.u0 : ua) * 2 < a.uw);
         const bool cutb = a.bandlimit && !(fabs(SH ? ub - a.u0 : ub) * 2 < a.uw);
         const double pca = SH ? a.ox * ua : 0.0, pcb = SH ? a.ox * ub : 0.0;
+        // K_ASM: H depends on v^2 and |v| only, so row i and its mirror L - i share
+        // one value.  The thread's rows below L/2 (slots q < E/2 of i = t + T q) are
+        // evaluated and published in LDS (free here: the forward transform ended with
+        // a barrier after its last LDS reads), and the rows past L/2 read their
+        // mirror's value.  Row L/2 is its own mirror: every thread evaluates it and
+        // stores it at entry t + (t == 0 ? L/2 : 0), computed without a select (a
+        // branch here makes the register allocator spill); for t > 0 that entry is
+        // then overwritten, in program order, with row t's value.  Entry (row, c)
+        // holds sets a and b.  The shifted form's window and phase are not
+        // symmetric: there every row is evaluated.
+        float4* tab = reinterpret_cast<float4*>(lds);
+        static_assert((L / 2 + 1) * 2 <= PF::LP, "mirror table fits in LDS");
+        if constexpr (!SH) {
+            const float2 hma = asm_h(a, Aa, cuta, L / 2, L);
+            __builtin_amdgcn_sched_barrier(0);
+            const float2 hmb = asm_h(a, Ab, cutb, L / 2, L);
+            const int e0 = t + (((t - 1) >> 31) & (L / 2));
+            tab[e0 * KR + c] = make_float4(hma.x, hma.y, hmb.x, hmb.y);
+            __builtin_amdgcn_sched_barrier(0);
+        }
 #pragma unroll
         for (int b = 0; b < E / RLF; b++)
 #pragma unroll
             for (int r = 0; r < RLF; r++) {
                 const int i = last_index<PF>(t, b, r);
-                va[b * RLF + r] = cmul(va[b * RLF + r], asm_h<SH>(a, Aa, cuta, i, L, pca));
-                vb[b * RLF + r] = cmul(vb[b * RLF + r], asm_h<SH>(a, Ab, cutb, i, L, pcb));
+                if (!SH && b + r * (E / RLF) >= E / 2) continue;
+                const float2 ha = asm_h<SH>(a, Aa, cuta, i, L, pca);
+                const float2 hb = asm_h<SH>(a, Ab, cutb, i, L, pcb);
+                va[b * RLF + r] = cmul(va[b * RLF + r], ha);
+                vb[b * RLF + r] = cmul(vb[b * RLF + r], hb);
+                if (!SH) tab[i * KR + c] = make_float4(ha.x, ha.y, hb.x, hb.y);
                 // keep the FP64 temporaries of one element pair live at a time
                 __builtin_amdgcn_sched_barrier(0);
             }
+        if constexpr (!SH) {
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < E / RLF; b++)
+#pragma unroll
+                for (int r = 0; r < RLF; r++) {
+                    if (b + r * (E / RLF) < E / 2) continue;
+                    const float4 h = tab[(L - last_index<PF>(t, b, r)) * KR + c];
+                    va[b * RLF + r] = cmul(va[b * RLF + r], make_float2(h.x, h.y));
+                    vb[b * RLF + r] = cmul(vb[b * RLF + r], make_float2(h.z, h.w));
+                }
+            // the inverse transform's first exchange overwrites the table
+            __syncthreads();
+        }
     }
     float2 wa[E], wb[E];
 #pragma unroll

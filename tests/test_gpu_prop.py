@@ -136,6 +136,28 @@ def test_asm_tiled_rectangular(shape, dy, dx, z):
         assert rel(u.cpu().numpy(), olref.asm_prop(u0, dy, dx, LAM, z, flags)) < TOL
 
 
+@pytest.mark.parametrize("shape", [(256, 256), (1024, 1024), (1024, 2048), (2048, 1024),
+                                   (2048, 4096), (4096, 4096), (8192, 8192)])
+def test_asm_bitwise_equals_shifted_at_zero_offset(shape):
+    """On the tiled path ASM pairs each spectral column u with its mirror -u and
+    evaluates one transfer value for both (columns 0 and Mx/2 pair with themselves);
+    the shifted form evaluates every column on its own.  At zero offset its phase,
+    window and scale are the same FP64 arithmetic, so the fields agree bit for bit,
+    window on and off, forward and adjoint."""
+    ny, nx = shape
+    g = torch.Generator(device="cuda").manual_seed(ny + 3 * nx)
+    u0 = torch.randn(ny, nx, dtype=torch.complex64, device="cuda", generator=g)
+    for bandlimit in (False, True):
+        for adjoint in (False, True):
+            u = u0.clone()
+            ol().AsmProp(u, ny, nx, 8e-6, 6e-6, LAM, 0.09, bandlimit=bandlimit, adjoint=adjoint)
+            v = torch.empty_like(u0)
+            ol().gshiftedAsmProp(u0, v, ny, nx, 8e-6, 6e-6, LAM, 0.09, 0.0, 0.0,
+                                 bandlimit=bandlimit, adjoint=adjoint)
+            assert torch.equal(u, v), (bandlimit, adjoint)
+            assert torch.isfinite(torch.view_as_real(u)).all()
+
+
 @pytest.mark.parametrize("variant,flags", [(None, 0), ("cpu", olref.FRESNEL_BL_CPU)])
 def test_fresnel_tiled(variant, flags):
     shape, d, z = (1024, 2048), 8e-6, 0.3
