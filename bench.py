@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--n", type=int, default=4096, help="field side (metric: 4096)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the ASM metric")
-    ap.add_argument("--only", choices=["ingest", "next"], default=None,
+    ap.add_argument("--only", choices=["ingest", "next", "cgh"], default=None,
                     help="run one secondary workload alone and print its figures (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (for PMC byte-counter calibration)")
@@ -264,19 +264,26 @@ def asm_sizes(ol, lib, torch):
     return out
 
 
-def extras(ol, lib, torch):
-    """Secondary workloads of the metric: direct vs layered CGH, WRP + ARSS,
-    the smaller ASM sizes, the multi-plane optimisation and the device ceilings."""
+# VALU lane instructions per (pixel, point) interaction of the direct FP32 kernels,
+# from rocprofv3 --pmc SQ_INSTS_VALU on the 10,000-point 2048^2 cloud (wave64
+# instructions per launch x 64 / interactions): the separable Fresnel kernel
+# (4 + 8 sine / cosine pairs per point for 32 pixels, then 4 FMA per pixel) and the
+# exact-r kernel (series and square-root forms).
+CGH_VALU_PER_INTERACTION = {"fresnel": 6.254, "exact": 10.77}
+
+
+def direct_cgh(ol, torch):
+    """Direct CGH of 10,000 points over 2048^2 (config 2), exact r and Fresnel, the
+    FP64 kernel, and each FP32 kernel's VALU issue roofline (both are bound by VALU
+    issue: sine and cosine are a minority of their instructions)."""
     from openol_amd import synth
     out = {}
-    out["ceilings"] = ceilings(lib, torch)
-    out.update(asm_sizes(ol, lib, torch))
-    # direct CGH, 10,000 points over 2048^2, exact r (config 2)
     N, n = 2048, 10000
     pts = synth.random_points(n, seed=2, xmax=N * PITCH, ymax=N * PITCH, zmin=0.10, zmax=0.20)
     obj = ol.Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
     U = torch.empty((N, N), dtype=torch.complex64, device="cuda")
     for exact in (True, False):
+        kind = "exact" if exact else "fresnel"
         ol.Cgh(obj, U, N, N, PITCH, PITCH, LAM, exact_r=exact)
         torch.cuda.synchronize()
         reps = 5
@@ -284,8 +291,10 @@ def extras(ol, lib, torch):
         for _ in range(reps):
             ol.Cgh(obj, U, N, N, PITCH, PITCH, LAM, exact_r=exact)
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
-        out["cgh_direct_" + ("exact" if exact else "fresnel") + "_Gptpx_s"] = round(n * N * N / dt / 1e9, 1)
+        rate = n * N * N * reps / (time.perf_counter() - t0)
+        out[f"cgh_direct_{kind}_Gptpx_s"] = round(rate / 1e9, 1)
+        out[f"cgh_direct_{kind}_valu_frac"] = round(
+            rate * CGH_VALU_PER_INTERACTION[kind] / 64 / VALU_PEAK, 3)
     # the FP64 kernel (exact r), complex128 output
     U64 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
     ol.CghFP64(obj, U64, N, N, PITCH, PITCH, LAM, exact_r=True)
@@ -294,11 +303,17 @@ def extras(ol, lib, torch):
     ol.CghFP64(obj, U64, N, N, PITCH, PITCH, LAM, exact_r=True)
     torch.cuda.synchronize()
     out["cgh_direct_exact_fp64_Gptpx_s"] = round(n * N * N / (time.perf_counter() - t0) / 1e9, 1)
-    del U64
-    # one interaction needs a sine and a cosine: the transcendental ceiling
-    sin_rate = out["ceilings"]["sin_f32_Gevals_s"] * 1e9
-    out["cgh_direct_fresnel_frac_of_sincos_ceiling"] = round(
-        out["cgh_direct_fresnel_Gptpx_s"] * 1e9 / (sin_rate / 2), 3)
+    return out
+
+
+def extras(ol, lib, torch):
+    """Secondary workloads of the metric: direct vs layered CGH, WRP + ARSS,
+    the smaller ASM sizes, the multi-plane optimisation and the device ceilings."""
+    from openol_amd import synth
+    out = {}
+    out["ceilings"] = ceilings(lib, torch)
+    out.update(asm_sizes(ol, lib, torch))
+    out.update(direct_cgh(ol, torch))
     # WRP (200,000 points) + ARSS to the hologram (config 3)
     N, n = 4096, 200000
     pts = synth.random_points(n, seed=3, xmax=N * PITCH, ymax=N * PITCH, zmin=0.151, zmax=0.155)
@@ -498,6 +513,9 @@ def main():
         return
     if args.only == "next":
         print(json.dumps({"next_rows": next_rows(ol, torch)}), flush=True)
+        return
+    if args.only == "cgh":
+        print(json.dumps({"direct_cgh": direct_cgh(ol, torch)}), flush=True)
         return
     n = args.n
     dt, rep = bench_asm(ol, lib, torch, n, args.steps, args.warmup)

@@ -5,14 +5,18 @@
 //             (the reference's 2 pi (dx^2+dy^2)/(2 dz lambda), include/olcgh.h:304-308)
 //   exact r:  theta = sqrt(dx^2 + dy^2 + z_j^2) / lambda + phi_j / 2 pi
 //
-// A work-group owns a 32 x 32 pixel tile (256 threads, 4 pixels each, one row of
-// 32 contiguous pixels per 32 lanes, so every store is a 256-byte line).  Points
-// are staged through LDS 256 at a time: each thread converts one point into the
-// tile's FP64 anchor (phase at the tile centre reduced mod 1, and the linear and
-// quadratic phase slopes across the tile), rounded to FP32 once.  The inner loop
-// then evaluates every (pixel, point) pair in FP32 from the anchor with the
-// pixel's small offset, so the 1e4..1e5-cycle phases never meet FP32 rounding.
-// sin/cos use the hardware transcendental unit in revolutions.
+// Points are staged through LDS 256 at a time: each thread converts one point into
+// the tile's FP64 anchors (phase at an anchor pixel reduced mod 1, and the linear
+// and quadratic phase slopes around it), rounded to FP32 once.  The inner loop then
+// evaluates every (pixel, point) pair in FP32 from an anchor with the pixel's small
+// offset (at most 16 pixels), so the 1e4..1e5-cycle phases never meet FP32
+// rounding.  sin/cos use the hardware transcendental unit in revolutions.  Each
+// chunk of 256 points is summed on its own and then added to the pixel's total, so
+// the roundings of the running sum happen once per chunk.
+//
+// Exact r (k_cgh_exact): a work-group owns a 32 x 32 pixel tile (256 threads, 4
+// pixels each, one row of 32 contiguous pixels per 32 lanes, so every store is a
+// 256-byte line) with one anchor at its centre.
 //
 // Exact r, per (pixel, point): with dq = r^2 - ra^2 and e = dq / ra^2,
 // r - ra = ra (sqrt(1 + e) - 1).  When e stays small over the whole tile, the
@@ -20,7 +24,6 @@
 // (5/128) e^4 ra, kept under 4e-8 cycles by the per-point test.  Other points
 // (near field, wide tiles) use dq / (ra + sqrt(ra^2 + dq)).  Each 256-point chunk
 // is partitioned so both groups run as uniform loops.
-#include <type_traits>
 
 #include "olx_common.hpp"
 
@@ -36,28 +39,22 @@ __device__ __forceinline__ void sincos_rev(float t, float* s, float* c) {
     *c = __builtin_amdgcn_cosf(t);
 }
 
-struct PtAnchor {  // per-(tile, point) constants, FP32
-    float th;      // phase at the anchor pixel, cycles mod 1 (phi folded in)
-    float gx, gy;  // Fresnel: 2 c dxa px, 2 c dya py (cycles per pixel)
-                   // exact, series: 2 dxa px / ra^2, 2 dya py / ra^2; exact, sqrt: 2 dxa px, 2 dya py
-    float qx, qy;  // Fresnel: c px^2, c py^2; exact, series: px^2 / ra^2, py^2 / ra^2; exact, sqrt: ra, ra^2
+struct PtAnchorR {  // per-(tile, point) constants of the exact-r kernel, FP32
+    float th;       // phase at the anchor pixel, cycles mod 1 (phi folded in)
+    float gx, gy;   // series: 2 dxa px / ra^2, 2 dya py / ra^2; sqrt: 2 dxa px, 2 dya py
+    float qx, qy;   // series: px^2 / ra^2, py^2 / ra^2; sqrt: ra, ra^2
     float amp;
-};
-
-struct PtAnchorR : PtAnchor {  // exact r: series coefficients (ra / lambda) (1/2, -1/8, 1/16)
-    float c1, c2, c3;
+    float c1, c2, c3;  // series coefficients (ra / lambda) (1/2, -1/8, 1/16)
 };
 
 // series bound: (5/128) e^4 ra / lambda <= 4e-8 cycles
 constexpr double CGH_SERIES_E4K = 1.0e-6;
 
-template <bool EXACT>
 __global__ void __launch_bounds__(CGH_NT)
-k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
-          float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx, double py,
-          double px, double lambda, int accumulate) {
-    typedef typename std::conditional<EXACT, PtAnchorR, PtAnchor>::type Anchor;
-    __shared__ Anchor sp[CGH_NT];
+k_cgh_exact(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
+            float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx, double py,
+            double px, double lambda, int accumulate) {
+    __shared__ PtAnchorR sp[CGH_NT];
     __shared__ int wcount[2][CGH_NT / 64];
     const int tid = threadIdx.x, tx = tid % CGH_TX, ty = tid / CGH_TX;
     const int w0 = blockIdx.x * CGH_TX, h0 = blockIdx.y * CGH_TY;
@@ -76,52 +73,40 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
 
     for (int base = 0; base < n; base += CGH_NT) {
         const int j = base + tid;
-        Anchor pa;
+        PtAnchorR pa;
         bool series = false;
         if (j < n) {
             const olx_point p = pts[j];
             const double dxa = px * wa - p.x, dya = py * ha - p.y;
-            double th;
-            if constexpr (!EXACT) {
-                const double c = 1.0 / (2.0 * lambda * p.z);
-                th = c * (dxa * dxa + dya * dya);
-                pa.gx = (float)(2.0 * c * dxa * px);
-                pa.gy = (float)(2.0 * c * dya * py);
-                pa.qx = (float)(c * px * px);
-                pa.qy = (float)(c * py * py);
+            const double ra2 = dxa * dxa + dya * dya + p.z * p.z;
+            const double ra = sqrt(ra2);
+            double th = ra / lambda;
+            // |e| over the tile
+            const double em = (2.0 * (fabs(dxa) * hx + fabs(dya) * hy) + hx * hx + hy * hy) / ra2;
+            series = em <= 0.01 && em * em * em * em * th <= CGH_SERIES_E4K;
+            if (series) {
+                const double ir2 = 1.0 / ra2;
+                pa.gx = (float)(2.0 * dxa * px * ir2);
+                pa.gy = (float)(2.0 * dya * py * ir2);
+                pa.qx = (float)(px * px * ir2);
+                pa.qy = (float)(py * py * ir2);
+                pa.c1 = (float)(0.5 * th);
+                pa.c2 = (float)(-0.125 * th);
+                pa.c3 = (float)(0.0625 * th);
             } else {
-                const double ra2 = dxa * dxa + dya * dya + p.z * p.z;
-                const double ra = sqrt(ra2);
-                th = ra / lambda;
-                // |e| over the tile
-                const double em = (2.0 * (fabs(dxa) * hx + fabs(dya) * hy) + hx * hx + hy * hy) / ra2;
-                series = em <= 0.01 && em * em * em * em * th <= CGH_SERIES_E4K;
-                if (series) {
-                    const double ir2 = 1.0 / ra2;
-                    pa.gx = (float)(2.0 * dxa * px * ir2);
-                    pa.gy = (float)(2.0 * dya * py * ir2);
-                    pa.qx = (float)(px * px * ir2);
-                    pa.qy = (float)(py * py * ir2);
-                    pa.c1 = (float)(0.5 * th);
-                    pa.c2 = (float)(-0.125 * th);
-                    pa.c3 = (float)(0.0625 * th);
-                } else {
-                    pa.gx = (float)(2.0 * dxa * px);
-                    pa.gy = (float)(2.0 * dya * py);
-                    pa.qx = (float)ra;
-                    pa.qy = (float)ra2;
-                    pa.c1 = pa.c2 = pa.c3 = 0.f;
-                }
+                pa.gx = (float)(2.0 * dxa * px);
+                pa.gy = (float)(2.0 * dya * py);
+                pa.qx = (float)ra;
+                pa.qy = (float)ra2;
+                pa.c1 = pa.c2 = pa.c3 = 0.f;
             }
             th += (double)p.phase * (0.5 / M_PI);
             pa.th = (float)(th - rint(th));
             pa.amp = p.amp;
         }
         const int cnt = min(CGH_NT, n - base);
-        int nser = 0;  // exact r: slots [0, nser) hold series points, [nser, cnt) the others
-        if constexpr (!EXACT) {
-            if (j < n) sp[tid] = pa;
-        } else {
+        int nser = 0;  // slots [0, nser) hold series points, [nser, cnt) the others
+        {
             const int lane = tid & 63, wv = tid >> 6;
             const unsigned long long ms = __ballot(j < n && series);
             const unsigned long long mo = __ballot(j < n && !series);
@@ -144,48 +129,41 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
                 sp[series ? sb + __popcll(ms & below) : nser + ob + __popcll(mo & below)] = pa;
         }
         __syncthreads();
-        if constexpr (!EXACT) {
-            for (int q = 0; q < cnt; q++) {
-                const PtAnchor P = sp[q];
-                const float thx = P.th + a * (P.gx + a * P.qx);
+        float cre[CGH_RPT], cim[CGH_RPT];  // this chunk's sums
 #pragma unroll
-                for (int k = 0; k < CGH_RPT; k++) {
-                    const float t = thx + bk[k] * (P.gy + bk[k] * P.qy);
-                    float s, c;
-                    sincos_rev(t, &s, &c);
-                    re[k] += P.amp * c;
-                    im[k] += P.amp * s;
-                }
-            }
-        } else {
-            for (int q = 0; q < nser; q++) {
-                const PtAnchorR P = sp[q];
-                const float ex = a * (P.gx + a * P.qx);
+        for (int k = 0; k < CGH_RPT; k++) cre[k] = cim[k] = 0.f;
+        for (int q = 0; q < nser; q++) {
+            const PtAnchorR P = sp[q];
+            const float ex = a * (P.gx + a * P.qx);
 #pragma unroll
-                for (int k = 0; k < CGH_RPT; k++) {
-                    const float e = ex + bk[k] * (P.gy + bk[k] * P.qy);  // dq / ra^2
-                    const float poly = fmaf(e, fmaf(e, P.c3, P.c2), P.c1);
-                    const float t = fmaf(e, poly, P.th);  // + (r - ra) / lambda
-                    float s, c;
-                    sincos_rev(t, &s, &c);
-                    re[k] += P.amp * c;
-                    im[k] += P.amp * s;
-                }
+            for (int k = 0; k < CGH_RPT; k++) {
+                const float e = ex + bk[k] * (P.gy + bk[k] * P.qy);  // dq / ra^2
+                const float poly = fmaf(e, fmaf(e, P.c3, P.c2), P.c1);
+                const float t = fmaf(e, poly, P.th);  // + (r - ra) / lambda
+                float s, c;
+                sincos_rev(t, &s, &c);
+                cre[k] += P.amp * c;
+                cim[k] += P.amp * s;
             }
-            for (int q = nser; q < cnt; q++) {
-                const PtAnchorR P = sp[q];
-                const float dqx = a * (P.gx + a * px2);
+        }
+        for (int q = nser; q < cnt; q++) {
+            const PtAnchorR P = sp[q];
+            const float dqx = a * (P.gx + a * px2);
 #pragma unroll
-                for (int k = 0; k < CGH_RPT; k++) {
-                    const float dq = dqx + bk[k] * (P.gy + bk[k] * py2);
-                    const float dr = dq / (P.qx + sqrtf(P.qy + dq));  // r - ra
-                    const float t = P.th + dr * inv_lambda;
-                    float s, c;
-                    sincos_rev(t, &s, &c);
-                    re[k] += P.amp * c;
-                    im[k] += P.amp * s;
-                }
+            for (int k = 0; k < CGH_RPT; k++) {
+                const float dq = dqx + bk[k] * (P.gy + bk[k] * py2);
+                const float dr = dq / (P.qx + sqrtf(P.qy + dq));  // r - ra
+                const float t = P.th + dr * inv_lambda;
+                float s, c;
+                sincos_rev(t, &s, &c);
+                cre[k] += P.amp * c;
+                cim[k] += P.amp * s;
             }
+        }
+#pragma unroll
+        for (int k = 0; k < CGH_RPT; k++) {
+            re[k] += cre[k];
+            im[k] += cim[k];
         }
         __syncthreads();
     }
@@ -204,6 +182,133 @@ k_cgh_f32(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
         if (U) U[idx] = v;
         if (phase_out) phase_out[idx] = atan2f(v.y, v.x);
         if (amp_out) amp_out[idx] = sqrtf(v.x * v.x + v.y * v.y);
+    }
+}
+
+// Fresnel, separable.  theta = X(w) + Y(h) with X = c (px w - x_j)^2 + phi_j / 2 pi and
+// Y = c (py h - y_j)^2, c = 1 / (2 lambda z_j), so exp(2 pi i theta) = exp(2 pi i X)
+// exp(2 pi i Y): a thread holding 4 columns x 8 rows takes 4 + 8 sine / cosine pairs
+// per point for its 32 pixels, and each pixel is then the complex product accumulated
+// (4 FMA, no transcendental) instead of a phase, a reduction and a sine / cosine pair.
+// A work-group owns a 128 x 64 tile (256 threads): thread (tx, ty) holds columns
+// w0 + 32 i + tx and rows h0 + ty + 8 k, so every store is a 256-byte line.  Each
+// column group i and each 32-row half m has its own FP64 anchor (its phase mod 1 and
+// slope at the group's centre), so the FP32 offsets stay within 16 pixels, as in the
+// exact-r kernel.
+constexpr int SEP_TX = 128, SEP_TY = 64, SEP_NT = 256, SEP_NI = 4, SEP_NK = 8;
+
+struct SepAnchor {  // per (tile, point), FP32
+    float thx[SEP_NI];  // X at column anchor i, cycles mod 1 (phi folded in)
+    float gx[SEP_NI];   // 2 c dx_i px (cycles per pixel)
+    float thy[2];       // Y at row anchor m, cycles mod 1
+    float gy[2];        // 2 c dy_m py
+    float qx, qy;       // c px^2, c py^2
+    float amp, pad;
+};
+
+__global__ void __launch_bounds__(SEP_NT)
+k_cgh_fresnel_sep(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
+                  float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx,
+                  double py, double px, double lambda, int accumulate) {
+    __shared__ SepAnchor sp[SEP_NT];
+    const int tid = threadIdx.x, tx = tid % 32, ty = tid / 32;
+    const int w0 = blockIdx.x * SEP_TX, h0 = blockIdx.y * SEP_TY;
+    const float a = (float)(tx - 16);  // offset from every column anchor
+    float bk[4];                       // row offsets from the half's anchor
+#pragma unroll
+    for (int k = 0; k < 4; k++) bk[k] = (float)(ty + 8 * k - 16);
+    float re[SEP_NI][SEP_NK], im[SEP_NI][SEP_NK];
+#pragma unroll
+    for (int i = 0; i < SEP_NI; i++)
+#pragma unroll
+        for (int k = 0; k < SEP_NK; k++) re[i][k] = im[i][k] = 0.f;
+
+    for (int base = 0; base < n; base += SEP_NT) {
+        const int j = base + tid;
+        if (j < n) {
+            const olx_point p = pts[j];
+            const double c = 1.0 / (2.0 * lambda * p.z);
+            const double ph = (double)p.phase * (0.5 / M_PI);
+            SepAnchor pa;
+#pragma unroll
+            for (int i = 0; i < SEP_NI; i++) {
+                const double dx = px * (w0 + 32 * i + 16) - p.x;
+                const double th = c * dx * dx + ph;
+                pa.thx[i] = (float)(th - rint(th));
+                pa.gx[i] = (float)(2.0 * c * dx * px);
+            }
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const double dy = py * (h0 + 32 * m + 16) - p.y;
+                const double th = c * dy * dy;
+                pa.thy[m] = (float)(th - rint(th));
+                pa.gy[m] = (float)(2.0 * c * dy * py);
+            }
+            pa.qx = (float)(c * px * px);
+            pa.qy = (float)(c * py * py);
+            pa.amp = p.amp;
+            pa.pad = 0.f;
+            sp[tid] = pa;
+        }
+        __syncthreads();
+        const int cnt = min(SEP_NT, n - base);
+        // the chunk's sum first: its roundings happen at the chunk's magnitude (about
+        // 16 amp) rather than the whole sum's (sqrt(n) amp), once per chunk there
+        float cre[SEP_NI][SEP_NK], cim[SEP_NI][SEP_NK];
+#pragma unroll
+        for (int i = 0; i < SEP_NI; i++)
+#pragma unroll
+            for (int k = 0; k < SEP_NK; k++) cre[i][k] = cim[i][k] = 0.f;
+        for (int q = 0; q < cnt; q++) {
+            const SepAnchor P = sp[q];
+            float cx[SEP_NI], sx[SEP_NI], cy[SEP_NK], sy[SEP_NK];
+#pragma unroll
+            for (int i = 0; i < SEP_NI; i++) {
+                sincos_rev(P.thx[i] + a * (P.gx[i] + a * P.qx), &sx[i], &cx[i]);
+                cx[i] *= P.amp;
+                sx[i] *= P.amp;
+            }
+#pragma unroll
+            for (int k = 0; k < SEP_NK; k++) {
+                const int m = k / 4;
+                const float b = bk[k % 4];
+                sincos_rev(P.thy[m] + b * (P.gy[m] + b * P.qy), &sy[k], &cy[k]);
+            }
+#pragma unroll
+            for (int i = 0; i < SEP_NI; i++)
+#pragma unroll
+                for (int k = 0; k < SEP_NK; k++) {
+                    cre[i][k] = fmaf(cx[i], cy[k], fmaf(-sx[i], sy[k], cre[i][k]));
+                    cim[i][k] = fmaf(sx[i], cy[k], fmaf(cx[i], sy[k], cim[i][k]));
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < SEP_NI; i++)
+#pragma unroll
+            for (int k = 0; k < SEP_NK; k++) {
+                re[i][k] += cre[i][k];
+                im[i][k] += cim[i][k];
+            }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < SEP_NI; i++) {
+        const int w = w0 + 32 * i + tx;
+        if (w >= nx) continue;
+#pragma unroll
+        for (int k = 0; k < SEP_NK; k++) {
+            const int h = h0 + ty + 8 * k;
+            if (h >= ny) continue;
+            const size_t idx = (size_t)h * nx + w;
+            float2 v = make_float2(re[i][k], im[i][k]);
+            if (accumulate && U) {
+                const float2 o = U[idx];
+                v = make_float2(v.x + o.x, v.y + o.y);
+            }
+            if (U) U[idx] = v;
+            if (phase_out) phase_out[idx] = atan2f(v.y, v.x);
+            if (amp_out) amp_out[idx] = sqrtf(v.x * v.x + v.y * v.y);
+        }
     }
 }
 
@@ -269,18 +374,19 @@ extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     OLX_TRY(check_cgh(ctx, d_pts, n, ny, nx, py, px, lambda));
     if (!d_U && !d_phase && !d_amp) OLX_FAIL(OLX_EINVAL, "cgh: no output requested");
     if ((flags & OLX_CGH_ACCUMULATE) && !d_U) OLX_FAIL(OLX_EINVAL, "cgh: accumulate needs d_U");
-    dim3 grid((unsigned)((nx + CGH_TX - 1) / CGH_TX), (unsigned)((ny + CGH_TY - 1) / CGH_TY));
     const int acc = (flags & OLX_CGH_ACCUMULATE) ? 1 : 0;
     {
         TimedLaunch tl(ctx, (flags & OLX_CGH_EXACT_R) ? "cgh_f32_exact" : "cgh_f32_fresnel");
-        if (flags & OLX_CGH_EXACT_R)
-            k_cgh_f32<true><<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U, d_phase,
-                                                              d_amp, (int)ny, (int)nx, py, px,
-                                                              lambda, acc);
-        else
-            k_cgh_f32<false><<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U,
-                                                               d_phase, d_amp, (int)ny, (int)nx,
-                                                               py, px, lambda, acc);
+        if (flags & OLX_CGH_EXACT_R) {
+            dim3 grid((unsigned)((nx + CGH_TX - 1) / CGH_TX), (unsigned)((ny + CGH_TY - 1) / CGH_TY));
+            k_cgh_exact<<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U, d_phase, d_amp,
+                                                          (int)ny, (int)nx, py, px, lambda, acc);
+        } else {
+            dim3 grid((unsigned)((nx + SEP_TX - 1) / SEP_TX), (unsigned)((ny + SEP_TY - 1) / SEP_TY));
+            k_cgh_fresnel_sep<<<grid, SEP_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U,
+                                                                d_phase, d_amp, (int)ny, (int)nx,
+                                                                py, px, lambda, acc);
+        }
     }
     return check_launch("cgh_f32");
 }

@@ -41,7 +41,8 @@ def obj_of(pts, with_ap=True):
 
 
 @pytest.mark.parametrize("exact", [False, True])
-@pytest.mark.parametrize("shape,n", [((12, 16), 7), ((64, 96), 300), ((100, 70), 257), ((256, 256), 1000)])
+@pytest.mark.parametrize("shape,n", [((12, 16), 7), ((64, 96), 300), ((100, 70), 257), ((256, 256), 1000),
+                                     ((200, 300), 600)])
 def test_cgh_matches_fp64_oracle(exact, shape, n):
     ny, nx = shape
     pts = cloud(n, 40 + n, nx, ny)
