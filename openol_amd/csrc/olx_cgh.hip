@@ -31,6 +31,22 @@ namespace olx {
 
 constexpr int CGH_TX = 32, CGH_TY = 32, CGH_NT = 256, CGH_RPT = CGH_TY / (CGH_NT / CGH_TX);
 
+// A hologram of few tiles splits its points into gridDim.z slices of `slice` points
+// (a multiple of 256, so chunking is unchanged); slice z writes its partial plane to
+// part + z ny nx, and k_cgh_sum_slices adds the planes in slice order.
+__device__ __forceinline__ void cgh_slice(const olx_point* __restrict__& pts, int& n,
+                                          float2* __restrict__& U, float* __restrict__& phase_out,
+                                          float* __restrict__& amp_out, int& accumulate,
+                                          float2* part, int slice, int ny, int nx) {
+    if (!part) return;
+    const int start = (int)blockIdx.z * slice;
+    pts += start;
+    n = max(0, min(slice, n - start));
+    U = part + (size_t)blockIdx.z * ny * nx;
+    phase_out = amp_out = nullptr;
+    accumulate = 0;
+}
+
 // sin / cos of t revolutions: the hardware units take revolutions; t is first
 // reduced to [0, 1) with one v_fract (exact for a float)
 __device__ __forceinline__ void sincos_rev(float t, float* s, float* c) {
@@ -53,7 +69,8 @@ constexpr double CGH_SERIES_E4K = 1.0e-6;
 __global__ void __launch_bounds__(CGH_NT)
 k_cgh_exact(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
             float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx, double py,
-            double px, double lambda, int accumulate) {
+            double px, double lambda, int accumulate, float2* __restrict__ part, int slice) {
+    cgh_slice(pts, n, U, phase_out, amp_out, accumulate, part, slice, ny, nx);
     __shared__ PtAnchorR sp[CGH_NT];
     __shared__ int wcount[2][CGH_NT / 64];
     const int tid = threadIdx.x, tx = tid % CGH_TX, ty = tid / CGH_TX;
@@ -209,7 +226,9 @@ struct SepAnchor {  // per (tile, point), FP32
 __global__ void __launch_bounds__(SEP_NT)
 k_cgh_fresnel_sep(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
                   float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx,
-                  double py, double px, double lambda, int accumulate) {
+                  double py, double px, double lambda, int accumulate, float2* __restrict__ part,
+                  int slice) {
+    cgh_slice(pts, n, U, phase_out, amp_out, accumulate, part, slice, ny, nx);
     __shared__ SepAnchor sp[SEP_NT];
     const int tid = threadIdx.x, tx = tid % 32, ty = tid / 32;
     const int w0 = blockIdx.x * SEP_TX, h0 = blockIdx.y * SEP_TY;
@@ -343,6 +362,26 @@ k_cgh_f64(const olx_point* __restrict__ pts, int n, double2* __restrict__ U, int
     if (w < nx && h < ny) U[(size_t)h * nx + w] = make_double2(re, im);
 }
 
+__global__ void k_cgh_sum_slices(const float2* __restrict__ part, int S, int64_t np,
+                                 float2* __restrict__ U, float* __restrict__ ph,
+                                 float* __restrict__ am, int accumulate) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float2 v = part[i];
+        for (int s = 1; s < S; s++) {
+            const float2 w = part[(size_t)s * np + i];
+            v = make_float2(v.x + w.x, v.y + w.y);
+        }
+        if (accumulate && U) {
+            const float2 o = U[i];
+            v = make_float2(v.x + o.x, v.y + o.y);
+        }
+        if (U) U[i] = v;
+        if (ph) ph[i] = atan2f(v.y, v.x);
+        if (am) am[i] = sqrtf(v.x * v.x + v.y * v.y);
+    }
+}
+
 __global__ void k_encode(const float2* __restrict__ U, float* __restrict__ ph,
                          float* __restrict__ am, int64_t count) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
@@ -375,17 +414,38 @@ extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     if (!d_U && !d_phase && !d_amp) OLX_FAIL(OLX_EINVAL, "cgh: no output requested");
     if ((flags & OLX_CGH_ACCUMULATE) && !d_U) OLX_FAIL(OLX_EINVAL, "cgh: accumulate needs d_U");
     const int acc = (flags & OLX_CGH_ACCUMULATE) ? 1 : 0;
+    const bool exact = (flags & OLX_CGH_EXACT_R) != 0;
+    const int tx = exact ? CGH_TX : SEP_TX, ty = exact ? CGH_TY : SEP_TY;
+    dim3 grid((unsigned)((nx + tx - 1) / tx), (unsigned)((ny + ty - 1) / ty));
+    // fewer tiles than CUs: split the points into slices of at least 2048 (8 chunks),
+    // up to about 1024 work-groups, and sum the partial planes in slice order
+    const int64_t tiles = (int64_t)grid.x * grid.y, np = ny * nx;
+    int64_t S = 1, slice = n;
+    if (tiles < 256 && n >= 4096) {
+        S = std::min<int64_t>((1024 + tiles - 1) / tiles, n / 2048);
+        slice = ((n + S - 1) / S + 255) / 256 * 256;
+        S = (n + slice - 1) / slice;
+    }
+    float2* part = nullptr;
+    if (S > 1) {
+        OLX_TRY(ctx->cgh_part.reserve(sizeof(float2) * (size_t)(S * np)));
+        part = (float2*)ctx->cgh_part.p;
+        grid.z = (unsigned)S;
+    }
     {
-        TimedLaunch tl(ctx, (flags & OLX_CGH_EXACT_R) ? "cgh_f32_exact" : "cgh_f32_fresnel");
-        if (flags & OLX_CGH_EXACT_R) {
-            dim3 grid((unsigned)((nx + CGH_TX - 1) / CGH_TX), (unsigned)((ny + CGH_TY - 1) / CGH_TY));
+        TimedLaunch tl(ctx, exact ? "cgh_f32_exact" : "cgh_f32_fresnel");
+        if (exact)
             k_cgh_exact<<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U, d_phase, d_amp,
-                                                          (int)ny, (int)nx, py, px, lambda, acc);
-        } else {
-            dim3 grid((unsigned)((nx + SEP_TX - 1) / SEP_TX), (unsigned)((ny + SEP_TY - 1) / SEP_TY));
-            k_cgh_fresnel_sep<<<grid, SEP_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U,
-                                                                d_phase, d_amp, (int)ny, (int)nx,
-                                                                py, px, lambda, acc);
+                                                          (int)ny, (int)nx, py, px, lambda, acc, part,
+                                                          (int)slice);
+        else
+            k_cgh_fresnel_sep<<<grid, SEP_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U, d_phase,
+                                                                d_amp, (int)ny, (int)nx, py, px,
+                                                                lambda, acc, part, (int)slice);
+        if (part) {
+            const int64_t blocks = std::min<int64_t>((np + 255) / 256, 65536);
+            k_cgh_sum_slices<<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+                part, (int)S, np, (float2*)d_U, d_phase, d_amp, acc);
         }
     }
     return check_launch("cgh_f32");

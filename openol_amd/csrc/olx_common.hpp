@@ -69,6 +69,7 @@ struct olx_ctx {
     olx::DevBuf obj_keys, obj_tmp;
     olx::DevBuf split_acc;  // split propagation: the summed destination field
     olx::DevBuf mp_work;    // multi-plane loss: exp(i phi), one plane's field, partial sums
+    olx::DevBuf cgh_part;   // direct CGH split over point slices: one partial plane per slice
     // timing
     bool timing = false;
     std::map<std::string, olx::KernelStat> stats;

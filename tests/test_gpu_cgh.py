@@ -110,6 +110,54 @@ def test_cgh_outputs_phase_amp_accumulate():
     assert rel(V.cpu().numpy(), u) < 1e-5
 
 
+@pytest.mark.parametrize("exact", [False, True])
+def test_cgh_small_hologram_split_over_points(exact):
+    """A hologram of fewer tiles than CUs splits its points over work-groups, whose
+    partial planes a second kernel sums in slice order: 64 x 96 from 20,000 points
+    (9 slices) against the FP64 oracle, with the phase / amplitude outputs and
+    accumulation taken on the split path."""
+    ny, nx, n = 64, 96, 20000
+    pts = cloud(n, 77, nx, ny)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ph = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    am = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    ol().Cgh(obj_of(pts), U, ny, nx, P, P, LAM, exact_r=exact, phase_out=ph, amp_out=am)
+    ref = olref.cgh(pts["x"], pts["y"], pts["z"], ny, nx, P, P, LAM, pts["amp"], pts["phase"],
+                    flags=olref.CGH_EXACT_R if exact else 0, dtype=np.complex128)
+    u = U.cpu().numpy()
+    assert rel(u, ref) < 1e-5
+    assert np.allclose(ph.cpu().numpy(), np.angle(u), atol=1e-5)
+    assert np.allclose(am.cpu().numpy(), np.abs(u), rtol=1e-6)
+    # phase only (no complex output) takes the same path
+    ph2 = torch.empty_like(ph)
+    ol().Cgh(obj_of(pts), ph2, ny, nx, P, P, LAM, exact_r=exact)
+    assert torch.equal(ph2, ph)
+    # two halves of 10,000 points (each split) accumulate to the whole
+    k = n // 2
+    half = lambda sl: ol().Object(pts["x"][sl], pts["y"][sl], pts["z"][sl], pts["amp"][sl], pts["phase"][sl])
+    V = torch.empty_like(U)
+    ol().Cgh(half(slice(0, k)), V, ny, nx, P, P, LAM, exact_r=exact)
+    ol().Cgh(half(slice(k, n)), V, ny, nx, P, P, LAM, exact_r=exact, accumulate=True)
+    assert rel(V.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("exact,tol", [(False, 6e-6), (True, 8e-6)])
+def test_cgh_large_cloud_sums_stay_accurate(exact, tol):
+    """200,000 points over 2048^2 (no split), a 64 x 128 window against the FP64 kernel
+    on the shifted cloud.  Each 256-point chunk is summed before it is added to the
+    pixel's total; accumulating every term into the running sum (|U| ~ 450, two
+    roundings per term in the separable kernel) would leave about 1e-5."""
+    N, n, h0, w0, wy, wx = 2048, 200000, 1000, 700, 64, 128
+    pts = cloud(n, 91, N, N)
+    U = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    ol().Cgh(obj_of(pts), U, N, N, P, P, LAM, exact_r=exact)
+    sub = ol().Object(pts["x"] - w0 * P, pts["y"] - h0 * P, pts["z"], pts["amp"], pts["phase"])
+    D = torch.empty((wy, wx), dtype=torch.complex128, device="cuda")
+    ol().CghFP64(sub, D, wy, wx, P, P, LAM, exact_r=exact)
+    got = U[h0:h0 + wy, w0:w0 + wx].to(torch.complex128)
+    assert (torch.linalg.norm(got - D) / torch.linalg.norm(D)).item() < tol
+
+
 def test_cgh_gauss_sum_closed_form():
     z = 2 * P * P / LAM
     o = ol().Object([5 * P], [3 * P], [z])
