@@ -16,6 +16,7 @@
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "fft_core.hpp"
 #include "olx_common.hpp"
@@ -304,8 +305,18 @@ constexpr int PAIR_MIN = 2048;  // shortest padded length on the tiled path
 // largest KR: 1024 threads and KR padded transforms of LDS (136 KiB)
 template <int L>
 constexpr int kr_max() { return 16384 / L; }
-template <int L>
-constexpr int kr_lo() { return kr_max<L>() > 1 ? kr_max<L>() / 2 : 1; }
+// the work-group shapes instantiated: KR = kr_max / 2^d, d < 4, at least 1
+template <int L, int D>
+constexpr int kr_div() { return kr_max<L>() >> D >= 1 ? kr_max<L>() >> D : 1; }
+
+// calls f(std::integral_constant<int, KR>) with the largest instantiated KR <= kr
+template <int L, class F>
+static void with_kr(int kr, F&& f) {
+    if (kr >= kr_div<L, 0>()) f(std::integral_constant<int, kr_div<L, 0>()>());
+    else if (kr >= kr_div<L, 1>()) f(std::integral_constant<int, kr_div<L, 1>()>());
+    else if (kr >= kr_div<L, 2>()) f(std::integral_constant<int, kr_div<L, 2>()>());
+    else f(std::integral_constant<int, kr_div<L, 3>()>());
+}
 // waves per SIMD the register budget must allow: two work-groups per CU when a
 // work-group is at most 512 threads and its LDS fits twice
 template <int L, int KR>
@@ -569,8 +580,8 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
 }
 
 // work-group shapes and radix orders of the tiled path; OLX_TILE="ka,kb,kc,cbl"
-// and OLX_ORDER="oa,of,oi,oc" override them for tuning (each KR is kr_max or
-// kr_max / 2, each order 0 = remainder radix first, 1 = last)
+// and OLX_ORDER="oa,of,oi,oc" override them for tuning (each KR is taken down to
+// kr_max / 2^d, d < 4; each order 0 = remainder radix first, 1 = last)
 struct TileCfg {
     int kra, krb, krc, cbl;
     int oa, of, oi, oc;
@@ -871,15 +882,26 @@ static int launch_cols(int cb, const PropArgs& a, float2* W, const float2* tw, h
 
 static int kr_max_of(int L) { return 16384 / L; }
 
-// Defaults (measured at 4096^2, padded 8192): 512-thread work-groups, so two fit on
-// a CU and one's memory phases overlap the other's arithmetic; tiles 8 columns
-// wide, so a row pass's two rows fill whole 128-byte lines; the column pass with
-// the remainder radix last in both transforms (no twiddles in its first radix-16
-// stage), the row passes with it first.
+// KR for a pass of `lines` transforms: half the largest (measured at 4096^2, padded
+// 8192: 512-thread work-groups, so two fit on a CU and one's memory phases overlap
+// the other's arithmetic), halved while the pass would have fewer than `groups`
+// work-groups.  Measured on smaller fields: the row passes, which stream, run
+// fastest with at least 1024 groups; the column pass, which computes, with 512 and
+// larger groups (1024^2: 13.7 + 25.9 + 10.5 us against 21.1 + 27.4 + 15.6 us with
+// the 4096^2 shapes; 2048^2: 22.7 + 55.7 + 24.3 us against 23.5 + 55.9 + 26.4 us).
+static int kr_default(int L, int64_t lines, int64_t groups) {
+    int kr = std::max(1, kr_max_of(L) / 2);
+    while (kr > 1 && kr_max_of(L) / kr < 8 && lines / (2 * kr) < groups) kr /= 2;
+    return kr;
+}
+
+// Defaults: KR per pass as above; tiles 8 columns wide, so a row pass's two rows
+// fill whole 128-byte lines; the column pass with the remainder radix last in both
+// transforms (no twiddles in its first radix-16 stage), the row passes with it first.
 static TileCfg tile_cfg(int Mx, int My) {
     TileCfg c;
-    c.kra = c.krc = std::max(1, kr_max_of(Mx) / 2);
-    c.krb = std::max(1, kr_max_of(My) / 2);
+    c.kra = c.krc = kr_default(Mx, My / 2, 1024);
+    c.krb = kr_default(My, Mx, 512);
     c.cbl = std::max(8, 2 * c.krb);
     c.oa = c.oc = REM_FIRST;
     c.of = c.oi = REM_LAST;
@@ -897,7 +919,11 @@ static TileCfg tile_cfg(int Mx, int My) {
     if (env) {
         int v[4];
         if (sscanf(env, "%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3]) == 4) {
-            auto pick = [](int want, int mx) { return want >= mx ? mx : std::max(1, mx / 2); };
+            auto pick = [](int want, int mx) {
+                int k = mx;
+                while (k > 1 && k > want && mx / k < 8) k /= 2;
+                return k;
+            };
             c.kra = pick(v[0], kr_max_of(Mx));
             c.krb = pick(v[1], kr_max_of(My));
             c.krc = pick(v[2], kr_max_of(Mx));
@@ -933,8 +959,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
             TimedLaunch tl(ctx, "prop_rows_pad_fwd");
             OLX_LEN_SWITCH(ilog2(Mx), {
                 if constexpr (L >= PAIR_MIN) {
-                    if (cfg.kra == kr_max<L>()) launch_rows_fwd<L, kr_max<L>()>(cfg, a, src, W, ctx->twiddle, s);
-                    else launch_rows_fwd<L, kr_lo<L>()>(cfg, a, src, W, ctx->twiddle, s);
+                    with_kr<L>(cfg.kra, [&](auto K) { launch_rows_fwd<L, decltype(K)::value>(cfg, a, src, W, ctx->twiddle, s); });
                 }
             });
         }
@@ -943,8 +968,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
             TimedLaunch tl(ctx, "prop_cols_conv");
             OLX_LEN_SWITCH(ilog2(My), {
                 if constexpr (L >= PAIR_MIN) {
-                    if (cfg.krb == kr_max<L>()) launch_cols_conv<L, kr_max<L>()>(cfg, a, W, ctx->twiddle, s);
-                    else launch_cols_conv<L, kr_lo<L>()>(cfg, a, W, ctx->twiddle, s);
+                    with_kr<L>(cfg.krb, [&](auto K) { launch_cols_conv<L, decltype(K)::value>(cfg, a, W, ctx->twiddle, s); });
                 }
             });
         }
@@ -953,8 +977,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
             TimedLaunch tl(ctx, "prop_rows_crop_inv");
             OLX_LEN_SWITCH(ilog2(Mx), {
                 if constexpr (L >= PAIR_MIN) {
-                    if (cfg.krc == kr_max<L>()) launch_rows_inv<L, kr_max<L>()>(cfg, a, W, dst, ctx->twiddle, s);
-                    else launch_rows_inv<L, kr_lo<L>()>(cfg, a, W, dst, ctx->twiddle, s);
+                    with_kr<L>(cfg.krc, [&](auto K) { launch_rows_inv<L, decltype(K)::value>(cfg, a, W, dst, ctx->twiddle, s); });
                 }
             });
         }
