@@ -103,4 +103,8 @@ const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa,
 
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
+// olx_asm_c64 from src into dst (which may alias src) (olx_prop.hip)
+int asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
+             double dx, double lambda, double z, uint32_t flags);
+
 }  // namespace olx

@@ -6,12 +6,12 @@
 //   dL/df_k = 2 f_k dL/dI_k  (gradient w.r.t. Re f plus i times w.r.t. Im f)
 //   dL/dphi = sum_p Im(conj(u) A_p^H dL/df_p)                    (A^H: the adjoint ASM)
 // Each plane is propagated, reduced, turned into its gradient and propagated back
-// before the next one, so no plane field is kept.  Sums are FP64; every block
-// re-reduces the block partials in the same order, so results are deterministic.
+// before the next one, so no plane field is kept.  One pass takes the FP64 sums
+// S_I, S_T, S_II, S_IT, S_TT, from which
+//   sum r^2 = S_II / mI^2 - 2 S_IT / (mI mT) + S_TT / mT^2,  sum r I = S_II / mI - S_IT / mT;
+// every block re-reduces the block partials in the same order, so results are
+// deterministic.
 #include "olx_common.hpp"
-
-extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
-                           double lambda, double z, uint32_t flags);
 
 namespace olx {
 
@@ -27,78 +27,62 @@ __global__ void __launch_bounds__(MP_T) k_mp_expi(const float* __restrict__ phi,
     }
 }
 
-// sum over the block of two FP64 values per thread, in a fixed order
-__device__ __forceinline__ double2 block_sum2(double a, double b) {
-    __shared__ double sa[MP_T], sb[MP_T];
-    sa[threadIdx.x] = a;
-    sb[threadIdx.x] = b;
+constexpr int MP_NS = 5;  // S_I, S_T, S_II, S_IT, S_TT
+
+// sum over the block of MP_NS FP64 values per thread, in a fixed order
+__device__ __forceinline__ void block_sums(double* v) {
+    __shared__ double sh[MP_NS][MP_T];
+#pragma unroll
+    for (int k = 0; k < MP_NS; k++) sh[k][threadIdx.x] = v[k];
     __syncthreads();
     for (int off = MP_T / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) {
-            sa[threadIdx.x] += sa[threadIdx.x + off];
-            sb[threadIdx.x] += sb[threadIdx.x + off];
+#pragma unroll
+            for (int k = 0; k < MP_NS; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
         }
         __syncthreads();
     }
-    const double2 r = make_double2(sa[0], sb[0]);
+#pragma unroll
+    for (int k = 0; k < MP_NS; k++) v[k] = sh[k][0];
     __syncthreads();
-    return r;
 }
 
-// totals of the MP_B block partials (pairs), identical in every block
-__device__ __forceinline__ double2 total2(const double2* __restrict__ part) {
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < MP_B; i += MP_T) {
-        a += part[i].x;
-        b += part[i].y;
-    }
-    return block_sum2(a, b);
-}
-
-// pass 1: sum I, sum T
+// pass 1: the block's S_I, S_T, S_II, S_IT, S_TT (part[k * MP_B + block])
 __global__ void __launch_bounds__(MP_T) k_mp_sums(const float2* __restrict__ f,
                                                   const float* __restrict__ t, int64_t n,
-                                                  double2* __restrict__ part) {
-    double si = 0.0, st = 0.0;
+                                                  double* __restrict__ part) {
+    double v[MP_NS] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)MP_B * MP_T) {
-        const float2 v = f[i];
-        si += (double)v.x * v.x + (double)v.y * v.y;
-        st += (double)t[i];
+        const float2 x = f[i];
+        const double I = (double)x.x * x.x + (double)x.y * x.y, T = (double)t[i];
+        v[0] += I;
+        v[1] += T;
+        v[2] += I * I;
+        v[3] += I * T;
+        v[4] += T * T;
     }
-    const double2 s = block_sum2(si, st);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    block_sums(v);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < MP_NS; k++) part[k * MP_B + blockIdx.x] = v[k];
 }
 
-// pass 2: sum r^2, sum r I
-__global__ void __launch_bounds__(MP_T) k_mp_rstats(const float2* __restrict__ f,
-                                                    const float* __restrict__ t, int64_t n,
-                                                    const double2* __restrict__ part,
-                                                    double2* __restrict__ part2) {
-    const double2 tot = total2(part);
-    const double inv_mi = (double)n / tot.x, inv_mt = (double)n / tot.y;
-    double s2 = 0.0, sri = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)MP_B * MP_T) {
-        const float2 v = f[i];
-        const double I = (double)v.x * v.x + (double)v.y * v.y;
-        const double r = I * inv_mi - (double)t[i] * inv_mt;
-        s2 += r * r;
-        sri += r * I;
-    }
-    const double2 s = block_sum2(s2, sri);
-    if (threadIdx.x == 0) part2[blockIdx.x] = s;
-}
-
-// pass 3: dL/df in place; block 0 adds this plane's loss
+// pass 2: dL/df in place; block 0 adds this plane's loss
 __global__ void __launch_bounds__(MP_T) k_mp_grad(float2* __restrict__ f, const float* __restrict__ t,
-                                                  int64_t n, const double2* __restrict__ part,
-                                                  const double2* __restrict__ part2,
+                                                  int64_t n, const double* __restrict__ part,
                                                   double* __restrict__ loss) {
-    const double2 tot = total2(part);
-    const double2 rs = total2(part2);
+    double S[MP_NS];
+#pragma unroll
+    for (int k = 0; k < MP_NS; k++) {
+        S[k] = 0.0;
+        for (int i = threadIdx.x; i < MP_B; i += MP_T) S[k] += part[k * MP_B + i];
+    }
+    block_sums(S);
     const double N = (double)n;
-    const double inv_mi = N / tot.x, inv_mt = N / tot.y;
-    const double c1 = 4.0 / N * inv_mi;                       // 2 * (2/N) / mI
-    const double c2 = 4.0 / N * rs.y / N * inv_mi * inv_mi;   // 2 * (2/N) * sum rI / (N mI^2)
+    const double inv_mi = N / S[0], inv_mt = N / S[1];
+    const double sr2 = S[2] * inv_mi * inv_mi - 2.0 * S[3] * inv_mi * inv_mt + S[4] * inv_mt * inv_mt;
+    const double sri = S[2] * inv_mi - S[3] * inv_mt;
+    const double c1 = 4.0 / N * inv_mi;                    // 2 * (2/N) / mI
+    const double c2 = 4.0 / N * sri / N * inv_mi * inv_mi;  // 2 * (2/N) * sum rI / (N mI^2)
     for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)MP_B * MP_T) {
         const float2 v = f[i];
         const double I = (double)v.x * v.x + (double)v.y * v.y;
@@ -106,7 +90,7 @@ __global__ void __launch_bounds__(MP_T) k_mp_grad(float2* __restrict__ f, const 
         const double g = c1 * r - c2;
         f[i] = make_float2((float)(g * v.x), (float)(g * v.y));
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *loss += rs.x / N;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *loss += sr2 / N;
 }
 
 __global__ void k_mp_zero(double* loss) { *loss = 0.0; }
@@ -138,29 +122,25 @@ extern "C" int olx_multiplane_mse_c64(olx_ctx* ctx, const float* d_phi, int64_t 
         if (!d_targets[p]) OLX_FAIL(OLX_EINVAL, "multiplane: null target %d", p);
     const int64_t n = ny * nx;
     const size_t fb = sizeof(float2) * (size_t)n;
-    OLX_TRY(ctx->mp_work.reserve(2 * fb + 2 * sizeof(double2) * MP_B));
+    OLX_TRY(ctx->mp_work.reserve(2 * fb + sizeof(double) * MP_NS * MP_B));
     float2* u = (float2*)ctx->mp_work.p;
     float2* f = (float2*)((char*)ctx->mp_work.p + fb);
-    double2* part = (double2*)((char*)ctx->mp_work.p + 2 * fb);
-    double2* part2 = part + MP_B;
+    double* part = (double*)((char*)ctx->mp_work.p + 2 * fb);
     hipStream_t s = ctx->stream;
     const unsigned eb = (unsigned)std::min<int64_t>((n + MP_T - 1) / MP_T, 65536);
     k_mp_expi<<<eb, MP_T, 0, s>>>(d_phi, u, n);
     k_mp_zero<<<1, 1, 0, s>>>(d_loss);
     OLX_TRY(check_launch("multiplane setup"));
     for (int p = 0; p < nplanes; p++) {
-        OLX_HIP(hipMemcpyAsync(f, u, fb, hipMemcpyDeviceToDevice, s));
-        OLX_TRY(olx_asm_c64(ctx, f, ny, nx, dy, dx, lambda, zs[p], flags & OLX_BANDLIMIT));
+        OLX_TRY(asm_prop(ctx, u, f, ny, nx, dy, dx, lambda, zs[p], flags & OLX_BANDLIMIT));
         {
             TimedLaunch tl(ctx, "multiplane_loss");
             k_mp_sums<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part);
-            k_mp_rstats<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part, part2);
-            k_mp_grad<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part, part2, d_loss);
+            k_mp_grad<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part, d_loss);
         }
         OLX_TRY(check_launch("multiplane loss"));
         if (!d_grad) continue;
-        OLX_TRY(olx_asm_c64(ctx, f, ny, nx, dy, dx, lambda, zs[p],
-                            (flags & OLX_BANDLIMIT) | OLX_ADJOINT));
+        OLX_TRY(asm_prop(ctx, f, f, ny, nx, dy, dx, lambda, zs[p], (flags & OLX_BANDLIMIT) | OLX_ADJOINT));
         k_mp_phigrad<<<eb, MP_T, 0, s>>>(u, f, n, d_grad, p > 0 ? 1 : 0);
         OLX_TRY(check_launch("multiplane gradient"));
     }

@@ -1037,9 +1037,10 @@ static PropArgs base_args(int64_t ny, int64_t nx) {
 
 using namespace olx;
 
-extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
-                           double dx, double lambda, double z, uint32_t flags) {
-    OLX_TRY(check_field(ctx, d_u, ny, nx));
+int olx::asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
+                  double dx, double lambda, double z, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_src, ny, nx));
+    if (!d_dst) OLX_FAIL(OLX_EINVAL, "asm: null destination");
     if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !std::isfinite(z))
         OLX_FAIL(OLX_EINVAL, "asm: lambda, dx, dy must be > 0 and z finite");
     PropArgs a = base_args(ny, nx);
@@ -1061,7 +1062,12 @@ extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, doub
         a.uw = ulp + ulm;
         a.vw = vlp + vlm;
     }
-    return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a);
+    return run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a);
+}
+
+extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
+                           double dx, double lambda, double z, uint32_t flags) {
+    return asm_prop(ctx, d_u, d_u, ny, nx, dy, dx, lambda, z, flags);
 }
 
 extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
