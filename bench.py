@@ -120,8 +120,8 @@ def roofline_of(rep, n):
 
 def gpu_outputs(ol, torch, n):
     """GPU results the CPU leg checks against the oracle: one AsmProp of the metric's
-    seeded field, and the first 16 rows of the direct CGH of config 2 (Fresnel and
-    exact r)."""
+    seeded field, and the first 16 rows of the direct CGH of 10,000 points over 2048^2
+    (Fresnel and exact r)."""
     from openol_amd import synth
     u = torch.from_numpy(synth.random_field(n, n, seed=0)).cuda()
     ol.AsmProp(u, n, n, PITCH, PITCH, LAM, 0.10, bandlimit=True)
@@ -183,12 +183,12 @@ def cpu_baseline(n, gpu=None):
         # the device stores amplitude and phase as float32
         amp = pts["amp"].astype(np.float32).astype(np.float64)
         ph = pts["phase"].astype(np.float32).astype(np.float64)
-        errors["cgh_config2_fresnel_rows_vs_c64_restatement"] = float(
+        errors["cgh_10k_2048_fresnel_rows_vs_c64_restatement"] = float(
             f"{olref.rel_l2(gpu['cgh_fresnel'], band):.3e}")
         for name, fl in (("fresnel", 0), ("exact", olref.CGH_EXACT_R)):
             b64 = olref.cgh(pts["x"], pts["y"], pts["z"], rows, N, PITCH, PITCH, LAM, amp, ph,
                             flags=fl, dtype=np.complex128)
-            errors[f"cgh_config2_{name}_rows_vs_fp64"] = float(f"{olref.rel_l2(gpu['cgh_' + name], b64):.3e}")
+            errors[f"cgh_10k_2048_{name}_rows_vs_fp64"] = float(f"{olref.rel_l2(gpu['cgh_' + name], b64):.3e}")
     # one thread: AsmProp at 1024^2 (OpenMP pinned to one thread through libgomp)
     one = None
     try:
@@ -221,7 +221,7 @@ def cpu_baseline(n, gpu=None):
                        "complex2img_amp_4096_ms": round(t_img * 1e3, 1),
                        "numpy_rows": "objsort and complex2img: numpy, 1 core"},
             "rel_l2_vs_oracle": errors,
-            "rel_l2_sample": "AsmProp at the metric size; direct CGH of config 2, rows 0-15"}
+            "rel_l2_sample": "AsmProp at the metric size; direct CGH of 10,000 points over 2048^2, rows 0-15"}
 
 
 def ceilings(lib, torch):
@@ -273,7 +273,7 @@ CGH_VALU_PER_INTERACTION = {"fresnel": 6.254, "exact": 10.77}
 
 
 def direct_cgh(ol, torch):
-    """Direct CGH of 10,000 points over 2048^2 (config 2), exact r and Fresnel, the
+    """Direct CGH of 10,000 points over 2048^2, exact r and Fresnel, the
     FP64 kernel, and each FP32 kernel's VALU issue roofline (both are bound by VALU
     issue: sine and cosine are a minority of their instructions)."""
     from openol_amd import synth
@@ -314,7 +314,7 @@ def extras(ol, lib, torch):
     out["ceilings"] = ceilings(lib, torch)
     out.update(asm_sizes(ol, lib, torch))
     out.update(direct_cgh(ol, torch))
-    # WRP (200,000 points) + ARSS to the hologram (config 3)
+    # WRP (200,000 points into 4096^2) + ARSS to the hologram
     N, n = 4096, 200000
     pts = synth.random_points(n, seed=3, xmax=N * PITCH, ymax=N * PITCH, zmin=0.151, zmax=0.155)
     obj = ol.Object(pts["x"], pts["y"], pts["z"])

@@ -176,8 +176,8 @@ def test_cgh_empty_cloud():
     assert torch.count_nonzero(U).item() == 0
 
 
-def test_cgh_config2_sampled_fp64():
-    """Config: 10,000 points, 2048^2 hologram, exact r and Fresnel; FP64 oracle on
+def test_cgh_10k_points_2048_sampled_fp64():
+    """10,000 points, 2048^2 hologram, exact r and Fresnel; FP64 oracle on
     4096 sampled pixels (each pixel is an independent sum)."""
     n, N = 10000, 2048
     pts = cloud(n, 2, N, N, 0.10, 0.20)

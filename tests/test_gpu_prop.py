@@ -63,8 +63,8 @@ def test_asm_bandlimit_active_matches_golden():
         assert rel(u.cpu().numpy(), g[f"asm_{tag}"]) < TOL
 
 
-def test_asm_config1_full_size():
-    """Config: 1024^2 field, 8 um pitch, z = 0.10 m, padded to 2048^2, band-limited TF."""
+def test_asm_1024_full_size():
+    """1024^2 field, 8 um pitch, z = 0.10 m, padded to 2048^2, band-limited TF."""
     n = 1024
     u0 = synth.random_field(n, n, seed=1)
     u = dev(u0)
@@ -128,7 +128,7 @@ def test_asm_matches_oracle_large(n):
     ((1024, 2048), 8e-6, 6e-6, 0.08), ((2048, 1024), 6e-6, 8e-6, 0.11), ((8192, 1024), 8e-6, 8e-6, 0.1)])
 def test_asm_tiled_rectangular(shape, dy, dx, z):
     """Padded sides >= 2048 run the tiled passes, whose work-group shapes follow each
-    axis separately (4, 8 or 16 rows / columns per group)."""
+    axis separately (2 to 16 rows / columns per group)."""
     u0 = synth.random_field(*shape, seed=35)
     for flags, adj in ((olref.BANDLIMIT, False), (olref.BANDLIMIT | olref.ADJOINT, True)):
         u = dev(u0)
@@ -139,11 +139,11 @@ def test_asm_tiled_rectangular(shape, dy, dx, z):
 @pytest.mark.parametrize("shape", [(256, 256), (1024, 1024), (1024, 2048), (2048, 1024),
                                    (2048, 4096), (4096, 4096), (8192, 8192)])
 def test_asm_bitwise_equals_shifted_at_zero_offset(shape):
-    """On the tiled path ASM pairs each spectral column u with its mirror -u and
-    evaluates one transfer value for both (columns 0 and Mx/2 pair with themselves);
-    the shifted form evaluates every column on its own.  At zero offset its phase,
-    window and scale are the same FP64 arithmetic, so the fields agree bit for bit,
-    window on and off, forward and adjoint."""
+    """On the tiled path ASM evaluates the transfer function once per mirror pair of
+    spectral rows (v and -v, shared through LDS; row L/2 on its own); the shifted
+    form evaluates every row.  At zero offset its phase, window and scale are the
+    same FP64 arithmetic, so the fields agree bit for bit, window on and off, forward
+    and adjoint."""
     ny, nx = shape
     g = torch.Generator(device="cuda").manual_seed(ny + 3 * nx)
     u0 = torch.randn(ny, nx, dtype=torch.complex64, device="cuda", generator=g)
@@ -237,8 +237,8 @@ def test_arss_matches_oracle(shape, dy, dx, z, s, oy, ox):
     assert rel(src.cpu().numpy(), ref) < TOL
 
 
-def test_arss_config3_size_properties():
-    """4096^2 WRP -> ARSS s = 0.5, z = 0.15 m (config 3 shape): linearity and a
+def test_arss_wrp_4096_properties():
+    """4096^2 WRP -> ARSS s = 0.5, z = 0.15 m (the WRP workload's shape): linearity and a
     2048^2 oracle comparison at the same optical parameters."""
     n = 2048
     u0 = synth.random_field(n, n, seed=31)
