@@ -58,6 +58,9 @@ def pass_bytes(n):
             "prop_rows_crop_inv": 24 * n * n}
 
 
+TIMING_PERIOD = 8
+
+
 def bench_asm(ol, lib, torch, n, steps, warmup):
     from openol_amd import synth
     u = torch.from_numpy(synth.random_field(n, n, seed=0)).cuda()
@@ -67,7 +70,9 @@ def bench_asm(ol, lib, torch, n, steps, warmup):
         ol.AsmProp(u, n, n, PITCH, PITCH, LAM, 0.10, bandlimit=True)
     torch.cuda.synchronize()
     L.olx_ctx_reset_timing(c)
-    L.olx_ctx_enable_timing(c, 1)
+    # HIP events bracket every 8th launch (each of the three passes is sampled in
+    # turn): timing every launch adds about 20 us per propagation to the steps
+    L.olx_ctx_enable_timing(c, TIMING_PERIOD)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):

@@ -72,6 +72,8 @@ struct olx_ctx {
     olx::DevBuf cgh_part;   // direct CGH split over point slices: one partial plane per slice
     // timing
     bool timing = false;
+    int timing_period = 1;   // time every timing_period-th launch
+    uint64_t timing_seq = 0;
     std::map<std::string, olx::KernelStat> stats;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;

@@ -64,6 +64,7 @@ static hipEvent_t get_event(olx_ctx* ctx) {
 
 TimedLaunch::TimedLaunch(olx_ctx* c, const char* n) : ctx(c), name(n) {
     if (!ctx->timing) return;
+    if (ctx->timing_seq++ % (uint64_t)ctx->timing_period != 0) return;  // not sampled: a, b stay null
     a = get_event(ctx);
     b = get_event(ctx);
     if (a) (void)hipEventRecord(a, ctx->stream);
@@ -203,8 +204,11 @@ extern "C" int olx_dev_memset(olx_ctx* ctx, void* d_ptr, int value, size_t bytes
 
 extern "C" int olx_ctx_enable_timing(olx_ctx* ctx, int on) {
     if (!ctx) OLX_FAIL(OLX_EINVAL, "null context");
+    if (on < 0) OLX_FAIL(OLX_EINVAL, "timing: negative sampling period");
     if (!on) flush_timing(ctx);
     ctx->timing = on != 0;
+    ctx->timing_period = on > 1 ? on : 1;
+    ctx->timing_seq = 0;
     return OLX_OK;
 }
 

@@ -287,3 +287,23 @@ def test_invalid_arguments_raise():
         ol().AsmProp(u.to(torch.complex128), 16, 16, 8e-6, 8e-6, LAM, 0.1)
     with pytest.raises(OlxInvalid):
         ol().Prop(u, 16, 16, 8e-6, 8e-6, LAM, 0.1, propmode=7)
+
+
+def test_launch_timing_sampling():
+    """olx_ctx_enable_timing(ctx, k): every k-th launch is bracketed by HIP events
+    (the three passes of a tiled propagation in turn); 0 stops, a negative period is
+    rejected."""
+    from openol_amd import _lib
+    L, c = _lib.load(), _lib.ctx()
+    n = 1024
+    u = dev(synth.random_field(n, n, seed=40))
+    names = ("prop_rows_pad_fwd", "prop_cols_conv", "prop_rows_crop_inv")
+    for period, props, per_pass in ((1, 4, 4), (4, 8, 2)):
+        L.olx_ctx_reset_timing(c)
+        assert L.olx_ctx_enable_timing(c, period) == 0
+        for _ in range(props):
+            ol().AsmProp(u, n, n, 8e-6, 8e-6, LAM, 0.1, bandlimit=True)
+        L.olx_ctx_enable_timing(c, 0)
+        rep = _lib.timing_report()
+        assert all(rep[k][0] == per_pass and rep[k][1] > 0 for k in names), rep
+    assert L.olx_ctx_enable_timing(c, -2) != 0
