@@ -399,14 +399,22 @@ def ingest_encode(ol, torch):
 
 
 def next_rows(ol, torch):
-    """Off-axis, split and tilted propagation and the optical elements at full size:
-    device time per call (HIP events) and, for the elementwise passes, GB/s."""
+    """Fresnel, off-axis, split and tilted propagation and the optical elements at full
+    size: device time per call (HIP events) and, for the elementwise passes, GB/s."""
     out = {}
     N = 4096
     rng = np.random.default_rng(8)
     u0 = torch.from_numpy((rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
                           .astype(np.complex64)).cuda()
     u = u0.clone()
+    # FresnelProp and the CPU-form band-limited variant (cached separable chirp spectra)
+    f = u0.clone()
+    ms = _event_ms(torch, lambda: ol.FresnelProp(f, N, N, PITCH, PITCH, LAM, 0.3), 10)
+    out["fresnel_4096_ms"] = round(ms, 3)
+    ms = _event_ms(torch, lambda: ol.FresnelPropBandLimit(f, N, N, PITCH, PITCH, LAM, 0.3,
+                                                          variant="cpu"), 10)
+    out["fresnel_bl_4096_ms"] = round(ms, 3)
+    del f
     ms = _event_ms(torch, lambda: ol.gshiftedAsmProp(u, u, N, N, PITCH, PITCH, LAM, 0.01, 1e-3, -2e-3,
                                                      bandlimit=True), 10)
     out["shifted_asm_4096_ms"] = round(ms, 3)
