@@ -101,10 +101,28 @@ __global__ void k_twiddle(float2* tw, int n) {
     tw[m] = make_float2((float)c, (float)s);
 }
 
+// plain copy, 16 bytes per lane.  Each work-group copies a contiguous span in 16 KiB
+// blocks, four independent loads per thread before their stores (a grid-stride
+// loop with one load per iteration reached 4.56 TB/s at 1 GiB).
 __global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    constexpr int U = 4;
+    const size_t blk = (size_t)U * blockDim.x;                      // vectors per block
+    const size_t nblk = (n + blk - 1) / blk;
+    const size_t per = (nblk + gridDim.x - 1) / gridDim.x;          // blocks per work-group
+    const size_t b0 = blockIdx.x * per, b1 = b0 + per < nblk ? b0 + per : nblk;
+    for (size_t b = b0; b < b1; b++) {
+        const size_t base = b * blk + threadIdx.x;
+        if (base + (U - 1) * blockDim.x < n) {
+            uint4 v[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) v[k] = src[base + k * blockDim.x];
+#pragma unroll
+            for (int k = 0; k < U; k++) dst[base + k * blockDim.x] = v[k];
+        } else {
+            for (int k = 0; k < U; k++)
+                if (base + k * blockDim.x < n) dst[base + k * blockDim.x] = src[base + k * blockDim.x];
+        }
+    }
 }
 
 }  // namespace olx
