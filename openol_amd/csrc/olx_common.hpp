@@ -108,5 +108,10 @@ inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 // olx_asm_c64 from src into dst (which may alias src) (olx_prop.hip)
 int asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
              double dx, double lambda, double z, uint32_t flags);
+// ASM of g (per flags, normally OLX_ADJOINT), consumed as grad (+)= Im(conj(u) A g)
+// instead of being stored; g is overwritten (olx_prop.hip)
+int asm_phase_grad(olx_ctx* ctx, void* d_g, const void* d_u, float* d_grad, int accumulate,
+                   int64_t ny, int64_t nx, double dy, double dx, double lambda, double z,
+                   uint32_t flags);
 
 }  // namespace olx

@@ -95,17 +95,6 @@ __global__ void __launch_bounds__(MP_T) k_mp_grad(float2* __restrict__ f, const 
 
 __global__ void k_mp_zero(double* loss) { *loss = 0.0; }
 
-// dL/dphi (+)= Im(conj(u) a)
-__global__ void __launch_bounds__(MP_T) k_mp_phigrad(const float2* __restrict__ u,
-                                                     const float2* __restrict__ a, int64_t n,
-                                                     float* __restrict__ grad, int accumulate) {
-    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)gridDim.x * MP_T) {
-        const float2 x = u[i], y = a[i];
-        const float g = x.x * y.y - x.y * y.x;
-        grad[i] = accumulate ? grad[i] + g : g;
-    }
-}
-
 }  // namespace olx
 
 using namespace olx;
@@ -140,9 +129,9 @@ extern "C" int olx_multiplane_mse_c64(olx_ctx* ctx, const float* d_phi, int64_t 
         }
         OLX_TRY(check_launch("multiplane loss"));
         if (!d_grad) continue;
-        OLX_TRY(asm_prop(ctx, f, f, ny, nx, dy, dx, lambda, zs[p], (flags & OLX_BANDLIMIT) | OLX_ADJOINT));
-        k_mp_phigrad<<<eb, MP_T, 0, s>>>(u, f, n, d_grad, p > 0 ? 1 : 0);
-        OLX_TRY(check_launch("multiplane gradient"));
+        // adjoint propagation whose last pass writes dL/dphi (+)= Im(conj(u) A^H dL/df)
+        OLX_TRY(asm_phase_grad(ctx, f, u, d_grad, p > 0 ? 1 : 0, ny, nx, dy, dx, lambda, zs[p],
+                               (flags & OLX_BANDLIMIT) | OLX_ADJOINT));
     }
     return OLX_OK;
 }

@@ -46,6 +46,11 @@ struct PropArgs {
     double extra_scale;  // multiplies the 1/(My Mx) output scale
     int cbl;  // column-block width of W (power of two): W[(m / cbl) ny + h][m % cbl]
     int cbl_log;
+    // phase gradient in place of the output field: pg_grad (+)= Im(conj(pg_u) x) for
+    // each output pixel x (pg_acc: add), by pass C on the tiled path, else by k_phigrad
+    const float2* pg_u;
+    float* pg_grad;
+    int pg_acc;
 };
 
 // index of (row h, padded column m) in the column-blocked intermediate
@@ -289,6 +294,13 @@ __device__ __forceinline__ void bstore(Rsrc b, uint32_t voff, uint32_t soff, flo
     using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), b.r, voff, soff, 0);
 }
+__device__ __forceinline__ float bloadf(Rsrc b, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstoref(Rsrc b, uint32_t voff, uint32_t soff, float v) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, v), b.r, voff, soff, 0);
+}
 // two adjacent complex values in one 16-byte access
 __device__ __forceinline__ void bload2(Rsrc b, uint32_t voff, uint32_t soff, float2& x, float2& y) {
     const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, voff, soff, 0));
@@ -515,7 +527,8 @@ __device__ __forceinline__ void bstore_acc(Rsrc b, uint32_t voff, uint32_t soff,
     bstore(b, voff, soff, v);
 }
 
-template <int L, int KR, int ORD, bool ACC = false>
+// PG: the phase gradient (PropArgs::pg_*) in place of the output field
+template <int L, int KR, int ORD, bool ACC = false, bool PG = false>
 __global__ void __launch_bounds__(KR * Plan<L>::T, (kr_waves<L, KR>()))
 k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a,
                  const float2* __restrict__ tw) {
@@ -544,6 +557,28 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
     run_stages_pair<P, true, TWN, KR>(va, vb, t, lds, c, tw);
     const uint32_t so = (uint32_t)(((size_t)ha * a.ds + t) * sizeof(float2));
     const uint32_t rowb = (uint32_t)(KR * a.ds * sizeof(float2));
+    if constexpr (PG) {
+        // float offsets are half the complex ones
+        const Rsrc ub = make_rsrc(a.pg_u, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float2));
+        const Rsrc gb = make_rsrc(a.pg_grad, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float));
+#pragma unroll
+        for (int b = 0; b < P::E / RL; b++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                if (!last_kept<P>(b, r)) continue;
+                const uint32_t D = (last_index<P>(0, b, r) - L / 4) * sizeof(float2);
+                const float2 xa = cscale(va[b * RL + r], a.scale), xb = cscale(vb[b * RL + r], a.scale);
+                const float2 ua = bload(ub, so, D), ubv = bload(ub, so, D + rowb);
+                float ga = ua.x * xa.y - ua.y * xa.x, gbv = ubv.x * xb.y - ubv.y * xb.x;
+                if (a.pg_acc) {
+                    ga += bloadf(gb, so / 2, D / 2);
+                    gbv += bloadf(gb, so / 2, (D + rowb) / 2);
+                }
+                bstoref(gb, so / 2, D / 2, ga);
+                bstoref(gb, so / 2, (D + rowb) / 2, gbv);
+            }
+        return;
+    }
     if (a.cx == nullptr) {
 #pragma unroll
         for (int b = 0; b < P::E / RL; b++)
@@ -614,7 +649,10 @@ template <int L, int KR>
 static void launch_rows_inv(const TileCfg& g, const PropArgs& a, const float2* W, float2* dst,
                             const float2* tw, hipStream_t s) {
     const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
-    if (a.accum) {
+    if (a.pg_grad) {
+        if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
+        else k_rows_crop_inv2<L, KR, REM_FIRST, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
+    } else if (a.accum) {
         if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
         else k_rows_crop_inv2<L, KR, REM_FIRST, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
     } else {
@@ -935,6 +973,18 @@ static TileCfg tile_cfg(int Mx, int My) {
     return c;
 }
 
+// pg_grad (+)= Im(conj(u) x) over a packed ny x nx field x (the untiled path's
+// phase-gradient epilogue)
+__global__ void k_phigrad(const float2* __restrict__ u, const float2* __restrict__ x, int64_t n,
+                          float* __restrict__ grad, int accumulate) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float2 p = u[i], q = x[i];
+        const float g = p.x * q.y - p.y * q.x;
+        grad[i] = accumulate ? grad[i] + g : g;
+    }
+}
+
 static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a) {
     const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
     OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
@@ -1010,7 +1060,14 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
             k_rows_crop_inv<L><<<grid, row_nt<L>(), 0, s>>>(W, dst, a, ctx->twiddle);
         });
     }
-    return check_launch("prop pass C");
+    OLX_TRY(check_launch("prop pass C"));
+    if (a.pg_grad) {
+        const int64_t n = (int64_t)a.ny * a.nx;
+        const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+        k_phigrad<<<blocks, 256, 0, s>>>(a.pg_u, dst, n, a.pg_grad, a.pg_acc);
+        return check_launch("phase gradient");
+    }
+    return OLX_OK;
 }
 
 static int check_field(olx_ctx* ctx, const void* p, int64_t ny, int64_t nx) {
@@ -1037,13 +1094,11 @@ static PropArgs base_args(int64_t ny, int64_t nx) {
 
 using namespace olx;
 
-int olx::asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
-                  double dx, double lambda, double z, uint32_t flags) {
-    OLX_TRY(check_field(ctx, d_src, ny, nx));
-    if (!d_dst) OLX_FAIL(OLX_EINVAL, "asm: null destination");
+static int asm_args(PropArgs& a, int64_t ny, int64_t nx, double dy, double dx, double lambda,
+                    double z, uint32_t flags) {
     if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !std::isfinite(z))
         OLX_FAIL(OLX_EINVAL, "asm: lambda, dx, dy must be > 0 and z finite");
-    PropArgs a = base_args(ny, nx);
+    a = base_args(ny, nx);
     a.mode = K_ASM;
     a.adjoint = (flags & OLX_ADJOINT) ? 1 : 0;
     const int64_t nx2 = 2 * nx, ny2 = 2 * ny;
@@ -1062,7 +1117,29 @@ int olx::asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int6
         a.uw = ulp + ulm;
         a.vw = vlp + vlm;
     }
+    return OLX_OK;
+}
+
+int olx::asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
+                  double dx, double lambda, double z, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_src, ny, nx));
+    if (!d_dst) OLX_FAIL(OLX_EINVAL, "asm: null destination");
+    PropArgs a;
+    OLX_TRY(asm_args(a, ny, nx, dy, dx, lambda, z, flags));
     return run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a);
+}
+
+int olx::asm_phase_grad(olx_ctx* ctx, void* d_g, const void* d_u, float* d_grad, int accumulate,
+                        int64_t ny, int64_t nx, double dy, double dx, double lambda, double z,
+                        uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_g, ny, nx));
+    if (!d_u || !d_grad) OLX_FAIL(OLX_EINVAL, "asm phase gradient: null argument");
+    PropArgs a;
+    OLX_TRY(asm_args(a, ny, nx, dy, dx, lambda, z, flags));
+    a.pg_u = (const float2*)d_u;
+    a.pg_grad = d_grad;
+    a.pg_acc = accumulate ? 1 : 0;
+    return run_three_pass(ctx, (const float2*)d_g, (float2*)d_g, a);
 }
 
 extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,

@@ -127,11 +127,13 @@ def test_multiplane_gradient_mean_norm_vs_fp64(n):
         assert olref.rel_l2(phi.grad.cpu().numpy().ravel()[idx], np.array(fd)) < 1e-3
 
 
-@pytest.mark.parametrize("shape", [(128, 128), (64, 256)])
+@pytest.mark.parametrize("shape", [(128, 128), (64, 256), (1024, 1024), (1024, 2048)])
 def test_native_multiplane_matches_composed(shape):
     """The native mean-normalised loss and gradient (one pass per plane) against the
     same loss composed from differentiable ASM propagations and torch operations;
-    gout scales the gradient; without grad mode the loss alone is computed."""
+    gout scales the gradient; without grad mode the loss alone is computed.  Padded
+    sides of 2048 and more take the tiled passes, whose last adjoint pass writes the
+    phase gradient itself; smaller fields add it in a separate kernel."""
     ny, nx = shape
     zs = [0.010, 0.013]
     phi0 = synth.uniform(9, synth.S_PHASE, ny * nx, 0, 2 * np.pi).reshape(ny, nx).astype(np.float32)
