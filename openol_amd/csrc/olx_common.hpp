@@ -109,9 +109,11 @@ inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 int asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
              double dx, double lambda, double z, uint32_t flags);
 // ASM of g (per flags, normally OLX_ADJOINT), consumed as grad (+)= Im(conj(u) A g)
-// instead of being stored; g is overwritten (olx_prop.hip)
+// instead of being stored; g is overwritten.  With d_t and d_c, g is first replaced
+// by the loss gradient c1 (|g|^2 m_i - t m_t) - c2 times g, d_c = {m_i, m_t, c1, c2}
+// (olx_prop.hip)
 int asm_phase_grad(olx_ctx* ctx, void* d_g, const void* d_u, float* d_grad, int accumulate,
                    int64_t ny, int64_t nx, double dy, double dx, double lambda, double z,
-                   uint32_t flags);
+                   uint32_t flags, const float* d_t = nullptr, const double* d_c = nullptr);
 
 }  // namespace olx

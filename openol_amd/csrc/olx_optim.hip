@@ -9,8 +9,9 @@
 // before the next one, so no plane field is kept.  One pass takes the FP64 sums
 // S_I, S_T, S_II, S_IT, S_TT, from which
 //   sum r^2 = S_II / mI^2 - 2 S_IT / (mI mT) + S_TT / mT^2,  sum r I = S_II / mI - S_IT / mT;
-// every block re-reduces the block partials in the same order, so results are
-// deterministic.
+// one block adds the block partials in a fixed order, so results are deterministic.
+// The adjoint propagation forms dL/df from f and T as its first pass reads them and
+// writes dL/dphi from its last pass, so neither dL/df nor A^H dL/df is stored.
 #include "olx_common.hpp"
 
 namespace olx {
@@ -66,10 +67,12 @@ __global__ void __launch_bounds__(MP_T) k_mp_sums(const float2* __restrict__ f,
         for (int k = 0; k < MP_NS; k++) part[k * MP_B + blockIdx.x] = v[k];
 }
 
-// pass 2: dL/df in place; block 0 adds this plane's loss
-__global__ void __launch_bounds__(MP_T) k_mp_grad(float2* __restrict__ f, const float* __restrict__ t,
-                                                  int64_t n, const double* __restrict__ part,
-                                                  double* __restrict__ loss) {
+// pass 2 (one block): the totals, this plane's loss (added to *loss) and the
+// constants of dL/df = 2 f dL/dI = (c1 r - c2) f, r = I m_i - T m_t:
+// c = {m_i, m_t, c1, c2} with m = 1 / mean, c1 = 4 m_i / N, c2 = 4 m_i^2 sum(r I) / N^2
+__global__ void __launch_bounds__(MP_T) k_mp_consts(const double* __restrict__ part, int64_t n,
+                                                    double* __restrict__ c,
+                                                    double* __restrict__ loss) {
     double S[MP_NS];
 #pragma unroll
     for (int k = 0; k < MP_NS; k++) {
@@ -77,20 +80,16 @@ __global__ void __launch_bounds__(MP_T) k_mp_grad(float2* __restrict__ f, const 
         for (int i = threadIdx.x; i < MP_B; i += MP_T) S[k] += part[k * MP_B + i];
     }
     block_sums(S);
+    if (threadIdx.x != 0) return;
     const double N = (double)n;
     const double inv_mi = N / S[0], inv_mt = N / S[1];
     const double sr2 = S[2] * inv_mi * inv_mi - 2.0 * S[3] * inv_mi * inv_mt + S[4] * inv_mt * inv_mt;
     const double sri = S[2] * inv_mi - S[3] * inv_mt;
-    const double c1 = 4.0 / N * inv_mi;                    // 2 * (2/N) / mI
-    const double c2 = 4.0 / N * sri / N * inv_mi * inv_mi;  // 2 * (2/N) * sum rI / (N mI^2)
-    for (int64_t i = blockIdx.x * (int64_t)MP_T + threadIdx.x; i < n; i += (int64_t)MP_B * MP_T) {
-        const float2 v = f[i];
-        const double I = (double)v.x * v.x + (double)v.y * v.y;
-        const double r = I * inv_mi - (double)t[i] * inv_mt;
-        const double g = c1 * r - c2;
-        f[i] = make_float2((float)(g * v.x), (float)(g * v.y));
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *loss += sr2 / N;
+    c[0] = inv_mi;
+    c[1] = inv_mt;
+    c[2] = 4.0 / N * inv_mi;                    // 2 * (2/N) / mI
+    c[3] = 4.0 / N * sri / N * inv_mi * inv_mi;  // 2 * (2/N) * sum rI / (N mI^2)
+    *loss += sr2 / N;
 }
 
 __global__ void k_mp_zero(double* loss) { *loss = 0.0; }
@@ -111,10 +110,11 @@ extern "C" int olx_multiplane_mse_c64(olx_ctx* ctx, const float* d_phi, int64_t 
         if (!d_targets[p]) OLX_FAIL(OLX_EINVAL, "multiplane: null target %d", p);
     const int64_t n = ny * nx;
     const size_t fb = sizeof(float2) * (size_t)n;
-    OLX_TRY(ctx->mp_work.reserve(2 * fb + sizeof(double) * MP_NS * MP_B));
+    OLX_TRY(ctx->mp_work.reserve(2 * fb + sizeof(double) * (MP_NS * MP_B + 4)));
     float2* u = (float2*)ctx->mp_work.p;
     float2* f = (float2*)((char*)ctx->mp_work.p + fb);
     double* part = (double*)((char*)ctx->mp_work.p + 2 * fb);
+    double* consts = part + MP_NS * MP_B;
     hipStream_t s = ctx->stream;
     const unsigned eb = (unsigned)std::min<int64_t>((n + MP_T - 1) / MP_T, 65536);
     k_mp_expi<<<eb, MP_T, 0, s>>>(d_phi, u, n);
@@ -125,13 +125,14 @@ extern "C" int olx_multiplane_mse_c64(olx_ctx* ctx, const float* d_phi, int64_t 
         {
             TimedLaunch tl(ctx, "multiplane_loss");
             k_mp_sums<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part);
-            k_mp_grad<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part, d_loss);
+            k_mp_consts<<<1, MP_T, 0, s>>>(part, n, consts, d_loss);
         }
         OLX_TRY(check_launch("multiplane loss"));
         if (!d_grad) continue;
-        // adjoint propagation whose last pass writes dL/dphi (+)= Im(conj(u) A^H dL/df)
+        // adjoint propagation of dL/df = (c1 r - c2) f, formed as its first pass reads f,
+        // whose last pass writes dL/dphi (+)= Im(conj(u) A^H dL/df)
         OLX_TRY(asm_phase_grad(ctx, f, u, d_grad, p > 0 ? 1 : 0, ny, nx, dy, dx, lambda, zs[p],
-                               (flags & OLX_BANDLIMIT) | OLX_ADJOINT));
+                               (flags & OLX_BANDLIMIT) | OLX_ADJOINT, d_targets[p], consts));
     }
     return OLX_OK;
 }

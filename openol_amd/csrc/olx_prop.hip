@@ -51,6 +51,11 @@ struct PropArgs {
     const float2* pg_u;
     float* pg_grad;
     int pg_acc;
+    // loss-gradient prologue: each source value x becomes g x with
+    // g = c1 (|x|^2 m_i - t m_t) - c2, lg_c = {m_i, m_t, c1, c2} (FP64, on the device)
+    // and t from lg_t; by pass A on the tiled path, else by k_loss_grad
+    const float* lg_t;
+    const double* lg_c;
 };
 
 // index of (row h, padded column m) in the column-blocked intermediate
@@ -336,7 +341,15 @@ constexpr int kr_waves() {
     return (KR * Plan<L>::T <= 512 && 2 * KR * Plan<L>::LP * 8 <= 160 * 1024) ? 2 * KR * Plan<L>::T / 256 : 1;
 }
 
-template <int L, int KR, int ORD>
+__device__ __forceinline__ float2 loss_grad(float2 x, float t, double mi, double mt, double c1,
+                                            double c2) {
+    const double I = (double)x.x * x.x + (double)x.y * x.y;
+    const double g = c1 * (I * mi - (double)t * mt) - c2;
+    return make_float2((float)(g * x.x), (float)(g * x.y));
+}
+
+// LG: the loss-gradient prologue (PropArgs::lg_*) on the source values
+template <int L, int KR, int ORD, bool LG = false>
 __global__ void __launch_bounds__(KR * Plan<L>::T, (kr_waves<L, KR>()))
 k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a,
                 const float2* __restrict__ tw) {
@@ -357,6 +370,15 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
         aya = a.ay[ha];
         ayb = a.ay[hb];
     }
+    Rsrc tb;
+    double mi = 0.0, mt = 0.0, c1 = 0.0, c2 = 0.0;
+    if constexpr (LG) {
+        tb = make_rsrc(a.lg_t, ((size_t)(a.ny - 1) * a.ss + nx) * sizeof(float));
+        mi = a.lg_c[0];
+        mt = a.lg_c[1];
+        c1 = a.lg_c[2];
+        c2 = a.lg_c[3];
+    }
     float2 va[P::E], vb[P::E];
 #pragma unroll
     for (int b = 0; b < P::E / R0; b++)
@@ -368,6 +390,10 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
                 const int D = first_index<P>(0, b, r) - L / 4;
                 xa = bload(sb, so, D * sizeof(float2));
                 xb = bload(sb, so, D * sizeof(float2) + rowb);
+                if constexpr (LG) {  // float offsets are half the complex ones
+                    xa = loss_grad(xa, bloadf(tb, so / 2, D * sizeof(float)), mi, mt, c1, c2);
+                    xb = loss_grad(xb, bloadf(tb, so / 2, D * sizeof(float) + rowb / 2), mi, mt, c1, c2);
+                }
                 if (a.ax) {
                     const float2 ax = a.ax[t + D];
                     xa = cmul(xa, cmul(ax, aya));
@@ -626,8 +652,13 @@ template <int L, int KR>
 static void launch_rows_fwd(const TileCfg& g, const PropArgs& a, const float2* src, float2* W,
                             const float2* tw, hipStream_t s) {
     const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
-    if (g.oa) k_rows_pad_fwd2<L, KR, REM_LAST><<<grid, nt, 0, s>>>(src, W, a, tw);
-    else k_rows_pad_fwd2<L, KR, REM_FIRST><<<grid, nt, 0, s>>>(src, W, a, tw);
+    if (a.lg_t) {
+        if (g.oa) k_rows_pad_fwd2<L, KR, REM_LAST, true><<<grid, nt, 0, s>>>(src, W, a, tw);
+        else k_rows_pad_fwd2<L, KR, REM_FIRST, true><<<grid, nt, 0, s>>>(src, W, a, tw);
+    } else {
+        if (g.oa) k_rows_pad_fwd2<L, KR, REM_LAST><<<grid, nt, 0, s>>>(src, W, a, tw);
+        else k_rows_pad_fwd2<L, KR, REM_FIRST><<<grid, nt, 0, s>>>(src, W, a, tw);
+    }
 }
 template <int L, int KR, int MODE>
 static void launch_cols_mode(const TileCfg& g, const PropArgs& a, float2* W, const float2* tw,
@@ -973,6 +1004,15 @@ static TileCfg tile_cfg(int Mx, int My) {
     return c;
 }
 
+// the loss-gradient prologue in place over a packed ny x nx field (untiled path)
+__global__ void k_loss_grad(float2* __restrict__ x, const float* __restrict__ t, int64_t n,
+                            const double* __restrict__ c) {
+    const double mi = c[0], mt = c[1], c1 = c[2], c2 = c[3];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = loss_grad(x[i], t[i], mi, mt, c1, c2);
+}
+
 // pg_grad (+)= Im(conj(u) x) over a packed ny x nx field x (the untiled path's
 // phase-gradient epilogue)
 __global__ void k_phigrad(const float2* __restrict__ u, const float2* __restrict__ x, int64_t n,
@@ -1032,6 +1072,12 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
             });
         }
         return check_launch("prop pass C");
+    }
+    if (a.lg_t) {  // the prologue in place (the caller's source is writable in this mode)
+        const int64_t n = (int64_t)a.ny * a.nx;
+        const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+        k_loss_grad<<<blocks, 256, 0, s>>>(const_cast<float2*>(src), a.lg_t, n, a.lg_c);
+        OLX_TRY(check_launch("loss gradient"));
     }
     int cb = 1;
     OLX_LEN_SWITCH(ilog2(My), { cb = col_batch<L>(); });
@@ -1131,7 +1177,7 @@ int olx::asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int6
 
 int olx::asm_phase_grad(olx_ctx* ctx, void* d_g, const void* d_u, float* d_grad, int accumulate,
                         int64_t ny, int64_t nx, double dy, double dx, double lambda, double z,
-                        uint32_t flags) {
+                        uint32_t flags, const float* d_t, const double* d_c) {
     OLX_TRY(check_field(ctx, d_g, ny, nx));
     if (!d_u || !d_grad) OLX_FAIL(OLX_EINVAL, "asm phase gradient: null argument");
     PropArgs a;
@@ -1139,6 +1185,10 @@ int olx::asm_phase_grad(olx_ctx* ctx, void* d_g, const void* d_u, float* d_grad,
     a.pg_u = (const float2*)d_u;
     a.pg_grad = d_grad;
     a.pg_acc = accumulate ? 1 : 0;
+    if (d_t && d_c) {
+        a.lg_t = d_t;
+        a.lg_c = d_c;
+    }
     return run_three_pass(ctx, (const float2*)d_g, (float2*)d_g, a);
 }
 
