@@ -108,6 +108,12 @@ inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 // olx_asm_c64 from src into dst (which may alias src) (olx_prop.hip)
 int asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
              double dx, double lambda, double z, uint32_t flags);
+// asm_prop that also writes, on the tiled path, each pass-C work-group's sums of I, T,
+// I^2, I T, T^2 (I = |dst|^2, T = d_t) to d_part[k * G + group] and sets *groups = G
+// (at most ny / 2); *groups = 0 when the untiled path ran (olx_prop.hip)
+int asm_prop_sums(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
+                  double dx, double lambda, double z, uint32_t flags, const float* d_t,
+                  double* d_part, int* groups);
 // ASM of g (per flags, normally OLX_ADJOINT), consumed as grad (+)= Im(conj(u) A g)
 // instead of being stored; g is overwritten.  With d_t and d_c, g is first replaced
 // by the loss gradient c1 (|g|^2 m_i - t m_t) - c2 times g, d_c = {m_i, m_t, c1, c2}

@@ -6,7 +6,8 @@
 //   dL/df_k = 2 f_k dL/dI_k  (gradient w.r.t. Re f plus i times w.r.t. Im f)
 //   dL/dphi = sum_p Im(conj(u) A_p^H dL/df_p)                    (A^H: the adjoint ASM)
 // Each plane is propagated, reduced, turned into its gradient and propagated back
-// before the next one, so no plane field is kept.  One pass takes the FP64 sums
+// before the next one, so no plane field is kept.  The forward propagation's last
+// pass (or, for small fields, a pass of its own) takes the FP64 sums
 // S_I, S_T, S_II, S_IT, S_TT, from which
 //   sum r^2 = S_II / mI^2 - 2 S_IT / (mI mT) + S_TT / mT^2,  sum r I = S_II / mI - S_IT / mT;
 // one block adds the block partials in a fixed order, so results are deterministic.
@@ -67,17 +68,18 @@ __global__ void __launch_bounds__(MP_T) k_mp_sums(const float2* __restrict__ f,
         for (int k = 0; k < MP_NS; k++) part[k * MP_B + blockIdx.x] = v[k];
 }
 
-// pass 2 (one block): the totals, this plane's loss (added to *loss) and the
+// pass 2 (one block): the totals of `groups` partials (part[k * groups + i]), this
+// plane's loss (added to *loss) and the
 // constants of dL/df = 2 f dL/dI = (c1 r - c2) f, r = I m_i - T m_t:
 // c = {m_i, m_t, c1, c2} with m = 1 / mean, c1 = 4 m_i / N, c2 = 4 m_i^2 sum(r I) / N^2
-__global__ void __launch_bounds__(MP_T) k_mp_consts(const double* __restrict__ part, int64_t n,
-                                                    double* __restrict__ c,
+__global__ void __launch_bounds__(MP_T) k_mp_consts(const double* __restrict__ part, int groups,
+                                                    int64_t n, double* __restrict__ c,
                                                     double* __restrict__ loss) {
     double S[MP_NS];
 #pragma unroll
     for (int k = 0; k < MP_NS; k++) {
         S[k] = 0.0;
-        for (int i = threadIdx.x; i < MP_B; i += MP_T) S[k] += part[k * MP_B + i];
+        for (int i = threadIdx.x; i < groups; i += MP_T) S[k] += part[k * groups + i];
     }
     block_sums(S);
     if (threadIdx.x != 0) return;
@@ -110,22 +112,29 @@ extern "C" int olx_multiplane_mse_c64(olx_ctx* ctx, const float* d_phi, int64_t 
         if (!d_targets[p]) OLX_FAIL(OLX_EINVAL, "multiplane: null target %d", p);
     const int64_t n = ny * nx;
     const size_t fb = sizeof(float2) * (size_t)n;
-    OLX_TRY(ctx->mp_work.reserve(2 * fb + sizeof(double) * (MP_NS * MP_B + 4)));
+    const int64_t maxg = std::max<int64_t>(MP_B, ny / 2);  // partials: sums kernel or pass C
+    OLX_TRY(ctx->mp_work.reserve(2 * fb + sizeof(double) * (MP_NS * maxg + 4)));
     float2* u = (float2*)ctx->mp_work.p;
     float2* f = (float2*)((char*)ctx->mp_work.p + fb);
     double* part = (double*)((char*)ctx->mp_work.p + 2 * fb);
-    double* consts = part + MP_NS * MP_B;
+    double* consts = part + MP_NS * maxg;
     hipStream_t s = ctx->stream;
     const unsigned eb = (unsigned)std::min<int64_t>((n + MP_T - 1) / MP_T, 65536);
     k_mp_expi<<<eb, MP_T, 0, s>>>(d_phi, u, n);
     k_mp_zero<<<1, 1, 0, s>>>(d_loss);
     OLX_TRY(check_launch("multiplane setup"));
     for (int p = 0; p < nplanes; p++) {
-        OLX_TRY(asm_prop(ctx, u, f, ny, nx, dy, dx, lambda, zs[p], flags & OLX_BANDLIMIT));
+        // forward propagation; on the tiled path its last pass also takes the sums
+        int groups = 0;
+        OLX_TRY(asm_prop_sums(ctx, u, f, ny, nx, dy, dx, lambda, zs[p], flags & OLX_BANDLIMIT,
+                              d_targets[p], part, &groups));
         {
             TimedLaunch tl(ctx, "multiplane_loss");
-            k_mp_sums<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part);
-            k_mp_consts<<<1, MP_T, 0, s>>>(part, n, consts, d_loss);
+            if (groups == 0) {
+                k_mp_sums<<<MP_B, MP_T, 0, s>>>(f, d_targets[p], n, part);
+                groups = MP_B;
+            }
+            k_mp_consts<<<1, MP_T, 0, s>>>(part, groups, n, consts, d_loss);
         }
         OLX_TRY(check_launch("multiplane loss"));
         if (!d_grad) continue;

@@ -56,6 +56,12 @@ struct PropArgs {
     // and t from lg_t; by pass A on the tiled path, else by k_loss_grad
     const float* lg_t;
     const double* lg_c;
+    // loss sums beside the output field: pass C (tiled path) writes each work-group's
+    // sums of I, T, I^2, I T, T^2 (I = |x|^2, T from ls_t) to ls_part[k * G + group]
+    // and sets ls_groups = G; the untiled path leaves ls_groups 0
+    const float* ls_t;
+    double* ls_part;
+    int ls_groups;
 };
 
 // index of (row h, padded column m) in the column-blocked intermediate
@@ -554,7 +560,8 @@ __device__ __forceinline__ void bstore_acc(Rsrc b, uint32_t voff, uint32_t soff,
 }
 
 // PG: the phase gradient (PropArgs::pg_*) in place of the output field
-template <int L, int KR, int ORD, bool ACC = false, bool PG = false>
+// LS: the loss sums (PropArgs::ls_*) beside it
+template <int L, int KR, int ORD, bool ACC = false, bool PG = false, bool LS = false>
 __global__ void __launch_bounds__(KR * Plan<L>::T, (kr_waves<L, KR>()))
 k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a,
                  const float2* __restrict__ tw) {
@@ -606,6 +613,9 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
         return;
     }
     if (a.cx == nullptr) {
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        Rsrc tb;
+        if constexpr (LS) tb = make_rsrc(a.ls_t, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float));
 #pragma unroll
         for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
@@ -613,9 +623,41 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
                 if (!last_kept<P>(b, r)) continue;
                 // column t + D
                 const uint32_t D = (last_index<P>(0, b, r) - L / 4) * sizeof(float2);
-                bstore_acc(db, so, D, cscale(va[b * RL + r], a.scale), ACC);
-                bstore_acc(db, so, D + rowb, cscale(vb[b * RL + r], a.scale), ACC);
+                const float2 xa = cscale(va[b * RL + r], a.scale), xb = cscale(vb[b * RL + r], a.scale);
+                bstore_acc(db, so, D, xa, ACC);
+                bstore_acc(db, so, D + rowb, xb, ACC);
+                if constexpr (LS) {  // float offsets are half the complex ones
+                    const double Ia = (double)xa.x * xa.x + (double)xa.y * xa.y;
+                    const double Ib = (double)xb.x * xb.x + (double)xb.y * xb.y;
+                    const double Ta = (double)bloadf(tb, so / 2, D / 2);
+                    const double Tb = (double)bloadf(tb, so / 2, (D + rowb) / 2);
+                    acc[0] += Ia + Ib;
+                    acc[1] += Ta + Tb;
+                    acc[2] += Ia * Ia + Ib * Ib;
+                    acc[3] += Ia * Ta + Ib * Tb;
+                    acc[4] += Ta * Ta + Tb * Tb;
+                }
             }
+        if constexpr (LS) {
+            // work-group sums in a fixed order through LDS (free again: the inverse
+            // transform's last LDS reads are behind it, then this barrier)
+            constexpr int NT = KR * P::T;
+            static_assert(5 * NT * sizeof(double) <= KR * P::LP * sizeof(float2), "sums fit in LDS");
+            double* sh = reinterpret_cast<double*>(lds);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 5; k++) sh[k * NT + tid] = acc[k];
+            __syncthreads();
+            for (int off = NT / 2; off > 0; off >>= 1) {
+                if (tid < off) {
+#pragma unroll
+                    for (int k = 0; k < 5; k++) sh[k * NT + tid] += sh[k * NT + tid + off];
+                }
+                __syncthreads();
+            }
+            if (tid == 0)
+                for (int k = 0; k < 5; k++) a.ls_part[k * gridDim.x + blockIdx.x] = sh[k * NT];
+        }
         return;
     }
     const bool posta = abs(ha - a.ny / 2) < a.chl;
@@ -680,7 +722,10 @@ template <int L, int KR>
 static void launch_rows_inv(const TileCfg& g, const PropArgs& a, const float2* W, float2* dst,
                             const float2* tw, hipStream_t s) {
     const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
-    if (a.pg_grad) {
+    if (a.ls_t) {
+        if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST, false, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
+        else k_rows_crop_inv2<L, KR, REM_FIRST, false, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
+    } else if (a.pg_grad) {
         if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
         else k_rows_crop_inv2<L, KR, REM_FIRST, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
     } else if (a.accum) {
@@ -1067,7 +1112,10 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
             TimedLaunch tl(ctx, "prop_rows_crop_inv");
             OLX_LEN_SWITCH(ilog2(Mx), {
                 if constexpr (L >= PAIR_MIN) {
-                    with_kr<L>(cfg.krc, [&](auto K) { launch_rows_inv<L, decltype(K)::value>(cfg, a, W, dst, ctx->twiddle, s); });
+                    with_kr<L>(cfg.krc, [&](auto K) {
+                        launch_rows_inv<L, decltype(K)::value>(cfg, a, W, dst, ctx->twiddle, s);
+                        if (a.ls_t) a.ls_groups = a.ny / (2 * decltype(K)::value);
+                    });
                 }
             });
         }
@@ -1173,6 +1221,20 @@ int olx::asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int6
     PropArgs a;
     OLX_TRY(asm_args(a, ny, nx, dy, dx, lambda, z, flags));
     return run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a);
+}
+
+int olx::asm_prop_sums(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx,
+                       double dy, double dx, double lambda, double z, uint32_t flags,
+                       const float* d_t, double* d_part, int* groups) {
+    OLX_TRY(check_field(ctx, d_src, ny, nx));
+    if (!d_dst || !d_t || !d_part || !groups) OLX_FAIL(OLX_EINVAL, "asm sums: null argument");
+    PropArgs a;
+    OLX_TRY(asm_args(a, ny, nx, dy, dx, lambda, z, flags));
+    a.ls_t = d_t;
+    a.ls_part = d_part;
+    OLX_TRY(run_three_pass(ctx, (const float2*)d_src, (float2*)d_dst, a));
+    *groups = a.ls_groups;
+    return OLX_OK;
 }
 
 int olx::asm_phase_grad(olx_ctx* ctx, void* d_g, const void* d_u, float* d_grad, int accumulate,
