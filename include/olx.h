@@ -13,10 +13,21 @@
  *
  * Threading: a context is not thread safe; independent contexts are.  All work
  * is enqueued on the context's stream (olx_ctx_set_stream) and is asynchronous
- * with respect to the host unless an entry point says otherwise.
+ * with respect to the host unless an entry point says otherwise.  A context's
+ * workspaces and cached tables serve every stream it is bound to: rebinding it to
+ * another stream makes that stream wait for the work queued on the previous one,
+ * so calls on different streams through one context are ordered, not overlapped.
  *
- * Sizes: propagation needs ny, nx to be powers of two with 4 <= n <= 8192
- * (padded transform length 2n <= 16384).
+ * Cached tables (Fresnel / ARSS chirp spectra, pre- and post-chirps, twiddles of
+ * non-power-of-two lengths) are kept per context, keyed by their parameters; the
+ * least recently used is freed beyond 64 entries, and olx_ctx_clear_tables frees
+ * them all.
+ *
+ * Sizes: propagation takes 2 <= ny, nx <= 8192 whose padded lengths 2n have no
+ * prime factor above 13 (the reference's FFTW / cuFFT take any length; 1920 x
+ * 1080 and 3840 x 2160, its demo sizes, qualify).  Power-of-two sides from 4 run
+ * the fused three-pass kernels; other sides the mixed-radix passes.  Other sizes
+ * fail with OLX_EUNSUPPORTED.
  */
 #ifndef OLX_H
 #define OLX_H
@@ -78,6 +89,7 @@ typedef struct olx_point {
 int olx_ctx_create(olx_ctx** out, int device);
 int olx_ctx_destroy(olx_ctx* ctx);
 int olx_ctx_set_stream(olx_ctx* ctx, void* hip_stream); /* NULL = default stream */
+int olx_ctx_clear_tables(olx_ctx* ctx); /* synchronises the stream, frees every cached table */
 int olx_ctx_synchronize(olx_ctx* ctx);
 const char* olx_last_error(void);
 int olx_version(void);
@@ -121,7 +133,7 @@ int olx_shifted_asm_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny
  * gshiftedAsmProp per pair (gsplitProp, include/golprop.h:773-813), or with
  * OLX_SHIFTED_CPU the CPU shiftedAsmProp (splitAsmProp, include/olprop.h:614-651);
  * OLX_MODE_FRESNEL: ARSS with s = 1 per pair (gsplitProp's FRESNEL branch through
- * shiftedProp, include/golprop.h:756-770).  Blocks must be powers of two in [4, 8192]. */
+ * shiftedProp, include/golprop.h:756-770).  Blocks follow the propagation size rule. */
 int olx_split_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
                        double lambda, double z, int mode, int by, int bx, uint32_t flags);
 /* tilted-plane propagation in place: kind OLX_TILT_SOURCE is TiltedSourceProp

@@ -56,6 +56,7 @@ SIGNATURES = {
     "olx_ctx_create": [ctypes.POINTER(_p), _i],
     "olx_ctx_destroy": [_p],
     "olx_ctx_set_stream": [_p, _p],
+    "olx_ctx_clear_tables": [_p],
     "olx_ctx_synchronize": [_p],
     "olx_version": [],
     "olx_dev_alloc": [_p, ctypes.POINTER(_p), ctypes.c_size_t],
@@ -143,10 +144,18 @@ def check(rc, what=""):
 
 
 _ctx = {}
+# Held from binding a context to the caller's stream through the launch, so that
+# two threads cannot interleave a bind and another thread's launch on one context.
+_call_lock = threading.RLock()
 
 
 def ctx(device=None):
-    """Per-device library context bound to torch's current stream."""
+    """Per-device library context bound to torch's current stream.
+
+    One context (and so one set of workspaces) serves every stream of a device.
+    When the bound stream changes, the library makes the new stream wait for the
+    work already queued on the previous one (olx_ctx_set_stream), so calls issued
+    on different streams are ordered, never overlapped, on the shared workspaces."""
     import torch
     if not torch.cuda.is_available():
         raise OlxError(EDEVICE, "no HIP device visible: the MI355X path has no CPU fallback")
@@ -163,6 +172,14 @@ def ctx(device=None):
             _ctx[device] = c
     check(L.olx_ctx_set_stream(c, _p(torch.cuda.current_stream(device).cuda_stream)))
     return c
+
+
+def call(name, device, *args):
+    """Bind the device's context to torch's current stream and call entry point
+    `name` with the context as its first argument, both under one lock."""
+    L = load()
+    with _call_lock:
+        return getattr(L, name)(ctx(device), *args)
 
 
 def timing_report(device=None):

@@ -14,9 +14,11 @@ from . import ops
 
 
 def _asm_copy(u, dy, dx, lam, z, bandlimit, adjoint):
+    """AsmProp of a copy of u; leading dimensions are independent fields."""
     out = u.contiguous().clone()
-    ops.AsmProp(out, out.shape[-2], out.shape[-1], dy, dx, lam, z, bandlimit=bandlimit,
-                adjoint=adjoint)
+    ny, nx = out.shape[-2], out.shape[-1]
+    for plane in out.reshape(-1, ny, nx):
+        ops.AsmProp(plane, ny, nx, dy, dx, lam, z, bandlimit=bandlimit, adjoint=adjoint)
     return out
 
 

@@ -60,8 +60,16 @@ struct olx_ctx {
     // propagation workspace (padded intermediate)
     olx::DevBuf work;
     olx::DevBuf scratch2;  // second work buffer (sources aliasing, staging)
-    // cached 1-D tables (chirp spectra, pre/post chirps), keyed by parameters
-    std::map<std::string, std::shared_ptr<olx::DevBuf>> tables;
+    // ordering event: recorded on the previous stream when the bound stream changes
+    hipEvent_t order_event = nullptr;
+    // cached 1-D tables (chirp spectra, pre/post chirps, twiddles), keyed by parameters,
+    // least recently used evicted beyond olx::TABLE_CAP entries
+    struct Table {
+        std::shared_ptr<olx::DevBuf> buf;
+        uint64_t last_use = 0;
+    };
+    std::map<std::string, Table> tables;
+    uint64_t table_tick = 0;
     // WRP / CGH scratch
     olx::DevBuf wrp_counts, wrp_lists, wrp_pts;
     olx::DevBuf lay_pts, lay_counts;
@@ -72,8 +80,8 @@ struct olx_ctx {
     olx::DevBuf cgh_part;   // direct CGH split over point slices: one partial plane per slice
     // timing
     bool timing = false;
-    int timing_period = 1;   // time every timing_period-th launch
-    uint64_t timing_seq = 0;
+    int timing_period = 1;   // time every timing_period-th launch of each kernel
+    std::map<std::string, uint64_t> timing_seq;  // launches seen per kernel name
     std::map<std::string, olx::KernelStat> stats;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
@@ -82,6 +90,7 @@ struct olx_ctx {
 namespace olx {
 
 constexpr int TWN = 16384;  // twiddle table length = largest transform length
+constexpr size_t TABLE_CAP = 64;  // cached tables kept per context
 
 // timing bracket around launches (no-op unless ctx->timing)
 struct TimedLaunch {
@@ -104,6 +113,14 @@ const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa,
                              int adj, int* rc, double qb = 0.0);
 
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// mixed-radix transforms (olx_prop_mr.hip): whether length L (2 .. 16384) factors
+// into 2, 3, 5, 7, 11, 13; the three propagation passes over the row-major
+// intermediate W (ny x 2nx); the in-place 2-D FFT (sign of the exponent: inv)
+struct PropArgs;
+bool mr_supported(int64_t L);
+int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, float2* W);
+int mr_fft2d(olx_ctx* ctx, float2* u, int64_t ny, int64_t nx, bool inv);
 
 // olx_asm_c64 from src into dst (which may alias src) (olx_prop.hip)
 int asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,

@@ -42,11 +42,21 @@ float2* table(olx_ctx* ctx, const std::string& key, size_t count, bool* fresh, i
     auto it = ctx->tables.find(key);
     if (it != ctx->tables.end()) {
         *fresh = false;
-        return (float2*)it->second->p;
+        it->second.last_use = ++ctx->table_tick;
+        return (float2*)it->second.buf->p;
+    }
+    if (ctx->tables.size() >= TABLE_CAP) {
+        // evict the least recently used table; queued work may still read it.  A call
+        // fetches at most six tables, so the victim is never one of the current call's.
+        auto victim = ctx->tables.begin();
+        for (auto i = ctx->tables.begin(); i != ctx->tables.end(); ++i)
+            if (i->second.last_use < victim->second.last_use) victim = i;
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->tables.erase(victim);
     }
     auto buf = std::make_shared<DevBuf>();
     if ((*rc = buf->reserve(sizeof(float2) * count)) != OLX_OK) return nullptr;
-    ctx->tables[key] = buf;
+    ctx->tables[key] = {buf, ++ctx->table_tick};
     *fresh = true;
     return (float2*)buf->p;
 }
@@ -64,7 +74,8 @@ static hipEvent_t get_event(olx_ctx* ctx) {
 
 TimedLaunch::TimedLaunch(olx_ctx* c, const char* n) : ctx(c), name(n) {
     if (!ctx->timing) return;
-    if (ctx->timing_seq++ % (uint64_t)ctx->timing_period != 0) return;  // not sampled: a, b stay null
+    // each kernel name is sampled on every timing_period-th of its own launches
+    if (ctx->timing_seq[name]++ % (uint64_t)ctx->timing_period != 0) return;  // a, b stay null
     a = get_event(ctx);
     b = get_event(ctx);
     if (a) (void)hipEventRecord(a, ctx->stream);
@@ -159,8 +170,13 @@ extern "C" int olx_ctx_destroy(olx_ctx* ctx) {
     if (!ctx) return OLX_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
+    for (auto& p : ctx->pending) {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
     ctx->pending.clear();
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->order_event) (void)hipEventDestroy(ctx->order_event);
     if (ctx->twiddle) (void)hipFree(ctx->twiddle);
     delete ctx;
     return OLX_OK;
@@ -168,7 +184,21 @@ extern "C" int olx_ctx_destroy(olx_ctx* ctx) {
 
 extern "C" int olx_ctx_set_stream(olx_ctx* ctx, void* hip_stream) {
     if (!ctx) OLX_FAIL(OLX_EINVAL, "null context");
-    ctx->stream = (hipStream_t)hip_stream;
+    hipStream_t s = (hipStream_t)hip_stream;
+    if (s == ctx->stream) return OLX_OK;
+    // The workspaces and tables are shared by every stream the context is bound to:
+    // work queued from now on waits for what the previous stream already holds.
+    if (!ctx->order_event) OLX_HIP(hipEventCreateWithFlags(&ctx->order_event, hipEventDisableTiming));
+    OLX_HIP(hipEventRecord(ctx->order_event, ctx->stream));
+    OLX_HIP(hipStreamWaitEvent(s, ctx->order_event, 0));
+    ctx->stream = s;
+    return OLX_OK;
+}
+
+extern "C" int olx_ctx_clear_tables(olx_ctx* ctx) {
+    if (!ctx) OLX_FAIL(OLX_EINVAL, "null context");
+    OLX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->tables.clear();
     return OLX_OK;
 }
 
@@ -226,7 +256,7 @@ extern "C" int olx_ctx_enable_timing(olx_ctx* ctx, int on) {
     if (!on) flush_timing(ctx);
     ctx->timing = on != 0;
     ctx->timing_period = on > 1 ? on : 1;
-    ctx->timing_seq = 0;
+    ctx->timing_seq.clear();
     return OLX_OK;
 }
 

@@ -20,49 +20,10 @@
 
 #include "fft_core.hpp"
 #include "olx_common.hpp"
+#include "prop_common.hpp"
 
 namespace olx {
 
-enum { K_ASM = 0, K_TABLE = 1, K_ASHIFT = 2 };
-
-struct PropArgs {
-    int ny, nx;  // unpadded field
-    int mode;    // K_ASM / K_TABLE / K_ASHIFT
-    int adjoint;
-    int bandlimit;
-    double du, dv, inv_l2, zs, uw, vw;  // ASM transfer function (zs: z, -z for the
-                                        // adjoint) and band limit
-    double u0, v0, ox, oy;  // K_ASHIFT: window centres and the shift's linear phase
-    int ss, ds;             // row strides (elements) of the source and destination
-    int accum;              // pass C adds into the destination instead of storing
-    const float2* sx;                  // K_TABLE: natural-order spectra, length Mx / My
-    const float2* sy;
-    const float2* ax;  // pre-chirp tables (length nx / ny), zero outside the window
-    const float2* ay;
-    const float2* cx;  // post-chirp tables, applied inside |w-nx/2|<cwl, |h-ny/2|<chl
-    const float2* cy;
-    int cwl, chl;
-    float scale;
-    double extra_scale;  // multiplies the 1/(My Mx) output scale
-    int cbl;  // column-block width of W (power of two): W[(m / cbl) ny + h][m % cbl]
-    int cbl_log;
-    // phase gradient in place of the output field: pg_grad (+)= Im(conj(pg_u) x) for
-    // each output pixel x (pg_acc: add), by pass C on the tiled path, else by k_phigrad
-    const float2* pg_u;
-    float* pg_grad;
-    int pg_acc;
-    // loss-gradient prologue: each source value x becomes g x with
-    // g = c1 (|x|^2 m_i - t m_t) - c2, lg_c = {m_i, m_t, c1, c2} (FP64, on the device)
-    // and t from lg_t; by pass A on the tiled path, else by k_loss_grad
-    const float* lg_t;
-    const double* lg_c;
-    // loss sums beside the output field: pass C (tiled path) writes each work-group's
-    // sums of I, T, I^2, I T, T^2 (I = |x|^2, T from ls_t) to ls_part[k * G + group]
-    // and sets ls_groups = G; the untiled path leaves ls_groups 0
-    const float* ls_t;
-    double* ls_part;
-    int ls_groups;
-};
 
 // index of (row h, padded column m) in the column-blocked intermediate
 __device__ __forceinline__ size_t widx(const PropArgs& a, int h, int m) {
@@ -80,13 +41,6 @@ template <int L>
 constexpr int col_nt() { return col_cb<L>() * Plan<L>::T; }
 
 // ------------------------------------------------------------------ pass A
-// Work-groups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Neighbouring column groups share 128-byte lines of W, so the block index is
-// remapped to give every XCD a contiguous run of column groups.
-__device__ __forceinline__ int xcd_remap(int b, int nblk) {
-    if (nblk % 8) return b;
-    return (b % 8) * (nblk / 8) + b / 8;
-}
 
 template <int L>
 __global__ void __launch_bounds__(row_nt<L>())
@@ -124,36 +78,6 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
     }
 }
 
-// transfer function at spectrum row i (natural order) of a column whose
-// 1/lambda^2 - u^2 is A (FP64).  w = sqrt(A - v^2) comes from an FP32 rsqrt seed
-// and one FP64 Newton step (relative error ~1e-14); the phase w z is reduced to
-// [0, 1) cycle in FP64 and the sine / cosine are taken by the hardware unit in
-// revolutions (absolute error ~1e-7).  zs is z, negated for the adjoint (which
-// conjugates H).  Branch-free: every lane evaluates and a 0/1 factor zeroes the
-// cut and evanescent ones (a select of these values is turned into a branch by
-// the compiler); the argument of the root is |w2| + 1e-30, so every lane's
-// value is finite and the product is exact zero where masked.
-// The shifted form (SHIFT, shiftedAsmTransferF include/olprop.h:250-305) adds the
-// linear phase ox u + oy v of a destination window offset by (ox, oy) (pc = ox u
-// of the column, sign-flipped with z for the adjoint) and centres the band-limit
-// window of each axis on u0 / v0.
-template <bool SHIFT = false>
-__device__ __forceinline__ float2 asm_h(const PropArgs& a, double A, bool ucut, int i, int My,
-                                        double pc = 0.0) {
-    const int fy = i < My / 2 ? i : i - My;
-    const double v = (double)fy * a.dv;
-    const double w2 = A - v * v;
-    const double vc = SHIFT ? v - a.v0 : v;
-    const bool keep = (!ucut) & ((!a.bandlimit) | (fabs(vc) * 2 < a.vw)) & (w2 >= 0.0);
-    const double w2c = fabs(w2) + 1e-30;
-    const double y0 = (double)__builtin_amdgcn_rsqf((float)w2c);
-    const double s0 = w2c * y0;
-    const double w = fma(fma(-s0, s0, w2c), 0.5 * y0, s0);
-    const float t = SHIFT ? (float)__builtin_amdgcn_fract(fma(w, a.zs, fma(a.oy, v, pc)))
-                          : (float)__builtin_amdgcn_fract(w * a.zs);
-    const float m = keep ? 1.f : 0.f;
-    return make_float2(__builtin_amdgcn_cosf(t) * m, __builtin_amdgcn_sinf(t) * m);
-}
 
 // ------------------------------------------------------------------ pass B
 template <int L, int CB>
@@ -347,12 +271,6 @@ constexpr int kr_waves() {
     return (KR * Plan<L>::T <= 512 && 2 * KR * Plan<L>::LP * 8 <= 160 * 1024) ? 2 * KR * Plan<L>::T / 256 : 1;
 }
 
-__device__ __forceinline__ float2 loss_grad(float2 x, float t, double mi, double mt, double c1,
-                                            double c2) {
-    const double I = (double)x.x * x.x + (double)x.y * x.y;
-    const double g = c1 * (I * mi - (double)t * mt) - c2;
-    return make_float2((float)(g * x.x), (float)(g * x.y));
-}
 
 // LG: the loss-gradient prologue (PropArgs::lg_*) on the source values
 template <int L, int KR, int ORD, bool LG = false>
@@ -823,9 +741,9 @@ __global__ void k_dft_f64(float2* __restrict__ S, const double2* __restrict__ hc
     if (k >= M) return;
     double re = 0.0, im = 0.0;
     for (int n = 0; n < M; n++) {
-        const double2 h = hc[(n + M / 2) & (M - 1)];
+        const double2 h = hc[(n + M / 2) % M];
         if (h.x == 0.0 && h.y == 0.0) continue;
-        const long long nk = ((long long)n * k) & (M - 1);
+        const long long nk = ((long long)n * k) % M;
         double s, c;
         sincospi(-2.0 * (double)nk / (double)M, &s, &c);
         re += h.x * c - h.y * s;
@@ -1127,6 +1045,16 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
         k_loss_grad<<<blocks, 256, 0, s>>>(const_cast<float2*>(src), a.lg_t, n, a.lg_c);
         OLX_TRY(check_launch("loss gradient"));
     }
+    if (!is_pow2(a.nx) || !is_pow2(a.ny) || a.nx < 4 || a.ny < 4) {
+        OLX_TRY(mr_three_pass(ctx, src, dst, a, W));
+        if (a.pg_grad) {
+            const int64_t n = (int64_t)a.ny * a.nx;
+            const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+            k_phigrad<<<blocks, 256, 0, s>>>(a.pg_u, dst, n, a.pg_grad, a.pg_acc);
+            return check_launch("phase gradient");
+        }
+        return OLX_OK;
+    }
     int cb = 1;
     OLX_LEN_SWITCH(ilog2(My), { cb = col_batch<L>(); });
     if (cb > Mx) cb = (int)Mx;
@@ -1164,13 +1092,18 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
     return OLX_OK;
 }
 
+// Power-of-two sides in [4, 8192] take the fused passes; other sides in [2, 8192]
+// whose padded lengths 2n factor into 2, 3, 5, 7, 11 and 13 the mixed-radix ones.
 static int check_field(olx_ctx* ctx, const void* p, int64_t ny, int64_t nx) {
     if (!ctx) OLX_FAIL(OLX_EINVAL, "null context");
     if (!p) OLX_FAIL(OLX_EINVAL, "null field pointer");
-    if (!is_pow2(ny) || !is_pow2(nx) || ny < 4 || nx < 4 || ny > 8192 || nx > 8192)
+    if (is_pow2(ny) && is_pow2(nx) && ny >= 4 && nx >= 4 && ny <= 8192 && nx <= 8192)
+        return OLX_OK;
+    if (ny < 2 || nx < 2 || ny > 8192 || nx > 8192 || !mr_supported(2 * ny) ||
+        !mr_supported(2 * nx))
         OLX_FAIL(OLX_EUNSUPPORTED,
-                 "field %lld x %lld: sides must be powers of two in [4, 8192]", (long long)ny,
-                 (long long)nx);
+                 "field %lld x %lld: sides must be in [2, 8192] with prime factors 2, 3, 5, 7, "
+                 "11 and 13 only (powers of two from 4)", (long long)ny, (long long)nx);
     return OLX_OK;
 }
 
@@ -1525,9 +1458,13 @@ extern "C" int olx_prop_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, dou
 
 extern "C" int olx_fft2d_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, int sign) {
     if (!ctx || !d_u) OLX_FAIL(OLX_EINVAL, "fft2d: null argument");
-    if (!is_pow2(ny) || !is_pow2(nx) || ny < 8 || nx < 8 || ny > 16384 || nx > 16384)
-        OLX_FAIL(OLX_EUNSUPPORTED, "fft2d: sides must be powers of two in [8, 16384]");
     float2* u = (float2*)d_u;
+    if (!is_pow2(ny) || !is_pow2(nx) || ny < 8 || nx < 8 || ny > 16384 || nx > 16384) {
+        if (!mr_supported(ny) || !mr_supported(nx))
+            OLX_FAIL(OLX_EUNSUPPORTED, "fft2d: sides must be in [2, 16384] with prime factors 2, "
+                     "3, 5, 7, 11 and 13 only");
+        return mr_fft2d(ctx, u, ny, nx, sign > 0);
+    }
     hipStream_t s = ctx->stream;
     const bool inv = sign > 0;
     {

@@ -1,0 +1,483 @@
+// olx_prop_mr.hip -- propagation and 2-D FFT for field sides that are not powers of
+// two (the reference pads any ny x nx field to 2ny x 2nx, include/olutils.h:133-178,
+// and transforms it with FFTW / cuFFT, which are mixed radix).
+//
+// Transforms of length L whose prime factors are 2, 3, 5, 7, 11 and 13 run as a
+// runtime-planned Stockham FFT in LDS: stage s of radix R with NS = product of
+// the earlier radices takes butterfly j in [0, L/R) from x[j + r L/R], multiplies
+// by W_{NS R}^{k r} (k = j mod NS, from an FP64-built table of W_L), applies the
+// R-point DFT and writes y[(j - k) R + k + r NS], leaving natural order after the
+// last stage.  Radix 16 (remainder 8, 4, 2) takes the factors of two; odd primes
+// use the symmetric-pair DFT.  Each stage loads every butterfly a thread owns into
+// registers, waits, then writes, so one LDS image per transform suffices.
+//
+// The three propagation passes mirror the power-of-two ones over a row-major
+// intermediate W of ny x Mx values: A pads, transforms and stores rows; B loads
+// columns (ny of My values nonzero), transforms, multiplies by the transfer
+// function, transforms back and stores the centre rows; C transforms rows back,
+// crops, scales and applies the post-chirp.  Zero padding is centred at offset
+// (M - n) / 2 = n / 2, as the reference's zeropadding / del_zero.
+#include <climits>
+#include <cstring>
+
+#include "olx_common.hpp"
+#include "prop_common.hpp"
+
+namespace olx {
+
+constexpr int MR_MAXST = 20;
+
+struct MrStage {
+    int R;        // radix
+    int ns;       // product of the earlier radices
+    int lr;       // L / R (butterflies per transform)
+    int tws;      // twiddle stride L / (ns R)
+    uint32_t mns; // fast-division magic of ns (unused when ns == 1)
+    uint32_t mlr; // fast-division magic of lr
+};
+
+// a plan lives in device memory right after its length's twiddle table, so kernels
+// index its stages through a pointer (uniform scalar loads)
+struct MrPlan {
+    int L, nst;
+    const float2* tw;  // W_L^m = exp(-2 pi i m / L), m in [0, L)
+    MrStage st[MR_MAXST];
+};
+
+// n / d for n, d < 2^16 (n d < 2^32): magic m = floor(2^32 / d) + 1
+__host__ __device__ constexpr uint32_t fdiv_magic(int d) {
+    return d <= 1 ? 0u : (uint32_t)((0x100000000ull / (uint64_t)d) + 1);
+}
+__device__ __forceinline__ int fdiv(int n, int d, uint32_t m) {
+    return d == 1 ? n : (int)__umulhi((uint32_t)n, m);
+}
+
+// compile-time cos / sin of 2 pi n / R (Taylor series after reduction to
+// [-pi, pi], double precision)
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double ct_sin(double x) {
+    while (x > kPi) x -= 2 * kPi;
+    while (x < -kPi) x += 2 * kPi;
+    double term = x, sum = x;
+    for (int i = 1; i < 40; i++) {
+        term *= -x * x / ((2.0 * i) * (2.0 * i + 1));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ct_cos(double x) { return ct_sin(x + kPi / 2); }
+
+// R-point DFT in registers, natural-order output; forward e^{-2 pi i nk/R}
+template <int R, bool INV>
+__device__ __forceinline__ void dft_any(float2* v) {
+    if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
+        dft<R, INV>(v);
+    } else {
+        // odd R: a_n = x_n + x_{R-n}, b_n = x_n - x_{R-n} (n = 1 .. H), then
+        // y_k = A_k -/+ i B_k, y_{R-k} = A_k +/- i B_k with A_k = x_0 + sum a_n cos,
+        // B_k = sum b_n sin (angles 2 pi n k / R; upper signs forward)
+        constexpr int H = (R - 1) / 2;
+        float2 a[H], b[H];
+        float2 y0 = v[0];
+#pragma unroll
+        for (int n = 1; n <= H; n++) {
+            a[n - 1] = cadd(v[n], v[R - n]);
+            b[n - 1] = csub(v[n], v[R - n]);
+            y0 = cadd(y0, a[n - 1]);
+        }
+        float2 out[R];
+#pragma unroll
+        for (int k = 1; k <= H; k++) {
+            float2 A = v[0], B = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int n = 1; n <= H; n++) {
+                const float c = (float)ct_cos(2.0 * kPi * ((n * k) % R) / R);
+                const float s = (float)ct_sin(2.0 * kPi * ((n * k) % R) / R);
+                A = make_float2(fmaf(a[n - 1].x, c, A.x), fmaf(a[n - 1].y, c, A.y));
+                B = make_float2(fmaf(b[n - 1].x, s, B.x), fmaf(b[n - 1].y, s, B.y));
+            }
+            if (INV) {
+                out[k] = make_float2(A.x - B.y, A.y + B.x);
+                out[R - k] = make_float2(A.x + B.y, A.y - B.x);
+            } else {
+                out[k] = make_float2(A.x + B.y, A.y - B.x);
+                out[R - k] = make_float2(A.x - B.y, A.y + B.x);
+            }
+        }
+        v[0] = y0;
+#pragma unroll
+        for (int k = 1; k < R; k++) v[k] = out[k];
+    }
+}
+
+// one Stockham stage over nb transforms (LDS stride LP apart) of nb * lr butterflies
+template <int R, bool INV, int NT, int LCAP>
+__device__ __forceinline__ void mr_stage(float2* lds, int LP, int nb, const MrStage& S,
+                                         const float2* __restrict__ tw, int tid) {
+    constexpr int Q = (LCAP / R + NT - 1) / NT;
+    const int tot = nb * S.lr;
+    float2 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int g = tid + NT * q;
+        if (g < tot) {
+            const int tr = fdiv(g, S.lr, S.mlr), j = g - tr * S.lr;
+            const float2* p = lds + tr * LP;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = p[lpad(j + r * S.lr)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int g = tid + NT * q;
+        if (g < tot) {
+            const int tr = fdiv(g, S.lr, S.mlr), j = g - tr * S.lr;
+            const int k = j - fdiv(j, S.ns, S.mns) * S.ns;
+            if (S.ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    float2 w = tw[k * r * S.tws];
+                    if (INV) w = cconj(w);
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft_any<R, INV>(v[q]);
+            float2* p = lds + tr * LP;
+            const int o = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) p[lpad(o + r * S.ns)] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// the whole transform of nb LDS-resident sequences (natural order in and out)
+template <bool INV, int NT, int LCAP>
+__device__ void mr_run(float2* lds, int LP, int nb, const MrPlan* __restrict__ P, int tid) {
+    const int nst = P->nst;
+    const float2* tw = P->tw;
+    for (int s = 0; s < nst; s++) {
+        const MrStage S = P->st[s];
+        switch (S.R) {
+            case 2: mr_stage<2, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 3: mr_stage<3, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 4: mr_stage<4, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 5: mr_stage<5, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 7: mr_stage<7, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 8: mr_stage<8, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 11: mr_stage<11, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 13: mr_stage<13, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            default: mr_stage<16, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+        }
+    }
+}
+
+// LDS capacity of a work-group: LCAP values plus the padding of every 16th
+template <int LCAP>
+constexpr int mr_lds() { return LCAP + LCAP / 16 + 32; }
+__host__ __device__ inline int mr_lp(int L) { return L + L / 16 + 1; }
+
+// ------------------------------------------------------------------ pass A
+template <int NT, int LCAP>
+__global__ void __launch_bounds__(NT)
+k_mr_rows_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P, int nb) {
+    __shared__ float2 lds[mr_lds<LCAP>()];
+    const int tid = threadIdx.x, L = P->L, LP = mr_lp(L), nx = a.nx, off = nx / 2;
+    const int row0 = xcd_remap(blockIdx.x, gridDim.x) * nb;
+    for (int tr = 0; tr < nb; tr++) {
+        const int row = row0 + tr;
+        const float2 ay = (a.ay && row < a.ny) ? a.ay[row] : make_float2(1.f, 0.f);
+        for (int i = tid; i < L; i += NT) {
+            float2 x = make_float2(0.f, 0.f);
+            const int col = i - off;
+            if (row < a.ny && col >= 0 && col < nx) {
+                x = src[(size_t)row * a.ss + col];
+                if (a.ax) x = cmul(x, cmul(a.ax[col], ay));
+            }
+            lds[tr * LP + lpad(i)] = x;
+        }
+    }
+    __syncthreads();
+    mr_run<false, NT, LCAP>(lds, LP, nb, P, tid);
+    for (int tr = 0; tr < nb; tr++) {
+        const int row = row0 + tr;
+        if (row >= a.ny) break;
+        for (int i = tid; i < L; i += NT) W[(size_t)row * L + i] = lds[tr * LP + lpad(i)];
+    }
+}
+
+// ------------------------------------------------------------------ pass B
+template <int NT, int LCAP, int MODE>
+__global__ void __launch_bounds__(NT)
+k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P, int nb) {
+    __shared__ float2 lds[mr_lds<LCAP>()];
+    const int tid = threadIdx.x, My = P->L, LP = mr_lp(My), ny = a.ny, off = ny / 2;
+    const int Mx = 2 * a.nx;
+    const int col0 = xcd_remap(blockIdx.x, gridDim.x) * nb;
+    for (int tr = 0; tr < nb; tr++) {
+        const int col = col0 + tr;
+        for (int i = tid; i < My; i += NT) {
+            const int h = i - off;
+            float2 x = make_float2(0.f, 0.f);
+            if (col < Mx && h >= 0 && h < ny) x = W[(size_t)h * Mx + col];
+            lds[tr * LP + lpad(i)] = x;
+        }
+    }
+    __syncthreads();
+    mr_run<false, NT, LCAP>(lds, LP, nb, P, tid);
+    for (int tr = 0; tr < nb; tr++) {
+        const int col = col0 + tr;
+        if (col >= Mx) break;
+        double Acol = 0.0, pc = 0.0;
+        bool ucut = false;
+        float2 sxc = make_float2(0.f, 0.f);
+        if constexpr (MODE == K_TABLE) {
+            sxc = a.sx[col];
+        } else {
+            const int fx = col < Mx / 2 ? col : col - Mx;
+            const double u = (double)fx * a.du;
+            Acol = a.inv_l2 - u * u;
+            ucut = a.bandlimit && !(fabs(MODE == K_ASHIFT ? u - a.u0 : u) * 2 < a.uw);
+            pc = a.ox * u;
+        }
+        for (int i = tid; i < My; i += NT) {
+            float2 h;
+            if constexpr (MODE == K_TABLE) {
+                h = cmul(sxc, a.sy[i]);
+                if (a.adjoint) h = cconj(h);
+            } else if constexpr (MODE == K_ASHIFT) {
+                h = asm_h<true>(a, Acol, ucut, i, My, pc);
+            } else {
+                h = asm_h(a, Acol, ucut, i, My);
+            }
+            float2* p = &lds[tr * LP + lpad(i)];
+            *p = cmul(*p, h);
+        }
+    }
+    __syncthreads();
+    mr_run<true, NT, LCAP>(lds, LP, nb, P, tid);
+    for (int tr = 0; tr < nb; tr++) {
+        const int col = col0 + tr;
+        if (col >= Mx) break;
+        for (int h = tid; h < ny; h += NT) W[(size_t)h * Mx + col] = lds[tr * LP + lpad(h + off)];
+    }
+}
+
+// ------------------------------------------------------------------ pass C
+template <int NT, int LCAP>
+__global__ void __launch_bounds__(NT)
+k_mr_rows_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a, const MrPlan* __restrict__ P, int nb) {
+    __shared__ float2 lds[mr_lds<LCAP>()];
+    const int tid = threadIdx.x, L = P->L, LP = mr_lp(L), nx = a.nx, off = nx / 2;
+    const int row0 = xcd_remap(blockIdx.x, gridDim.x) * nb;
+    for (int tr = 0; tr < nb; tr++) {
+        const int row = row0 + tr;
+        for (int i = tid; i < L; i += NT)
+            lds[tr * LP + lpad(i)] = row < a.ny ? W[(size_t)row * L + i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    mr_run<true, NT, LCAP>(lds, LP, nb, P, tid);
+    for (int tr = 0; tr < nb; tr++) {
+        const int row = row0 + tr;
+        if (row >= a.ny) break;
+        const bool post = a.cx != nullptr && abs(row - a.ny / 2) < a.chl;
+        const float2 cy = post ? a.cy[row] : make_float2(1.f, 0.f);
+        for (int col = tid; col < nx; col += NT) {
+            float2 x = cscale(lds[tr * LP + lpad(col + off)], a.scale);
+            if (post && abs(col - nx / 2) < a.cwl) x = cmul(x, cmul(a.cx[col], cy));
+            float2* o = &dst[(size_t)row * a.ds + col];
+            if (a.accum) x = make_float2(o->x + x.x, o->y + x.y);
+            *o = x;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ plain 2-D FFT
+template <int NT, int LCAP, bool INV>
+__global__ void __launch_bounds__(NT)
+k_mr_fft_rows(float2* __restrict__ u, int rows, const MrPlan* __restrict__ P, int nb) {
+    __shared__ float2 lds[mr_lds<LCAP>()];
+    const int tid = threadIdx.x, L = P->L, LP = mr_lp(L);
+    const int row0 = blockIdx.x * nb;
+    for (int tr = 0; tr < nb; tr++) {
+        const int row = row0 + tr;
+        for (int i = tid; i < L; i += NT)
+            lds[tr * LP + lpad(i)] = row < rows ? u[(size_t)row * L + i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    mr_run<INV, NT, LCAP>(lds, LP, nb, P, tid);
+    for (int tr = 0; tr < nb; tr++) {
+        const int row = row0 + tr;
+        if (row >= rows) break;
+        for (int i = tid; i < L; i += NT) u[(size_t)row * L + i] = lds[tr * LP + lpad(i)];
+    }
+}
+
+template <int NT, int LCAP, bool INV>
+__global__ void __launch_bounds__(NT)
+k_mr_fft_cols(float2* __restrict__ u, int cols, const MrPlan* __restrict__ P, int nb) {
+    __shared__ float2 lds[mr_lds<LCAP>()];
+    const int tid = threadIdx.x, L = P->L, LP = mr_lp(L);
+    const int col0 = xcd_remap(blockIdx.x, gridDim.x) * nb;
+    // adjacent threads read adjacent columns of one row
+    for (int e = tid; e < nb * L; e += NT) {
+        const int i = e / nb, tr = e - i * nb, col = col0 + tr;
+        lds[tr * LP + lpad(i)] = col < cols ? u[(size_t)i * cols + col] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    mr_run<INV, NT, LCAP>(lds, LP, nb, P, tid);
+    for (int e = tid; e < nb * L; e += NT) {
+        const int i = e / nb, tr = e - i * nb, col = col0 + tr;
+        if (col < cols) u[(size_t)i * cols + col] = lds[tr * LP + lpad(i)];
+    }
+}
+
+// W_L^m = exp(-2 pi i m / L) in FP64, rounded once; thread 0 also stores the plan
+__global__ void k_mr_twiddle(float2* __restrict__ tw, int L, MrPlan plan, MrPlan* __restrict__ dst) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0) *dst = plan;
+    if (m >= L) return;
+    double s, c;
+    sincospi(-2.0 * (double)m / (double)L, &s, &c);
+    tw[m] = make_float2((float)c, (float)s);
+}
+
+// ------------------------------------------------------------------ host side
+// Stage radices of L: sixteens then the remaining 8 / 4 / 2, then 3, 5, 7, 11, 13.
+// Returns false when L has another prime factor or needs too many stages.
+static bool mr_factor(int L, int* radix, int* nst) {
+    int n = L, k = 0;
+    while (n % 16 == 0 && k < MR_MAXST) { radix[k++] = 16; n /= 16; }
+    for (int r : {8, 4, 2})
+        if (n % r == 0 && k < MR_MAXST) { radix[k++] = r; n /= r; }
+    for (int p : {3, 5, 7, 11, 13})
+        while (n % p == 0 && k < MR_MAXST) { radix[k++] = p; n /= p; }
+    *nst = k;
+    return n == 1 && L >= 2;
+}
+
+bool mr_supported(int64_t L) {
+    if (L < 2 || L > 16384) return false;
+    int r[MR_MAXST], k;
+    return mr_factor((int)L, r, &k);
+}
+
+// the device-resident plan of length L (built with its twiddles on first use)
+static int mr_plan(olx_ctx* ctx, int L, const MrPlan** out) {
+    int radix[MR_MAXST], nst = 0;
+    if (L > 16384 || !mr_factor(L, radix, &nst))
+        OLX_FAIL(OLX_EUNSUPPORTED, "transform length %d: prime factors must be 2, 3, 5, 7, 11, 13 "
+                 "and the length at most 16384", L);
+    char key[64];
+    snprintf(key, sizeof key, "mrplan|%d", L);
+    bool fresh = false;
+    int rc = OLX_OK;
+    constexpr size_t plan_f2 = (sizeof(MrPlan) + sizeof(float2) - 1) / sizeof(float2);
+    float2* tw = table(ctx, key, (size_t)L + plan_f2, &fresh, &rc);
+    if (!tw) return rc;
+    MrPlan* dplan = reinterpret_cast<MrPlan*>(tw + L);
+    if (fresh) {
+        MrPlan P;
+        memset(&P, 0, sizeof P);
+        P.L = L;
+        P.nst = nst;
+        P.tw = tw;
+        int ns = 1;
+        for (int s = 0; s < nst; s++) {
+            MrStage& S = P.st[s];
+            S.R = radix[s];
+            S.ns = ns;
+            S.lr = L / S.R;
+            S.tws = L / (ns * S.R);
+            S.mns = fdiv_magic(ns);
+            S.mlr = fdiv_magic(S.lr);
+            ns *= S.R;
+        }
+        k_mr_twiddle<<<(L + 255) / 256, 256, 0, ctx->stream>>>(tw, L, P, dplan);
+        OLX_TRY(check_launch("mixed-radix plan"));
+    }
+    *out = dplan;
+    return OLX_OK;
+}
+
+// transforms per work-group: as many as fill LCAP values, at most 8
+template <int LCAP>
+static int mr_nb(int L) {
+    int nb = LCAP / L;
+    return nb < 1 ? 1 : (nb > 8 ? 8 : nb);
+}
+
+// calls f(integral NT, integral LCAP) for the work-group shape of length L
+template <class F>
+static void mr_shape(int L, F&& f) {
+    if (L <= 8192) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
+    else f(std::integral_constant<int, 512>(), std::integral_constant<int, 16384>());
+}
+
+int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, float2* W) {
+    const int Mx = 2 * a.nx, My = 2 * a.ny;
+    const MrPlan *px, *py;
+    OLX_TRY(mr_plan(ctx, Mx, &px));
+    OLX_TRY(mr_plan(ctx, My, &py));
+    hipStream_t s = ctx->stream;
+    {
+        TimedLaunch tl(ctx, "prop_rows_pad_fwd");
+        mr_shape(Mx, [&](auto NT, auto LC) {
+            const int nb = mr_nb<LC.value>(Mx);
+            k_mr_rows_fwd<NT.value, LC.value><<<(a.ny + nb - 1) / nb, NT.value, 0, s>>>(src, W, a, px, nb);
+        });
+    }
+    OLX_TRY(check_launch("mixed-radix pass A"));
+    {
+        TimedLaunch tl(ctx, "prop_cols_conv");
+        mr_shape(My, [&](auto NT, auto LC) {
+            const int nb = mr_nb<LC.value>(My);
+            const int grid = (Mx + nb - 1) / nb;
+            if (a.mode == K_ASM)
+                k_mr_cols_conv<NT.value, LC.value, K_ASM><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+            else if (a.mode == K_ASHIFT)
+                k_mr_cols_conv<NT.value, LC.value, K_ASHIFT><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+            else
+                k_mr_cols_conv<NT.value, LC.value, K_TABLE><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+        });
+    }
+    OLX_TRY(check_launch("mixed-radix pass B"));
+    {
+        TimedLaunch tl(ctx, "prop_rows_crop_inv");
+        mr_shape(Mx, [&](auto NT, auto LC) {
+            const int nb = mr_nb<LC.value>(Mx);
+            k_mr_rows_inv<NT.value, LC.value><<<(a.ny + nb - 1) / nb, NT.value, 0, s>>>(W, dst, a, px, nb);
+        });
+    }
+    return check_launch("mixed-radix pass C");
+}
+
+int mr_fft2d(olx_ctx* ctx, float2* u, int64_t ny, int64_t nx, bool inv) {
+    const MrPlan *px, *py;
+    OLX_TRY(mr_plan(ctx, (int)nx, &px));
+    OLX_TRY(mr_plan(ctx, (int)ny, &py));
+    hipStream_t s = ctx->stream;
+    {
+        TimedLaunch tl(ctx, "fft_rows");
+        mr_shape((int)nx, [&](auto NT, auto LC) {
+            const int nb = mr_nb<LC.value>((int)nx);
+            const int grid = (int)((ny + nb - 1) / nb);
+            if (inv) k_mr_fft_rows<NT.value, LC.value, true><<<grid, NT.value, 0, s>>>(u, (int)ny, px, nb);
+            else k_mr_fft_rows<NT.value, LC.value, false><<<grid, NT.value, 0, s>>>(u, (int)ny, px, nb);
+        });
+    }
+    OLX_TRY(check_launch("mixed-radix fft rows"));
+    {
+        TimedLaunch tl(ctx, "fft_cols");
+        mr_shape((int)ny, [&](auto NT, auto LC) {
+            const int nb = mr_nb<LC.value>((int)ny);
+            const int grid = (int)((nx + nb - 1) / nb);
+            if (inv) k_mr_fft_cols<NT.value, LC.value, true><<<grid, NT.value, 0, s>>>(u, (int)nx, py, nb);
+            else k_mr_fft_cols<NT.value, LC.value, false><<<grid, NT.value, 0, s>>>(u, (int)nx, py, nb);
+        });
+    }
+    return check_launch("mixed-radix fft cols");
+}
+
+}  // namespace olx

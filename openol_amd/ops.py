@@ -34,7 +34,7 @@ def _field(u, height, width, name="u"):
         raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be complex64, got {u.dtype}")
     if not u.is_contiguous():
         raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be contiguous")
-    if tuple(u.shape[-2:]) != (int(height), int(width)):
+    if u.dim() != 2 or tuple(u.shape) != (int(height), int(width)):
         raise _lib.OlxInvalid(_lib.EINVAL, f"{name} shape {tuple(u.shape)} != ({height}, {width})")
     return u
 
@@ -44,8 +44,7 @@ def AsmProp(u, height, width, dy, dx, lam, z, bandlimit=False, adjoint=False):
     """ol::AsmProp (include/olprop.h:227-248): in place, net scale 1."""
     _field(u, height, width)
     flags = (_lib.BANDLIMIT if bandlimit else 0) | (_lib.ADJOINT if adjoint else 0)
-    L = _lib.load()
-    check(L.olx_asm_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z, flags),
+    check(_lib.call("olx_asm_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z, flags),
           "AsmProp")
     return u
 
@@ -69,8 +68,7 @@ def multiplane_mse(phi, targets, zs, dy, dx, lam, bandlimit=True, grad=True):
     g = torch.empty_like(phi) if grad else None
     zarr = (ctypes.c_double * len(zs))(*[float(z) for z in zs])
     tarr = (ctypes.c_void_p * len(targets))(*[t.data_ptr() for t in targets])
-    check(_lib.load().olx_multiplane_mse_c64(
-        _lib.ctx(phi.device.index), _ptr(phi), ny, nx, len(zs), ctypes.cast(zarr, ctypes.c_void_p),
+    check(_lib.call("olx_multiplane_mse_c64", phi.device.index, _ptr(phi), ny, nx, len(zs), ctypes.cast(zarr, ctypes.c_void_p),
         ctypes.cast(tarr, ctypes.c_void_p), dy, dx, lam, _lib.BANDLIMIT if bandlimit else 0,
         _ptr(loss), _ptr(g) if grad else None), "multiplane_mse")
     return loss, g
@@ -81,7 +79,7 @@ def mAsmProp(u, height, width, dy, dx, lam, z):
     transforms the reference stages through host memory, net scale 1/M^2 (it scales
     before and after the inverse transform).  The field stays in HBM here."""
     _field(u, height, width)
-    check(_lib.load().olx_asm_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z,
+    check(_lib.call("olx_asm_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
                                   _lib.EXTRA_INV_M2), "mAsmProp")
     return u
 
@@ -89,8 +87,7 @@ def mAsmProp(u, height, width, dy, dx, lam, z):
 def FresnelProp(u, height, width, dy, dx, lam, z, adjoint=False):
     """ol::FresnelProp (include/olprop.h:125-148): circconv(pad u, h) / M^2, in place."""
     _field(u, height, width)
-    L = _lib.load()
-    check(L.olx_fresnel_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z,
+    check(_lib.call("olx_fresnel_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
                             _lib.ADJOINT if adjoint else 0), "FresnelProp")
     return u
 
@@ -100,8 +97,7 @@ def FresnelPropBandLimit(u, height, width, dy, dx, lam, z, variant="cpu"):
     variant="gpu", gFresnelPropBandLimit (include/golprop.h:258-302, x-limit on both axes)."""
     _field(u, height, width)
     flags = {"cpu": _lib.FRESNEL_BL_CPU, "gpu": _lib.FRESNEL_BL_GPU}[variant]
-    L = _lib.load()
-    check(L.olx_fresnel_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx, lam, z,
+    check(_lib.call("olx_fresnel_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
                             flags), "FresnelPropBandLimit")
     return u
 
@@ -114,8 +110,7 @@ def ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lam, z, s, oy=0.0, ox=0.0):
     """ol::ARSSFresnelProp (include/golprop.h:941-995); usrc and udst may alias."""
     _field(usrc, ny, nx, "usrc")
     _field(udst, ny, nx, "udst")
-    L = _lib.load()
-    check(L.olx_arss_c64(_lib.ctx(usrc.device.index), _ptr(usrc), _ptr(udst), ny, nx, dy, dx,
+    check(_lib.call("olx_arss_c64", usrc.device.index, _ptr(usrc), _ptr(udst), ny, nx, dy, dx,
                          lam, z, s, oy, ox, 0), "ARSSFresnelProp")
     return udst
 
@@ -123,7 +118,7 @@ def ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lam, z, s, oy=0.0, ox=0.0):
 def _shifted(usrc, udst, ny, nx, dy, dx, lam, z, oy, ox, flags, name):
     _field(usrc, ny, nx, "usrc")
     _field(udst, ny, nx, "udst")
-    check(_lib.load().olx_shifted_asm_c64(_lib.ctx(usrc.device.index), _ptr(usrc), _ptr(udst), ny,
+    check(_lib.call("olx_shifted_asm_c64", usrc.device.index, _ptr(usrc), _ptr(udst), ny,
                                           nx, dy, dx, lam, z, oy, ox, flags), name)
     return udst
 
@@ -157,7 +152,7 @@ def splitAsmProp(u, height, width, dy, dx, lam, z, blocknum_y=2, blocknum_x=2):
     """ol::splitAsmProp (include/olprop.h:614-651): every block to every block through
     shiftedAsmProp, summed, in place."""
     _field(u, height, width)
-    check(_lib.load().olx_split_prop_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx,
+    check(_lib.call("olx_split_prop_c64", u.device.index, _ptr(u), height, width, dy, dx,
                                          lam, z, _lib.MODE_ASM, blocknum_y, blocknum_x,
                                          _lib.SHIFTED_CPU), "splitAsmProp")
     return u
@@ -168,7 +163,7 @@ def gsplitProp(u, height, width, dy, dx, lam, z, propmode, blocknum_y=2, blocknu
     _field(u, height, width)
     if propmode not in (ASM, FRESNEL):
         raise OlxInvalid(_lib.EINVAL, "gsplitProp: propmode must be ASM or FRESNEL")
-    check(_lib.load().olx_split_prop_c64(_lib.ctx(u.device.index), _ptr(u), height, width, dy, dx,
+    check(_lib.call("olx_split_prop_c64", u.device.index, _ptr(u), height, width, dy, dx,
                                          lam, z, propmode, blocknum_y, blocknum_x, 0), "gsplitProp")
     return u
 
@@ -180,7 +175,7 @@ def _tilted(u, ny, nx, dy, dx, lam, z, angle, inp, propmode, kind, name):
     _field(u, ny, nx)
     if propmode not in (ASM, FRESNEL):
         raise OlxInvalid(_lib.EINVAL, f"{name}: propmode must be ASM or FRESNEL")
-    check(_lib.load().olx_tilted_c64(_lib.ctx(u.device.index), _ptr(u), ny, nx, dy, dx, lam, z,
+    check(_lib.call("olx_tilted_c64", u.device.index, _ptr(u), ny, nx, dy, dx, lam, z,
                                      angle, inp, propmode, kind), name)
     return u
 
@@ -203,9 +198,8 @@ def Prop(u, ny, nx, dy, dx, lam, d, propmode=AUTO):
     """ol::Prop (include/olprop.h:656-682).  Returns the mode used (ASM, FRESNEL or
     FRESNEL_BL when AUTO falls back to the band-limited Fresnel)."""
     _field(u, ny, nx)
-    L = _lib.load()
     ch = ctypes.c_int(-1)
-    check(L.olx_prop_c64(_lib.ctx(u.device.index), _ptr(u), ny, nx, dy, dx, lam, d, propmode,
+    check(_lib.call("olx_prop_c64", u.device.index, _ptr(u), ny, nx, dy, dx, lam, d, propmode,
                          ctypes.byref(ch)), "Prop")
     return ch.value
 
@@ -223,7 +217,7 @@ def FresnelPropCheck(ny, nx, dy, dx, lam, z):
 def fft(u):
     """Unnormalised forward 2-D FFT in place (ol::fft, include/olutils.h:19-41)."""
     _field(u, u.shape[-2], u.shape[-1])
-    check(_lib.load().olx_fft2d_c64(_lib.ctx(u.device.index), _ptr(u), u.shape[-2], u.shape[-1],
+    check(_lib.call("olx_fft2d_c64", u.device.index, _ptr(u), u.shape[-2], u.shape[-1],
                                     -1), "fft")
     return u
 
@@ -231,7 +225,7 @@ def fft(u):
 def ifft(u):
     """Unnormalised backward 2-D FFT in place (ol::ifft, include/olutils.h:44-66)."""
     _field(u, u.shape[-2], u.shape[-1])
-    check(_lib.load().olx_fft2d_c64(_lib.ctx(u.device.index), _ptr(u), u.shape[-2], u.shape[-1],
+    check(_lib.call("olx_fft2d_c64", u.device.index, _ptr(u), u.shape[-2], u.shape[-1],
                                     1), "ifft")
     return u
 
@@ -318,7 +312,7 @@ def Cgh(obj, out, ny, nx, py, px, lam, exact_r=False, phase_out=None, amp_out=No
     _out2d(amp_out, ny, nx, torch.float32, "amp_out")
     flags = (_lib.CGH_EXACT_R if exact_r else 0) | (_lib.CGH_ACCUMULATE if accumulate else 0)
     nul = lambda t: None if t is None else _ptr(t)
-    check(_lib.load().olx_cgh_c64(_lib.ctx(obj.device.index), obj.ptr(), obj.size, nul(out),
+    check(_lib.call("olx_cgh_c64", obj.device.index, obj.ptr(), obj.size, nul(out),
                                   nul(phase_out), nul(amp_out), ny, nx, py, px, lam, flags), "Cgh")
     return out if out is not None else phase_out
 
@@ -326,7 +320,7 @@ def Cgh(obj, out, ny, nx, py, px, lam, exact_r=False, phase_out=None, amp_out=No
 def CghFP64(obj, out, ny, nx, py, px, lam, exact_r=False):
     """Same sum in FP64 into a complex128 tensor (the FP64 column)."""
     _out2d(out, ny, nx, torch.complex128, "out")
-    check(_lib.load().olx_cgh_c128(_lib.ctx(obj.device.index), obj.ptr(), obj.size, _ptr(out), ny,
+    check(_lib.call("olx_cgh_c128", obj.device.index, obj.ptr(), obj.size, _ptr(out), ny,
                                    nx, py, px, lam, _lib.CGH_EXACT_R if exact_r else 0), "CghFP64")
     return out
 
@@ -337,7 +331,7 @@ def encode(U, phase_out=None, amp_out=None):
         phase_out = torch.empty(U.shape, dtype=torch.float32, device=U.device)
     if amp_out is None:
         amp_out = torch.empty(U.shape, dtype=torch.float32, device=U.device)
-    check(_lib.load().olx_encode_c64(_lib.ctx(U.device.index), _ptr(U), _ptr(phase_out),
+    check(_lib.call("olx_encode_c64", U.device.index, _ptr(U), _ptr(phase_out),
                                      _ptr(amp_out), U.numel()), "encode")
     return phase_out, amp_out
 
@@ -347,7 +341,7 @@ def WRPStep1(obj, U, ny, nx, p, lam, d, plimit=None, square=False):
     """WRPStep1 (include/golcgh.h:369-389); square=True is WRPStep1_D (:471-491).
     d is the WRP plane position; plimit the pitch that sets the window."""
     _out2d(U, ny, nx, torch.complex64, "U")
-    check(_lib.load().olx_wrp_c64(_lib.ctx(U.device.index), obj.ptr(), obj.size, _ptr(U), ny, nx,
+    check(_lib.call("olx_wrp_c64", U.device.index, obj.ptr(), obj.size, _ptr(U), ny, nx,
                                   p, lam, d, p if plimit is None else plimit,
                                   _lib.WRP_SQUARE if square else 0), "WRPStep1")
     return U
@@ -385,7 +379,7 @@ def WRPMethod(obj, U, ny, nx, p, lam, zmin, s=0.9, propmode=AUTO, square=False, 
     Returns the WRP plane position used."""
     _out2d(U, ny, nx, torch.complex64, "U")
     pz = ctypes.c_double(0.0)
-    check(_lib.load().olx_wrp_method_c64(_lib.ctx(U.device.index), obj.ptr(), obj.size, _ptr(U),
+    check(_lib.call("olx_wrp_method_c64", U.device.index, obj.ptr(), obj.size, _ptr(U),
                                          ny, nx, p, lam, zmin, p if plimit is None else plimit, s,
                                          propmode, _lib.WRP_SQUARE if square else 0,
                                          ctypes.byref(pz)), "WRPMethod")
@@ -407,7 +401,7 @@ def CghLayered(obj, U, ny, nx, py, px, lam, nlayers, zmin, zmax, precision="f16"
     tile128 forces the 128 x 128 work-group tile where 256 x 128 would be used."""
     _out2d(U, ny, nx, torch.complex64, "U")
     flags = (_lib.CGH_ACCUMULATE if accumulate else 0) | (_lib.LAYERED_TILE128 if tile128 else 0)
-    check(_lib.load().olx_cgh_layered_c64(_lib.ctx(U.device.index), obj.ptr(), obj.size, _ptr(U),
+    check(_lib.call("olx_cgh_layered_c64", U.device.index, obj.ptr(), obj.size, _ptr(U),
                                           ny, nx, py, px, lam, nlayers, zmin, zmax,
                                           _PREC[precision], flags), "CghLayered")
     return U
@@ -446,7 +440,7 @@ def objinfo(obj):
     """Bounding box of a device cloud, ((xmin, ymin, zmin), (xmax, ymax, zmax))
     (ol::objinfo, include/olobject.h:164-175, without printing)."""
     mn, mx = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
-    check(_lib.load().olx_obj_bounds(_lib.ctx(obj.device.index), obj.ptr(), obj.size, mn, mx),
+    check(_lib.call("olx_obj_bounds", obj.device.index, obj.ptr(), obj.size, mn, mx),
           "objinfo")
     return tuple(mn), tuple(mx)
 
@@ -457,7 +451,7 @@ def objset(obj, x_range, y_range=None, offset=(0.0, 0.0, 0.0), mode=ZMIN, f32=Fa
     f32=True reproduces the arithmetic of an Object<float> (float coordinates,
     scale and offsets); the default that of an Object<double>."""
     m = mode | (_lib.OBJSET_F32 if f32 else 0)
-    check(_lib.load().olx_obj_set(_lib.ctx(obj.device.index), obj.ptr(), obj.size, x_range,
+    check(_lib.call("olx_obj_set", obj.device.index, obj.ptr(), obj.size, x_range,
                                   -1.0 if y_range is None else y_range, *offset, m), "objset")
     obj._touched()
     return obj
@@ -466,7 +460,7 @@ def objset(obj, x_range, y_range=None, offset=(0.0, 0.0, 0.0), mode=ZMIN, f32=Fa
 def objsort(obj):
     """ol::objsort (include/olobject.h:177-182) in place on the device: descending z,
     equal depths in input order."""
-    check(_lib.load().olx_obj_sort_z(_lib.ctx(obj.device.index), obj.ptr(), obj.size), "objsort")
+    check(_lib.call("olx_obj_sort_z", obj.device.index, obj.ptr(), obj.size), "objsort")
     obj._touched()
     return obj
 
@@ -502,7 +496,7 @@ def phase8bit(phase, img=None, offset=False):
     x86 build converts it; offset=True maps round((phi + pi) / 2pi * 255)."""
     f = _typed(phase, _REAL_T, "phase")
     img = _img_like(phase, img)
-    check(_lib.load().olx_phase8bit(_lib.ctx(phase.device.index), _ptr(phase), _ptr(img),
+    check(_lib.call("olx_phase8bit", phase.device.index, _ptr(phase), _ptr(img),
                                     phase.numel(), f | (_lib.PHASE8_OFFSET if offset else 0)),
           "phase8bit")
     return img
@@ -512,7 +506,7 @@ def quant8bit(real, img=None):
     """ol::quant8bit (include/olimg.h:178-199): min/max scaled to 0..255."""
     f = _typed(real, _REAL_T, "real")
     img = _img_like(real, img)
-    check(_lib.load().olx_quant8bit(_lib.ctx(real.device.index), _ptr(real), _ptr(img), real.numel(),
+    check(_lib.call("olx_quant8bit", real.device.index, _ptr(real), _ptr(img), real.numel(),
                                     f), "quant8bit")
     return img
 
@@ -522,7 +516,7 @@ def complex2img(u, img=None, mode="phase"):
     field; mode "phase", "real", "imag", "amp" or "real_legacy" (the bool overload)."""
     f = _typed(u, _COMPLEX_T, "u")
     img = _img_like(u, img)
-    check(_lib.load().olx_complex2img(_lib.ctx(u.device.index), _ptr(u), _ptr(img), u.numel(),
+    check(_lib.call("olx_complex2img", u.device.index, _ptr(u), _ptr(img), u.numel(),
                                       _IMGMODE[mode] | f), "complex2img")
     return img
 
@@ -535,7 +529,7 @@ def complex2real(u, out=None, mode="phase"):
         out = torch.empty(u.shape, dtype=rt, device=u.device)
     if out.dtype != rt or out.numel() != u.numel() or not out.is_contiguous():
         raise OlxInvalid(_lib.EINVAL, f"out must be a contiguous {rt} tensor of u's size")
-    check(_lib.load().olx_complex2real(_lib.ctx(u.device.index), _ptr(u), _ptr(out), u.numel(),
+    check(_lib.call("olx_complex2real", u.device.index, _ptr(u), _ptr(out), u.numel(),
                                        _IMGMODE[mode] | f), "complex2real")
     return out
 
@@ -576,7 +570,7 @@ def uniform(n, seed=1, device=None):
     """n uniform floats in (0, 1] on the device (Philox4x32-10 stream keyed by seed)."""
     dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
     out = torch.empty(n, dtype=torch.float32, device=dev)
-    check(_lib.load().olx_uniform_f32(_lib.ctx(dev.index), _ptr(out), n, seed), "uniform")
+    check(_lib.call("olx_uniform_f32", dev.index, _ptr(out), n, seed), "uniform")
     return out
 
 
@@ -590,7 +584,7 @@ class Diffuser:
         """The CPU form (include/Diffuser.h:191-215): the std::mt19937(seed) stream in
         pixel order, phases range (v - 0.5) (PLAY) or their negative (RECORD)."""
         _field(u, self.height, self.width)
-        check(_lib.load().olx_diffuser_random_phase_c64(_lib.ctx(u.device.index), _ptr(u), self.height,
+        check(_lib.call("olx_diffuser_random_phase_c64", u.device.index, _ptr(u), self.height,
                                                         self.width, step, seed, range, oy, ox),
               "random_phase")
         return u
@@ -599,7 +593,7 @@ class Diffuser:
         _field(u, self.height, self.width)
         if b.dtype != torch.float32 or not b.is_contiguous():
             raise OlxInvalid(_lib.EINVAL, "uniforms must be a contiguous float32 tensor")
-        check(_lib.load().olx_diffuser_apply_c64(_lib.ctx(u.device.index), _ptr(u), self.height,
+        check(_lib.call("olx_diffuser_apply_c64", u.device.index, _ptr(u), self.height,
                                                  self.width, _ptr(b), scalar, _DIFFUSER_KIND[kind],
                                                  blocksize), "diffuser")
         return u
@@ -648,7 +642,7 @@ class Lensarry:
 
     def _run(self, u, step, oy, ox, p_limit, write):
         flags = (_lib.LENS_CIRCLE if self.circle else 0) | (_lib.LENS_WRITE if write else 0)
-        check(_lib.load().olx_lens_array_c64(_lib.ctx(u.device.index), _ptr(u), self.height, self.width,
+        check(_lib.call("olx_lens_array_c64", u.device.index, _ptr(u), self.height, self.width,
                                              self.pitch, self.blockwidth, self.f, self.lam, step, oy,
                                              ox, p_limit, flags), "Lensarry")
         return u

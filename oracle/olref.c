@@ -27,9 +27,15 @@ static int64_t win_lim(double v) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* FFT: radix-2 iterative (power-of-two lengths) or direct DFT (other lengths). */
+/* FFT: radix-2 iterative (power-of-two lengths) or recursive mixed radix by    */
+/* the smallest prime factor (other lengths; prime lengths by a direct DFT).    */
 /* Twiddles are computed in double and rounded to the working precision.       */
 /* ------------------------------------------------------------------------- */
+static int64_t small_factor(int64_t n) {
+    for (int64_t p = 2; p * p <= n && p <= 256; p++)
+        if (n % p == 0) return p;
+    return n;
+}
 #define DEFINE_FFT(SUF, CT, RT, COSF, SINF)                                         \
 static CT* twiddles_##SUF(int64_t n, int sign) {                                   \
     CT* tw = (CT*)malloc(sizeof(CT) * (size_t)(n > 1 ? n : 1));                     \
@@ -39,14 +45,38 @@ static CT* twiddles_##SUF(int64_t n, int sign) {                                
     }                                                                               \
     return tw;                                                                      \
 }                                                                                   \
+/* out[k] = sum_j in[j s] W_n^{jk}, W_n^x = tw[x st] (tw holds W_N, st = N / n).  \
+   Decimation in time by p, the smallest prime factor of n = p m: the p          \
+   subsequences in[(r + p j) s] transform into out[r m .. r m + m), then          \
+   X[k + m q] = sum_r W_p^{rq} (W_n^{rk} Y_r[k]) reads and writes the same set    \
+   {k + m r} for each k.  A prime n is a direct DFT. */                          \
+static void mrfft_##SUF(const CT* in, int64_t s, CT* out, int64_t n, const CT* tw,  \
+                        int64_t st) {                                               \
+    if (n == 1) { out[0] = in[0]; return; }                                         \
+    const int64_t p = small_factor(n), m = n / p;                                   \
+    if (p == n) {                                                                   \
+        for (int64_t k = 0; k < n; k++) {                                           \
+            CT acc = 0;                                                             \
+            for (int64_t j = 0; j < n; j++) acc += in[j * s] * tw[((j * k) % n) * st]; \
+            out[k] = acc;                                                           \
+        }                                                                           \
+        return;                                                                     \
+    }                                                                               \
+    for (int64_t r = 0; r < p; r++) mrfft_##SUF(in + r * s, s * p, out + r * m, m, tw, st * p); \
+    CT t[256];                                                                      \
+    for (int64_t k = 0; k < m; k++) {                                               \
+        for (int64_t r = 0; r < p; r++) t[r] = out[r * m + k] * tw[r * k * st];     \
+        for (int64_t q = 0; q < p; q++) {                                           \
+            CT acc = 0;                                                             \
+            for (int64_t r = 0; r < p; r++) acc += t[r] * tw[((r * q) % p) * m * st]; \
+            out[k + m * q] = acc;                                                   \
+        }                                                                           \
+    }                                                                               \
+}                                                                                   \
 static void fft1d_##SUF(CT* a, int64_t n, const CT* tw, CT* tmp) {                 \
     if (n <= 1) return;                                                             \
-    if (!is_pow2(n)) { /* direct DFT, exact index reduction */                      \
-        for (int64_t k = 0; k < n; k++) {                                           \
-            CT s = 0;                                                               \
-            for (int64_t j = 0; j < n; j++) s += a[j] * tw[(j * k) % n];            \
-            tmp[k] = s;                                                             \
-        }                                                                           \
+    if (!is_pow2(n)) {                                                              \
+        mrfft_##SUF(a, 1, tmp, n, tw, 1);                                           \
         memcpy(a, tmp, sizeof(CT) * (size_t)n);                                     \
         return;                                                                     \
     }                                                                               \

@@ -24,7 +24,8 @@ def rel(a, b):
 
 
 # --------------------------------------------------------------------------- FFT
-@pytest.mark.parametrize("shape", [(8, 16), (16, 16), (6, 10), (32, 4)])
+@pytest.mark.parametrize("shape", [(8, 16), (16, 16), (6, 10), (32, 4), (12, 18), (30, 15),
+                                   (27, 20), (7, 11), (26, 13), (49, 24)])
 def test_fft_matches_dft_matrices(shape):
     ny, nx = shape
     u = synth.random_field(ny, nx, seed=3, dtype=np.complex128)
@@ -35,6 +36,19 @@ def test_fft_matches_dft_matrices(shape):
     assert rel(olref.fft2(u.astype(np.complex64), -1), ref) < 2e-6
     refb = np.conj(Fy) @ u @ np.conj(Fx).T
     assert rel(olref.fft2(u, +1), refb) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(270, 3840), (2160, 96), (4320, 30), (64, 7680), (1250, 486),
+                                   (1001, 17)])
+def test_fft_mixed_radix_matches_pocketfft(shape):
+    """Non-power-of-two lengths (the reference's 1920 x 1080 and 3840 x 2160 demo
+    fields pad to 3840 x 2160 and 7680 x 4320: 2^8 3 5, 2^4 3^3 5, 2^9 3 5, 2^5 3^3 5)
+    and prime factors up to 17, against numpy's pocketfft, an independent mixed-radix
+    implementation."""
+    u = synth.random_field(*shape, seed=5, dtype=np.complex128)
+    assert rel(olref.fft2(u, -1), np.fft.fft2(u)) < 1e-13
+    assert rel(olref.fft2(u, +1), np.fft.ifft2(u) * u.size) < 1e-13
+    assert rel(olref.fft2(u.astype(np.complex64), -1), np.fft.fft2(u)) < 3e-6
 
 
 def test_fft_parseval_and_inverse():
