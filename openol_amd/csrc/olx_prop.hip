@@ -1004,6 +1004,22 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
         return e && strcmp(e, "0") == 0;
     }();
     hipStream_t s = ctx->stream;
+    if (!is_pow2(a.nx) || !is_pow2(a.ny) || a.nx < 4 || a.ny < 4) {
+        // mixed-radix passes; the loss-gradient prologue and the phase-gradient
+        // epilogue run as kernels of their own, as on the untiled path
+        const int64_t n = (int64_t)a.ny * a.nx;
+        const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+        if (a.lg_t) {
+            k_loss_grad<<<blocks, 256, 0, s>>>(const_cast<float2*>(src), a.lg_t, n, a.lg_c);
+            OLX_TRY(check_launch("loss gradient"));
+        }
+        OLX_TRY(mr_three_pass(ctx, src, dst, a, W));
+        if (a.pg_grad) {
+            k_phigrad<<<blocks, 256, 0, s>>>(a.pg_u, dst, n, a.pg_grad, a.pg_acc);
+            return check_launch("phase gradient");
+        }
+        return OLX_OK;
+    }
     if (!pair_off && Mx >= PAIR_MIN && My >= PAIR_MIN) {
         TileCfg cfg = tile_cfg((int)Mx, (int)My);
         a.cbl = cfg.cbl;
@@ -1044,16 +1060,6 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
         const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
         k_loss_grad<<<blocks, 256, 0, s>>>(const_cast<float2*>(src), a.lg_t, n, a.lg_c);
         OLX_TRY(check_launch("loss gradient"));
-    }
-    if (!is_pow2(a.nx) || !is_pow2(a.ny) || a.nx < 4 || a.ny < 4) {
-        OLX_TRY(mr_three_pass(ctx, src, dst, a, W));
-        if (a.pg_grad) {
-            const int64_t n = (int64_t)a.ny * a.nx;
-            const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
-            k_phigrad<<<blocks, 256, 0, s>>>(a.pg_u, dst, n, a.pg_grad, a.pg_acc);
-            return check_launch("phase gradient");
-        }
-        return OLX_OK;
     }
     int cb = 1;
     OLX_LEN_SWITCH(ilog2(My), { cb = col_batch<L>(); });

@@ -187,11 +187,36 @@ int main(int argc, char** argv) {
         std::cout << "error: " << e.what() << "\n";
         return 1;
     }
-    // errors are exceptions, not exit(1)
+    // tests/Prop_test.cpp:46 unchanged, with its tests/para.h:8-18 parameters: the
+    // reference's 3840 x 2160 demo field (padded 7680 x 4320, mixed radix)
+    {
+        std::ifstream probe(dir + "/field4k.bin");
+        if (probe) {
+            typedef float PREC_T;
+            const int WIDTH = 1920 * 2;
+            const int HEIGHT = 1080 * 2;
+            PREC_T p = 3.74e-6;
+            PREC_T lambda = 532e-9;
+            PREC_T d = -0.1 + 0;
+            const auto f4 = load<std::complex<float>>(dir + "/field4k.bin", (size_t)WIDTH * HEIGHT);
+            std::unique_ptr<std::complex<PREC_T>[]> u;
+            u = std::make_unique<std::complex<PREC_T>[]>(WIDTH * HEIGHT);
+            for (size_t i = 0; i < (size_t)WIDTH * HEIGHT; i++) u[i] = f4[i];
+            try {
+                ol::FresnelProp(u,HEIGHT,WIDTH,p,p,lambda,-d);
+            } catch (const std::exception& e) {
+                std::cout << "error: " << e.what() << "\n";
+                return 1;
+            }
+            save(dir + "/fresnel4k.bin", u.get(), (size_t)WIDTH * HEIGHT);
+        }
+    }
+    // errors are exceptions, not exit(1): a side whose padded length 2 x 17 has a
+    // prime factor above 13 is refused
     try {
-        auto bad = std::make_unique<std::complex<float>[]>(12 * 12);
-        ol::AsmProp(bad, 12, 12, p, p, lambda, 0.05);
-        std::cout << "no exception for a 12x12 field\n";
+        auto bad = std::make_unique<std::complex<float>[]>(17 * 17);
+        ol::AsmProp(bad, 17, 17, p, p, lambda, 0.05);
+        std::cout << "no exception for a 17x17 field\n";
         return 1;
     } catch (const std::runtime_error& e) {
         std::cout << "caught: " << e.what() << "\n";

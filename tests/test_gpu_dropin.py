@@ -25,12 +25,18 @@ def test_dropin_program_matches_oracle(tmp_path):
     u0.tofile(tmp_path / "field.bin")
     xyz.astype(np.float64).tofile(tmp_path / "points.bin")
     (tmp_path / "params.txt").write_text(f"{n} {m}\n")
+    u4k = synth.random_field(2160, 3840, seed=82)
+    u4k.tofile(tmp_path / "field4k.bin")
     r = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "done" in r.stdout and "caught:" in r.stdout
     assert "allocation changed" not in r.stdout
     rd = lambda name, dt=np.complex64, k=1: np.fromfile(tmp_path / f"{name}.bin", dtype=dt).reshape(k * n, k * n)
     rel = olref.rel_l2
+    # tests/Prop_test.cpp:46: FresnelProp(u, 2160, 3840, p, p, lambda, -d), float parameters
+    f4k = np.fromfile(tmp_path / "fresnel4k.bin", dtype=np.complex64).reshape(2160, 3840)
+    pf, lf, zf = (float(np.float32(v)) for v in (3.74e-6, 532e-9, 0.1))
+    assert rel(f4k, olref.fresnel_prop(u4k, pf, pf, lf, zf)) < 1e-5
     assert rel(rd("asm"), olref.asm_prop(u0, P, P, LAM, 0.05)) < 1e-5
     assert rel(rd("asm_c128", np.complex128), olref.asm_prop(u0.astype(np.complex128), P, P, LAM, 0.05)) < 1e-5
     assert rel(rd("fresnel"), olref.fresnel_prop(u0, P, P, LAM, 0.2)) < 1e-5

@@ -273,9 +273,9 @@ def test_prop_dispatcher():
 
 def test_invalid_arguments_raise():
     from openol_amd._lib import OlxError, OlxInvalid
-    u = dev(synth.random_field(12, 16, seed=1))
+    u = dev(synth.random_field(17, 16, seed=1))
     with pytest.raises(OlxError):
-        ol().AsmProp(u, 12, 16, 8e-6, 8e-6, LAM, 0.1)  # not a power of two
+        ol().AsmProp(u, 17, 16, 8e-6, 8e-6, LAM, 0.1)  # padded 34 = 2 x 17: prime above 13
     u = dev(synth.random_field(16, 16, seed=1))
     with pytest.raises(OlxInvalid):
         ol().AsmProp(u, 16, 16, 8e-6, 8e-6, -LAM, 0.1)

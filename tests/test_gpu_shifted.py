@@ -105,4 +105,4 @@ def test_split_errors():
     with pytest.raises(Exception):
         ol.gsplitProp(u, 64, 64, P, P, LAM, 0.01, ol.AUTO, 2, 2)
     with pytest.raises(Exception):
-        ol.gsplitProp(u, 64, 64, P, P, LAM, 0.01, ol.ASM, 32, 32)   # 2 x 2 blocks: too small
+        ol.gsplitProp(u, 64, 64, P, P, LAM, 0.01, ol.ASM, 64, 64)   # 1 x 1 blocks: too small
