@@ -107,6 +107,16 @@ int olx_dev_memset(olx_ctx* ctx, void* d_ptr, int value, size_t bytes);
  * Replaces ol::AsmProp (include/olprop.h:227-248) and gAsmProp (include/golprop.h:506-534). */
 int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
                 double lambda, double z, uint32_t flags);
+/* olx_asm_c64 without any device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, flags and results as
+ * olx_asm_c64; the padded intermediate (ny x 2nx complex64) lives in a static
+ * 64 MiB device array per GPU, set up by olx_ctx_create, so ny * nx <= 2048 * 2048
+ * (larger fields fail with OLX_EINVAL before any work is queued).  Every call on one
+ * GPU shares that array, across contexts and streams: calls that overlap on one
+ * GPU race and must be ordered by the caller. */
+int olx_asm_c64_noalloc(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
+                        double lambda, double z, uint32_t flags);
 /* FresnelProp / FresnelPropBandLimit: circconv(pad u, h)/M^2, crop.  Replaces
  * ol::FresnelProp (include/olprop.h:125-148), FresnelPropBandLimit (:150-170),
  * gFresnelProp (include/golprop.h:167-198), gFresnelPropBandLimit (:258-302). */

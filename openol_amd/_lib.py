@@ -65,6 +65,7 @@ SIGNATURES = {
     "olx_copy_d2h": [_p, _p, _p, ctypes.c_size_t],
     "olx_dev_memset": [_p, _p, _i, ctypes.c_size_t],
     "olx_asm_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_asm_c64_noalloc": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_fresnel_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_arss_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _u32],
     "olx_shifted_asm_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _u32],

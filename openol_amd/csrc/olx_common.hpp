@@ -122,6 +122,10 @@ bool mr_supported(int64_t L);
 int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, float2* W);
 int mr_fft2d(olx_ctx* ctx, float2* u, int64_t ny, int64_t nx, bool inv);
 
+// fetches (once per device, with the device current) the device address of the
+// static workspace of olx_asm_c64_noalloc (olx_prop.hip)
+int noalloc_init(int device);
+
 // olx_asm_c64 from src into dst (which may alias src) (olx_prop.hip)
 int asm_prop(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx, double dy,
              double dx, double lambda, double z, uint32_t flags);

@@ -156,6 +156,11 @@ extern "C" int olx_ctx_create(olx_ctx** out, int device) {
         delete ctx;
         OLX_FAIL(OLX_ENOMEM, "twiddle table allocation failed");
     }
+    if (int rc = noalloc_init(device); rc != OLX_OK) {
+        (void)hipFree(ctx->twiddle);
+        delete ctx;
+        return rc;
+    }
     k_twiddle<<<TWN / 256, 256>>>(ctx->twiddle, TWN);
     if (hipDeviceSynchronize() != hipSuccess) {
         (void)hipFree(ctx->twiddle);

@@ -16,6 +16,7 @@
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "fft_core.hpp"
@@ -988,10 +989,16 @@ __global__ void k_phigrad(const float2* __restrict__ u, const float2* __restrict
     }
 }
 
-static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a) {
+// Wext: a caller-provided intermediate of at least ny x 2nx complex64 values in
+// place of the context's workspace (which is then neither grown nor touched)
+static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a,
+                          float2* Wext = nullptr) {
     const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
-    OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
-    float2* W = (float2*)ctx->work.p;
+    float2* W = Wext;
+    if (!W) {
+        OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
+        W = (float2*)ctx->work.p;
+    }
     a.scale = (float)(a.extra_scale / ((double)Mx * (double)My));
     // W layout: row-major (the default), or blocked by the column batch when
     // OLX_WLAYOUT=blocked (contiguous column pass, scattered row passes)
@@ -1196,6 +1203,48 @@ int olx::asm_phase_grad(olx_ctx* ctx, void* d_g, const void* d_u, float* d_grad,
 extern "C" int olx_asm_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
                            double dx, double lambda, double z, uint32_t flags) {
     return asm_prop(ctx, d_u, d_u, ny, nx, dy, dx, lambda, z, flags);
+}
+
+// ---------------------------------------------------------------- no-allocation ASM
+// The intermediate W of olx_asm_c64_noalloc lives in this file-scope device array
+// (one copy per GPU, held until the process ends).  Host code never passes the
+// array's name to a kernel: its device address is fetched once per device by
+// noalloc_init (called from olx_ctx_create with the device current) and kept in
+// g_noalloc_ptr.
+constexpr size_t NOALLOC_BYTES = (size_t)64 << 20;  // ny * 2nx * 8 bytes: up to 2048 x 2048
+static __device__ __align__(256) float2 scratch[NOALLOC_BYTES / sizeof(float2)];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+int olx::noalloc_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        OLX_FAIL(OLX_EUNSUPPORTED, "no-allocation workspace: device %d beyond %d", device,
+                 NOALLOC_MAX_DEVICES);
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    if (g_noalloc_ptr[device]) return OLX_OK;
+    void* p = nullptr;
+    OLX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) OLX_FAIL(OLX_EDEVICE, "no-allocation workspace: null symbol address on device %d", device);
+    g_noalloc_ptr[device] = p;
+    return OLX_OK;
+}
+
+extern "C" int olx_asm_c64_noalloc(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
+                                   double dx, double lambda, double z, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_u, ny, nx));
+    const size_t need = sizeof(float2) * (size_t)ny * 2 * (size_t)nx;
+    if (need > NOALLOC_BYTES)
+        OLX_FAIL(OLX_EINVAL, "asm noalloc: %lld x %lld needs %zu workspace bytes, above the "
+                 "static %zu (at most 2048 x 2048 elements)", (long long)ny, (long long)nx, need,
+                 NOALLOC_BYTES);
+    void* w = ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES ? g_noalloc_ptr[ctx->device]
+                                                                   : nullptr;
+    if (!w) OLX_FAIL(OLX_EDEVICE, "asm noalloc: workspace address of device %d not set up",
+                     ctx->device);
+    PropArgs a;
+    OLX_TRY(asm_args(a, ny, nx, dy, dx, lambda, z, flags));
+    return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a, (float2*)w);
 }
 
 extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
