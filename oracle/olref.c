@@ -1034,18 +1034,55 @@ int olref_cgh_pixels_c128(const double* x, const double* y, const double* z,
 /* accumulation into U) and WRPStep1Pre_D / WRPStep1_D :391-468 (square,     */
 /* PREC_T accumulation).                                                     */
 /* ------------------------------------------------------------------------- */
+/* per-point window of WRPStep1Pre (golcgh.h:291-299): W_j = |z_j - z_wrp| /          */
+/* sqrt(4 (p / lambda)^2 - 1), times sqrt(2) for the square window (WRPStep1Pre_D)    */
+static double* wrp_windows(const double* z, int64_t n, double lambda, double plane_z,
+                           double p_limit, unsigned flags) {
+    double* W = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double rt2 = (double)sqrtf(2.0f);
+    for (int64_t j = 0; j < n; j++) {
+        double Wj = 1 / (sqrt(4 * pow((p_limit / lambda), 2) - 1)) * fabs(z[j] - plane_z);
+        W[j] = (flags & OLREF_WRP_SQUARE) ? Wj * rt2 : Wj;
+    }
+    return W;
+}
+
+/* the WRP sum at the listed pixels only, FP64 accumulation (sampled check at
+ * configuration size); the window tests and terms are those of olref_wrp_* */
+int olref_wrp_pixels_c128(const double* x, const double* y, const double* z, int64_t n,
+                          const int64_t* ph, const int64_t* pw, int64_t npix, double* out,
+                          double p, double lambda, double plane_z, double p_limit,
+                          unsigned flags) {
+    if (!x || !y || !z || !ph || !pw || !out || n < 0 || npix < 0) return -1;
+    cd* o = (cd*)out;
+    double* W = wrp_windows(z, n, lambda, plane_z, p_limit, flags);
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < npix; i++) {
+        cd acc = 0;
+        const int64_t h = ph[i], w = pw[i];
+        for (int64_t j = 0; j < n; j++) {
+            double dxv = p * w - x[j], dyv = p * h - y[j], dz = z[j] - plane_z;
+            double tmp = dxv * dxv + dyv * dyv;
+            int in = (flags & OLREF_WRP_SQUARE) ? (fabs(dxv) * 2 <= W[j] && fabs(dyv) * 2 <= W[j])
+                                                : (tmp < W[j] * W[j]);
+            if (in) {
+                double th = 2 * M_PI * tmp / (2 * dz * lambda);
+                acc += (cos(th) + I * sin(th)) / fabs(dz);
+            }
+        }
+        o[i] = acc;
+    }
+    free(W);
+    return 0;
+}
+
 #define DEFINE_WRP(SUF, CT, RT)                                                     \
 int olref_wrp_##SUF(const double* x, const double* y, const double* z, int64_t n,   \
                     RT* Uin, int64_t ny, int64_t nx, double p, double lambda,       \
                     double plane_z, double p_limit, unsigned flags) {               \
     if (!x || !y || !z || !Uin || n < 0 || ny <= 0 || nx <= 0) return -1;           \
     CT* U = (CT*)Uin;                                                               \
-    double* W = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));          \
-    double rt2 = (double)sqrtf(2.0f);                                               \
-    for (int64_t j = 0; j < n; j++) {                                               \
-        double Wj = 1 / (sqrt(4 * pow((p_limit / lambda), 2) - 1)) * fabs(z[j] - plane_z);\
-        W[j] = (flags & OLREF_WRP_SQUARE) ? Wj * rt2 : Wj;                          \
-    }                                                                               \
+    double* W = wrp_windows(z, n, lambda, plane_z, p_limit, flags);                 \
     _Pragma("omp parallel for schedule(static) collapse(2)")                        \
     for (int64_t h = 0; h < ny; h++)                                                \
         for (int64_t w = 0; w < nx; w++) {                                          \

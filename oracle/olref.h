@@ -144,6 +144,11 @@ int olref_wrp_c128(const double* x, const double* y, const double* z, int64_t n,
                    double* U, int64_t ny, int64_t nx, double p, double lambda,
                    double plane_z, double p_limit, unsigned flags);
 /* WRPMethod(_D) plane position rule, golcgh.h:519-581 (returns float value) */
+/* the c128 WRP sum at the listed pixels (ph[i], pw[i]) only, into out (complex128) */
+int olref_wrp_pixels_c128(const double* x, const double* y, const double* z, int64_t n,
+                          const int64_t* ph, const int64_t* pw, int64_t npix, double* out,
+                          double p, double lambda, double plane_z, double p_limit,
+                          unsigned flags);
 double olref_wrp_plane_z(int64_t ny, int64_t nx, double p, double lambda,
                          double zmin, double p_limit, float s, int square);
 

@@ -73,6 +73,7 @@ def lib():
             "olref_cgh_pixels_c128": [p, p, p, p, p, i64, p, p, i64, p, d, d, d, u32],
             "olref_wrp_c64": [p, p, p, i64, p, i64, i64, d, d, d, d, u32],
             "olref_wrp_c128": [p, p, p, i64, p, i64, i64, d, d, d, d, u32],
+            "olref_wrp_pixels_c128": [p, p, p, i64, p, p, i64, p, d, d, d, d, u32],
             "olref_wrp_plane_z": [i64, i64, d, d, d, d, ctypes.c_float, ctypes.c_int],
         }
         for name, args in sig.items():
@@ -298,6 +299,19 @@ def wrp(x, y, z, ny, nx, p, lam, plane_z, p_limit=None, square=False, dtype=np.c
     _check(fn(_ptr(x), _ptr(y), _ptr(z), len(x), _ptr(U), ny, nx, p, lam, plane_z,
               p if p_limit is None else p_limit, WRP_SQUARE if square else 0), "wrp")
     return U
+
+
+def wrp_pixels(x, y, z, ph, pw, p, lam, plane_z, p_limit=None, square=False):
+    """The FP64 WRP sum (WRPStep1 / WRPStep1_D windows) at the listed pixels only."""
+    x, y, z, _, _ = _pts(x, y, z)
+    ph = np.ascontiguousarray(ph, dtype=np.int64)
+    pw = np.ascontiguousarray(pw, dtype=np.int64)
+    out = np.empty(len(ph), dtype=np.complex128)
+    _check(lib().olref_wrp_pixels_c128(_ptr(x), _ptr(y), _ptr(z), len(x), _ptr(ph), _ptr(pw),
+                                       len(ph), _ptr(out), p, lam, plane_z,
+                                       p if p_limit is None else p_limit,
+                                       WRP_SQUARE if square else 0), "wrp_pixels")
+    return out
 
 
 def wrp_plane_z(ny, nx, p, lam, zmin, s=0.9, p_limit=None, square=False):

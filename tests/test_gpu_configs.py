@@ -1,0 +1,156 @@
+"""GPU parity at the full sizes of the WRP + ARSS, layered-CGH and multi-plane
+configurations (the same seeded inputs the bench measures).
+
+* WRP + ARSS: 200,000 points (x, y uniform over the 4096^2 x 8 um aperture, z in
+  [0.151, 0.155) m) accumulated into a 4096^2 WRP at 0.15 m (WRPStep1,
+  include/golcgh.h:291-378), then ARSS with s = 0.5 to the hologram plane
+  (include/golprop.h:941-995).  WRP: relative L2 <= 1e-5 against the FP64
+  per-pixel oracle on 4096 seeded pixels plus four 32 x 32 windows (corners,
+  centre); ARSS: <= 1e-5 against the oracle's complex64 and FP64 restatements of
+  the same WRP plane.
+* Layered CGH: 1,000,000 points in 64 layers over z in [0.10, 0.20) m, 4096^2
+  hologram; FP16 and split-BF16 factors <= 1e-3 relative L2 against the FP64
+  oracle on 4096 seeded pixels of the layer-snapped cloud (plain BF16 <= 6e-3,
+  its 8-bit mantissa); the exact direct FP32 kernel on the snapped cloud <= 1e-4.
+* Multi-plane: 2048^2 phase, planes at 0.10 / 0.12 / 0.15 m, mean-normalised
+  loss: the native gradient against central finite differences on 64 seeded
+  pixels, the loss evaluated in FP64 from the propagated fields.
+"""
+import numpy as np
+import pytest
+import torch
+
+import olref
+from openol_amd import synth
+
+pytestmark = pytest.mark.gpu
+LAM = 532e-9
+P = 8e-6
+
+
+def ol():
+    import openol_amd
+    return openol_amd
+
+
+def f32(a):
+    return np.asarray(a).astype(np.float32).astype(np.float64)
+
+
+def sample_pixels(n, seed, count=4096, windows=()):
+    rng = np.random.default_rng(seed)
+    ph, pw = [rng.integers(0, n, count)], [rng.integers(0, n, count)]
+    for h0, w0, s in windows:
+        hh, ww = np.meshgrid(np.arange(h0, h0 + s), np.arange(w0, w0 + s), indexing="ij")
+        ph.append(hh.ravel())
+        pw.append(ww.ravel())
+    return np.concatenate(ph), np.concatenate(pw)
+
+
+def test_wrp_200k_4096_then_arss():
+    N, n = 4096, 200_000
+    pts = synth.random_points(n, seed=3, xmax=N * P, ymax=N * P, zmin=0.151, zmax=0.155)
+    obj = ol().Object(pts["x"], pts["y"], pts["z"])
+    W = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    ol().WRPStep1(obj, W, N, N, P, LAM, 0.15)
+    W2 = torch.empty_like(W)
+    ol().WRPStep1(obj, W2, N, N, P, LAM, 0.15)
+    assert torch.equal(W, W2), "WRP summation order is deterministic"
+    del W2
+    wins = [(0, 0, 32), (0, N - 32, 32), (N - 32, 0, 32), (2032, 2032, 32)]
+    ph, pw = sample_pixels(N, 31, windows=wins)
+    ref = olref.wrp_pixels(pts["x"], pts["y"], pts["z"], ph, pw, P, LAM, 0.15)
+    Wh = W.cpu().numpy()
+    err = olref.rel_l2(Wh[ph, pw], ref)
+    print(f"WRP 200k -> 4096^2: rel L2 vs FP64 on {len(ph)} pixels = {err:.3e}")
+    assert np.count_nonzero(ref) > 0.9 * len(ref)
+    assert err < 1e-5
+    # ARSS s = 0.5 of this plane (in place, as the pipeline runs it)
+    ol().ARSSFresnelProp(W, W, N, N, P, P, LAM, 0.15, 0.5)
+    got = W.cpu().numpy()
+    del W
+    e32 = olref.rel_l2(got, olref.arss_prop(Wh, P, P, LAM, 0.15, 0.5))
+    e64 = olref.rel_l2(got, olref.arss_prop(Wh.astype(np.complex128), P, P, LAM, 0.15, 0.5))
+    print(f"ARSS s=0.5 4096^2: rel L2 vs complex64 oracle {e32:.3e}, vs FP64 {e64:.3e}")
+    assert e32 < 1e-5 and e64 < 1e-5
+
+
+def test_layered_1M_64_layers_4096_vs_fp64():
+    N, n, nl, zmin, zmax = 4096, 1_000_000, 64, 0.10, 0.20
+    pts = synth.random_points(n, seed=4, xmax=N * P, ymax=N * P, zmin=zmin, zmax=zmax)
+    amp, phase = f32(pts["amp"]), f32(pts["phase"])
+    obj = ol().Object(pts["x"], pts["y"], pts["z"], amp, phase)
+    # layer snapping, vectorised; checked against the library's own on a subset
+    l = np.clip(np.floor((pts["z"] - zmin) / (zmax - zmin) * nl), 0, nl - 1)
+    zl = zmin + (l + 0.5) * (zmax - zmin) / nl
+    assert all(zl[k] == ol().layer_depth(pts["z"][k], zmin, zmax, nl) for k in range(0, n, 97))
+    ph, pw = sample_pixels(N, 32)
+    ref = olref.cgh_pixels(pts["x"], pts["y"], zl, ph, pw, P, P, LAM, amp, phase)
+    U = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    for prec, tol in (("f16", 1e-3), ("bf16x3", 1e-3), ("bf16", 6e-3)):
+        ol().CghLayered(obj, U, N, N, P, P, LAM, nl, zmin, zmax, precision=prec)
+        got = U.cpu().numpy()[ph, pw]
+        err = olref.rel_l2(got, ref)
+        dphi = np.abs(np.angle(got * np.conj(ref)))
+        wmean = float(np.sum(dphi * np.abs(ref)) / np.sum(np.abs(ref)))
+        print(f"layered {prec} 1M/64/4096^2: rel L2 vs FP64 {err:.3e}, amplitude-weighted "
+              f"phase error {wmean:.3e} rad, p99.9 {np.quantile(dphi, 0.999):.3e} rad")
+        assert err < tol, prec
+    objs = ol().Object(pts["x"], pts["y"], zl, amp, phase)
+    ol().Cgh(objs, U, N, N, P, P, LAM)
+    err = olref.rel_l2(U.cpu().numpy()[ph, pw], ref)
+    print(f"direct FP32 on the snapped cloud: rel L2 vs FP64 {err:.3e}")
+    assert err < 1e-4
+
+
+def _mean_norm_loss_delta(F, dF, T):
+    """L(F + dF) - L(F) for L = sum_p mean((I_p / mean I_p - T_p / mean T_p)^2), in FP64
+    and without cancellation: I' - I = 2 Re(conj F dF) + |dF|^2 enters directly."""
+    total = torch.zeros((), dtype=torch.float64, device=F[0].device)
+    for Fp, dFp, Tp in zip(F, dF, T):
+        I = Fp.real ** 2 + Fp.imag ** 2
+        dI = 2 * (Fp.real * dFp.real + Fp.imag * dFp.imag) + dFp.real ** 2 + dFp.imag ** 2
+        m, dm = I.mean(), dI.mean()
+        t = Tp / Tp.mean()
+        a0 = I / m - t
+        da = dI / (m + dm) - I * dm / (m * (m + dm))  # (I + dI) / (m + dm) - I / m
+        total = total + torch.mean(da * (2 * a0 + da))
+    return total
+
+
+def test_multiplane_2048_gradient_vs_central_fd():
+    n, zs = 2048, [0.10, 0.12, 0.15]
+    phi0 = synth.uniform(0, synth.S_PHASE, n * n, 0.0, 2 * np.pi).astype(np.float32).reshape(n, n)
+    T = [torch.from_numpy(synth.gaussian_targets(n, n, 5 + 101 * i)).cuda() for i in range(3)]
+    phi = torch.from_numpy(phi0).cuda()
+    _, g = ol().multiplane_mse(phi, T, zs, P, P, LAM)
+    g = g.cpu().numpy().ravel()
+    u = torch.exp(1j * phi.double()).to(torch.complex64)
+    F = []
+    for z in zs:
+        f = u.clone()
+        ol().AsmProp(f, n, n, P, P, LAM, z, bandlimit=True)
+        F.append(f.to(torch.complex128))
+    Td = [t.double() for t in T]
+    rng = np.random.default_rng(33)
+    idx = rng.choice(n * n, 64, replace=False)
+    h = 1e-3
+    fd = []
+    for k in idx:
+        d = torch.zeros((n, n), dtype=torch.complex64, device="cuda")
+        d.view(-1)[k] = 1.0
+        resp = []
+        for z in zs:  # column k of each plane's propagation operator
+            r = d.clone()
+            ol().AsmProp(r, n, n, P, P, LAM, z, bandlimit=True)
+            resp.append(r.to(torch.complex128))
+        uk = np.exp(1j * np.float64(phi0.ravel()[k]))
+        dl = []
+        for s in (1.0, -1.0):
+            c = complex(uk * (np.exp(1j * s * h) - 1.0))
+            dl.append(_mean_norm_loss_delta(F, [c * r for r in resp], Td).item())
+        fd.append((dl[0] - dl[1]) / (2 * h))
+    fd = np.array(fd)
+    err = olref.rel_l2(g[idx], fd)
+    print(f"multi-plane 2048^2 gradient vs central FD on 64 pixels: rel L2 {err:.3e}")
+    assert err < 1e-3

@@ -132,6 +132,17 @@ def test_wrp_golden():
         assert rel(out32, GOLD[key]) < 1e-6
 
 
+def test_wrp_pixels_match_the_golden_plane():
+    """The per-pixel WRP oracle (used at configuration size) gives the golden
+    brute-force plane's values at every pixel, for both window shapes."""
+    g = {k: GOLD[f"wrp_{k}"] for k in ("x", "y", "z")}
+    hh, ww = np.meshgrid(np.arange(24), np.arange(20), indexing="ij")
+    for sq, key in ((False, "wrp_circle"), (True, "wrp_square")):
+        out = olref.wrp_pixels(g["x"], g["y"], g["z"], hh.ravel(), ww.ravel(), 8e-6, LAM, 0.15,
+                               square=sq)
+        assert rel(out.reshape(24, 20), GOLD[key]) < 1e-12
+
+
 # ------------------------------------------------------------- closed forms
 def test_asm_zero_distance_is_identity():
     u = synth.random_field(32, 48, seed=5, dtype=np.complex128)
