@@ -159,12 +159,39 @@ def gpu_outputs(ol, torch, n):
     for exact in (False, True):
         ol.Cgh(obj, U, N, N, PITCH, PITCH, LAM, exact_r=exact)
         out["cgh_exact" if exact else "cgh_fresnel"] = U[:16].cpu().numpy()
+    del U, u
+    # WRP of 200,000 points into 4096^2 (sampled pixels) and its ARSS (whole plane)
+    N, P = 4096, 200_000
+    pts = synth.random_points(P, seed=3, xmax=N * PITCH, ymax=N * PITCH, zmin=0.151, zmax=0.155)
+    obj = ol.Object(pts["x"], pts["y"], pts["z"])
+    W = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    ol.WRPStep1(obj, W, N, N, PITCH, LAM, 0.15)
+    ph, pw = check_pixels(N, 31)
+    out["wrp"] = (pts, ph, pw, W.cpu().numpy())
+    ol.ARSSFresnelProp(W, W, N, N, PITCH, PITCH, LAM, 0.15, 0.5)
+    out["arss"] = W.cpu().numpy()
+    # layered CGH of 1,000,000 points in 64 layers over 4096^2 (sampled pixels)
+    n, nl, zmin, zmax = 1_000_000, 64, 0.10, 0.20
+    pts = synth.random_points(n, seed=4, xmax=N * PITCH, ymax=N * PITCH, zmin=zmin, zmax=zmax)
+    obj = ol.Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    ph, pw = check_pixels(N, 32)
+    lay = {}
+    for prec in ("f16", "bf16x3", "bf16"):
+        ol.CghLayered(obj, W, N, N, PITCH, PITCH, LAM, nl, zmin, zmax, precision=prec)
+        lay[prec] = W.cpu().numpy()[ph, pw]
+    out["layered"] = (pts, ph, pw, lay)
     return out
+
+
+def check_pixels(n, seed, count=4096):
+    """Seeded pixel sample of an n x n plane for the sampled FP64 checks."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, n, count), rng.integers(0, n, count)
 
 
 def cpu_baseline(n, gpu=None):
     """The oracle's AsmProp (complex64 field, PREC_T = double) on the host cores:
-    1 warm-up, then the median of 3 runs, at the metric size.  With the GPU results
+    1 warm-up, then the median of 5 runs, at the metric size.  With the GPU results
     of gpu_outputs, also their relative L2 error against the oracle: its complex64
     restatement (PREC_T = double) and its FP64 (complex128) form."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -175,7 +202,7 @@ def cpu_baseline(n, gpu=None):
     u = synth.random_field(n, n, seed=0)
     olref.asm_prop(u, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
     ts = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         ref_asm = olref.asm_prop(u, PITCH, PITCH, LAM, 0.10, olref.BANDLIMIT)
         ts.append(time.perf_counter() - t0)
@@ -215,6 +242,29 @@ def cpu_baseline(n, gpu=None):
             b64 = olref.cgh(pts["x"], pts["y"], pts["z"], rows, N, PITCH, PITCH, LAM, amp, ph,
                             flags=fl, dtype=np.complex128)
             errors[f"cgh_10k_2048_{name}_rows_vs_fp64"] = float(f"{olref.rel_l2(gpu['cgh_' + name], b64):.3e}")
+        fmt = lambda v: float(f"{v:.3e}")
+        # WRP: FP64 per-pixel sums on the sampled pixels; ARSS of the GPU's WRP plane
+        wpts, ph, pw, Wg = gpu["wrp"]
+        ref = olref.wrp_pixels(wpts["x"], wpts["y"], wpts["z"], ph, pw, PITCH, LAM, 0.15)
+        errors["wrp_200k_4096_sampled_vs_fp64"] = fmt(olref.rel_l2(Wg[ph, pw], ref))
+        errors["arss_4096_vs_c64_restatement"] = fmt(olref.rel_l2(
+            gpu["arss"], olref.arss_prop(Wg, PITCH, PITCH, LAM, 0.15, 0.5)))
+        errors["arss_4096_vs_fp64"] = fmt(olref.rel_l2(
+            gpu["arss"], olref.arss_prop(Wg.astype(np.complex128), PITCH, PITCH, LAM, 0.15, 0.5)))
+        # layered CGH: FP64 sums of the layer-snapped cloud on the sampled pixels
+        lpts, ph, pw, lay = gpu["layered"]
+        nl, zmin, zmax = 64, 0.10, 0.20
+        lyr = np.clip(np.floor((lpts["z"] - zmin) / (zmax - zmin) * nl), 0, nl - 1)
+        zl = zmin + (lyr + 0.5) * (zmax - zmin) / nl
+        a32 = lpts["amp"].astype(np.float32).astype(np.float64)
+        p32 = lpts["phase"].astype(np.float32).astype(np.float64)
+        ref = olref.cgh_pixels(lpts["x"], lpts["y"], zl, ph, pw, PITCH, PITCH, LAM, a32, p32)
+        for prec, got in lay.items():
+            dphi = np.abs(np.angle(got * np.conj(ref)))
+            errors[f"cgh_layered_{prec}_1M_4096_sampled_vs_fp64"] = fmt(olref.rel_l2(got, ref))
+            errors[f"cgh_layered_{prec}_phase_err_wmean_rad_vs_fp64"] = fmt(
+                np.sum(dphi * np.abs(ref)) / np.sum(np.abs(ref)))
+            errors[f"cgh_layered_{prec}_phase_err_max_rad_vs_fp64"] = fmt(dphi.max())
     # one thread: AsmProp at 1024^2 (OpenMP pinned to one thread through libgomp)
     one = None
     try:
@@ -237,7 +287,7 @@ def cpu_baseline(n, gpu=None):
     except OSError:
         pass
     return {"value": round(1.0 / med, 4), "unit": "props/s", "cores": cores, "kind": "port",
-            "sample": f"{n}x{n} AsmProp (band-limited, pad {2 * n}), 1 warm-up + median of 3 "
+            "sample": f"{n}x{n} AsmProp (band-limited, pad {2 * n}), 1 warm-up + median of 5 "
                       f"= {med:.2f} s/prop; own OpenMP radix-2 FFT (FFTW unavailable - substitute FFT)",
             "cpu_model": model, "nproc": os.cpu_count(),
             "extras": {"asm_1024_props_s_one_thread": one,
@@ -247,7 +297,11 @@ def cpu_baseline(n, gpu=None):
                        "complex2img_amp_4096_ms": round(t_img * 1e3, 1),
                        "numpy_rows": "objsort and complex2img: numpy, 1 core"},
             "rel_l2_vs_oracle": errors,
-            "rel_l2_sample": "AsmProp at the metric size; direct CGH of 10,000 points over 2048^2, rows 0-15"}
+            "rel_l2_sample": "AsmProp at the metric size; direct CGH of 10,000 points over 2048^2, "
+                             "rows 0-15; WRP of 200,000 points into 4096^2 and layered CGH of "
+                             "1,000,000 points / 64 layers over 4096^2 on 4096 seeded pixels (FP64 "
+                             "per-pixel sums, layered on the layer-snapped cloud); ARSS s = 0.5 of "
+                             "the GPU's WRP plane, whole 4096^2 field"}
 
 
 def ceilings(lib, torch):
@@ -481,7 +535,7 @@ def layered(ol, torch):
     sub = ol.Object(pts["x"] - w0 * PITCH, pts["y"] - h0 * PITCH, zl, pts["amp"], pts["phase"])
     D = torch.empty((win, win), dtype=torch.complex64, device="cuda")
     ol.Cgh(sub, D, win, win, PITCH, PITCH, LAM)
-    for prec in ("f16", "bf16x3"):
+    for prec in ("f16", "bf16x3", "bf16"):
         ol.CghLayered(obj, U, N, N, PITCH, PITCH, LAM, nl, zmin, zmax, precision=prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -527,9 +581,19 @@ def multiplane(torch):
     targets = optim.gaussian_targets(n, n, zs, seed=5)
     optim.optimize_phase(targets, zs, steps=3, norm="mean")  # warm-up (tables, allocations)
     phi, losses, dt = optim.optimize_phase(targets, zs, steps=200, lr=0.05, norm="mean")
+    # the native gradient at the optimised phase against central differences of the
+    # loss (evaluated in FP64 from the propagated fields) on 64 seeded pixels
+    import openol_amd as ol
+    _, g = ol.multiplane_mse(phi, targets, zs, optim.synth.PITCH, optim.synth.PITCH,
+                             optim.synth.LAMBDA)
+    idx = np.random.default_rng(33).choice(n * n, 64, replace=False)
+    fd = optim.central_fd_gradient(phi, targets, zs, idx)
+    gk = g.reshape(-1).cpu().numpy()[idx].astype(np.float64)
+    fd_err = float(np.linalg.norm(gk - fd) / np.linalg.norm(fd))
     return {"multiplane_2048_steps_per_s": round(200 / dt, 1),
             "multiplane_2048_ms_per_step": round(dt / 200 * 1e3, 3),
-            "multiplane_loss_first_last": [round(losses[0], 5), round(losses[-1], 5)]}
+            "multiplane_loss_first_last": [round(losses[0], 5), round(losses[-1], 5)],
+            "multiplane_2048_grad_vs_central_fd_64px": float(f"{fd_err:.3e}")}
 
 
 def main():
