@@ -102,6 +102,33 @@ int olx_copy_h2d(olx_ctx* ctx, void* d_dst, const void* h_src, size_t bytes); /*
 int olx_copy_d2h(olx_ctx* ctx, void* h_dst, const void* d_src, size_t bytes); /* synchronous */
 int olx_dev_memset(olx_ctx* ctx, void* d_ptr, int value, size_t bytes);
 
+/* ---- array helpers (include/olutils.h, include/golutils.h) -------------- */
+/* Window copy: d_dst is dny x dnx, dst[h][w] = src[h - oy][w - ox] inside the
+ * sny x snx source and 0 elsewhere (elements of 1, 2, 4, 8 or 16 bytes; source and
+ * destination distinct).  With oy = (dny - sny) / 2, ox = (dnx - snx) / 2 it is
+ * zeropadding (include/olutils.h:133-178) / gzeropadding (include/golutils.h:
+ * 189-244); with oy = -sny / 4, ox = -snx / 4 into sny/2 x snx/2 del_zero
+ * (include/olutils.h:181-198) / gdel_zero (include/golutils.h:247-278); with
+ * oy = -y, ox = -x cut (include/olutils.h:200-211) / gcut (include/golutils.h:
+ * 281-308). */
+int olx_window_copy(olx_ctx* ctx, const void* d_src, int64_t sny, int64_t snx, void* d_dst,
+                    int64_t dny, int64_t dnx, int64_t oy, int64_t ox, size_t elem_bytes);
+/* fftshift (include/olutils.h:95-117) / gfftshift (include/golutils.h:80-110): the
+ * reference's quadrant swap in place, elements of 4, 8 or 16 bytes. */
+int olx_fftshift(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, size_t elem_bytes);
+/* c = a * b elementwise over n complex64 values (c may alias a or b): mul_complex
+ * (include/olutils.h:121-130) / mult (include/golutils.h:123-143). */
+int olx_mul_c64(olx_ctx* ctx, const void* d_a, const void* d_b, void* d_c, int64_t n);
+/* c = a + b elementwise (add, include/golutils.h:145-164). */
+int olx_add_c64(olx_ctx* ctx, const void* d_a, const void* d_b, void* d_c, int64_t n);
+/* u *= s for n complex64 values: mul_scalar (include/olutils.h:81-89,
+ * include/golutils.h:166-187), standardization with s = 1 / (ny nx) (:70-79). */
+int olx_scale_c64(olx_ctx* ctx, void* d_u, int64_t n, float s);
+/* complex128 twins of olx_mul_c64 and olx_scale_c64 (host std::complex<double>
+ * arrays of mul_complex / mul_scalar / standardization stay in FP64) */
+int olx_mul_c128(olx_ctx* ctx, const void* d_a, const void* d_b, void* d_c, int64_t n);
+int olx_scale_c128(olx_ctx* ctx, void* d_u, int64_t n, double s);
+
 /* ---- propagation (include/olprop.h, include/golprop.h) ------------------ */
 /* AsmProp: in place on an ny x nx field, pad 2x, IFFT(FFT(u) H)/M^2, crop.
  * Replaces ol::AsmProp (include/olprop.h:227-248) and gAsmProp (include/golprop.h:506-534). */
@@ -364,6 +391,17 @@ int olx_uniform_f32(olx_ctx* ctx, float* d_out, int64_t n, uint64_t seed);
 int olx_lens_array_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, float pitch,
                        float blockwidth, float f, float lambda, int step, int oy, int ox,
                        float p_limit, uint32_t flags);
+/* Lensarry::duplicate (kind 0), duplicate_random (kind 1) and duplicate_multrandom
+ * (kind 2) (include/Lensarry.h:137-252): the bp x bp block d_block tiles the ny x nx
+ * field d_out with local shift (on, om); for kinds 1 and 2, pixels whose local
+ * position lies outside the window |. - bp/2| < floor(|lambda f / (2 p_limit)| / pitch)
+ * take exp(i phase) (kind 1, phase 2 pi (v - 0.5)) or the block value times
+ * exp(i phase) (kind 2, phase pi/2 (v - 0.5)), v the 53-bit value of two draws of
+ * std::mt19937(0) taken in the reference's block traversal order (generated on the
+ * host; the call then waits for its upload). */
+int olx_lens_tile_c64(olx_ctx* ctx, const void* d_block, int bp, void* d_out, int64_t ny,
+                      int64_t nx, int on, int om, float pitch, float f, float lambda,
+                      float p_limit, int kind);
 
 /* ---- instrumentation ---------------------------------------------------- */
 /* per-kernel device time measured with HIP events on the context stream.  on = 0

@@ -1,6 +1,6 @@
 // openol.h -- umbrella header of the MI355X drop-in (role of the reference's
-// include/openol.h:3-22): the propagation, CGH, point-cloud and image APIs over
-// libolx.so.
+// include/openol.h:3-22): the propagation, CGH, array-helper, point-cloud and image
+// APIs over libolx.so.
 // Link with -L<repo>/openol_amd -lolx (rpath to the same directory).
 #ifndef OPENOL_H
 #define OPENOL_H
@@ -16,5 +16,6 @@
 #include "olimg.h"
 #include "olobject.h"
 #include "olprop.h"
+#include "olutils.h"
 #include "olx.h"
 #endif

@@ -76,10 +76,19 @@ SIGNATURES = {
     "olx_uniform_f32": [_p, _p, _i64, ctypes.c_uint64],
     "olx_lens_array_c64": [_p, _p, _i64, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                            ctypes.c_float, _i, _i, _i, ctypes.c_float, _u32],
+    "olx_lens_tile_c64": [_p, _p, _i, _p, _i64, _i64, _i, _i, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, ctypes.c_float, _i],
     "olx_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "olx_asm_prop_check": [_i64, _i64, _d, _d, _d, _d],
     "olx_fresnel_prop_check": [_i64, _i64, _d, _d, _d, _d],
     "olx_fft2d_c64": [_p, _p, _i64, _i64, _i],
+    "olx_window_copy": [_p, _p, _i64, _i64, _p, _i64, _i64, _i64, _i64, ctypes.c_size_t],
+    "olx_fftshift": [_p, _p, _i64, _i64, ctypes.c_size_t],
+    "olx_mul_c64": [_p, _p, _p, _p, _i64],
+    "olx_add_c64": [_p, _p, _p, _p, _i64],
+    "olx_scale_c64": [_p, _p, _i64, ctypes.c_float],
+    "olx_mul_c128": [_p, _p, _p, _p, _i64],
+    "olx_scale_c128": [_p, _p, _i64, _d],
     "olx_asm_transfer_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_fresnel_response_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i64, _i64],
     "olx_cgh_c64": [_p, _p, _i64, _p, _p, _p, _i64, _i64, _d, _d, _d, _u32],

@@ -203,6 +203,29 @@ extern "C" int olx_diffuser_random_phase_c64(olx_ctx* ctx, void* d_u, int64_t ny
     return OLX_OK;
 }
 
+// Lensarry::duplicate / duplicate_random / duplicate_multrandom (include/Lensarry.h:
+// 137-252): the bp x bp block tiles the field, each pixel taking the block value at
+// its shifted local position; with a phase array, pixels outside the +-wlim window
+// of the block centre take exp(i phase) (kind 1) or the block value times it (kind
+// 2), cos / sin in double rounded to float (std::cos / std::sin of a float)
+__global__ void k_lens_tile(const float2* __restrict__ block, int bp, float2* __restrict__ out,
+                            int ny, int nx, int on, int om, const float* __restrict__ phase,
+                            int kind, int wlim) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y;
+    if (m >= nx) return;
+    int ms = (m % bp - om) % bp, ns = (n % bp - on) % bp;
+    ms = ms >= 0 ? ms : ms + bp;
+    ns = ns >= 0 ? ns : ns + bp;
+    float2 v = block[(size_t)ns * bp + ms];
+    if (kind != 0 && (abs(ms - bp / 2) >= wlim || abs(ns - bp / 2) >= wlim)) {
+        double s, c;
+        sincos((double)phase[(size_t)n * nx + m], &s, &c);
+        const float2 e = make_float2((float)c, (float)s);
+        v = kind == 1 ? e : cmulf(v, e);
+    }
+    out[(size_t)n * nx + m] = v;
+}
+
 extern "C" int olx_lens_array_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, float pitch,
                                   float blockwidth, float f, float lambda, int step, int oy, int ox,
                                   float p_limit, uint32_t flags) {
@@ -239,4 +262,57 @@ extern "C" int olx_lens_array_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t n
     a.write = (flags & OLX_LENS_WRITE) ? 1 : 0;
     k_lens<<<dim3((unsigned)((nx + 255) / 256), (unsigned)ny), 256, 0, ctx->stream>>>((float2*)d_u, a);
     return check_launch("lens array");
+}
+
+extern "C" int olx_lens_tile_c64(olx_ctx* ctx, const void* d_block, int bp, void* d_out,
+                                 int64_t ny, int64_t nx, int on, int om, float pitch, float f,
+                                 float lambda, float p_limit, int kind) {
+    if (!ctx || !d_block || !d_out) OLX_FAIL(OLX_EINVAL, "lens tile: null argument");
+    if (ny <= 0 || nx <= 0 || ny > 65535 || nx > (1 << 24) || bp <= 0)
+        OLX_FAIL(OLX_EINVAL, "lens tile: bad size");
+    if (kind < 0 || kind > 2) OLX_FAIL(OLX_EINVAL, "lens tile: kind must be 0, 1 or 2");
+    int wlim = INT_MAX;
+    const float* dph = nullptr;
+    if (kind != 0) {
+        if (!(p_limit > 0) || !(pitch > 0)) OLX_FAIL(OLX_EINVAL, "lens tile: random forms need p_limit, pitch > 0");
+        // include/Lensarry.h:167-169: xlim = |lambda f 0.5 / p_limit| (float), floor(xlim / pitch)
+        const float xlim = (float)std::fabs(lambda * f * 0.5 / p_limit);
+        wlim = (int)std::floor(xlim / pitch);
+        // the std::mt19937(0) stream in the reference's traversal: blocks row-major, then
+        // the block's rows and columns, a row cut at the field edge (its break), two draws
+        // per pixel outside the window
+        const size_t npx = (size_t)ny * nx;
+        std::vector<float> ph(npx, 0.f);
+        std::mt19937 e2(0);
+        const float span = kind == 1 ? 2 * 3.14159265f : 0.5f * 3.14159265f;
+        for (int bn = 0; bn < ny; bn += bp)
+            for (int bm = 0; bm < nx; bm += bp)
+                for (int ln = 0; ln < bp; ln++)
+                    for (int lm = 0; lm < bp; lm++) {
+                        int lms = (lm - om) % bp, lns = (ln - on) % bp;
+                        lms = lms >= 0 ? lms : lms + bp;
+                        lns = lns >= 0 ? lns : lns + bp;
+                        const int m = bm + lm, n = bn + ln;
+                        if (m >= nx || n >= ny) break;
+                        if (std::abs(lms - bp / 2) >= wlim || std::abs(lns - bp / 2) >= wlim) {
+                            const int a = (int)(e2() >> 5);
+                            const int b = (int)(e2() >> 6);
+                            const double value = (a * 67108864.0 + b) / 9007199254740992.0;
+                            ph[(size_t)n * nx + m] = (float)(span * (value - 0.5f));
+                        }
+                    }
+        OLX_TRY(ctx->obj_tmp.reserve(sizeof(float) * npx));
+        OLX_HIP(hipMemcpyAsync(ctx->obj_tmp.p, ph.data(), sizeof(float) * npx, hipMemcpyHostToDevice,
+                               ctx->stream));
+        dph = (const float*)ctx->obj_tmp.p;
+        k_lens_tile<<<dim3((unsigned)((nx + 255) / 256), (unsigned)ny), 256, 0, ctx->stream>>>(
+            (const float2*)d_block, bp, (float2*)d_out, (int)ny, (int)nx, on, om, dph, kind, wlim);
+        OLX_TRY(check_launch("lens tile"));
+        // the host vector is freed on return: wait for the copy
+        OLX_HIP(hipStreamSynchronize(ctx->stream));
+        return OLX_OK;
+    }
+    k_lens_tile<<<dim3((unsigned)((nx + 255) / 256), (unsigned)ny), 256, 0, ctx->stream>>>(
+        (const float2*)d_block, bp, (float2*)d_out, (int)ny, (int)nx, on, om, dph, 0, wlim);
+    return check_launch("lens tile");
 }

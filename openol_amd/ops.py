@@ -230,6 +230,115 @@ def ifft(u):
     return u
 
 
+# ------------------------------------------------------------------ array helpers
+_ELEM_OK = (1, 2, 4, 8, 16)
+
+
+def _arr2d(t, name):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dim() != 2 or not t.is_contiguous():
+        raise OlxInvalid(_lib.EINVAL, f"{name} must be a contiguous 2-D CUDA tensor")
+    if t.element_size() not in _ELEM_OK:
+        raise OlxInvalid(_lib.EINVAL, f"{name}: element size {t.element_size()} not in {_ELEM_OK}")
+    return t
+
+
+def _window(src, dny, dnx, oy, ox, name):
+    _arr2d(src, "src")
+    dst = torch.empty((dny, dnx), dtype=src.dtype, device=src.device)
+    sny, snx = src.shape
+    check(_lib.call("olx_window_copy", src.device.index, _ptr(src), sny, snx, _ptr(dst), dny, dnx,
+                    oy, ox, src.element_size()), name)
+    return dst
+
+
+def zeropadding(src, out_height=None, out_width=None):
+    """ol::zeropadding / gzeropadding (include/olutils.h:133-178, include/golutils.h:
+    189-244): a new (out_height, out_width) array (default twice the size) holding src
+    centred, left pads (out - in) // 2, zeros elsewhere."""
+    ny, nx = src.shape
+    oh = 2 * ny if out_height is None else int(out_height)
+    ow = 2 * nx if out_width is None else int(out_width)
+    return _window(src, oh, ow, (oh - ny) // 2, (ow - nx) // 2, "zeropadding")
+
+
+def del_zero(src):
+    """ol::del_zero / gdel_zero (include/olutils.h:181-198, include/golutils.h:247-278):
+    the centre [h/4, 3h/4) x [w/4, 3w/4) of src as a new (h/2, w/2) array."""
+    ny, nx = src.shape
+    return _window(src, ny // 2, nx // 2, -(ny // 4), -(nx // 4), "del_zero")
+
+
+def cut(src, y, x, ysize, xsize):
+    """ol::cut / gcut (include/olutils.h:200-211, include/golutils.h:281-308): the
+    (ysize, xsize) window at row y, column x as a new array."""
+    ny, nx = _arr2d(src, "src").shape
+    if y < 0 or x < 0 or ysize < 0 or xsize < 0 or y + ysize > ny or x + xsize > nx:
+        raise OlxInvalid(_lib.EINVAL, "cut: window outside the array")
+    return _window(src, ysize, xsize, -y, -x, "cut")
+
+
+def fftshift(u):
+    """ol::fftshift / gfftshift (include/olutils.h:95-117, include/golutils.h:80-110):
+    the quadrant swap, in place."""
+    _arr2d(u, "u")
+    if u.element_size() not in (4, 8, 16):
+        raise OlxInvalid(_lib.EINVAL, "fftshift: element size must be 4, 8 or 16")
+    check(_lib.call("olx_fftshift", u.device.index, _ptr(u), u.shape[0], u.shape[1],
+                    u.element_size()), "fftshift")
+    return u
+
+
+def _same(a, b, c):
+    for t, nm in ((a, "a"), (b, "b"), (c, "c")):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
+            raise OlxInvalid(_lib.EINVAL, f"{nm} must be a contiguous CUDA tensor")
+    if not (a.shape == b.shape == c.shape and a.dtype == b.dtype == c.dtype):
+        raise OlxInvalid(_lib.EINVAL, "operands must share shape and dtype")
+    if a.dtype not in (torch.complex64, torch.complex128):
+        raise OlxInvalid(_lib.EINVAL, "operands must be complex64 or complex128")
+
+
+def mult(a, b, c=None):
+    """ol::mult / mul_complex (include/golutils.h:123-143, include/olutils.h:121-130):
+    c = a * b elementwise (c may be a or b; a new tensor when omitted)."""
+    c = torch.empty_like(a) if c is None else c
+    _same(a, b, c)
+    name = "olx_mul_c64" if a.dtype == torch.complex64 else "olx_mul_c128"
+    check(_lib.call(name, a.device.index, _ptr(a), _ptr(b), _ptr(c), a.numel()), "mult")
+    return c
+
+
+mul_complex = mult
+
+
+def add(a, b, c=None):
+    """ol::add (include/golutils.h:145-164): c = a + b elementwise (complex64)."""
+    c = torch.empty_like(a) if c is None else c
+    _same(a, b, c)
+    if a.dtype != torch.complex64:
+        raise OlxInvalid(_lib.EINVAL, "add: complex64 operands")
+    check(_lib.call("olx_add_c64", a.device.index, _ptr(a), _ptr(b), _ptr(c), a.numel()), "add")
+    return c
+
+
+def mul_scalar(u, c):
+    """ol::mul_scalar (include/olutils.h:81-89, include/golutils.h:166-187): u *= c in
+    place (c rounded to float for complex64 data)."""
+    _same(u, u, u)
+    if u.dtype == torch.complex64:
+        check(_lib.call("olx_scale_c64", u.device.index, _ptr(u), u.numel(), float(c)), "mul_scalar")
+    else:
+        check(_lib.call("olx_scale_c128", u.device.index, _ptr(u), u.numel(), float(c)), "mul_scalar")
+    return u
+
+
+def standardization(u):
+    """ol::standardization (include/olutils.h:70-79): u *= 1 / (ny nx), the factor
+    rounded to float as there."""
+    ny, nx = u.shape[-2], u.shape[-1]
+    return mul_scalar(u, float(np.float32(1.0 / (ny * nx))))
+
+
 # ------------------------------------------------------------------ point clouds
 POINT_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("amp", "<f4"),
                         ("phase", "<f4")])
@@ -668,6 +777,62 @@ class Lensarry:
         """include/Lensarry.h:312-317."""
         _field(u, self.height, self.width)
         return self._run(u, step, oy, ox, p_limit, False)
+
+    def lenslet(self, step=PLAY, p_limit=0.0, device=None):
+        """The blockpx x blockpx lenslet (FresnelResponse at -+f, its band-limited form
+        with p_limit > 0) that the array tiles."""
+        dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        b = torch.empty((self.blockpx, self.blockpx), dtype=torch.complex64, device=dev)
+        bp = self.blockpx
+        check(_lib.call("olx_lens_array_c64", dev.index, _ptr(b), bp, bp, self.pitch,
+                        self.blockwidth, self.f, self.lam, step, 0, 0, p_limit, _lib.LENS_WRITE),
+              "Lensarry")
+        return b
+
+    def _tile(self, block, on, om, p_limit, kind):
+        if block.dtype != torch.complex64 or tuple(block.shape) != (self.blockpx, self.blockpx) \
+                or not block.is_cuda or not block.is_contiguous():
+            raise OlxInvalid(_lib.EINVAL, "duplicate: block must be a contiguous blockpx x blockpx "
+                             "complex64 CUDA tensor")
+        out = torch.empty((self.height, self.width), dtype=torch.complex64, device=block.device)
+        check(_lib.call("olx_lens_tile_c64", block.device.index, _ptr(block), self.blockpx,
+                        _ptr(out), self.height, self.width, on, om, self.pitch, self.f, self.lam,
+                        p_limit, kind), "duplicate")
+        return out
+
+    def duplicate(self, block, on=0, om=0):
+        """include/Lensarry.h:136-161: the block tiled over the field, shifted by (on, om)."""
+        return self._tile(block, on, om, 0.0, 0)
+
+    def duplicate_random(self, block, p_limit, on=0, om=0):
+        """include/Lensarry.h:163-206: outside the p_limit window each pixel takes
+        exp(i 2 pi (v - 0.5)), v from std::mt19937(0) in the reference's traversal."""
+        return self._tile(block, on, om, p_limit, 1)
+
+    def duplicate_multrandom(self, block, p_limit, on=0, om=0):
+        """include/Lensarry.h:207-250: outside the window the block value times
+        exp(i pi/2 (v - 0.5))."""
+        return self._tile(block, on, om, p_limit, 2)
+
+    def gen_complex_random(self, p_limit, step=PLAY, oy=0, ox=0, device=None):
+        """include/Lensarry.h:270-281: the unwindowed lenslet tiled with a random phase
+        outside the p_limit window (duplicate_multrandom)."""
+        return self.duplicate_multrandom(self.lenslet(step, 0.0, device), p_limit, oy, ox)
+
+    def gen_complex_limit_random(self, p_limit, step=PLAY, oy=0, ox=0, device=None):
+        """include/Lensarry.h:310-321: the band-limited lenslet, random phases outside its
+        window (duplicate_random)."""
+        return self.duplicate_random(self.lenslet(step, p_limit, device), p_limit, oy, ox)
+
+    def mult_complex_random(self, u, p_limit, step=PLAY, oy=0, ox=0):
+        """include/Lensarry.h:282-288: u times gen_complex_random's field, in place."""
+        _field(u, self.height, self.width)
+        return mult(u, self.gen_complex_random(p_limit, step, oy, ox, u.device.index), u)
+
+    def mult_complex_limit_random(self, u, p_limit, step=PLAY, oy=0, ox=0):
+        """include/Lensarry.h:322-328."""
+        _field(u, self.height, self.width)
+        return mult(u, self.gen_complex_limit_random(p_limit, step, oy, ox, u.device.index), u)
 
 
 class LensarryCircle(Lensarry):

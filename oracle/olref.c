@@ -871,6 +871,65 @@ int olref_lens_array(float* out, int64_t height, int64_t width, float pitch, flo
     return 0;
 }
 
+/* Lensarry::duplicate (kind 0), duplicate_random (kind 1), duplicate_multrandom
+ * (kind 2), include/Lensarry.h:137-252: the bp x bp block tiles the field in the
+ * reference's traversal (blocks row-major, then the block's rows and columns, a row
+ * cut at the field edge); outside the window |local - bp/2| < wlim, wlim =
+ * floor(|lambda f 0.5 / p_limit| / pitch) (float), each pixel draws a 53-bit value
+ * from std::mt19937(0) and takes exp(i phase) (kind 1, phase = 2 F32_PI (v - 0.5f))
+ * or the block value times it (kind 2, phase = 0.5 F32_PI (v - 0.5f)); cos / sin of
+ * the float phase in float. */
+int olref_lens_tile(const float* blockin, int bp, float* out, int64_t height, int64_t width,
+                    int on, int om, float pitch, float f, float lambda, float p_limit, int kind) {
+    if (!blockin || !out || bp <= 0 || kind < 0 || kind > 2) return -1;
+    const cf* ub = (const cf*)blockin;
+    cf* u = (cf*)out;
+    const float F32PI = 3.14159265f;
+    int wlim = INT_MAX;
+    if (kind) {
+        float xlim = (float)fabs(lambda * f * 0.5 / p_limit);
+        wlim = (int)floorf(xlim / pitch);
+    }
+    mt19937 g;
+    mt_seed(&g, 0);
+    for (int64_t i = 0; i < height * width; i++) u[i] = 0;
+    for (int bn = 0; bn < height; bn += bp)
+        for (int bm = 0; bm < width; bm += bp)
+            for (int ln = 0; ln < bp; ln++)
+                for (int lm = 0; lm < bp; lm++) {
+                    int ms = (lm - om) % bp, ns = (ln - on) % bp;
+                    ms = ms >= 0 ? ms : ms + bp;
+                    ns = ns >= 0 ? ns : ns + bp;
+                    int m = bm + lm, n = bn + ln;
+                    if (m >= width || n >= height) break;
+                    cf v = ub[ms + ns * bp];
+                    if (kind && (abs(ms - bp / 2) >= wlim || abs(ns - bp / 2) >= wlim)) {
+                        double value = mt_res53(&g);
+                        float phase = kind == 1 ? (float)(2 * F32PI * (value - 0.5f))
+                                                : (float)(0.5 * F32PI * (value - 0.5f));
+                        cf e = cosf(phase) + I * sinf(phase);
+                        if (kind == 1) v = e;
+                        else {
+                            float re = crealf(v) * crealf(e) - cimagf(v) * cimagf(e);
+                            float im = crealf(v) * cimagf(e) + cimagf(v) * crealf(e);
+                            v = re + I * im;
+                        }
+                    }
+                    u[m + n * width] = v;
+                }
+    return 0;
+}
+
+/* the bp x bp lenslet of Lensarry (FresnelResponse at z = -f for PLAY, +f for
+ * RECORD, PREC_T = float; p_limit > 0: its band-limited form) */
+int olref_lens_block(float* out, int bp, float pitch, float f, float lambda, int step,
+                     float p_limit) {
+    if (!out || bp <= 0) return -1;
+    float tmpf = step == 0 ? -f : f;
+    lens_block((cf*)out, bp, pitch, lambda, -tmpf, p_limit);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Split propagation: splitAsmProp olprop.h:614-651, gsplitProp golprop.h:    */
 /* 773-813 (shiftedProp golprop.h:756-770 picks ASM or ARSS with s = 1).      */

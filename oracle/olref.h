@@ -104,6 +104,10 @@ int olref_diffuser_random_phase(float* u, int64_t height, int64_t width, int ste
 int olref_diffuser_apply(float* u, int64_t height, int64_t width, const float* b, float scalar,
                          int kind, int blocksize);
 /* Lensarry / LensarryCircle lens field (include/Lensarry.h:137-311, 406-447) */
+int olref_lens_tile(const float* block, int bp, float* out, int64_t height, int64_t width,
+                    int on, int om, float pitch, float f, float lambda, float p_limit, int kind);
+int olref_lens_block(float* out, int bp, float pitch, float f, float lambda, int step,
+                     float p_limit);
 int olref_lens_array(float* out, int64_t height, int64_t width, float pitch, float blockwidth,
                      float f, float lambda, int step, int on, int om, float p_limit, int circle);
 

@@ -63,6 +63,11 @@ def lib():
             "olref_diffuser_random_phase": [p, i64, i64, ctypes.c_int, u32, ctypes.c_float,
                                             ctypes.c_int, ctypes.c_int],
             "olref_diffuser_apply": [p, i64, i64, p, ctypes.c_float, ctypes.c_int, ctypes.c_int],
+            "olref_lens_tile": [p, ctypes.c_int, p, i64, i64, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_int],
+            "olref_lens_block": [p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_int, ctypes.c_float],
             "olref_lens_array": [p, i64, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_float, ctypes.c_int],
@@ -221,6 +226,23 @@ def lens_array(height, width, pitch, blockwidth, f, lam, step=PLAY, on=0, om=0, 
     out = np.empty((height, width), dtype=np.complex64)
     _check(lib().olref_lens_array(_ptr(out), height, width, pitch, blockwidth, f, lam, step, on, om,
                                   p_limit, 1 if circle else 0), "lens_array")
+    return out
+
+
+def lens_block(bp, pitch, f, lam, step=PLAY, p_limit=0.0):
+    """The bp x bp lenslet of Lensarry::gen_complex(_limit) (FresnelResponse at -+f)."""
+    out = np.empty((bp, bp), dtype=np.complex64)
+    _check(lib().olref_lens_block(_ptr(out), bp, pitch, f, lam, step, p_limit), "lens_block")
+    return out
+
+
+def lens_tile(block, height, width, on=0, om=0, pitch=0.0, f=0.0, lam=0.0, p_limit=0.0, kind=0):
+    """Lensarry::duplicate / duplicate_random / duplicate_multrandom (kind 0 / 1 / 2),
+    include/Lensarry.h:137-252."""
+    block = np.ascontiguousarray(block, dtype=np.complex64)
+    out = np.empty((height, width), dtype=np.complex64)
+    _check(lib().olref_lens_tile(_ptr(block), block.shape[0], _ptr(out), height, width, on, om,
+                                 pitch, f, lam, p_limit, kind), "lens_tile")
     return out
 
 

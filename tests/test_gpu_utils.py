@@ -1,0 +1,116 @@
+"""GPU checks of the array helpers (include/olutils.h host overloads, golutils.h device
+overloads, through tests/cpp/utils_test) and of the random lenslet forms of
+include/Lensarry.h, against numpy and the oracle.
+
+Data movement (padding, cropping, cutting, the quadrant swap) must be bit exact;
+complex64 products and scalings within a few float roundings (relative 1e-6); the
+complex64 FFT within 3e-6 of numpy's FP64 pocketfft; the lenslet fields within
+4e-7 of the oracle (float cos / sin of the same float phases)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+import olref
+from openol_amd import synth
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+EXE = os.path.join(ROOT, "tests", "cpp", "utils_test")
+LAM = 532e-9
+
+
+def rel(a, b):
+    return olref.rel_l2(a, b)
+
+
+def pad(a, oh, ow):
+    out = np.zeros((oh, ow), a.dtype)
+    y, x = (oh - a.shape[0]) // 2, (ow - a.shape[1]) // 2
+    out[y:y + a.shape[0], x:x + a.shape[1]] = a
+    return out
+
+
+def test_utils_program(tmp_path):
+    assert os.path.exists(EXE), "build() compiles tests/cpp/utils_test"
+    ny, nx = 90, 150
+    a = synth.random_field(ny, nx, seed=90)
+    b = synth.random_field(ny, nx, seed=91)
+    a.tofile(tmp_path / "a.bin")
+    b.tofile(tmp_path / "b.bin")
+    (tmp_path / "params.txt").write_text(f"{ny} {nx}\n")
+    r = subprocess.run([EXE, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "done" in r.stdout and "caught:" in r.stdout
+
+    def rd(name, shape=(ny, nx), dt=np.complex64):
+        return np.fromfile(tmp_path / f"{name}.bin", dtype=dt).reshape(shape)
+
+    a64, b64 = a.astype(np.complex128), b.astype(np.complex128)
+    F = np.fft.fft2(a64)
+    assert rel(rd("fft"), F) < 3e-6 and rel(rd("gfft"), F) < 3e-6
+    assert rel(rd("fft_c128", dt=np.complex128), F) < 3e-6
+    assert rel(rd("ifft"), np.fft.ifft2(a64) * a.size) < 3e-6
+    assert rel(rd("gfft_back"), a64 * a.size) < 3e-6
+    assert rel(rd("mul"), a64 * b64) < 1e-6 and rel(rd("gmult"), a64 * b64) < 1e-6
+    assert rel(rd("mul_c128", dt=np.complex128), a64 * b64) < 1e-15
+    assert rel(rd("gadd"), a64 + b64) < 1e-7
+    assert rel(rd("gscalar"), (a64 + b64) * 2.5) < 1e-6
+    assert rel(rd("scalar"), a64 * np.float32(0.37)) < 1e-7
+    assert rel(rd("standard"), a64 * np.float32(1.0 / (ny * nx))) < 1e-7
+    # data movement, bit for bit
+    assert np.array_equal(rd("shift"), np.fft.fftshift(a)) and np.array_equal(rd("gshift"), np.fft.fftshift(a))
+    assert np.array_equal(rd("pad2", (2 * ny, 2 * nx)), pad(a, 2 * ny, 2 * nx))
+    assert np.array_equal(rd("gpad2", (2 * ny, 2 * nx)), pad(a, 2 * ny, 2 * nx))
+    assert np.array_equal(rd("padg", (ny + 7, nx + 4)), pad(a, ny + 7, nx + 4))
+    assert np.array_equal(rd("delzero"), a) and np.array_equal(rd("gdelzero"), a)
+    assert np.array_equal(rd("cut", (ny // 2, nx // 3)), a[3:3 + ny // 2, 5:5 + nx // 3])
+    assert np.array_equal(rd("gcut", (ny // 2, nx // 3)), a[3:3 + ny // 2, 5:5 + nx // 3])
+    img = ((np.arange(ny * nx) * 7 + 3) % 256).astype(np.uint8).reshape(ny, nx)
+    assert np.array_equal(rd("padu8", (ny + 2, nx + 6), np.uint8), pad(img, ny + 2, nx + 6))
+    # random lenslet forms against the oracle
+    pitch, bw, f, pl = (float(np.float32(v)) for v in (8e-6, 16 * 8e-6, 0.01, 60e-6))
+    lam = float(np.float32(LAM))
+    bp = 16
+    ref = olref.lens_tile(olref.lens_block(bp, pitch, f, lam, olref.PLAY), ny, nx, 2, -3, pitch, f,
+                          lam, pl, 2)
+    assert np.abs(rd("lens_random") - ref).max() <= 4e-7
+    ref = olref.lens_tile(olref.lens_block(bp, pitch, f, lam, olref.RECORD, pl), ny, nx, 0, 0,
+                          pitch, f, lam, pl, 1)
+    assert np.abs(rd("lens_limit_random") - ref).max() <= 4e-7
+    ref = olref.lens_tile(olref.lens_block(bp, pitch, f, lam, olref.PLAY), ny, nx, 1, 1, pitch, f,
+                          lam, pl, 2)
+    assert np.abs(rd("lens_mult_random") - a * ref).max() <= 1e-6
+    ref = olref.lens_array(ny, nx, pitch, bw, f, lam, olref.PLAY, 0, 0)
+    assert np.abs(rd("lens_mult") - a * ref).max() <= 1e-6
+    blk = (np.arange(bp * bp) + 1j * (1 - np.arange(bp * bp))).astype(np.complex64).reshape(bp, bp)
+    ref = olref.lens_tile(blk, ny, nx, 0, 0, pitch, f, lam, pl, 2)
+    assert np.abs(rd("lens_dup_multrandom") - ref).max() <= 4e-7 * np.abs(blk).max()
+
+
+def test_python_helpers():
+    import openol_amd as ol
+    u0 = synth.random_field(64, 96, seed=92)
+    u = torch.from_numpy(u0).cuda()
+    assert np.array_equal(ol.zeropadding(u).cpu().numpy(), pad(u0, 128, 192))
+    assert np.array_equal(ol.zeropadding(u, 70, 99).cpu().numpy(), pad(u0, 70, 99))
+    assert np.array_equal(ol.del_zero(ol.zeropadding(u)).cpu().numpy(), u0)
+    assert np.array_equal(ol.cut(u, 4, 7, 20, 30).cpu().numpy(), u0[4:24, 7:37])
+    s = u.clone()
+    assert np.array_equal(ol.fftshift(s).cpu().numpy(), np.fft.fftshift(u0))
+    v = torch.from_numpy(synth.random_field(64, 96, seed=93)).cuda()
+    assert rel(ol.mult(u, v).cpu().numpy(), u0.astype(np.complex128) * v.cpu().numpy()) < 1e-6
+    assert rel(ol.add(u, v).cpu().numpy(), u0.astype(np.complex128) + v.cpu().numpy()) < 1e-7
+    d = u.to(torch.complex128)
+    assert rel(ol.mul_scalar(d, 0.3).cpu().numpy(), u0.astype(np.complex128) * 0.3) < 1e-15
+    with pytest.raises(ValueError):
+        ol.cut(u, 60, 0, 10, 10)
+    # random lenslets through the Python mirror
+    L = ol.Lensarry(64, 96, 8e-6, 16 * 8e-6, 0.01, LAM)
+    pitch, f, pl, lam = (float(np.float32(x)) for x in (8e-6, 0.01, 60e-6, LAM))
+    got = L.gen_complex_limit_random(60e-6, ol.PLAY, 3, 1).cpu().numpy()
+    ref = olref.lens_tile(olref.lens_block(16, pitch, f, lam, olref.PLAY, pl), 64, 96, 3, 1, pitch, f,
+                          lam, pl, 1)
+    assert np.abs(got - ref).max() <= 4e-7

@@ -104,3 +104,47 @@ def test_lens_circle_layout():
     # the second row (odd) is shifted by half a lenslet
     row1 = got[14 + 8, :]
     assert not np.array_equal(row1, got[8, :])
+
+
+def _traversal(ny, nx, bp):
+    """(n, m) of every pixel in the reference's duplicate traversal: lexicographic in
+    (block row, block column, local row, local column), pixels past the field edge
+    dropped (the loop's break skips the rest of a row; rows past the edge draw
+    nothing)."""
+    bn, bm, ln, lm = np.meshgrid(np.arange(0, ny, bp), np.arange(0, nx, bp), np.arange(bp),
+                                 np.arange(bp), indexing="ij")
+    n, m = (bn + ln).ravel(), (bm + lm).ravel()
+    keep = (n < ny) & (m < nx)
+    return n[keep], m[keep]
+
+
+@pytest.mark.parametrize("shape,on,om", [((48, 64), 0, 0), ((50, 71), 3, -5)])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_lens_tile_forms(shape, on, om, kind):
+    ny, nx = shape
+    pitch, f, pl, bp = np.float32(8e-6), np.float32(0.01), np.float32(60e-6), 16
+    block = olref.lens_block(bp, float(pitch), float(f), LAM, olref.PLAY, float(pl) if kind == 1 else 0.0)
+    got = olref.lens_tile(block, ny, nx, on, om, float(pitch), float(f), LAM, float(pl), kind)
+    n, m = np.mgrid[0:ny, 0:nx]
+    ns, ms = (n % bp - on) % bp, (m % bp - om) % bp
+    tiled = block[ns, ms]
+    if kind == 0:
+        # duplicate of the lenslet is gen_complex's field
+        assert np.array_equal(got, tiled)
+        assert np.array_equal(got, olref.lens_array(ny, nx, float(pitch), float(pitch) * bp, float(f),
+                                                     LAM, olref.PLAY, on, om))
+        return
+    wlim = int(np.floor(np.float32(abs(np.float32(LAM) * f * 0.5 / pl)) / pitch))
+    assert 0 < wlim < bp // 2
+    rnd = (np.abs(ms - bp // 2) >= wlim) | (np.abs(ns - bp // 2) >= wlim)
+    assert np.array_equal(got[~rnd], tiled[~rnd])
+    # the random pixels, in traversal order, carry numpy's independent res53 stream
+    tn, tm = _traversal(ny, nx, bp)
+    sel = rnd[tn, tm]
+    tn, tm = tn[sel], tm[sel]
+    v = np.random.RandomState(0).random_sample(len(tn))
+    span = 2 * np.float32(np.pi) if kind == 1 else 0.5 * np.float32(np.pi)
+    ph = (span * (v - 0.5)).astype(np.float32)
+    e = np.exp(1j * ph.astype(np.float64))
+    want = e if kind == 1 else tiled[tn, tm] * e
+    assert np.abs(got[tn, tm] - want).max() < 2e-6
