@@ -78,6 +78,28 @@ void Cgh(Object<_Tp>& obj, std::unique_ptr<PHASE_T[]>& phase, int ny, int nx, PR
     Cgh(scaled, phase, ny, nx, p, p, lambda);
 }
 
+// include/olcgh.h:315-349: the recurrence-formula CGH, coordinates in pixel units
+// (theta = p z / lambda + gamma / 2 (dx^2 + dy^2) cycles, gamma = p / (lambda z)),
+// phase output.  The reference's theta and delta are uninitialised for w > 0; the
+// device evaluates the recurrence's exact value at every pixel.
+template <typename _Tp, typename PHASE_T, typename PREC_T>
+void _CghFresnelDiff(Object<_Tp>& obj, std::unique_ptr<PHASE_T[]>& phase, int ny, int nx, PREC_T p,
+                     PREC_T lambda) {
+    static_assert(std::is_floating_point<PHASE_T>::value, "phase output must be float or double");
+    void* pts = detail::upload_points(obj);
+    const size_t n = (size_t)ny * nx;
+    void* d = detail::staging(0).get(sizeof(float) * n);
+    detail::check(olx_cgh_fresnel_diff_c64(detail::ctx(), (const olx_point*)pts, obj.size, nullptr,
+                                           (float*)d, ny, nx, p, lambda), "_CghFresnelDiff");
+    if constexpr (std::is_same<PHASE_T, float>::value) {
+        detail::check(olx_copy_d2h(detail::ctx(), phase.get(), d, sizeof(float) * n), "_CghFresnelDiff");
+    } else {
+        std::vector<float> t(n);
+        detail::check(olx_copy_d2h(detail::ctx(), t.data(), d, sizeof(float) * n), "_CghFresnelDiff");
+        for (size_t i = 0; i < n; i++) phase[i] = (PHASE_T)t[i];
+    }
+}
+
 // complex output, include/olcgh.h:367-379 (_CghFresnel :296-313)
 template <typename _Tp, typename COMPLEX_T, typename PREC_T>
 void Cgh(Object<_Tp>& obj, std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int ny, int nx, PREC_T p,

@@ -230,6 +230,15 @@ int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U, floa
 /* same sum with FP64 arithmetic and a complex128 output (the FP64 column) */
 int olx_cgh_c128(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U, int64_t ny,
                  int64_t nx, double py, double px, double lambda, uint32_t flags);
+/* _CghFresnelDiff (include/olcgh.h:315-349): point coordinates x, y, z in pixel
+ * units, the phase in cycles theta_j(h, w) = p z_j / lambda + gamma_j / 2 ((w - x_j)^2 +
+ * (h - y_j)^2), gamma_j = p / (lambda z_j), which the reference carries along each row
+ * by theta <- theta + delta, delta <- delta + gamma (its theta and delta are declared
+ * inside the column loop, so only w = 0 is defined there; here every w takes the
+ * recurrence's exact value).  U = sum_j A_j exp(i (2 pi theta_j + phi_j)) (A = 1,
+ * phi = 0 reproduce the reference); outputs d_U (complex64) and / or d_phase (atan2). */
+int olx_cgh_fresnel_diff_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
+                             float* d_phase, int64_t ny, int64_t nx, double p, double lambda);
 /* phase-only / amplitude encodings of a complex64 field (either output may be NULL) */
 int olx_encode_c64(olx_ctx* ctx, const void* d_U, float* d_phase, float* d_amp, int64_t count);
 

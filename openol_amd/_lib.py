@@ -93,6 +93,7 @@ SIGNATURES = {
     "olx_fresnel_response_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i64, _i64],
     "olx_cgh_c64": [_p, _p, _i64, _p, _p, _p, _i64, _i64, _d, _d, _d, _u32],
     "olx_cgh_c128": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _u32],
+    "olx_cgh_fresnel_diff_c64": [_p, _p, _i64, _p, _p, _i64, _i64, _d, _d],
     "olx_encode_c64": [_p, _p, _p, _p, _i64],
     "olx_wrp_c64": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_wrp_method_c64": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _d, ctypes.c_float, _i,

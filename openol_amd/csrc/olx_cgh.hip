@@ -477,3 +477,45 @@ extern "C" int olx_encode_c64(olx_ctx* ctx, const void* d_U, float* d_phase, flo
     k_encode<<<(unsigned)blocks, 256, 0, ctx->stream>>>((const float2*)d_U, d_phase, d_amp, count);
     return check_launch("encode");
 }
+
+// _CghFresnelDiff (include/olcgh.h:315-349): coordinates in pixel units, the phase in
+// cycles carried along each row by theta <- theta + delta, delta <- delta + gamma
+// from theta(0) = p z / lambda + gamma / 2 (dx0^2 + dy0^2), delta(0) = gamma / 2
+// (2 dx0 + 1), gamma = p / (lambda z).  In exact arithmetic that is theta(w) =
+// p z / lambda + p ((w - x)^2 + (h - y)^2) / (2 lambda z): the Fresnel sum of the
+// cloud scaled to metres (p x, p y, p z) plus a per-point piston 2 pi frac(p z /
+// lambda).  Each point is converted once (FP64 piston reduced mod 1, added to the
+// point's phase) and the direct kernel evaluates the sum.
+__global__ void k_diff_points(const olx_point* __restrict__ in, olx_point* __restrict__ out,
+                              int n, double p, double lambda) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    olx_point q = in[j];
+    const double piston = p * q.z / lambda;
+    const double cyc = piston - floor(piston);
+    q.phase = (float)((double)q.phase + 2.0 * M_PI * cyc);
+    q.x *= p;
+    q.y *= p;
+    q.z *= p;
+    out[j] = q;
+}
+
+extern "C" int olx_cgh_fresnel_diff_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n,
+                                        void* d_U, float* d_phase, int64_t ny, int64_t nx,
+                                        double p, double lambda) {
+    if (!ctx) OLX_FAIL(OLX_EINVAL, "cgh diff: null context");
+    if (!(p > 0)) OLX_FAIL(OLX_EINVAL, "cgh diff: pitch must be > 0");
+    if (n < 0 || (n > 0 && !d_pts)) OLX_FAIL(OLX_EINVAL, "cgh diff: bad point array");
+    if (n > INT32_MAX) OLX_FAIL(OLX_EUNSUPPORTED, "cgh diff: more than 2^31-1 points");
+    if (!d_U && !d_phase) OLX_FAIL(OLX_EINVAL, "cgh diff: no output requested");
+    const olx_point* pts = d_pts;
+    if (n > 0) {
+        OLX_TRY(ctx->lay_pts.reserve(sizeof(olx_point) * (size_t)n));
+        k_diff_points<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
+            d_pts, (olx_point*)ctx->lay_pts.p, (int)n, p, lambda);
+        OLX_TRY(check_launch("cgh diff points"));
+        pts = (const olx_point*)ctx->lay_pts.p;
+    }
+    // unit pixel pitch in the scaled cloud: p w - p x = p (w - x)
+    return olx_cgh_c64(ctx, pts, n, d_U, d_phase, nullptr, ny, nx, p, p, lambda, 0);
+}

@@ -402,7 +402,8 @@ def Object2cuda(obj_or_xyz):
 def _out2d(t, ny, nx, dtype, name):
     if t is None:
         return None
-    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous() or tuple(t.shape[-2:]) != (ny, nx):
+    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous() or t.dim() != 2 \
+            or tuple(t.shape) != (ny, nx):
         raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be a contiguous CUDA {dtype} ({ny}, {nx})")
     return t
 
@@ -501,6 +502,24 @@ def WRPMethod_D(obj, U, ny, nx, p, lam, zmin, s=0.9, propmode=AUTO, plimit=None)
 
 # ------------------------------------------------------------------ layered CGH
 _PREC = {"f16": _lib.LAYERED_F16, "bf16": _lib.LAYERED_BF16, "bf16x3": _lib.LAYERED_BF16X3}
+
+
+def CghFresnelDiff(obj, ny, nx, p, lam, U=None, phase=None):
+    """ol::_CghFresnelDiff (include/olcgh.h:315-349): the recurrence-formula CGH of a cloud
+    in pixel units, theta_j = p z_j / lambda + gamma_j / 2 ((w - x_j)^2 + (h - y_j)^2)
+    cycles, gamma_j = p / (lambda z_j), every pixel at the recurrence's exact value.
+    Writes U (complex64) and / or phase (float32, atan2); returns them."""
+    if U is None and phase is None:
+        phase = torch.empty((ny, nx), dtype=torch.float32, device=obj.device)
+    if U is not None:
+        _out2d(U, ny, nx, torch.complex64, "U")
+    if phase is not None:
+        _out2d(phase, ny, nx, torch.float32, "phase")
+    check(_lib.call("olx_cgh_fresnel_diff_c64", obj.device.index, obj.ptr(), obj.size,
+                    _ptr(U) if U is not None else None,
+                    _ptr(phase) if phase is not None else None, ny, nx, p, lam),
+          "CghFresnelDiff")
+    return U, phase
 
 
 def CghLayered(obj, U, ny, nx, py, px, lam, nlayers, zmin, zmax, precision="f16",

@@ -1088,6 +1088,41 @@ int olref_cgh_pixels_c128(const double* x, const double* y, const double* z,
     return 0;
 }
 
+/* _CghFresnelDiff, include/olcgh.h:315-349, complex sum in FP64: coordinates in pixel
+ * units; per point and row, theta (cycles) and delta start at w = 0 from
+ * theta = p z / lambda + gamma / 2 (dx0^2 + dy0^2), delta = gamma / 2 (2 dx0 + 1),
+ * gamma = p / (lambda z), and are carried along the row (theta += delta, delta +=
+ * gamma), the reference's evident intent (it declares them inside the column loop).
+ * amp / phase (may be NULL) weight each point as in the direct sum. */
+int olref_cgh_fresnel_diff(const double* x, const double* y, const double* z, const double* amp,
+                           const double* phase, int64_t n, int64_t ny, int64_t nx, double p,
+                           double lambda, double* out) {
+    if (!x || !y || !z || !out || n < 0 || ny <= 0 || nx <= 0) return -1;
+    cd* U = (cd*)out;
+    #pragma omp parallel for schedule(static)
+    for (int64_t h = 0; h < ny; h++) {
+        for (int64_t w = 0; w < nx; w++) U[h * nx + w] = 0;
+        for (int64_t j = 0; j < n; j++) {
+            double gam = p / (lambda * z[j]);
+            double theta = 0, delta = 0;
+            double a = amp ? amp[j] : 1.0, ph = phase ? phase[j] : 0.0;
+            for (int64_t w = 0; w < nx; w++) {
+                if (w == 0) {
+                    double dx0 = w - x[j], dy0 = h - y[j];
+                    delta = gam / 2 * (2 * dx0 + 1);
+                    theta = p * z[j] / lambda + gam / 2 * (dx0 * dx0 + dy0 * dy0);
+                } else {
+                    theta = theta + delta;
+                    delta = delta + gam;
+                }
+                double t = 2 * M_PI * theta + ph;
+                U[h * nx + w] += a * cos(t) + I * (a * sin(t));
+            }
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* WRP step 1: WRPStep1Pre / WRPStep1 golcgh.h:291-367 (circle, float        */
 /* accumulation into U) and WRPStep1Pre_D / WRPStep1_D :391-468 (square,     */

@@ -141,6 +141,11 @@ int olref_cgh_pixels_c128(const double* x, const double* y, const double* z,
 
 /* WRP step 1, include/golcgh.h:291-378 (circle) and :391-491 (square, flag);
  * p_limit is the pitch used for the window (p for the plain overloads). */
+/* _CghFresnelDiff (include/olcgh.h:315-349): the recurrence CGH of a cloud in pixel
+ * units, complex128 output (amp / phase may be NULL) */
+int olref_cgh_fresnel_diff(const double* x, const double* y, const double* z, const double* amp,
+                           const double* phase, int64_t n, int64_t ny, int64_t nx, double p,
+                           double lambda, double* out);
 int olref_wrp_c64(const double* x, const double* y, const double* z, int64_t n,
                   float* U, int64_t ny, int64_t nx, double p, double lambda,
                   double plane_z, double p_limit, unsigned flags);

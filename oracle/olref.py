@@ -77,6 +77,7 @@ def lib():
             "olref_cgh_c128": [p, p, p, p, p, i64, p, i64, i64, d, d, d, u32],
             "olref_cgh_pixels_c128": [p, p, p, p, p, i64, p, p, i64, p, d, d, d, u32],
             "olref_wrp_c64": [p, p, p, i64, p, i64, i64, d, d, d, d, u32],
+            "olref_cgh_fresnel_diff": [p, p, p, p, p, i64, i64, i64, d, d, p],
             "olref_wrp_c128": [p, p, p, i64, p, i64, i64, d, d, d, d, u32],
             "olref_wrp_pixels_c128": [p, p, p, i64, p, p, i64, p, d, d, d, d, u32],
             "olref_wrp_plane_z": [i64, i64, d, d, d, d, ctypes.c_float, ctypes.c_int],
@@ -310,6 +311,16 @@ def cgh_pixels(x, y, z, ph, pw, py, px, lam, amp=None, phase=None, flags=0):
     _check(lib().olref_cgh_pixels_c128(_ptr(x), _ptr(y), _ptr(z), _ptr(amp), _ptr(phase),
                                        len(x), _ptr(ph), _ptr(pw), len(ph), _ptr(out), py, px,
                                        lam, flags), "cgh_pixels")
+    return out
+
+
+def cgh_fresnel_diff(x, y, z, ny, nx, p, lam, amp=None, phase=None):
+    """_CghFresnelDiff (include/olcgh.h:315-349), the recurrence carried along each row
+    in FP64; complex128 field (the reference outputs its atan2)."""
+    x, y, z, amp, phase = _pts(x, y, z, amp, phase)
+    out = np.empty((ny, nx), dtype=np.complex128)
+    _check(lib().olref_cgh_fresnel_diff(_ptr(x), _ptr(y), _ptr(z), _ptr(amp), _ptr(phase), len(x),
+                                        ny, nx, p, lam, _ptr(out)), "cgh_fresnel_diff")
     return out
 
 

@@ -187,6 +187,18 @@ int main(int argc, char** argv) {
         std::cout << "error: " << e.what() << "\n";
         return 1;
     }
+    // _CghFresnelDiff (include/olcgh.h:315-349): the cloud in pixel units
+    try {
+        ol::Object<double> pix(m);
+        for (int j = 0; j < m; j++)
+            pix[j] = ol::Vect3<double>(xyz[3 * j] / p, xyz[3 * j + 1] / p, xyz[3 * j + 2] / p);
+        auto ph = std::make_unique<float[]>((size_t)n * n);
+        ol::_CghFresnelDiff(pix, ph, n, n, p, lambda);
+        save(dir + "/cgh_diff_phase.bin", ph.get(), (size_t)n * n);
+    } catch (const std::exception& e) {
+        std::cout << "error: " << e.what() << "\n";
+        return 1;
+    }
     // tests/Prop_test.cpp:46 unchanged, with its tests/para.h:8-18 parameters: the
     // reference's 3840 x 2160 demo field (padded 7680 x 4320, mixed radix)
     {

@@ -276,3 +276,30 @@ def test_wrp_method_pipeline():
                or not olref.fresnel_prop_check(ny, nx, P, P, LAM, pz)) else 1
     ref = olref.asm_prop(w, P, P, LAM, pz) if ch == 0 else olref.fresnel_prop(w, P, P, LAM, pz)
     assert rel(U.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("shape,n", [((128, 256), 2000), ((97, 61), 300)])
+def test_cgh_fresnel_diff_matches_fp64_oracle(shape, n):
+    """_CghFresnelDiff (include/olcgh.h:315-349): cloud in pixel units, the p z / lambda
+    piston and the row recurrence's exact value; complex field <= 1e-4 against the FP64
+    oracle (which carries the recurrence), phase output = atan2 of the field."""
+    ny, nx = shape
+    p = P
+    pts = synth.random_points(n, seed=44, xmax=nx, ymax=ny, zmin=0.10 / p, zmax=0.20 / p)
+    for k in ("amp", "phase"):
+        pts[k] = pts[k].astype(np.float32).astype(np.float64)
+    o = ol().Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ph = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    ol().CghFresnelDiff(o, ny, nx, p, LAM, U=U, phase=ph)
+    ref = olref.cgh_fresnel_diff(pts["x"], pts["y"], pts["z"], ny, nx, p, LAM, pts["amp"], pts["phase"])
+    assert rel(U.cpu().numpy(), ref) < 1e-4
+    u = U.cpu().numpy()
+    assert np.abs(np.angle(np.exp(1j * (ph.cpu().numpy() - np.angle(u))))).max() < 1e-5
+    # unit amplitude, no per-point phase: the reference's own sum
+    o1 = ol().Object(pts["x"], pts["y"], pts["z"])
+    _, ph1 = ol().CghFresnelDiff(o1, ny, nx, p, LAM)
+    ref1 = olref.cgh_fresnel_diff(pts["x"], pts["y"], pts["z"], ny, nx, p, LAM)
+    w = np.abs(ref1)
+    dphi = np.abs(np.angle(np.exp(1j * (ph1.cpu().numpy() - np.angle(ref1)))))
+    assert np.sum(dphi * w) / np.sum(w) < 1e-4

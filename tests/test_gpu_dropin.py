@@ -33,6 +33,11 @@ def test_dropin_program_matches_oracle(tmp_path):
     assert "allocation changed" not in r.stdout
     rd = lambda name, dt=np.complex64, k=1: np.fromfile(tmp_path / f"{name}.bin", dtype=dt).reshape(k * n, k * n)
     rel = olref.rel_l2
+    # _CghFresnelDiff of the same cloud in pixel units: phase, amplitude-weighted
+    ref = olref.cgh_fresnel_diff(xyz[:, 0] / P, xyz[:, 1] / P, xyz[:, 2] / P, n, n, P, LAM)
+    got = np.fromfile(tmp_path / "cgh_diff_phase.bin", dtype=np.float32).reshape(n, n)
+    dphi = np.abs(np.angle(np.exp(1j * (got - np.angle(ref)))))
+    assert np.sum(dphi * np.abs(ref)) / np.sum(np.abs(ref)) < 1e-4
     # tests/Prop_test.cpp:46: FresnelProp(u, 2160, 3840, p, p, lambda, -d), float parameters
     f4k = np.fromfile(tmp_path / "fresnel4k.bin", dtype=np.complex64).reshape(2160, 3840)
     pf, lf, zf = (float(np.float32(v)) for v in (3.74e-6, 532e-9, 0.1))

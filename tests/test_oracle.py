@@ -357,3 +357,24 @@ def test_prec_f32_variant_is_informational_only():
     f64 = olref.asm_prop(u, 8e-6, 8e-6, LAM, 0.1)
     assert rel(f64, d) < 1e-6
     assert rel(f32, d) > 1e-4
+
+
+def test_cgh_fresnel_diff_recurrence():
+    """_CghFresnelDiff's row recurrence: one point gives the closed form
+    theta = p z / lambda + gamma / 2 ((w - x)^2 + (h - y)^2) cycles at every pixel, and a
+    cloud equals the direct Fresnel sum of the cloud scaled to metres with the
+    p z / lambda piston as a per-point phase (independent code paths)."""
+    p, ny, nx = 8e-6, 24, 40
+    x, y, z = np.array([13.3]), np.array([7.6]), np.array([0.125 / p])
+    got = olref.cgh_fresnel_diff(x, y, z, ny, nx, p, LAM)
+    hh, ww = np.mgrid[0:ny, 0:nx]
+    gam = p / (LAM * z[0])
+    theta = p * z[0] / LAM + gam / 2 * ((ww - x[0]) ** 2 + (hh - y[0]) ** 2)
+    assert np.abs(got - np.exp(2j * np.pi * theta)).max() < 1e-8
+    pts = synth.random_points(50, seed=17, xmax=nx, ymax=ny, zmin=0.10 / p, zmax=0.20 / p)
+    amp = pts["amp"]
+    got = olref.cgh_fresnel_diff(pts["x"], pts["y"], pts["z"], ny, nx, p, LAM, amp, pts["phase"])
+    piston = 2 * np.pi * np.mod(p * pts["z"] / LAM, 1.0)
+    ref = olref.cgh(p * pts["x"], p * pts["y"], p * pts["z"], ny, nx, p, p, LAM, amp,
+                    pts["phase"] + piston, dtype=np.complex128)
+    assert rel(got, ref) < 1e-8
