@@ -32,11 +32,11 @@ void* upload_points(const Object<_Tp>& obj) {
 // complex hologram on the device for obj, copied into u (complex<C>)
 template <typename _Tp, typename C>
 void cgh_complex(const Object<_Tp>& obj, std::unique_ptr<std::complex<C>[]>& u, int ny, int nx,
-                 double py, double px, double lambda) {
+                 double py, double px, double lambda, uint32_t flags = 0) {
     void* pts = upload_points(obj);
     on_device(u, (int64_t)ny * nx, [&](void* d) {
         check(olx_cgh_c64(ctx(), (const olx_point*)pts, obj.size, d, nullptr, nullptr, ny, nx, py,
-                          px, lambda, 0), "Cgh");
+                          px, lambda, flags), "Cgh");
     });
 }
 
@@ -51,7 +51,7 @@ void Cgh(Object<_Tp>& obj, std::unique_ptr<PHASE_T[]>& phase, int ny, int nx, PR
     const size_t n = (size_t)ny * nx;
     void* d = detail::staging(0).get(sizeof(float) * n);
     detail::check(olx_cgh_c64(detail::ctx(), (const olx_point*)pts, obj.size, nullptr, (float*)d,
-                              nullptr, ny, nx, py, px, lambda, 0), "Cgh");
+                              nullptr, ny, nx, py, px, lambda, detail::prec_flags<PREC_T>()), "Cgh");
     if constexpr (std::is_same<PHASE_T, float>::value) {
         detail::check(olx_copy_d2h(detail::ctx(), phase.get(), d, sizeof(float) * n), "Cgh");
     } else {
@@ -104,12 +104,12 @@ void _CghFresnelDiff(Object<_Tp>& obj, std::unique_ptr<PHASE_T[]>& phase, int ny
 template <typename _Tp, typename COMPLEX_T, typename PREC_T>
 void Cgh(Object<_Tp>& obj, std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int ny, int nx, PREC_T p,
          PREC_T lambda) {
-    detail::cgh_complex(obj, u, ny, nx, p, p, lambda);
+    detail::cgh_complex(obj, u, ny, nx, p, p, lambda, detail::prec_flags<PREC_T>());
 }
 template <typename _Tp, typename COMPLEX_T, typename PREC_T>
 void Cgh(Object<_Tp>& obj, std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int ny, int nx,
          PREC_T py, PREC_T px, PREC_T lambda) {
-    detail::cgh_complex(obj, u, ny, nx, py, px, lambda);
+    detail::cgh_complex(obj, u, ny, nx, py, px, lambda, detail::prec_flags<PREC_T>());
 }
 
 // include/olcgh.h:381-399: the CPU WRPStep1 has no window and ignores d (its sum is

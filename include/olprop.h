@@ -5,7 +5,8 @@
 // std::unique_ptr<std::complex<COMPLEX_T>[]> allocation, is staged to HBM, runs
 // through the fused HIP kernels of libolx.so, and is copied back in place.  The
 // device computes complex64 fields with FP64 phase arithmetic (PREC_T=double
-// semantics); COMPLEX_T = double fields are rounded to complex64 for the device.
+// semantics, unless OL_PREC_T_FLOAT_COMPAT asks for the reference's float arithmetic
+// where PREC_T = float); COMPLEX_T = double fields are rounded to complex64 for the device.
 // Device overloads mirror include/golprop.h on ol::device_ptr<std::complex<float>[]>.
 // Errors throw std::runtime_error; nothing is printed and nothing exits.
 #ifndef OLPROP_H
@@ -70,12 +71,26 @@ bool FresnelPropCheck(size_t ny, size_t nx, PREC_T dy, PREC_T dx, PREC_T lambda,
 }
 
 // ------------------------------------------------------------------ host API
-// include/olprop.h:227-248 (net scale 1)
+// include/olprop.h:227-248 (net scale 1).  Defining OL_PREC_T_FLOAT_COMPAT before
+// including the headers makes PREC_T = float calls use the reference's float
+// arithmetic for the transfer function (OLX_PREC_F32); otherwise every PREC_T gets
+// the FP64 phase.
+namespace detail {
+template <typename PREC_T>
+constexpr uint32_t prec_flags() {
+#ifdef OL_PREC_T_FLOAT_COMPAT
+    return std::is_same<PREC_T, float>::value ? OLX_PREC_F32 : 0u;
+#else
+    return 0u;
+#endif
+}
+}  // namespace detail
 template <typename PREC_T = double, typename COMPLEX_T>
 void AsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int64_t width,
              PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
     detail::on_device(u, height * width, [&](void* d) {
-        detail::check(olx_asm_c64(detail::ctx(), d, height, width, dy, dx, lambda, z, 0), "AsmProp");
+        detail::check(olx_asm_c64(detail::ctx(), d, height, width, dy, dx, lambda, z,
+                                  detail::prec_flags<PREC_T>()), "AsmProp");
     });
 }
 

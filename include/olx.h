@@ -53,6 +53,12 @@ extern "C" {
                                     net scale circconv / M^4 (include/olprop.h:338-408) */
 #define OLX_EXTRA_INV_M2 0x2000u /* ASM: multiply the result by one more 1/M^2, the net scale of
                                     mAsmProp (include/molprop.h:80-101) */
+#define OLX_PREC_F32 0x8000u      /* ASM and Fresnel CGH with the reference's PREC_T = float
+                                    arithmetic (its tests' default, tests/para.h:8): float
+                                    frequencies / coordinates and phases, cos / sin of the float
+                                    phase (AsmTransferF include/olprop.h:172-189, _CghFresnel
+                                    include/olcgh.h:296-313).  Far less accurate than the default
+                                    (FP64 phases); for reproducing float-PREC_T results */
 #define OLX_SHIFTED_CPU 0x1000u  /* shifted ASM with the CPU shiftedAsmProp semantics: the shifted
                                     band-limit window and net scale 1/M^2 (include/olprop.h:250-330) */
 

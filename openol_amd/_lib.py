@@ -22,6 +22,7 @@ CGH_ACCUMULATE = 0x400
 WRP_SQUARE = 0x200
 ARSS_CPU = 0x800
 SHIFTED_CPU = 0x1000
+PREC_F32 = 0x8000
 EXTRA_INV_M2 = 0x2000
 TILT_SOURCE, TILT_REFERENCE = 0, 1
 STEP_RECORD, STEP_PLAY = 0, 1

@@ -392,6 +392,54 @@ __global__ void k_encode(const float2* __restrict__ U, float* __restrict__ ph,
     }
 }
 
+// _CghFresnel with PREC_T = float (include/olcgh.h:296-313; the reference tests'
+// default): dx = px w - x_j, dy = py h - y_j, dz = z_j in float, theta = 2 pi (dx^2 +
+// dy^2) / (2 dz lambda) formed in double from the float terms and stored to float,
+// cos / sin of that float angle accumulated in float.  Each float operation rounds
+// (no contraction).  One thread per pixel; points staged through LDS in tiles.
+constexpr int F32C_NT = 256, F32C_TILE = 256;
+__global__ void __launch_bounds__(F32C_NT)
+k_cgh_f32compat(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
+                float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx,
+                float pyf, float pxf, float lf, int acc) {
+#pragma clang fp contract(off)
+    __shared__ float4 sp[F32C_TILE];
+    const int64_t pix = blockIdx.x * (int64_t)F32C_NT + threadIdx.x;
+    const bool valid = pix < (int64_t)ny * nx;
+    const int h = valid ? (int)(pix / nx) : 0, w = valid ? (int)(pix - (int64_t)h * nx) : 0;
+    const float xw = pxf * (float)w, yh = pyf * (float)h;
+    float re = 0.f, im = 0.f;
+    for (int j0 = 0; j0 < n; j0 += F32C_TILE) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < F32C_TILE && j0 + k < n; k += F32C_NT) {
+            const olx_point q = pts[j0 + k];
+            sp[k] = make_float4((float)q.x, (float)q.y, (float)q.z, q.amp);
+        }
+        __syncthreads();
+        const int m = min(F32C_TILE, n - j0);
+        for (int k = 0; k < m; k++) {
+            const float4 q = sp[k];
+            const float dx = xw - q.x, dy = yh - q.y, dz = q.z;
+            const float th = (float)(2.0 * M_PI * (double)(dx * dx + dy * dy) / (double)(2.0f * dz * lf));
+            float sn, cs;
+            sincosf(th, &sn, &cs);
+            re += cs;
+            im += sn;
+        }
+    }
+    if (!valid) return;
+    float2 v = make_float2(re, im);
+    if (U) {
+        if (acc) {
+            const float2 o = U[pix];
+            v = make_float2(o.x + v.x, o.y + v.y);
+        }
+        U[pix] = v;
+    }
+    if (phase_out) phase_out[pix] = atan2f(v.y, v.x);
+    if (amp_out) amp_out[pix] = sqrtf(v.x * v.x + v.y * v.y);
+}
+
 static int check_cgh(olx_ctx* ctx, const olx_point* pts, int64_t n, int64_t ny, int64_t nx,
                      double py, double px, double lambda) {
     if (!ctx) OLX_FAIL(OLX_EINVAL, "null context");
@@ -415,6 +463,16 @@ extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     if ((flags & OLX_CGH_ACCUMULATE) && !d_U) OLX_FAIL(OLX_EINVAL, "cgh: accumulate needs d_U");
     const int acc = (flags & OLX_CGH_ACCUMULATE) ? 1 : 0;
     const bool exact = (flags & OLX_CGH_EXACT_R) != 0;
+    if (flags & OLX_PREC_F32) {
+        if (exact) OLX_FAIL(OLX_EINVAL, "cgh: the float-PREC_T form is the Fresnel sum only");
+        const int64_t np = ny * nx;
+        if (np / F32C_NT >= INT32_MAX) OLX_FAIL(OLX_EUNSUPPORTED, "cgh: hologram too large");
+        TimedLaunch tl(ctx, "cgh_f32compat");
+        k_cgh_f32compat<<<(unsigned)((np + F32C_NT - 1) / F32C_NT), F32C_NT, 0, ctx->stream>>>(
+            d_pts, (int)n, (float2*)d_U, d_phase, d_amp, (int)ny, (int)nx, (float)py, (float)px,
+            (float)lambda, acc);
+        return check_launch("cgh_f32compat");
+    }
     const int tx = exact ? CGH_TX : SEP_TX, ty = exact ? CGH_TY : SEP_TY;
     dim3 grid((unsigned)((nx + tx - 1) / tx), (unsigned)((ny + ty - 1) / ty));
     // fewer tiles than CUs: split the points into slices of at least 2048 (8 chunks),

@@ -119,32 +119,13 @@ k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
         for (int r = 0; r < RL; r++) lds[c + lpad(last_index<P>(t, b, r)) * CB] = v[b * RL + r];
     __syncthreads();
     // transfer function, then inverse transform
-    double Acol = 0.0;  // 1/lambda^2 - u^2 of this column
-    double pc = 0.0;    // K_ASHIFT: ox u
-    bool ucut = false;
-    const float2 sxc = (MODE == K_TABLE && valid) ? a.sx[col] : make_float2(0.f, 0.f);
-    if (MODE != K_TABLE) {
-        const int fx = col < Mx / 2 ? col : col - Mx;
-        const double u = (double)fx * a.du;
-        Acol = a.inv_l2 - u * u;
-        ucut = a.bandlimit && !(fabs(MODE == K_ASHIFT ? u - a.u0 : u) * 2 < a.uw);
-        pc = a.ox * u;
-    }
+    const ColH ch = col_h<MODE>(a, valid ? col : 0, Mx);
     // multiply the spectrum in LDS; a short loop keeps few FP64 temporaries live
 #pragma unroll 2
     for (int q = 0; q < P::E; q++) {
         const int i = t + P::T * q;
         float2* p = &lds[c + lpad(i) * CB];
-        float2 h;
-        if constexpr (MODE == K_TABLE) {
-            h = cmul(sxc, a.sy[i]);
-            if (a.adjoint) h = cconj(h);
-        } else if constexpr (MODE == K_ASHIFT) {
-            h = asm_h<true>(a, Acol, ucut, i, L, pc);
-        } else {
-            h = asm_h(a, Acol, ucut, i, L);
-        }
-        *p = cmul(*p, h);
+        *p = cmul(*p, trans_h<MODE>(a, ch, i, L));
     }
     __syncthreads();
     stage_load_lds<P, 0, CB>(v, t, lds, c);
@@ -393,9 +374,11 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
             }
     } else {
         constexpr bool SH = MODE == K_ASHIFT;
+        constexpr bool F32 = MODE == K_ASM32;
         const int Mx = 2 * a.nx;
         const int fxa = cola < Mx / 2 ? cola : cola - Mx, fxb = colb < Mx / 2 ? colb : colb - Mx;
         const double ua = (double)fxa * a.du, ub = (double)fxb * a.du;
+        const float ufa = (float)fxa * a.duf, ufb = (float)fxb * a.duf;
         const double Aa = a.inv_l2 - ua * ua, Ab = a.inv_l2 - ub * ub;
         const bool cuta = a.bandlimit && !(fabs(SH ? ua - a.u0 : ua) * 2 < a.uw);
         const bool cutb = a.bandlimit && !(fabs(SH ? ub - a.u0 : ub) * 2 < a.uw);
@@ -413,9 +396,9 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
         float4* tab = reinterpret_cast<float4*>(lds);
         static_assert((L / 2 + 1) * 2 <= PF::LP, "mirror table fits in LDS");
         if constexpr (!SH) {
-            const float2 hma = asm_h(a, Aa, cuta, L / 2, L);
+            const float2 hma = F32 ? asm_h_f32(a, ufa, cuta, L / 2, L) : asm_h(a, Aa, cuta, L / 2, L);
             __builtin_amdgcn_sched_barrier(0);
-            const float2 hmb = asm_h(a, Ab, cutb, L / 2, L);
+            const float2 hmb = F32 ? asm_h_f32(a, ufb, cutb, L / 2, L) : asm_h(a, Ab, cutb, L / 2, L);
             const int e0 = t + (((t - 1) >> 31) & (L / 2));
             tab[e0 * KR + c] = make_float4(hma.x, hma.y, hmb.x, hmb.y);
             __builtin_amdgcn_sched_barrier(0);
@@ -426,8 +409,8 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
             for (int r = 0; r < RLF; r++) {
                 const int i = last_index<PF>(t, b, r);
                 if (!SH && b + r * (E / RLF) >= E / 2) continue;
-                const float2 ha = asm_h<SH>(a, Aa, cuta, i, L, pca);
-                const float2 hb = asm_h<SH>(a, Ab, cutb, i, L, pcb);
+                const float2 ha = F32 ? asm_h_f32(a, ufa, cuta, i, L) : asm_h<SH>(a, Aa, cuta, i, L, pca);
+                const float2 hb = F32 ? asm_h_f32(a, ufb, cutb, i, L) : asm_h<SH>(a, Ab, cutb, i, L, pcb);
                 va[b * RLF + r] = cmul(va[b * RLF + r], ha);
                 vb[b * RLF + r] = cmul(vb[b * RLF + r], hb);
                 if (!SH) tab[i * KR + c] = make_float4(ha.x, ha.y, hb.x, hb.y);
@@ -635,6 +618,11 @@ static void launch_cols_conv(const TileCfg& g, const PropArgs& a, float2* W, con
                              hipStream_t s) {
     if (a.mode == K_ASM) launch_cols_mode<L, KR, K_ASM>(g, a, W, tw, s);
     else if (a.mode == K_ASHIFT) launch_cols_mode<L, KR, K_ASHIFT>(g, a, W, tw, s);
+    else if (a.mode == K_ASM32) {
+        // the default radix orders only (a compatibility mode, not tuned)
+        const int grid = 2 * a.nx / (2 * KR), nt = KR * Plan<L>::T;
+        k_cols_conv2<L, KR, K_ASM32, REM_LAST, REM_LAST><<<grid, nt, 0, s>>>(W, a, tw);
+    }
     else launch_cols_mode<L, KR, K_TABLE>(g, a, W, tw, s);
 }
 template <int L, int KR>
@@ -898,6 +886,8 @@ static void launch_cols_cb(const PropArgs& a, float2* W, const float2* tw, hipSt
         k_cols_conv<L, CB, K_ASM><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
     else if (a.mode == K_ASHIFT)
         k_cols_conv<L, CB, K_ASHIFT><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
+    else if (a.mode == K_ASM32)
+        k_cols_conv<L, CB, K_ASM32><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
     else
         k_cols_conv<L, CB, K_TABLE><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
 }
@@ -1148,6 +1138,16 @@ static int asm_args(PropArgs& a, int64_t ny, int64_t nx, double dy, double dx, d
     a.zs = a.adjoint ? -z : z;
     a.bandlimit = (flags & OLX_BANDLIMIT) ? 1 : 0;
     if (flags & OLX_EXTRA_INV_M2) a.extra_scale = 1.0 / ((double)nx2 * (double)ny2);
+    if (flags & OLX_PREC_F32) {
+        // include/olprop.h:227-233 with PREC_T = float: du = 1 / (dx * width2) and
+        // 1 / lambda^2 in float (AsmTransferF :176)
+        a.mode = K_ASM32;
+        const float dxf = (float)dx, dyf = (float)dy, lf = (float)lambda;
+        a.duf = 1 / (dxf * (float)nx2);
+        a.dvf = 1 / (dyf * (float)ny2);
+        a.tmpf = 1 / (lf * lf);
+        a.zf = (float)z;
+    }
     if (a.bandlimit) {
         // shiftedAsmTransferF at ox = oy = 0 (include/olprop.h:258-270): uw = ulp + ulm
         const double ulm = 1.0 / std::sqrt(std::pow(0.0 - 0.5f / a.du, -2) * z * z + 1.0f) / lambda;

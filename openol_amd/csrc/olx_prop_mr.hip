@@ -229,30 +229,10 @@ k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P,
     for (int tr = 0; tr < nb; tr++) {
         const int col = col0 + tr;
         if (col >= Mx) break;
-        double Acol = 0.0, pc = 0.0;
-        bool ucut = false;
-        float2 sxc = make_float2(0.f, 0.f);
-        if constexpr (MODE == K_TABLE) {
-            sxc = a.sx[col];
-        } else {
-            const int fx = col < Mx / 2 ? col : col - Mx;
-            const double u = (double)fx * a.du;
-            Acol = a.inv_l2 - u * u;
-            ucut = a.bandlimit && !(fabs(MODE == K_ASHIFT ? u - a.u0 : u) * 2 < a.uw);
-            pc = a.ox * u;
-        }
+        const ColH ch = col_h<MODE>(a, col, Mx);
         for (int i = tid; i < My; i += NT) {
-            float2 h;
-            if constexpr (MODE == K_TABLE) {
-                h = cmul(sxc, a.sy[i]);
-                if (a.adjoint) h = cconj(h);
-            } else if constexpr (MODE == K_ASHIFT) {
-                h = asm_h<true>(a, Acol, ucut, i, My, pc);
-            } else {
-                h = asm_h(a, Acol, ucut, i, My);
-            }
             float2* p = &lds[tr * LP + lpad(i)];
-            *p = cmul(*p, h);
+            *p = cmul(*p, trans_h<MODE>(a, ch, i, My));
         }
     }
     __syncthreads();
@@ -438,6 +418,8 @@ int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, flo
                 k_mr_cols_conv<NT.value, LC.value, K_ASM><<<grid, NT.value, 0, s>>>(W, a, py, nb);
             else if (a.mode == K_ASHIFT)
                 k_mr_cols_conv<NT.value, LC.value, K_ASHIFT><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+            else if (a.mode == K_ASM32)
+                k_mr_cols_conv<NT.value, LC.value, K_ASM32><<<grid, NT.value, 0, s>>>(W, a, py, nb);
             else
                 k_mr_cols_conv<NT.value, LC.value, K_TABLE><<<grid, NT.value, 0, s>>>(W, a, py, nb);
         });

@@ -7,7 +7,9 @@
 
 namespace olx {
 
-enum { K_ASM = 0, K_TABLE = 1, K_ASHIFT = 2 };
+// transfer functions: ASM (FP64 phase), table product (Fresnel / ARSS), shifted ASM,
+// ASM with the reference's PREC_T = float arithmetic (OLX_PREC_F32)
+enum { K_ASM = 0, K_TABLE = 1, K_ASHIFT = 2, K_ASM32 = 3 };
 
 struct PropArgs {
     int ny, nx;  // unpadded field
@@ -17,6 +19,7 @@ struct PropArgs {
     double du, dv, inv_l2, zs, uw, vw;  // ASM transfer function (zs: z, -z for the
                                         // adjoint) and band limit
     double u0, v0, ox, oy;  // K_ASHIFT: window centres and the shift's linear phase
+    float duf, dvf, tmpf, zf;  // K_ASM32: du, dv, 1/lambda^2 and z rounded as PREC_T = float
     int ss, ds;             // row strides (elements) of the source and destination
     int accum;              // pass C adds into the destination instead of storing
     const float2* sx;                  // K_TABLE: natural-order spectra, length Mx / My
@@ -85,6 +88,65 @@ __device__ __forceinline__ float2 asm_h(const PropArgs& a, double A, bool ucut, 
                           : (float)__builtin_amdgcn_fract(w * a.zs);
     const float m = keep ? 1.f : 0.f;
     return make_float2(__builtin_amdgcn_cosf(t) * m, __builtin_amdgcn_sinf(t) * m);
+}
+
+// AsmTransferF with PREC_T = float (include/olprop.h:172-189): v = (n - ny/2) dv,
+// w = sqrt(1/lambda^2 - u^2 - v^2) in float, each operation rounded (no contraction),
+// the phase 2 pi w z formed in double and stored to float, then cos / sin of that
+// float phase.  uf is the column's float u.  The band-limit window (on the FP64
+// frequencies, as asm_h) and the evanescent cut zero the value; the adjoint takes
+// the conjugate.
+__device__ __forceinline__ float2 asm_h_f32(const PropArgs& a, float uf, bool ucut, int i, int My) {
+#pragma clang fp contract(off)
+    const int fy = i < My / 2 ? i : i - My;
+    const float v = (float)fy * a.dvf;
+    const float arg = a.tmpf - uf * uf - v * v;
+    const double vd = (double)fy * a.dv;
+    const bool keep = (!ucut) & ((!a.bandlimit) | (fabs(vd) * 2 < a.vw)) & (arg >= 0.f);
+    const float w = sqrtf(fmaxf(arg, 0.f));
+    const float ph = (float)(2.0 * M_PI * (double)w * (double)a.zf);
+    float s, c;
+    sincosf(ph, &s, &c);
+    const float m = keep ? 1.f : 0.f;
+    return make_float2(c * m, (a.adjoint ? -s : s) * m);
+}
+
+// one column's transfer-function state and the value at spectrum row i, for every
+// mode (the untiled and mixed-radix column passes)
+struct ColH {
+    double A = 0.0;   // 1/lambda^2 - u^2 (FP64 modes)
+    double pc = 0.0;  // K_ASHIFT: ox u
+    float uf = 0.f;   // K_ASM32: the float u
+    bool ucut = false;
+    float2 sx = {0.f, 0.f};  // K_TABLE: the column's spectrum value
+};
+template <int MODE>
+__device__ __forceinline__ ColH col_h(const PropArgs& a, int col, int Mx) {
+    ColH c;
+    if constexpr (MODE == K_TABLE) {
+        c.sx = a.sx[col];
+    } else {
+        const int fx = col < Mx / 2 ? col : col - Mx;
+        const double u = (double)fx * a.du;
+        c.A = a.inv_l2 - u * u;
+        c.ucut = a.bandlimit && !(fabs(MODE == K_ASHIFT ? u - a.u0 : u) * 2 < a.uw);
+        c.pc = a.ox * u;
+        c.uf = (float)fx * a.duf;
+    }
+    return c;
+}
+template <int MODE>
+__device__ __forceinline__ float2 trans_h(const PropArgs& a, const ColH& c, int i, int My) {
+    if constexpr (MODE == K_TABLE) {
+        const float2 h = cmul(c.sx, a.sy[i]);
+        return a.adjoint ? cconj(h) : h;
+    } else if constexpr (MODE == K_ASHIFT) {
+        return asm_h<true>(a, c.A, c.ucut, i, My, c.pc);
+    } else if constexpr (MODE == K_ASM32) {
+        return asm_h_f32(a, c.uf, c.ucut, i, My);
+    } else {
+        return asm_h(a, c.A, c.ucut, i, My);
+    }
 }
 
 __device__ __forceinline__ float2 loss_grad(float2 x, float t, double mi, double mt, double c1,

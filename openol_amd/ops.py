@@ -40,10 +40,12 @@ def _field(u, height, width, name="u"):
 
 
 # ------------------------------------------------------------------ propagation
-def AsmProp(u, height, width, dy, dx, lam, z, bandlimit=False, adjoint=False):
-    """ol::AsmProp (include/olprop.h:227-248): in place, net scale 1."""
+def AsmProp(u, height, width, dy, dx, lam, z, bandlimit=False, adjoint=False, prec_f32=False):
+    """ol::AsmProp (include/olprop.h:227-248): in place, net scale 1.  prec_f32 computes
+    the transfer function with the reference's PREC_T = float arithmetic."""
     _field(u, height, width)
-    flags = (_lib.BANDLIMIT if bandlimit else 0) | (_lib.ADJOINT if adjoint else 0)
+    flags = (_lib.BANDLIMIT if bandlimit else 0) | (_lib.ADJOINT if adjoint else 0) | \
+        (_lib.PREC_F32 if prec_f32 else 0)
     check(_lib.call("olx_asm_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z, flags),
           "AsmProp")
     return u
@@ -409,18 +411,20 @@ def _out2d(t, ny, nx, dtype, name):
 
 
 def Cgh(obj, out, ny, nx, py, px, lam, exact_r=False, phase_out=None, amp_out=None,
-        accumulate=False):
+        accumulate=False, prec_f32=False):
     """ol::Cgh / gCgh (include/olcgh.h:351-379, include/golcgh.h:127-145).
 
     out is a complex64 tensor (complex hologram, _CghFresnel include/olcgh.h:296-313)
     or a float32 tensor (phase-only hologram via atan2, include/olcgh.h:275-295).
-    Optional phase_out / amp_out receive the encodings in the same pass."""
+    Optional phase_out / amp_out receive the encodings in the same pass.  prec_f32
+    evaluates the Fresnel sum with the reference's PREC_T = float arithmetic."""
     if out is not None and out.dtype == torch.float32:
         phase_out, out = out, None
     _out2d(out, ny, nx, torch.complex64, "out")
     _out2d(phase_out, ny, nx, torch.float32, "phase_out")
     _out2d(amp_out, ny, nx, torch.float32, "amp_out")
-    flags = (_lib.CGH_EXACT_R if exact_r else 0) | (_lib.CGH_ACCUMULATE if accumulate else 0)
+    flags = (_lib.CGH_EXACT_R if exact_r else 0) | (_lib.CGH_ACCUMULATE if accumulate else 0) | \
+        (_lib.PREC_F32 if prec_f32 else 0)
     nul = lambda t: None if t is None else _ptr(t)
     check(_lib.call("olx_cgh_c64", obj.device.index, obj.ptr(), obj.size, nul(out),
                                   nul(phase_out), nul(amp_out), ny, nx, py, px, lam, flags), "Cgh")
