@@ -155,6 +155,21 @@ int olx_asm_c64_noalloc(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double 
  * gFresnelProp (include/golprop.h:167-198), gFresnelPropBandLimit (:258-302). */
 int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
                     double lambda, double z, uint32_t flags);
+/* Batched propagation with fused factor fields.  d_u holds nb fields of ny x nx,
+ * field k at d_u + k ny nx; each is replaced in place by post * Prop(pre * u_k),
+ * where Prop is olx_asm_c64 (mode OLX_MODE_ASM, its flags) or olx_fresnel_c64
+ * (OLX_MODE_FRESNEL, its flags) and pre / post are ny x nx complex64 factor fields
+ * shared by every field (null: 1).  The factors are the optical-element multiplies
+ * that the reference runs as passes of their own before or after a propagation
+ * (Lensarry::mult_complex, include/Lensarry.h:262-270; Diffuser's random phase,
+ * include/Diffuser.h:173-326; mult, include/golutils.h:123-143): here pass A
+ * multiplies as it loads and pass C as it stores.  nb in [1, 65535] with the
+ * padded intermediate (nb ny 2nx complex64) at most 2 GiB; fields follow the
+ * propagation size rule.  No counterpart for nb > 1 in the reference, which
+ * propagates one field per call. */
+int olx_prop_batch_c64(olx_ctx* ctx, int mode, void* d_u, int64_t nb, int64_t ny, int64_t nx,
+                       double dy, double dx, double lambda, double z, uint32_t flags,
+                       const void* d_pre, const void* d_post);
 /* ARSSFresnelProp: scaled/shifted Fresnel, src -> dst (may alias).  Replaces
  * ol::ARSSFresnelProp (include/golprop.h:941-995). */
 int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx,

@@ -29,7 +29,7 @@ namespace olx {
 // index of (row h, padded column m) in the column-blocked intermediate
 __device__ __forceinline__ size_t widx(const PropArgs& a, int h, int m) {
     const int g = m >> a.cbl_log, c = m & (a.cbl - 1);
-    return ((size_t)g * a.ny + h) * a.cbl + c;
+    return ((size_t)g * a.rows + h) * a.cbl + c;
 }
 
 template <int L>
@@ -53,7 +53,7 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
     __shared__ float2 lds[RPW * P::LP];
     const int tid = threadIdx.x, rr = tid / P::T, t = tid - rr * P::T;
     const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + rr;
-    const bool valid = row < a.ny;
+    const bool valid = row < a.rows;
     const int nx = a.nx;
     float2 ay = make_float2(1.f, 0.f);
     if (a.ay && valid) ay = a.ay[row];
@@ -67,6 +67,7 @@ k_rows_pad_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs 
                 const int col = first_index<P>(t, b, r) - L / 4;
                 x = src[(size_t)row * a.ss + col];
                 if (a.ax) x = cmul(x, cmul(a.ax[col], ay));
+                if (a.pre) x = cmul(x, a.pre[(size_t)(row % a.ny) * nx + col]);
             }
             v[b * R0 + r] = x;
         }
@@ -100,7 +101,8 @@ k_cols_conv(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) {
     const bool valid = col < Mx;
     // blocked layout: this group's contiguous block; row-major: stride Mx
     const bool rowmajor = a.cbl == Mx;
-    float2* Wb = rowmajor ? W + col : W + (size_t)grp * a.ny * CB + c;
+    const size_t fo = (size_t)blockIdx.y * a.ny;  // this field's first row
+    float2* Wb = rowmajor ? W + fo * Mx + col : W + ((size_t)grp * a.rows + fo) * CB + c;
     const size_t wst = rowmajor ? (size_t)Mx : (size_t)CB;
     float2 v[P::E];
 #pragma unroll
@@ -153,7 +155,7 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
     __shared__ float2 lds[RPW * P::LP];
     const int tid = threadIdx.x, rr = tid / P::T, t = tid - rr * P::T;
     const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + rr;
-    const bool valid = row < a.ny;
+    const bool valid = row < a.rows;
     const int nx = a.nx;
     float2 v[P::E];
 #pragma unroll
@@ -175,6 +177,7 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
             const int col = i - L / 4;
             float2 x = cscale(v[b * RL + r], a.scale);
             if (post && abs(col - nx / 2) < a.cwl) x = cmul(x, cmul(a.cx[col], cy));
+            if (a.post) x = cmul(x, a.post[(size_t)(row % a.ny) * nx + col]);
             float2* o = &dst[(size_t)row * a.ds + col];
             if (a.accum) x = make_float2(o->x + x.x, o->y + x.y);
             *o = x;
@@ -267,10 +270,14 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
     const int h0 = xcd_remap(blockIdx.x, gridDim.x) * 2 * KR;
     const int ha = h0 + c, hb = h0 + KR + c;
     const int nx = a.nx;
-    const Rsrc sb = make_rsrc(src, ((size_t)(a.ny - 1) * a.ss + nx) * sizeof(float2));
-    const Rsrc wb = make_rsrc(W, (size_t)a.ny * 2 * nx * sizeof(float2));
+    const Rsrc sb = make_rsrc(src, ((size_t)(a.rows - 1) * a.ss + nx) * sizeof(float2));
+    // the factor field: ny x nx, dense, shared by every field of a batch
+    const Rsrc pb = make_rsrc(a.pre, a.pre ? (size_t)a.ny * nx * sizeof(float2) : 0);
+    const Rsrc wb = make_rsrc(W, (size_t)a.rows * 2 * nx * sizeof(float2));
     const uint32_t rowb = (uint32_t)(KR * a.ss * sizeof(float2));  // set b rows past set a's
     const uint32_t so = (uint32_t)(((size_t)ha * a.ss + t) * sizeof(float2));
+    const uint32_t fo = (uint32_t)(((size_t)(ha % a.ny) * nx + t) * sizeof(float2));
+    const uint32_t frowb = (uint32_t)(KR * nx * sizeof(float2));
     float2 aya = make_float2(1.f, 0.f), ayb = aya;
     if (a.ay) {
         aya = a.ay[ha];
@@ -279,7 +286,7 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
     Rsrc tb;
     double mi = 0.0, mt = 0.0, c1 = 0.0, c2 = 0.0;
     if constexpr (LG) {
-        tb = make_rsrc(a.lg_t, ((size_t)(a.ny - 1) * a.ss + nx) * sizeof(float));
+        tb = make_rsrc(a.lg_t, ((size_t)(a.rows - 1) * a.ss + nx) * sizeof(float));
         mi = a.lg_c[0];
         mt = a.lg_c[1];
         c1 = a.lg_c[2];
@@ -305,6 +312,10 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
                     xa = cmul(xa, cmul(ax, aya));
                     xb = cmul(xb, cmul(ax, ayb));
                 }
+                if (a.pre) {
+                    xa = cmul(xa, bload(pb, fo, D * sizeof(float2)));
+                    xb = cmul(xb, bload(pb, fo, D * sizeof(float2) + frowb));
+                }
             }
             va[b * R0 + r] = xa;
             vb[b * R0 + r] = xb;
@@ -317,7 +328,7 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
     for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
         for (int r = 0; r < RL; r++) {
-            const uint32_t so2 = (uint32_t)(last_index<P>(0, b, r) * a.ny * sizeof(float2));
+            const uint32_t so2 = (uint32_t)(last_index<P>(0, b, r) * a.rows * sizeof(float2));
             bstore(wb, wo, so2, va[b * RL + r]);
             bstore(wb, wo, so2 + setb, vb[b * RL + r]);
         }
@@ -337,9 +348,10 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
     const int cbl = a.cbl;
     // sets a and b are two neighbouring columns: one 16-byte access per row serves both
     const int cola = xcd_remap(blockIdx.x, gridDim.x) * 2 * KR + 2 * c, colb = cola + 1;
-    const Rsrc wr = make_rsrc(W, (size_t)a.ny * 2 * a.nx * sizeof(float2));
-    // tile base; column within the tile
-    const uint32_t gb = (uint32_t)((size_t)(cola >> a.cbl_log) * a.ny * cbl * sizeof(float2));
+    const Rsrc wr = make_rsrc(W, (size_t)a.rows * 2 * a.nx * sizeof(float2));
+    // tile base plus this field's first row (blockIdx.y); column within the tile
+    const uint32_t gb = (uint32_t)(((size_t)(cola >> a.cbl_log) * a.rows + (size_t)blockIdx.y * a.ny) *
+                                   cbl * sizeof(float2));
     const uint32_t wo = (uint32_t)((t * cbl + (cola & (cbl - 1))) * sizeof(float2));
     const uint32_t rowst = (uint32_t)(cbl * sizeof(float2));
     float2 va[E], vb[E];
@@ -476,8 +488,9 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
     const int ha = h0 + c, hb = h0 + KR + c;
     const int nx = a.nx;
     // element t + D of row h sits at widx(h, t) + D ny (D a multiple of the tile width)
-    const Rsrc wb = make_rsrc(W, (size_t)a.ny * 2 * nx * sizeof(float2));
-    const Rsrc db = make_rsrc(dst, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float2));
+    const Rsrc wb = make_rsrc(W, (size_t)a.rows * 2 * nx * sizeof(float2));
+    const Rsrc db = make_rsrc(dst, ((size_t)(a.rows - 1) * a.ds + nx) * sizeof(float2));
+    const Rsrc qb = make_rsrc(a.post, a.post ? (size_t)a.ny * nx * sizeof(float2) : 0);
     const uint32_t wo = (uint32_t)(widx(a, ha, t) * sizeof(float2));
     const uint32_t setb = (uint32_t)(KR * a.cbl * sizeof(float2));
     float2 va[P::E], vb[P::E];
@@ -485,17 +498,19 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
     for (int b = 0; b < P::E / R0; b++)
 #pragma unroll
         for (int r = 0; r < R0; r++) {
-            const uint32_t so = (uint32_t)(first_index<P>(0, b, r) * a.ny * sizeof(float2));
+            const uint32_t so = (uint32_t)(first_index<P>(0, b, r) * a.rows * sizeof(float2));
             va[b * R0 + r] = bload(wb, wo, so);
             vb[b * R0 + r] = bload(wb, wo, so + setb);
         }
     run_stages_pair<P, true, TWN, KR>(va, vb, t, lds, c, tw);
     const uint32_t so = (uint32_t)(((size_t)ha * a.ds + t) * sizeof(float2));
     const uint32_t rowb = (uint32_t)(KR * a.ds * sizeof(float2));
+    const uint32_t fo = (uint32_t)(((size_t)(ha % a.ny) * nx + t) * sizeof(float2));
+    const uint32_t frowb = (uint32_t)(KR * nx * sizeof(float2));
     if constexpr (PG) {
         // float offsets are half the complex ones
-        const Rsrc ub = make_rsrc(a.pg_u, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float2));
-        const Rsrc gb = make_rsrc(a.pg_grad, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float));
+        const Rsrc ub = make_rsrc(a.pg_u, ((size_t)(a.rows - 1) * a.ds + nx) * sizeof(float2));
+        const Rsrc gb = make_rsrc(a.pg_grad, ((size_t)(a.rows - 1) * a.ds + nx) * sizeof(float));
 #pragma unroll
         for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
@@ -517,7 +532,7 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
     if (a.cx == nullptr) {
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         Rsrc tb;
-        if constexpr (LS) tb = make_rsrc(a.ls_t, ((size_t)(a.ny - 1) * a.ds + nx) * sizeof(float));
+        if constexpr (LS) tb = make_rsrc(a.ls_t, ((size_t)(a.rows - 1) * a.ds + nx) * sizeof(float));
 #pragma unroll
         for (int b = 0; b < P::E / RL; b++)
 #pragma unroll
@@ -525,7 +540,11 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
                 if (!last_kept<P>(b, r)) continue;
                 // column t + D
                 const uint32_t D = (last_index<P>(0, b, r) - L / 4) * sizeof(float2);
-                const float2 xa = cscale(va[b * RL + r], a.scale), xb = cscale(vb[b * RL + r], a.scale);
+                float2 xa = cscale(va[b * RL + r], a.scale), xb = cscale(vb[b * RL + r], a.scale);
+                if (a.post) {
+                    xa = cmul(xa, bload(qb, fo, D));
+                    xb = cmul(xb, bload(qb, fo, D + frowb));
+                }
                 bstore_acc(db, so, D, xa, ACC);
                 bstore_acc(db, so, D + rowb, xb, ACC);
                 if constexpr (LS) {  // float offsets are half the complex ones
@@ -579,6 +598,10 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
                 if (posta) xa = cmul(xa, cmul(cx, cya));
                 if (postb) xb = cmul(xb, cmul(cx, cyb));
             }
+            if (a.post) {
+                xa = cmul(xa, bload(qb, fo, D * sizeof(float2)));
+                xb = cmul(xb, bload(qb, fo, D * sizeof(float2) + frowb));
+            }
             bstore_acc(db, so, D * sizeof(float2), xa, ACC);
             bstore_acc(db, so, D * sizeof(float2) + rowb, xb, ACC);
         }
@@ -595,7 +618,7 @@ struct TileCfg {
 template <int L, int KR>
 static void launch_rows_fwd(const TileCfg& g, const PropArgs& a, const float2* src, float2* W,
                             const float2* tw, hipStream_t s) {
-    const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
+    const int grid = a.rows / (2 * KR), nt = KR * Plan<L>::T;
     if (a.lg_t) {
         if (g.oa) k_rows_pad_fwd2<L, KR, REM_LAST, true><<<grid, nt, 0, s>>>(src, W, a, tw);
         else k_rows_pad_fwd2<L, KR, REM_FIRST, true><<<grid, nt, 0, s>>>(src, W, a, tw);
@@ -607,7 +630,8 @@ static void launch_rows_fwd(const TileCfg& g, const PropArgs& a, const float2* s
 template <int L, int KR, int MODE>
 static void launch_cols_mode(const TileCfg& g, const PropArgs& a, float2* W, const float2* tw,
                              hipStream_t s) {
-    const int grid = 2 * a.nx / (2 * KR), nt = KR * Plan<L>::T;
+    const dim3 grid(2 * a.nx / (2 * KR), a.nb);
+    const int nt = KR * Plan<L>::T;
     if (g.of && g.oi) k_cols_conv2<L, KR, MODE, REM_LAST, REM_LAST><<<grid, nt, 0, s>>>(W, a, tw);
     else if (g.of) k_cols_conv2<L, KR, MODE, REM_LAST, REM_FIRST><<<grid, nt, 0, s>>>(W, a, tw);
     else if (g.oi) k_cols_conv2<L, KR, MODE, REM_FIRST, REM_LAST><<<grid, nt, 0, s>>>(W, a, tw);
@@ -620,7 +644,8 @@ static void launch_cols_conv(const TileCfg& g, const PropArgs& a, float2* W, con
     else if (a.mode == K_ASHIFT) launch_cols_mode<L, KR, K_ASHIFT>(g, a, W, tw, s);
     else if (a.mode == K_ASM32) {
         // the default radix orders only (a compatibility mode, not tuned)
-        const int grid = 2 * a.nx / (2 * KR), nt = KR * Plan<L>::T;
+        const dim3 grid(2 * a.nx / (2 * KR), a.nb);
+        const int nt = KR * Plan<L>::T;
         k_cols_conv2<L, KR, K_ASM32, REM_LAST, REM_LAST><<<grid, nt, 0, s>>>(W, a, tw);
     }
     else launch_cols_mode<L, KR, K_TABLE>(g, a, W, tw, s);
@@ -628,7 +653,7 @@ static void launch_cols_conv(const TileCfg& g, const PropArgs& a, float2* W, con
 template <int L, int KR>
 static void launch_rows_inv(const TileCfg& g, const PropArgs& a, const float2* W, float2* dst,
                             const float2* tw, hipStream_t s) {
-    const int grid = a.ny / (2 * KR), nt = KR * Plan<L>::T;
+    const int grid = a.rows / (2 * KR), nt = KR * Plan<L>::T;
     if (a.ls_t) {
         if (g.oc) k_rows_crop_inv2<L, KR, REM_LAST, false, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
         else k_rows_crop_inv2<L, KR, REM_FIRST, false, false, true><<<grid, nt, 0, s>>>(W, dst, a, tw);
@@ -881,7 +906,7 @@ static int col_batch() {
 template <int L, int CB>
 static void launch_cols_cb(const PropArgs& a, float2* W, const float2* tw, hipStream_t s) {
     const int Mx = 2 * a.nx;
-    const int grid = (Mx + CB - 1) / CB;
+    const dim3 grid((Mx + CB - 1) / CB, a.nb);
     if (a.mode == K_ASM)
         k_cols_conv<L, CB, K_ASM><<<grid, CB * Plan<L>::T, 0, s>>>(W, a, tw);
     else if (a.mode == K_ASHIFT)
@@ -986,7 +1011,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
     const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
     float2* W = Wext;
     if (!W) {
-        OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.ny * (size_t)Mx));
+        OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.rows * (size_t)Mx));
         W = (float2*)ctx->work.p;
     }
     a.scale = (float)(a.extra_scale / ((double)Mx * (double)My));
@@ -1066,7 +1091,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
     {
         TimedLaunch tl(ctx, "prop_rows_pad_fwd");
         OLX_LEN_SWITCH(ilog2(Mx), {
-            const int grid = (a.ny + row_rpw<L>() - 1) / row_rpw<L>();
+            const int grid = (a.rows + row_rpw<L>() - 1) / row_rpw<L>();
             k_rows_pad_fwd<L><<<grid, row_nt<L>(), 0, s>>>(src, W, a, ctx->twiddle);
         });
     }
@@ -1081,7 +1106,7 @@ static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs
     {
         TimedLaunch tl(ctx, "prop_rows_crop_inv");
         OLX_LEN_SWITCH(ilog2(Mx), {
-            const int grid = (a.ny + row_rpw<L>() - 1) / row_rpw<L>();
+            const int grid = (a.rows + row_rpw<L>() - 1) / row_rpw<L>();
             k_rows_crop_inv<L><<<grid, row_nt<L>(), 0, s>>>(W, dst, a, ctx->twiddle);
         });
     }
@@ -1115,6 +1140,8 @@ static PropArgs base_args(int64_t ny, int64_t nx) {
     memset(&a, 0, sizeof a);
     a.ny = (int)ny;
     a.nx = (int)nx;
+    a.nb = 1;
+    a.rows = (int)ny;
     a.ss = a.ds = (int)nx;
     a.extra_scale = 1.0;
     return a;
@@ -1247,14 +1274,14 @@ extern "C" int olx_asm_c64_noalloc(olx_ctx* ctx, void* d_u, int64_t ny, int64_t 
     return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a, (float2*)w);
 }
 
-extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
-                               double dx, double lambda, double z, uint32_t flags) {
-    OLX_TRY(check_field(ctx, d_u, ny, nx));
+// Fresnel arguments: the separable spectrum tables, built or fetched from the cache
+static int fresnel_args(olx_ctx* ctx, PropArgs& a, int64_t ny, int64_t nx, double dy, double dx,
+                        double lambda, double z, uint32_t flags) {
     if (!(lambda > 0) || !(dx > 0) || !(dy > 0) || !(z != 0) || !std::isfinite(z))
         OLX_FAIL(OLX_EINVAL, "fresnel: lambda, dx, dy must be > 0 and z finite, nonzero");
     if ((flags & OLX_FRESNEL_BL_CPU) && (flags & OLX_FRESNEL_BL_GPU))
         OLX_FAIL(OLX_EINVAL, "fresnel: choose one band-limit variant");
-    PropArgs a = base_args(ny, nx);
+    a = base_args(ny, nx);
     a.mode = K_TABLE;
     const int adj = (flags & OLX_ADJOINT) ? 1 : 0;
     int64_t wlim = INT_MAX, hlim = INT_MAX;
@@ -1270,6 +1297,44 @@ extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, 
     if (!a.sx) return rc;
     a.sy = spectrum_table(ctx, (int)(2 * ny), dy, K, 1.0, hlim, adj, &rc);
     if (!a.sy) return rc;
+    return OLX_OK;
+}
+
+extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
+                               double dx, double lambda, double z, uint32_t flags) {
+    OLX_TRY(check_field(ctx, d_u, ny, nx));
+    PropArgs a;
+    OLX_TRY(fresnel_args(ctx, a, ny, nx, dy, dx, lambda, z, flags));
+    return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a);
+}
+
+// nb fields stacked in d_u, each replaced by post * Prop(pre * u_k): the row passes
+// run over one tall field of nb ny rows, the column pass over the nb fields as
+// blockIdx.y.  The 32-bit buffer offsets of the tiled passes bound the
+// intermediate (nb ny 2nx complex64) to 2 GiB.
+extern "C" int olx_prop_batch_c64(olx_ctx* ctx, int mode, void* d_u, int64_t nb, int64_t ny,
+                                  int64_t nx, double dy, double dx, double lambda, double z,
+                                  uint32_t flags, const void* d_pre, const void* d_post) {
+    OLX_TRY(check_field(ctx, d_u, ny, nx));
+    if (nb < 1 || nb > 65535)
+        OLX_FAIL(OLX_EINVAL, "prop batch: nb = %lld not in [1, 65535]", (long long)nb);
+    const size_t wbytes = sizeof(float2) * (size_t)nb * (size_t)ny * 2 * (size_t)nx;
+    if (wbytes > ((size_t)2 << 30))
+        OLX_FAIL(OLX_EINVAL, "prop batch: %lld fields of %lld x %lld need a %zu-byte "
+                 "intermediate, above 2 GiB", (long long)nb, (long long)ny, (long long)nx, wbytes);
+    PropArgs a;
+    if (mode == OLX_MODE_ASM) {
+        OLX_TRY(asm_args(a, ny, nx, dy, dx, lambda, z, flags));
+    } else if (mode == OLX_MODE_FRESNEL) {
+        OLX_TRY(fresnel_args(ctx, a, ny, nx, dy, dx, lambda, z, flags));
+    } else {
+        OLX_FAIL(OLX_EINVAL, "prop batch: mode %d is neither OLX_MODE_ASM nor OLX_MODE_FRESNEL",
+                 mode);
+    }
+    a.nb = (int)nb;
+    a.rows = (int)(nb * ny);
+    a.pre = (const float2*)d_pre;
+    a.post = (const float2*)d_post;
     return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a);
 }
 

@@ -187,13 +187,14 @@ k_mr_rows_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a
     const int row0 = xcd_remap(blockIdx.x, gridDim.x) * nb;
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
-        const float2 ay = (a.ay && row < a.ny) ? a.ay[row] : make_float2(1.f, 0.f);
+        const float2 ay = (a.ay && row < a.rows) ? a.ay[row] : make_float2(1.f, 0.f);
         for (int i = tid; i < L; i += NT) {
             float2 x = make_float2(0.f, 0.f);
             const int col = i - off;
-            if (row < a.ny && col >= 0 && col < nx) {
+            if (row < a.rows && col >= 0 && col < nx) {
                 x = src[(size_t)row * a.ss + col];
                 if (a.ax) x = cmul(x, cmul(a.ax[col], ay));
+                if (a.pre) x = cmul(x, a.pre[(size_t)(row % a.ny) * nx + col]);
             }
             lds[tr * LP + lpad(i)] = x;
         }
@@ -202,7 +203,7 @@ k_mr_rows_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a
     mr_run<false, NT, LCAP>(lds, LP, nb, P, tid);
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
-        if (row >= a.ny) break;
+        if (row >= a.rows) break;
         for (int i = tid; i < L; i += NT) W[(size_t)row * L + i] = lds[tr * LP + lpad(i)];
     }
 }
@@ -215,6 +216,7 @@ k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P,
     const int tid = threadIdx.x, My = P->L, LP = mr_lp(My), ny = a.ny, off = ny / 2;
     const int Mx = 2 * a.nx;
     const int col0 = xcd_remap(blockIdx.x, gridDim.x) * nb;
+    W += (size_t)blockIdx.y * ny * Mx;  // this field's rows
     for (int tr = 0; tr < nb; tr++) {
         const int col = col0 + tr;
         for (int i = tid; i < My; i += NT) {
@@ -254,18 +256,19 @@ k_mr_rows_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
         for (int i = tid; i < L; i += NT)
-            lds[tr * LP + lpad(i)] = row < a.ny ? W[(size_t)row * L + i] : make_float2(0.f, 0.f);
+            lds[tr * LP + lpad(i)] = row < a.rows ? W[(size_t)row * L + i] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     mr_run<true, NT, LCAP>(lds, LP, nb, P, tid);
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
-        if (row >= a.ny) break;
+        if (row >= a.rows) break;
         const bool post = a.cx != nullptr && abs(row - a.ny / 2) < a.chl;
         const float2 cy = post ? a.cy[row] : make_float2(1.f, 0.f);
         for (int col = tid; col < nx; col += NT) {
             float2 x = cscale(lds[tr * LP + lpad(col + off)], a.scale);
             if (post && abs(col - nx / 2) < a.cwl) x = cmul(x, cmul(a.cx[col], cy));
+            if (a.post) x = cmul(x, a.post[(size_t)(row % a.ny) * nx + col]);
             float2* o = &dst[(size_t)row * a.ds + col];
             if (a.accum) x = make_float2(o->x + x.x, o->y + x.y);
             *o = x;
@@ -405,7 +408,7 @@ int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, flo
         TimedLaunch tl(ctx, "prop_rows_pad_fwd");
         mr_shape(Mx, [&](auto NT, auto LC) {
             const int nb = mr_nb<LC.value>(Mx);
-            k_mr_rows_fwd<NT.value, LC.value><<<(a.ny + nb - 1) / nb, NT.value, 0, s>>>(src, W, a, px, nb);
+            k_mr_rows_fwd<NT.value, LC.value><<<(a.rows + nb - 1) / nb, NT.value, 0, s>>>(src, W, a, px, nb);
         });
     }
     OLX_TRY(check_launch("mixed-radix pass A"));
@@ -413,7 +416,7 @@ int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, flo
         TimedLaunch tl(ctx, "prop_cols_conv");
         mr_shape(My, [&](auto NT, auto LC) {
             const int nb = mr_nb<LC.value>(My);
-            const int grid = (Mx + nb - 1) / nb;
+            const dim3 grid((Mx + nb - 1) / nb, a.nb);
             if (a.mode == K_ASM)
                 k_mr_cols_conv<NT.value, LC.value, K_ASM><<<grid, NT.value, 0, s>>>(W, a, py, nb);
             else if (a.mode == K_ASHIFT)
@@ -429,7 +432,7 @@ int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, flo
         TimedLaunch tl(ctx, "prop_rows_crop_inv");
         mr_shape(Mx, [&](auto NT, auto LC) {
             const int nb = mr_nb<LC.value>(Mx);
-            k_mr_rows_inv<NT.value, LC.value><<<(a.ny + nb - 1) / nb, NT.value, 0, s>>>(W, dst, a, px, nb);
+            k_mr_rows_inv<NT.value, LC.value><<<(a.rows + nb - 1) / nb, NT.value, 0, s>>>(W, dst, a, px, nb);
         });
     }
     return check_launch("mixed-radix pass C");

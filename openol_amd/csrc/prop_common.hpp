@@ -13,6 +13,8 @@ enum { K_ASM = 0, K_TABLE = 1, K_ASHIFT = 2, K_ASM32 = 3 };
 
 struct PropArgs {
     int ny, nx;  // unpadded field
+    int nb;      // fields in the batch, stacked row-wise: the row passes see a field of
+    int rows;    // rows = nb ny rows, the column pass takes field blockIdx.y
     int mode;    // K_ASM / K_TABLE / K_ASHIFT
     int adjoint;
     int bandlimit;
@@ -21,6 +23,9 @@ struct PropArgs {
     double u0, v0, ox, oy;  // K_ASHIFT: window centres and the shift's linear phase
     float duf, dvf, tmpf, zf;  // K_ASM32: du, dv, 1/lambda^2 and z rounded as PREC_T = float
     int ss, ds;             // row strides (elements) of the source and destination
+    const float2* pre;      // ny x nx factor fields (dense, shared by every field of a
+    const float2* post;     // batch) multiplied into the source as pass A loads it and
+                            // into the result as pass C stores it
     int accum;              // pass C adds into the destination instead of storing
     const float2* sx;                  // K_TABLE: natural-order spectra, length Mx / My
     const float2* sy;

@@ -108,6 +108,31 @@ def gFresnelPropBandLimit(u, height, width, dy, dx, lam, z):
     return FresnelPropBandLimit(u, height, width, dy, dx, lam, z, variant="gpu")
 
 
+def prop_batch(u, dy, dx, lam, z, propmode=ASM, flags=0, pre=None, post=None):
+    """Every field of the (nb, ny, nx) complex64 tensor u, in place, replaced by
+    post * Prop(pre * u[k]): Prop is AsmProp (propmode ASM, flags BANDLIMIT / ADJOINT /
+    PREC_F32) or FresnelProp (FRESNEL, flags FRESNEL_BL_CPU / FRESNEL_BL_GPU / ADJOINT).
+    pre / post are (ny, nx) complex64 factor fields shared by every field (a lens array,
+    a diffuser's phase, an aperture; None: 1), multiplied in as the first pass loads
+    and the last pass stores, where the reference multiplies in passes of their own
+    (Lensarry::mult_complex, include/Lensarry.h:262-270)."""
+    if not isinstance(u, torch.Tensor) or u.dim() != 3:
+        raise _lib.OlxInvalid(_lib.EINVAL, "prop_batch: u must be a (nb, ny, nx) tensor")
+    nb, ny, nx = u.shape
+    _field(u.view(nb * ny, nx), nb * ny, nx)
+    for f, name in ((pre, "pre"), (post, "post")):
+        if f is not None:
+            _field(f, ny, nx, name)
+            if f.device != u.device:
+                raise _lib.OlxInvalid(_lib.EINVAL, f"prop_batch: {name} on {f.device}, u on {u.device}")
+    if propmode not in (ASM, FRESNEL):
+        raise _lib.OlxInvalid(_lib.EINVAL, "prop_batch: propmode must be ASM or FRESNEL")
+    check(_lib.call("olx_prop_batch_c64", u.device.index, propmode, _ptr(u), nb, ny, nx, dy, dx,
+                    lam, z, flags, _ptr(pre) if pre is not None else None,
+                    _ptr(post) if post is not None else None), "prop_batch")
+    return u
+
+
 def ARSSFresnelProp(usrc, udst, ny, nx, dy, dx, lam, z, s, oy=0.0, ox=0.0):
     """ol::ARSSFresnelProp (include/golprop.h:941-995); usrc and udst may alias."""
     _field(usrc, ny, nx, "usrc")
