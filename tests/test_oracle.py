@@ -378,3 +378,62 @@ def test_cgh_fresnel_diff_recurrence():
     ref = olref.cgh(p * pts["x"], p * pts["y"], p * pts["z"], ny, nx, p, p, LAM, amp,
                     pts["phase"] + piston, dtype=np.complex128)
     assert rel(got, ref) < 1e-8
+
+
+# ------------------------------------------------------------- window edges
+# The band-limit windows are written here as explicit half-widths worked out by hand
+# from the parameters (a value v keeps offsets |d| < floor(v), i.e. |d| <= floor(v) - 1),
+# at parameters where every window is active and v is far from an integer.  Each
+# expected field is an FP64 direct sum over source pixels with those masks: a "<="
+# for "<", a round for the floor, a y window from dx for one from dy (or the reverse),
+# or a zeroed instead of unmodified outside region changes the result.
+def _mask(n, half):
+    d = np.arange(n) - n // 2
+    return np.abs(d) <= half
+
+
+def _kernel(n, d, lam, z, s, half):
+    off = np.arange(n)[:, None] - np.arange(n)[None, :]
+    k = np.exp(1j * np.pi * s * (off * d) ** 2 / (lam * z))
+    return np.where(np.abs(off) <= half, k, 0)
+
+
+@pytest.mark.parametrize("variant", ["per_axis", "x_both_axes"])
+def test_fresnel_band_limit_window_edges(variant):
+    ny, nx, dy, dx, z = 24, 20, 10e-6, 8e-6, 0.00115
+    # lambda z / (2 dx^2) = 4.78 -> |dx offset| <= 3;  lambda z / (2 dy^2) = 3.06 -> <= 2
+    # (the GPU form takes the x value for both axes: <= 3)
+    hx, hy = 3, (2 if variant == "per_axis" else 3)
+    u = synth.random_field(ny, nx, seed=31, dtype=np.complex128)
+    flag = olref.FRESNEL_BL_CPU if variant == "per_axis" else olref.FRESNEL_BL_GPU
+    out = olref.fresnel_prop(u, dy, dx, LAM, z, flag)
+    direct = _kernel(ny, dy, LAM, z, 1.0, hy) @ u @ _kernel(nx, dx, LAM, z, 1.0, hx).T
+    direct /= 4 * ny * nx
+    assert rel(out, direct) < 1e-12
+    # the same sum one step wider on either axis is far off
+    wide = _kernel(ny, dy, LAM, z, 1.0, hy + 1) @ u @ _kernel(nx, dx, LAM, z, 1.0, hx).T
+    assert rel(out, wide / (4 * ny * nx)) > 1e-2
+
+
+def test_arss_window_edges():
+    ny, nx, dy, dx, s, z, oy, ox = 30, 32, 10e-6, 8e-6, 0.25, 0.0006, 3e-6, -5e-6
+    # source window, lambda |z| / (2 s |s-1| d^2): x 13.30 -> <= 12, y 8.51 -> <= 7
+    # kernel window from the x value on both axes, lambda z / (2 s dx^2) = 9.975 -> <= 8
+    # output chirp window, lambda |z| / (2 |1-s| d^2): x 3.325 -> <= 2, y 2.128 -> <= 1;
+    # outside it the output is left as the convolution gave it
+    tmp = np.pi / (LAM * z)
+    u = synth.random_field(ny, nx, seed=32, dtype=np.complex128)
+    yc, xc = (np.arange(ny) - ny // 2) * dy, (np.arange(nx) - nx // 2) * dx
+    pre = np.exp(1j * tmp * (((s * s - s) * yc ** 2 - 2 * s * oy * yc)[:, None] +
+                             ((s * s - s) * xc ** 2 - 2 * s * ox * xc)[None, :]))
+    u1 = np.where(_mask(ny, 7)[:, None] & _mask(nx, 12)[None, :], u * pre, 0)
+    conv = _kernel(ny, dy, LAM, z, s, 8) @ u1 @ _kernel(nx, dx, LAM, z, s, 8).T / (4 * ny * nx)
+    post = np.exp(1j * tmp * (((1 - s) * yc ** 2 + 2 * oy * yc + oy * oy)[:, None] +
+                              ((1 - s) * xc ** 2 + 2 * ox * xc + ox * ox)[None, :]))
+    inside = _mask(ny, 1)[:, None] & _mask(nx, 2)[None, :]
+    direct = np.where(inside, conv * post, conv)
+    out = olref.arss_prop(u, dy, dx, LAM, z, s, oy, ox)
+    # 1e-7: the reference scales by 1.0f / (float)(M^2) in float (golprop.h:965, 973),
+    # inexact for M^2 = 60 x 64
+    assert rel(out, direct) < 1e-6
+    assert rel(out, conv * post) > 1e-2  # a chirp over the whole field is far off
