@@ -166,10 +166,14 @@ int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, 
  * multiplies as it loads and pass C as it stores.  nb in [1, 65535] with the
  * padded intermediate (nb ny 2nx complex64) at most 2 GiB; fields follow the
  * propagation size rule.  No counterpart for nb > 1 in the reference, which
- * propagates one field per call. */
+ * propagates one field per call.  d_img (nb ny nx bytes, optional) also receives
+ * phase8bit(atan2) of every output value, the byte olx_phase8bit gives for the
+ * output's phase (include/olimg.h:205-213), from the last pass's epilogue; img_flags
+ * 0 or OLX_PHASE8_OFFSET. */
 int olx_prop_batch_c64(olx_ctx* ctx, int mode, void* d_u, int64_t nb, int64_t ny, int64_t nx,
                        double dy, double dx, double lambda, double z, uint32_t flags,
-                       const void* d_pre, const void* d_post);
+                       const void* d_pre, const void* d_post, uint8_t* d_img,
+                       uint32_t img_flags);
 /* ARSSFresnelProp: scaled/shifted Fresnel, src -> dst (may alias).  Replaces
  * ol::ARSSFresnelProp (include/golprop.h:941-995). */
 int olx_arss_c64(olx_ctx* ctx, const void* d_src, void* d_dst, int64_t ny, int64_t nx,
@@ -248,6 +252,15 @@ int olx_fft2d_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, int sign);
 int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U, float* d_phase,
                 float* d_amp, int64_t ny, int64_t nx, double py, double px, double lambda,
                 uint32_t flags);
+/* olx_cgh_c64 with the hologram written as 8-bit phase: d_img[h nx + w] =
+ * phase8bit(atan2(U)) (include/olimg.h:205-213, the byte olx_phase8bit gives for
+ * the phase olx_cgh_c64 writes), computed in the CGH kernel's epilogue instead of a
+ * phase array and a second pass: the reference's Cgh -> phase8bit -> Save chain
+ * (include/olcgh.h:275-295, include/olimg.h:205-213).  d_U optional (also stored);
+ * flags as olx_cgh_c64 plus OLX_PHASE8_OFFSET (defined with olx_phase8bit below). */
+int olx_cgh_img8_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
+                     uint8_t* d_img, int64_t ny, int64_t nx, double py, double px, double lambda,
+                     uint32_t flags);
 /* same sum with FP64 arithmetic and a complex128 output (the FP64 column) */
 int olx_cgh_c128(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U, int64_t ny,
                  int64_t nx, double py, double px, double lambda, uint32_t flags);

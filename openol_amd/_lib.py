@@ -68,7 +68,7 @@ SIGNATURES = {
     "olx_asm_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_asm_c64_noalloc": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_fresnel_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
-    "olx_prop_batch_c64": [_p, _i, _p, _i64, _i64, _i64, _d, _d, _d, _d, _u32, _p, _p],
+    "olx_prop_batch_c64": [_p, _i, _p, _i64, _i64, _i64, _d, _d, _d, _d, _u32, _p, _p, _p, _u32],
     "olx_arss_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _u32],
     "olx_shifted_asm_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _u32],
     "olx_split_prop_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, _i, _i, _u32],
@@ -94,6 +94,7 @@ SIGNATURES = {
     "olx_asm_transfer_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_fresnel_response_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i64, _i64],
     "olx_cgh_c64": [_p, _p, _i64, _p, _p, _p, _i64, _i64, _d, _d, _d, _u32],
+    "olx_cgh_img8_c64": [_p, _p, _i64, _p, _p, _i64, _i64, _d, _d, _d, _u32],
     "olx_cgh_c128": [_p, _p, _i64, _p, _i64, _i64, _d, _d, _d, _u32],
     "olx_cgh_fresnel_diff_c64": [_p, _p, _i64, _p, _p, _i64, _i64, _d, _d],
     "olx_encode_c64": [_p, _p, _p, _p, _i64],

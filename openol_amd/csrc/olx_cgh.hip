@@ -26,25 +26,49 @@
 // is partitioned so both groups run as uniform loops.
 
 #include "olx_common.hpp"
+#include "img_common.hpp"
 
 namespace olx {
 
 constexpr int CGH_TX = 32, CGH_TY = 32, CGH_NT = 256, CGH_RPT = CGH_TY / (CGH_NT / CGH_TX);
 
+// The outputs of a CGH kernel, any of them null: the field (added to what U holds
+// with accumulate), its phase atan2(im, re), its amplitude, and the phase8bit byte of
+// that phase (include/olimg.h:205-213; img_offset: the +pi form) fused into the
+// epilogue in place of a phase array and a second pass.
+struct CghOut {
+    float2* U;
+    float* phase;
+    float* amp;
+    uint8_t* img;
+    int img_offset;
+    int accumulate;
+};
+
+__device__ __forceinline__ void cgh_store(const CghOut& o, size_t idx, float2 v) {
+    if (o.accumulate && o.U) {
+        const float2 q = o.U[idx];
+        v = make_float2(v.x + q.x, v.y + q.y);
+    }
+    if (o.U) o.U[idx] = v;
+    if (o.phase || o.img) {
+        const float ph = atan2f(v.y, v.x);
+        if (o.phase) o.phase[idx] = ph;
+        if (o.img) o.img[idx] = phase8_one<float>(ph, o.img_offset);
+    }
+    if (o.amp) o.amp[idx] = sqrtf(v.x * v.x + v.y * v.y);
+}
+
 // A hologram of few tiles splits its points into gridDim.z slices of `slice` points
 // (a multiple of 256, so chunking is unchanged); slice z writes its partial plane to
 // part + z ny nx, and k_cgh_sum_slices adds the planes in slice order.
-__device__ __forceinline__ void cgh_slice(const olx_point* __restrict__& pts, int& n,
-                                          float2* __restrict__& U, float* __restrict__& phase_out,
-                                          float* __restrict__& amp_out, int& accumulate,
+__device__ __forceinline__ void cgh_slice(const olx_point* __restrict__& pts, int& n, CghOut& o,
                                           float2* part, int slice, int ny, int nx) {
     if (!part) return;
     const int start = (int)blockIdx.z * slice;
     pts += start;
     n = max(0, min(slice, n - start));
-    U = part + (size_t)blockIdx.z * ny * nx;
-    phase_out = amp_out = nullptr;
-    accumulate = 0;
+    o = CghOut{part + (size_t)blockIdx.z * ny * nx, nullptr, nullptr, nullptr, 0, 0};
 }
 
 // sin / cos of t revolutions: the hardware units take revolutions; t is first
@@ -67,10 +91,9 @@ struct PtAnchorR {  // per-(tile, point) constants of the exact-r kernel, FP32
 constexpr double CGH_SERIES_E4K = 1.0e-6;
 
 __global__ void __launch_bounds__(CGH_NT)
-k_cgh_exact(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
-            float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx, double py,
-            double px, double lambda, int accumulate, float2* __restrict__ part, int slice) {
-    cgh_slice(pts, n, U, phase_out, amp_out, accumulate, part, slice, ny, nx);
+k_cgh_exact(const olx_point* __restrict__ pts, int n, CghOut o, int ny, int nx, double py,
+            double px, double lambda, float2* __restrict__ part, int slice) {
+    cgh_slice(pts, n, o, part, slice, ny, nx);
     __shared__ PtAnchorR sp[CGH_NT];
     __shared__ int wcount[2][CGH_NT / 64];
     const int tid = threadIdx.x, tx = tid % CGH_TX, ty = tid / CGH_TX;
@@ -190,15 +213,7 @@ k_cgh_exact(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
     for (int k = 0; k < CGH_RPT; k++) {
         const int h = h0 + ty + k * (CGH_NT / CGH_TX);
         if (h >= ny) continue;
-        const size_t idx = (size_t)h * nx + w;
-        float2 v = make_float2(re[k], im[k]);
-        if (accumulate && U) {
-            const float2 o = U[idx];
-            v = make_float2(v.x + o.x, v.y + o.y);
-        }
-        if (U) U[idx] = v;
-        if (phase_out) phase_out[idx] = atan2f(v.y, v.x);
-        if (amp_out) amp_out[idx] = sqrtf(v.x * v.x + v.y * v.y);
+        cgh_store(o, (size_t)h * nx + w, make_float2(re[k], im[k]));
     }
 }
 
@@ -224,11 +239,9 @@ struct SepAnchor {  // per (tile, point), FP32
 };
 
 __global__ void __launch_bounds__(SEP_NT)
-k_cgh_fresnel_sep(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
-                  float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx,
-                  double py, double px, double lambda, int accumulate, float2* __restrict__ part,
-                  int slice) {
-    cgh_slice(pts, n, U, phase_out, amp_out, accumulate, part, slice, ny, nx);
+k_cgh_fresnel_sep(const olx_point* __restrict__ pts, int n, CghOut o, int ny, int nx,
+                  double py, double px, double lambda, float2* __restrict__ part, int slice) {
+    cgh_slice(pts, n, o, part, slice, ny, nx);
     __shared__ SepAnchor sp[SEP_NT];
     const int tid = threadIdx.x, tx = tid % 32, ty = tid / 32;
     const int w0 = blockIdx.x * SEP_TX, h0 = blockIdx.y * SEP_TY;
@@ -318,15 +331,7 @@ k_cgh_fresnel_sep(const olx_point* __restrict__ pts, int n, float2* __restrict__
         for (int k = 0; k < SEP_NK; k++) {
             const int h = h0 + ty + 8 * k;
             if (h >= ny) continue;
-            const size_t idx = (size_t)h * nx + w;
-            float2 v = make_float2(re[i][k], im[i][k]);
-            if (accumulate && U) {
-                const float2 o = U[idx];
-                v = make_float2(v.x + o.x, v.y + o.y);
-            }
-            if (U) U[idx] = v;
-            if (phase_out) phase_out[idx] = atan2f(v.y, v.x);
-            if (amp_out) amp_out[idx] = sqrtf(v.x * v.x + v.y * v.y);
+            cgh_store(o, (size_t)h * nx + w, make_float2(re[i][k], im[i][k]));
         }
     }
 }
@@ -362,9 +367,7 @@ k_cgh_f64(const olx_point* __restrict__ pts, int n, double2* __restrict__ U, int
     if (w < nx && h < ny) U[(size_t)h * nx + w] = make_double2(re, im);
 }
 
-__global__ void k_cgh_sum_slices(const float2* __restrict__ part, int S, int64_t np,
-                                 float2* __restrict__ U, float* __restrict__ ph,
-                                 float* __restrict__ am, int accumulate) {
+__global__ void k_cgh_sum_slices(const float2* __restrict__ part, int S, int64_t np, CghOut o) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np;
          i += (int64_t)gridDim.x * blockDim.x) {
         float2 v = part[i];
@@ -372,13 +375,7 @@ __global__ void k_cgh_sum_slices(const float2* __restrict__ part, int S, int64_t
             const float2 w = part[(size_t)s * np + i];
             v = make_float2(v.x + w.x, v.y + w.y);
         }
-        if (accumulate && U) {
-            const float2 o = U[i];
-            v = make_float2(v.x + o.x, v.y + o.y);
-        }
-        if (U) U[i] = v;
-        if (ph) ph[i] = atan2f(v.y, v.x);
-        if (am) am[i] = sqrtf(v.x * v.x + v.y * v.y);
+        cgh_store(o, (size_t)i, v);
     }
 }
 
@@ -399,9 +396,8 @@ __global__ void k_encode(const float2* __restrict__ U, float* __restrict__ ph,
 // (no contraction).  One thread per pixel; points staged through LDS in tiles.
 constexpr int F32C_NT = 256, F32C_TILE = 256;
 __global__ void __launch_bounds__(F32C_NT)
-k_cgh_f32compat(const olx_point* __restrict__ pts, int n, float2* __restrict__ U,
-                float* __restrict__ phase_out, float* __restrict__ amp_out, int ny, int nx,
-                float pyf, float pxf, float lf, int acc) {
+k_cgh_f32compat(const olx_point* __restrict__ pts, int n, CghOut o, int ny, int nx,
+                float pyf, float pxf, float lf) {
 #pragma clang fp contract(off)
     __shared__ float4 sp[F32C_TILE];
     const int64_t pix = blockIdx.x * (int64_t)F32C_NT + threadIdx.x;
@@ -428,16 +424,7 @@ k_cgh_f32compat(const olx_point* __restrict__ pts, int n, float2* __restrict__ U
         }
     }
     if (!valid) return;
-    float2 v = make_float2(re, im);
-    if (U) {
-        if (acc) {
-            const float2 o = U[pix];
-            v = make_float2(o.x + v.x, o.y + v.y);
-        }
-        U[pix] = v;
-    }
-    if (phase_out) phase_out[pix] = atan2f(v.y, v.x);
-    if (amp_out) amp_out[pix] = sqrtf(v.x * v.x + v.y * v.y);
+    cgh_store(o, (size_t)pix, make_float2(re, im));
 }
 
 static int check_cgh(olx_ctx* ctx, const olx_point* pts, int64_t n, int64_t ny, int64_t nx,
@@ -455,13 +442,12 @@ static int check_cgh(olx_ctx* ctx, const olx_point* pts, int64_t n, int64_t ny, 
 
 using namespace olx;
 
-extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
-                           float* d_phase, float* d_amp, int64_t ny, int64_t nx, double py,
-                           double px, double lambda, uint32_t flags) {
+static int cgh_run(olx_ctx* ctx, const olx_point* d_pts, int64_t n, CghOut o, int64_t ny,
+                   int64_t nx, double py, double px, double lambda, uint32_t flags) {
     OLX_TRY(check_cgh(ctx, d_pts, n, ny, nx, py, px, lambda));
-    if (!d_U && !d_phase && !d_amp) OLX_FAIL(OLX_EINVAL, "cgh: no output requested");
-    if ((flags & OLX_CGH_ACCUMULATE) && !d_U) OLX_FAIL(OLX_EINVAL, "cgh: accumulate needs d_U");
-    const int acc = (flags & OLX_CGH_ACCUMULATE) ? 1 : 0;
+    if (!o.U && !o.phase && !o.amp && !o.img) OLX_FAIL(OLX_EINVAL, "cgh: no output requested");
+    if ((flags & OLX_CGH_ACCUMULATE) && !o.U) OLX_FAIL(OLX_EINVAL, "cgh: accumulate needs d_U");
+    o.accumulate = (flags & OLX_CGH_ACCUMULATE) ? 1 : 0;
     const bool exact = (flags & OLX_CGH_EXACT_R) != 0;
     if (flags & OLX_PREC_F32) {
         if (exact) OLX_FAIL(OLX_EINVAL, "cgh: the float-PREC_T form is the Fresnel sum only");
@@ -469,8 +455,7 @@ extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
         if (np / F32C_NT >= INT32_MAX) OLX_FAIL(OLX_EUNSUPPORTED, "cgh: hologram too large");
         TimedLaunch tl(ctx, "cgh_f32compat");
         k_cgh_f32compat<<<(unsigned)((np + F32C_NT - 1) / F32C_NT), F32C_NT, 0, ctx->stream>>>(
-            d_pts, (int)n, (float2*)d_U, d_phase, d_amp, (int)ny, (int)nx, (float)py, (float)px,
-            (float)lambda, acc);
+            d_pts, (int)n, o, (int)ny, (int)nx, (float)py, (float)px, (float)lambda);
         return check_launch("cgh_f32compat");
     }
     const int tx = exact ? CGH_TX : SEP_TX, ty = exact ? CGH_TY : SEP_TY;
@@ -493,20 +478,33 @@ extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     {
         TimedLaunch tl(ctx, exact ? "cgh_f32_exact" : "cgh_f32_fresnel");
         if (exact)
-            k_cgh_exact<<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U, d_phase, d_amp,
-                                                          (int)ny, (int)nx, py, px, lambda, acc, part,
-                                                          (int)slice);
+            k_cgh_exact<<<grid, CGH_NT, 0, ctx->stream>>>(d_pts, (int)n, o, (int)ny, (int)nx, py,
+                                                          px, lambda, part, (int)slice);
         else
-            k_cgh_fresnel_sep<<<grid, SEP_NT, 0, ctx->stream>>>(d_pts, (int)n, (float2*)d_U, d_phase,
-                                                                d_amp, (int)ny, (int)nx, py, px,
-                                                                lambda, acc, part, (int)slice);
+            k_cgh_fresnel_sep<<<grid, SEP_NT, 0, ctx->stream>>>(d_pts, (int)n, o, (int)ny, (int)nx,
+                                                                py, px, lambda, part, (int)slice);
         if (part) {
             const int64_t blocks = std::min<int64_t>((np + 255) / 256, 65536);
-            k_cgh_sum_slices<<<(unsigned)blocks, 256, 0, ctx->stream>>>(
-                part, (int)S, np, (float2*)d_U, d_phase, d_amp, acc);
+            k_cgh_sum_slices<<<(unsigned)blocks, 256, 0, ctx->stream>>>(part, (int)S, np, o);
         }
     }
     return check_launch("cgh_f32");
+}
+
+extern "C" int olx_cgh_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
+                           float* d_phase, float* d_amp, int64_t ny, int64_t nx, double py,
+                           double px, double lambda, uint32_t flags) {
+    return cgh_run(ctx, d_pts, n, CghOut{(float2*)d_U, d_phase, d_amp, nullptr, 0, 0}, ny, nx, py,
+                   px, lambda, flags);
+}
+
+extern "C" int olx_cgh_img8_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
+                                uint8_t* d_img, int64_t ny, int64_t nx, double py, double px,
+                                double lambda, uint32_t flags) {
+    if (!d_img) OLX_FAIL(OLX_EINVAL, "cgh img8: null image");
+    const int off = (flags & OLX_PHASE8_OFFSET) ? 1 : 0;
+    return cgh_run(ctx, d_pts, n, CghOut{(float2*)d_U, nullptr, nullptr, d_img, off, 0}, ny, nx,
+                   py, px, lambda, flags & ~OLX_PHASE8_OFFSET);
 }
 
 extern "C" int olx_cgh_c128(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,

@@ -181,6 +181,7 @@ k_rows_crop_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs
             float2* o = &dst[(size_t)row * a.ds + col];
             if (a.accum) x = make_float2(o->x + x.x, o->y + x.y);
             *o = x;
+            if (a.img8) store_img8(a, (size_t)row * a.ds + col, x);
         }
 }
 
@@ -547,6 +548,11 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
                 }
                 bstore_acc(db, so, D, xa, ACC);
                 bstore_acc(db, so, D + rowb, xb, ACC);
+                if (a.img8) {  // not with ACC (the entry points exclude it)
+                    const size_t e = (so + D) / sizeof(float2);
+                    store_img8(a, e, xa);
+                    store_img8(a, e + (size_t)KR * a.ds, xb);
+                }
                 if constexpr (LS) {  // float offsets are half the complex ones
                     const double Ia = (double)xa.x * xa.x + (double)xa.y * xa.y;
                     const double Ib = (double)xb.x * xb.x + (double)xb.y * xb.y;
@@ -604,6 +610,11 @@ k_rows_crop_inv2(const float2* __restrict__ W, float2* __restrict__ dst, PropArg
             }
             bstore_acc(db, so, D * sizeof(float2), xa, ACC);
             bstore_acc(db, so, D * sizeof(float2) + rowb, xb, ACC);
+            if (a.img8) {
+                const size_t e = so / sizeof(float2) + D;
+                store_img8(a, e, xa);
+                store_img8(a, e + (size_t)KR * a.ds, xb);
+            }
         }
 }
 
@@ -1314,7 +1325,8 @@ extern "C" int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, 
 // intermediate (nb ny 2nx complex64) to 2 GiB.
 extern "C" int olx_prop_batch_c64(olx_ctx* ctx, int mode, void* d_u, int64_t nb, int64_t ny,
                                   int64_t nx, double dy, double dx, double lambda, double z,
-                                  uint32_t flags, const void* d_pre, const void* d_post) {
+                                  uint32_t flags, const void* d_pre, const void* d_post,
+                                  uint8_t* d_img, uint32_t img_flags) {
     OLX_TRY(check_field(ctx, d_u, ny, nx));
     if (nb < 1 || nb > 65535)
         OLX_FAIL(OLX_EINVAL, "prop batch: nb = %lld not in [1, 65535]", (long long)nb);
@@ -1335,6 +1347,9 @@ extern "C" int olx_prop_batch_c64(olx_ctx* ctx, int mode, void* d_u, int64_t nb,
     a.rows = (int)(nb * ny);
     a.pre = (const float2*)d_pre;
     a.post = (const float2*)d_post;
+    if (img_flags & ~OLX_PHASE8_OFFSET) OLX_FAIL(OLX_EINVAL, "prop batch: unknown image flags");
+    a.img8 = d_img;
+    a.img8_off = (img_flags & OLX_PHASE8_OFFSET) ? 1 : 0;
     return run_three_pass(ctx, (const float2*)d_u, (float2*)d_u, a);
 }
 

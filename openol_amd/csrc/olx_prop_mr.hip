@@ -272,6 +272,7 @@ k_mr_rows_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a
             float2* o = &dst[(size_t)row * a.ds + col];
             if (a.accum) x = make_float2(o->x + x.x, o->y + x.y);
             *o = x;
+            if (a.img8) store_img8(a, (size_t)row * a.ds + col, x);
         }
     }
 }

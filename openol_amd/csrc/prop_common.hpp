@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_core.hpp"
+#include "img_common.hpp"
 
 namespace olx {
 
@@ -27,6 +28,8 @@ struct PropArgs {
     const float2* post;     // batch) multiplied into the source as pass A loads it and
                             // into the result as pass C stores it
     int accum;              // pass C adds into the destination instead of storing
+    uint8_t* img8;          // pass C also stores phase8bit(atan2(x)) of each output value
+    int img8_off;           // (include/olimg.h:205-213; img8_off: the +pi form)
     const float2* sx;                  // K_TABLE: natural-order spectra, length Mx / My
     const float2* sy;
     const float2* ax;  // pre-chirp tables (length nx / ny), zero outside the window
@@ -152,6 +155,10 @@ __device__ __forceinline__ float2 trans_h(const PropArgs& a, const ColH& c, int 
     } else {
         return asm_h(a, c.A, c.ucut, i, My);
     }
+}
+
+__device__ __forceinline__ void store_img8(const PropArgs& a, size_t idx, float2 x) {
+    a.img8[idx] = phase8_one<float>(atan2f(x.y, x.x), a.img8_off);
 }
 
 __device__ __forceinline__ float2 loss_grad(float2 x, float t, double mi, double mt, double c1,

@@ -108,14 +108,17 @@ def gFresnelPropBandLimit(u, height, width, dy, dx, lam, z):
     return FresnelPropBandLimit(u, height, width, dy, dx, lam, z, variant="gpu")
 
 
-def prop_batch(u, dy, dx, lam, z, propmode=ASM, flags=0, pre=None, post=None):
+def prop_batch(u, dy, dx, lam, z, propmode=ASM, flags=0, pre=None, post=None, img=None,
+               img_offset=False):
     """Every field of the (nb, ny, nx) complex64 tensor u, in place, replaced by
     post * Prop(pre * u[k]): Prop is AsmProp (propmode ASM, flags BANDLIMIT / ADJOINT /
     PREC_F32) or FresnelProp (FRESNEL, flags FRESNEL_BL_CPU / FRESNEL_BL_GPU / ADJOINT).
     pre / post are (ny, nx) complex64 factor fields shared by every field (a lens array,
     a diffuser's phase, an aperture; None: 1), multiplied in as the first pass loads
     and the last pass stores, where the reference multiplies in passes of their own
-    (Lensarry::mult_complex, include/Lensarry.h:262-270)."""
+    (Lensarry::mult_complex, include/Lensarry.h:262-270).  img, a (nb, ny, nx) uint8
+    tensor, also receives the 8-bit phase of every output value (phase8bit,
+    include/olimg.h:205-213; img_offset: the +pi form) from the last pass."""
     if not isinstance(u, torch.Tensor) or u.dim() != 3:
         raise _lib.OlxInvalid(_lib.EINVAL, "prop_batch: u must be a (nb, ny, nx) tensor")
     nb, ny, nx = u.shape
@@ -125,11 +128,15 @@ def prop_batch(u, dy, dx, lam, z, propmode=ASM, flags=0, pre=None, post=None):
             _field(f, ny, nx, name)
             if f.device != u.device:
                 raise _lib.OlxInvalid(_lib.EINVAL, f"prop_batch: {name} on {f.device}, u on {u.device}")
+    if img is not None:
+        _out2d(img.view(nb * ny, nx) if img.dim() == 3 and tuple(img.shape) == (nb, ny, nx)
+               else img, nb * ny, nx, torch.uint8, "img")
     if propmode not in (ASM, FRESNEL):
         raise _lib.OlxInvalid(_lib.EINVAL, "prop_batch: propmode must be ASM or FRESNEL")
+    nul = lambda t: None if t is None else _ptr(t)
     check(_lib.call("olx_prop_batch_c64", u.device.index, propmode, _ptr(u), nb, ny, nx, dy, dx,
-                    lam, z, flags, _ptr(pre) if pre is not None else None,
-                    _ptr(post) if post is not None else None), "prop_batch")
+                    lam, z, flags, nul(pre), nul(post), nul(img),
+                    _lib.PHASE8_OFFSET if img_offset else 0), "prop_batch")
     return u
 
 
@@ -454,6 +461,24 @@ def Cgh(obj, out, ny, nx, py, px, lam, exact_r=False, phase_out=None, amp_out=No
     check(_lib.call("olx_cgh_c64", obj.device.index, obj.ptr(), obj.size, nul(out),
                                   nul(phase_out), nul(amp_out), ny, nx, py, px, lam, flags), "Cgh")
     return out if out is not None else phase_out
+
+
+def CghPhase8(obj, img, ny, nx, py, px, lam, out=None, exact_r=False, offset=False,
+              accumulate=False, prec_f32=False):
+    """Cgh -> phase8bit in one kernel (include/olcgh.h:275-295, include/olimg.h:205-213):
+    img (uint8, ny x nx) receives the 8-bit phase of the hologram, the byte phase8bit
+    gives for the phase Cgh writes, without the phase array in between.  out
+    (complex64) optionally receives the field as well; offset selects the +pi mapping."""
+    if img is None:
+        raise _lib.OlxInvalid(_lib.EINVAL, "CghPhase8: img is required")
+    _out2d(img, ny, nx, torch.uint8, "img")
+    _out2d(out, ny, nx, torch.complex64, "out")
+    flags = (_lib.CGH_EXACT_R if exact_r else 0) | (_lib.CGH_ACCUMULATE if accumulate else 0) | \
+        (_lib.PREC_F32 if prec_f32 else 0) | (_lib.PHASE8_OFFSET if offset else 0)
+    check(_lib.call("olx_cgh_img8_c64", obj.device.index, obj.ptr(), obj.size,
+                    None if out is None else _ptr(out), _ptr(img), ny, nx, py, px, lam, flags),
+          "CghPhase8")
+    return img
 
 
 def CghFP64(obj, out, ny, nx, py, px, lam, exact_r=False):
