@@ -9,7 +9,11 @@
 //      (fixed summation order), stage each point's constants (position, window
 //      limit, 1/(2 dz lambda), 1/|dz|) in LDS 128 at a time, and accumulate
 //      exp(i pi rho^2/(lambda dz))/|dz| for the pixels inside each window (two
-//      pixels of one column per thread).
+//      pixels of one column per thread).  A list longer than the LDS sort holds
+//      (dense clouds) is not used: that tile scans the whole cloud in index order
+//      instead, keeping the points whose window touches it (the same test that
+//      built the lists), so every tile sums its points in ascending index order and
+//      the result does not depend on the order the atomic fill left.
 // This file is built without FMA contraction: dx*dx + dy*dy rounds each product,
 // as the reference's host arithmetic does, so window membership is decided alike.
 // Window tests and rho^2 use FP64 in the reference's expression order, so pixel
@@ -187,6 +191,7 @@ __device__ __forceinline__ WrpPt wrp_stage(const olx_point& q, double lambda, do
 // One work-group of WRP_NT = 128 threads per 16 x 16 tile: thread t owns column t % 16
 // and rows t / 16 and t / 16 + 8, which share dx and each point read from LDS.
 constexpr int WRP_NT = 128;
+static_assert(WRP_NT == 128, "the dense-tile compaction counts two waves");
 
 template <bool SQUARE>
 __device__ __forceinline__ void wrp_pixel(const WrpPt& P, double dx, double dy, float& re,
@@ -202,13 +207,25 @@ __device__ __forceinline__ void wrp_pixel(const WrpPt& P, double dx, double dy, 
     }
 }
 
+// does the window of point q touch tile (tx, ty)?  The test of k_wrp_count / k_wrp_fill.
+__device__ __forceinline__ bool wrp_touches(const olx_point& q, int tx, int ty, int ny, int nx,
+                                            double p, double lambda, double plane_z, double plim,
+                                            int square) {
+    const double R = window_extent(wrp_radius(q.z, plane_z, plim, lambda, square), square);
+    int x0, x1, y0, y1;
+    tile_span(q.x, R, p, nx, &x0, &x1);
+    tile_span(q.y, R, p, ny, &y0, &y1);
+    return tx >= x0 && tx <= x1 && ty >= y0 && ty <= y1;
+}
+
 template <bool SQUARE>
 __global__ void __launch_bounds__(WRP_NT)
-k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
+k_wrp_accum(const olx_point* __restrict__ pts, int n, const int* __restrict__ offs,
             const int* __restrict__ list, float2* __restrict__ U, int ny, int nx, double p,
             double lambda, double plane_z, double plim) {
     __shared__ int ids[WRP_SORT_CAP];
     __shared__ WrpPt sp[WRP_NT];
+    __shared__ int wcnt[WRP_NT / 64];
     const int tiles_x = (nx + WT - 1) / WT;
     const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
@@ -239,10 +256,39 @@ k_wrp_accum(const olx_point* __restrict__ pts, const int* __restrict__ offs,
                 __syncthreads();
             }
     }
-    for (int base = 0; base < cnt; base += WRP_NT) {
+    if (!sorted) {
+        // dense tile: the whole cloud in index order, 128 points a step, compacted
+        // (order kept) to those whose window touches this tile
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        for (int base = 0; base < n; base += WRP_NT) {
+            const int j = base + threadIdx.x;
+            olx_point q;
+            bool take = false;
+            if (j < n) {
+                q = pts[j];
+                take = wrp_touches(q, tx, ty, ny, nx, p, lambda, plane_z, plim, SQUARE);
+            }
+            const unsigned long long mk = __ballot(take);
+            if (lane == 0) wcnt[wv] = __popcll(mk);
+            __syncthreads();
+            const int before = wv ? wcnt[0] : 0;  // WRP_NT = 128: two waves
+            const int m = wcnt[0] + wcnt[1];
+            if (take) sp[before + __popcll(mk & ((1ull << lane) - 1ull))] =
+                wrp_stage(q, lambda, plane_z, plim, SQUARE);
+            __syncthreads();
+            for (int k = 0; k < m; k++) {
+                const WrpPt P = sp[k];
+                const double dx = xw - P.x;
+                wrp_pixel<SQUARE>(P, dx, yh0 - P.y, re0, im0);
+                wrp_pixel<SQUARE>(P, dx, yh1 - P.y, re1, im1);
+            }
+            __syncthreads();
+        }
+    }
+    for (int base = 0; sorted && base < cnt; base += WRP_NT) {
         const int m = min(WRP_NT, cnt - base);
         if ((int)threadIdx.x < m) {
-            const int j = sorted ? ids[base + threadIdx.x] : list[beg + base + threadIdx.x];
+            const int j = ids[base + threadIdx.x];
             sp[threadIdx.x] = wrp_stage(pts[j], lambda, plane_z, plim, SQUARE);
         }
         __syncthreads();
@@ -331,11 +377,11 @@ extern "C" int olx_wrp_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     {
         TimedLaunch tl(ctx, "wrp_accum");
         if (square)
-            k_wrp_accum<true><<<ntiles, WRP_NT, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
-                                                     (int)nx, p, lambda, plane_z, p_limit);
+            k_wrp_accum<true><<<ntiles, WRP_NT, 0, s>>>(d_pts, (int)n, offs, list, (float2*)d_U,
+                                                     (int)ny, (int)nx, p, lambda, plane_z, p_limit);
         else
-            k_wrp_accum<false><<<ntiles, WRP_NT, 0, s>>>(d_pts, offs, list, (float2*)d_U, (int)ny,
-                                                      (int)nx, p, lambda, plane_z, p_limit);
+            k_wrp_accum<false><<<ntiles, WRP_NT, 0, s>>>(d_pts, (int)n, offs, list, (float2*)d_U,
+                                                      (int)ny, (int)nx, p, lambda, plane_z, p_limit);
     }
     return check_launch("wrp accumulate");
 }

@@ -215,6 +215,31 @@ def test_wrp_matches_oracle(square):
     assert torch.equal(U, U2)
 
 
+@pytest.mark.parametrize("square", [False, True])
+def test_wrp_dense_tiles_deterministic(square):
+    """A cloud dense enough that tile lists pass the 4096 entries the LDS sort holds:
+    40,000 points within a 32 x 32-pixel patch plus 2,000 over the whole 160 x 200
+    plane, so tiles near the patch take the dense path and the rest the sorted
+    lists.  Sampled pixels against the FP64 oracle; two runs bit for bit alike."""
+    ny, nx = 160, 200
+    a = cloud(40000, 77, 32, 32, 0.151, 0.155)
+    b = cloud(2000, 78, nx, ny, 0.151, 0.155)
+    pts = {k: np.concatenate([a[k] + (60 * P if k in ("x", "y") else 0), b[k]]) for k in a}
+    o = obj_of(pts, with_ap=False)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ol().WRPStep1(o, U, ny, nx, P, LAM, 0.15, square=square)
+    rng = np.random.default_rng(5)
+    ph, pw = rng.integers(0, ny, 1500), rng.integers(0, nx, 1500)
+    ph[:200], pw[:200] = rng.integers(50, 110, 200), rng.integers(50, 110, 200)  # in the patch
+    ref = olref.wrp_pixels(pts["x"], pts["y"], pts["z"], ph, pw, P, LAM, 0.15, square=square)
+    got = U.cpu().numpy()[ph, pw]
+    assert rel(got, ref) < 1e-5
+    for _ in range(2):
+        U2 = torch.empty_like(U)
+        ol().WRPStep1(o, U2, ny, nx, P, LAM, 0.15, square=square)
+        assert torch.equal(U, U2)
+
+
 def test_wrp_large_plane_matches_windows():
     """A 33024^2 plane (4.26 M tiles of 16 x 16: the tile-count scan spans 1040 blocks
     and two rounds of its top level) with clusters of points in three far-apart
