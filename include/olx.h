@@ -296,7 +296,9 @@ double olx_wrp_plane_z(int64_t ny, int64_t nx, double p, double lambda, double z
 /* Points are binned into nlayers uniform depth layers over [zmin, zmax) and
  * snapped to the layer centres; each layer is the complex GEMM
  * U += Y_l diag(a_l) X_l^T of unit-modulus exponential factors, stored in
- * F16 / BF16 / split BF16 with FP32 accumulation. */
+ * F16 / BF16 / split BF16 with FP32 accumulation.  Sides in [1, 8192]; the GEMM
+ * runs on whole 128 x 128 or 256 x 128 tiles and stores only the pixels inside
+ * the hologram. */
 int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void* d_U,
                         int64_t ny, int64_t nx, double py, double px, double lambda,
                         int nlayers, double zmin, double zmax, int precision,

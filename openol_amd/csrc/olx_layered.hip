@@ -193,10 +193,12 @@ __device__ inline f16v_t mfma<__bf16>(const uint4& a, const uint4& b, f16v_t c) 
 
 // U[h][w] (+)= sum_k A[h][k] Br[w][k]  (re),  A[h][k] Bi[w][k]  (im), k < ldk.
 // Work-group: 256 threads, 128 x 128 output tile; wave (wr, wc) owns 64 x 64.
+// The factor matrices hold whole tiles (rows past ny / nx padded, any content: they
+// only reach outputs past the hologram's edge, which are not stored).
 template <typename T>
 __global__ void __launch_bounds__(256, 2)
 k_lay_gemm(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* __restrict__ U,
-           int nx, int accumulate) {
+           int ny, int nx, int accumulate) {
     __shared__ __align__(16) T sA[2][GT * KP];
     __shared__ __align__(16) T sB[2][GT * KP];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -281,6 +283,7 @@ k_lay_gemm(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* _
             for (int v = 0; v < 16; v++) {
                 const int h = h0 + wr * 64 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int w = w0 + wc * 64 + j * 32 + (lane & 31);
+                if (h >= ny || w >= nx) continue;
                 float2* p = &U[(size_t)h * nx + w];
                 float2 o = make_float2(accr[i][j][v], acci[i][j][v]);
                 if (accumulate) {
@@ -445,7 +448,7 @@ __device__ __forceinline__ void lay_stage(
 template <typename T, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES, 1)
 k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* __restrict__ U,
-            int nx, int accumulate) {
+            int ny, int nx, int accumulate) {
     __shared__ __align__(16) char sA0[GT2M * ROWB], sA1[GT2M * ROWB], sA2[GT2M * ROWB];
     __shared__ __align__(16) char sB0[GT2N * ROWB], sB1[GT2N * ROWB], sB2[GT2N * ROWB];
     char *a0 = sA0, *a1 = sA1, *a2 = sA2, *b0 = sB0, *b1 = sB1, *b2 = sB2;
@@ -506,6 +509,7 @@ k_lay_gemm2(const T* __restrict__ A, const T* __restrict__ Br, int ldk, float2* 
             for (int v = 0; v < 16; v++) {
                 const int h = h0 + wr * 32 * MI + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int w = w0 + wc * 64 + j * 32 + (lane & 31);
+                if (h >= ny || w >= nx) continue;
                 float2* p = &U[(size_t)h * nx + w];
                 float2 o = make_float2(accr[i][j][v], acci[i][j][v]);
                 if (accumulate) {
@@ -539,8 +543,8 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
     if (!ctx || !d_U) OLX_FAIL(OLX_EINVAL, "layered: null argument");
     if (n < 0 || (n > 0 && !d_pts)) OLX_FAIL(OLX_EINVAL, "layered: bad point array");
     if (n > INT32_MAX / 2) OLX_FAIL(OLX_EUNSUPPORTED, "layered: too many points");
-    if (ny <= 0 || nx <= 0 || ny % GT || nx % GT || ny > 8192 || nx > 8192)
-        OLX_FAIL(OLX_EUNSUPPORTED, "layered: hologram sides must be multiples of %d up to 8192", GT);
+    if (ny <= 0 || nx <= 0 || ny > 8192 || nx > 8192)
+        OLX_FAIL(OLX_EUNSUPPORTED, "layered: hologram sides must be in [1, 8192]");
     if (!(py > 0) || !(px > 0) || !(lambda > 0) || nlayers < 1 || !(zmax > zmin) || !(zmin > 0))
         OLX_FAIL(OLX_EINVAL, "layered: need pitch, lambda > 0, nlayers >= 1, 0 < zmin < zmax");
     if (precision < OLX_LAYERED_F16 || precision > OLX_LAYERED_BF16X3)
@@ -580,30 +584,33 @@ extern "C" int olx_cgh_layered_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t
     const int kc = (int)std::min<int64_t>(KC_PTS, ((n + KS / 2 - 1) / (KS / 2)) * (KS / 2));
     const int ldk = 2 * kc;
     const int nmat = precision == OLX_LAYERED_BF16X3 ? 4 : 2;
-    const size_t ybytes = sizeof(uint16_t) * (size_t)ny * ldk;
-    const size_t xbytes = sizeof(uint16_t) * (size_t)nx * ldk;
+    // factor rows padded to whole work-group tiles: 256 x 128 tiles unless that pads
+    // the rows by more than an eighth beyond the 128 x 128 tiles (or those are asked for)
+    auto up = [](int64_t v, int64_t m) { return (v + m - 1) / m * m; };
+    const bool big = !(flags & OLX_LAYERED_TILE128) && 8 * up(ny, GT2M) <= 9 * up(ny, GT);
+    const int64_t nyp = big ? up(ny, GT2M) : up(ny, GT), nxp = up(nx, GT);
+    const size_t ybytes = sizeof(uint16_t) * (size_t)nyp * ldk;
+    const size_t xbytes = sizeof(uint16_t) * (size_t)nxp * ldk;
     OLX_TRY(ctx->lay_counts.reserve((nmat / 2) * (ybytes + xbytes)));
     char* fb = (char*)ctx->lay_counts.p;
     void* Yh = fb;
     void* Xh = fb + ybytes;
     void* Yl = fb + ybytes + xbytes;
     void* Xl = fb + 2 * ybytes + xbytes;
-    const dim3 ggrid((unsigned)(nx / GT), (unsigned)(ny / GT));
-    // 256 x 128 tiles where the sides allow, unless the 128 x 128 kernel is asked for
-    const bool big = ny % GT2M == 0 && nx % GT2N == 0 && !(flags & OLX_LAYERED_TILE128);
-    const dim3 ggrid2((unsigned)(nx / GT2N), (unsigned)(ny / GT2M));
+    const dim3 ggrid((unsigned)(nxp / GT), (unsigned)(nyp / GT));
+    const dim3 ggrid2((unsigned)(nxp / GT2N), (unsigned)(nyp / GT2M));
 
     auto gemm = [&](const void* Ya, const void* Xb, int acc) {
         if (precision == OLX_LAYERED_F16) {
             if (big)
-                k_lay_gemm2<_Float16, GT2_WAVES><<<ggrid2, 64 * GT2_WAVES, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+                k_lay_gemm2<_Float16, GT2_WAVES><<<ggrid2, 64 * GT2_WAVES, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)ny, (int)nx, acc);
             else
-                k_lay_gemm<_Float16><<<ggrid, 256, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+                k_lay_gemm<_Float16><<<ggrid, 256, 0, s>>>((const _Float16*)Ya, (const _Float16*)Xb, ldk, (float2*)d_U, (int)ny, (int)nx, acc);
         } else {
             if (big)
-                k_lay_gemm2<__bf16, GT2_WAVES><<<ggrid2, 64 * GT2_WAVES, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+                k_lay_gemm2<__bf16, GT2_WAVES><<<ggrid2, 64 * GT2_WAVES, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)ny, (int)nx, acc);
             else
-                k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)nx, acc);
+                k_lay_gemm<__bf16><<<ggrid, 256, 0, s>>>((const __bf16*)Ya, (const __bf16*)Xb, ldk, (float2*)d_U, (int)ny, (int)nx, acc);
         }
     };
     bool first = true;

@@ -103,9 +103,45 @@ def test_layered_wide_tile(prec, shape):
     assert olref.rel_l2(U.cpu().numpy()[ph, pw], ref) < 1e-3
 
 
+@pytest.mark.parametrize("prec", ["f16", "bf16x3"])
+@pytest.mark.parametrize("shape", [(100, 128), (1080, 200), (90, 333), (2160, 136)])
+def test_layered_edge_tiles(prec, shape):
+    """Sides that are not multiples of the work-group tile (the reference's 1080 and
+    2160 rows among them): the GEMM covers whole tiles and stores the pixels inside
+    the hologram only.  Sampled pixels, the last row and column included, against
+    the FP64 oracle on the snapped cloud; the 128 x 128 tiles agree bit for bit."""
+    ny, nx = shape
+    n, nl, zmin, zmax = 3000, 8, 0.10, 0.20
+    pts = cloud(n, 63, ny, nx, zmin, zmax)
+    obj = ol().Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ol().CghLayered(obj, U, ny, nx, P, P, LAM, nl, zmin, zmax, precision=prec)
+    V = torch.empty_like(U)
+    ol().CghLayered(obj, V, ny, nx, P, P, LAM, nl, zmin, zmax, precision=prec, tile128=True)
+    assert torch.equal(U, V)
+    zl = snapped(pts, zmin, zmax, nl)
+    rng = np.random.default_rng(6)
+    ph = np.concatenate([rng.integers(0, ny, 300), np.full(20, ny - 1), rng.integers(0, ny, 20)])
+    pw = np.concatenate([rng.integers(0, nx, 300), rng.integers(0, nx, 20), np.full(20, nx - 1)])
+    ref = olref.cgh_pixels(pts["x"], pts["y"], zl, ph, pw, P, P, LAM, pts["amp"], pts["phase"])
+    assert olref.rel_l2(U.cpu().numpy()[ph, pw], ref) < 1e-3
+
+
+def test_layered_edge_tiles_stay_inside():
+    """Every store stays inside the hologram: a 100 x 130 output in the middle of a
+    larger buffer leaves the buffer around it untouched."""
+    ny, nx = 100, 130
+    pts = cloud(500, 64, ny, nx, 0.10, 0.20)
+    obj = ol().Object(pts["x"], pts["y"], pts["z"], pts["amp"], pts["phase"])
+    buf = torch.full((ny * nx + 4096,), 3 - 5j, dtype=torch.complex64, device="cuda")
+    U = buf[2048:2048 + ny * nx].view(ny, nx)
+    ol().CghLayered(obj, U, ny, nx, P, P, LAM, 8, 0.10, 0.20)
+    assert torch.all(buf[:2048] == 3 - 5j) and torch.all(buf[2048 + ny * nx:] == 3 - 5j)
+
+
 def test_layered_rejects_unsupported_shapes():
     from openol_amd._lib import OlxError
     obj = ol().Object([1e-4], [1e-4], [0.15])
-    U = torch.empty((100, 128), dtype=torch.complex64, device="cuda")
+    U = torch.empty((100, 8200), dtype=torch.complex64, device="cuda")
     with pytest.raises(OlxError):
-        ol().CghLayered(obj, U, 100, 128, P, P, LAM, 8, 0.1, 0.2)
+        ol().CghLayered(obj, U, 100, 8200, P, P, LAM, 8, 0.1, 0.2)
