@@ -3,10 +3,13 @@
 // Host overloads keep the signatures of include/olprop.h (namespace ol, same
 // template parameters and argument order): the field stays in the caller's
 // std::unique_ptr<std::complex<COMPLEX_T>[]> allocation, is staged to HBM, runs
-// through the fused HIP kernels of libolx.so, and is copied back in place.  The
-// device computes complex64 fields with FP64 phase arithmetic (PREC_T=double
-// semantics, unless OL_PREC_T_FLOAT_COMPAT asks for the reference's float arithmetic
-// where PREC_T = float); COMPLEX_T = double fields are rounded to complex64 for the device.
+// through the fused HIP kernels of libolx.so, and is copied back in place.
+// COMPLEX_T = float fields are computed in complex64 with FP64 phase arithmetic
+// (PREC_T=double semantics, unless OL_PREC_T_FLOAT_COMPAT asks for the reference's
+// float arithmetic where PREC_T = float).  COMPLEX_T = double fields of AsmProp,
+// mAsmProp, FresnelProp, FresnelPropBandLimit and Prop stay complex128 and run the
+// FP64 passes (sides up to 4096: larger ones throw); the other complex<double> calls
+// are rounded to complex64.
 // Device overloads mirror include/golprop.h on ol::device_ptr<std::complex<float>[]>.
 // Errors throw std::runtime_error; nothing is printed and nothing exits.
 #ifndef OLPROP_H
@@ -49,6 +52,16 @@ void on_device(std::unique_ptr<std::complex<C>[]>& u, int64_t count, F&& fn) {
     }
 }
 
+// stage a host field to HBM in its own precision, run fn(device pointer), copy back
+template <typename C, typename F>
+void on_device_native(std::unique_ptr<std::complex<C>[]>& u, int64_t count, F&& fn) {
+    const size_t bytes = sizeof(std::complex<C>) * (size_t)count;
+    void* d = staging(0).get(bytes);
+    check(olx_copy_h2d(ctx(), d, u.get(), bytes), "copy to device");
+    fn(d);
+    check(olx_copy_d2h(ctx(), u.get(), d, bytes), "copy to host");
+}
+
 inline int64_t win(double v) {
     const double f = std::floor(v);
     if (!(f == f) || f >= (double)INT_MAX) return INT_MAX;
@@ -88,6 +101,15 @@ constexpr uint32_t prec_flags() {
 template <typename PREC_T = double, typename COMPLEX_T>
 void AsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int64_t width,
              PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    if constexpr (std::is_same<COMPLEX_T, double>::value) {
+        if (detail::prec_flags<PREC_T>() == 0u) {  // the fftw3 path: complex128 in FP64
+            detail::on_device_native(u, height * width, [&](void* d) {
+                detail::check(olx_asm_c128(detail::ctx(), d, height, width, dy, dx, lambda, z, 0),
+                              "AsmProp");
+            });
+            return;
+        }
+    }
     detail::on_device(u, height * width, [&](void* d) {
         detail::check(olx_asm_c64(detail::ctx(), d, height, width, dy, dx, lambda, z,
                                   detail::prec_flags<PREC_T>()), "AsmProp");
@@ -99,6 +121,13 @@ void AsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int6
 template <typename PREC_T = double, typename COMPLEX_T>
 void mAsmProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int64_t height, int64_t width,
               PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    if constexpr (std::is_same<COMPLEX_T, double>::value) {
+        detail::on_device_native(u, height * width, [&](void* d) {
+            detail::check(olx_asm_c128(detail::ctx(), d, height, width, dy, dx, lambda, z,
+                                       OLX_EXTRA_INV_M2), "mAsmProp");
+        });
+        return;
+    }
     detail::on_device(u, height * width, [&](void* d) {
         detail::check(olx_asm_c64(detail::ctx(), d, height, width, dy, dx, lambda, z,
                                   OLX_EXTRA_INV_M2), "mAsmProp");
@@ -143,6 +172,13 @@ void ifft2d_hyblid(std::unique_ptr<std::complex<PREC_T>[]>& src,
 template <typename PREC_T = double, typename COMPLEX_T>
 void FresnelProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, size_t height, size_t width,
                  PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    if constexpr (std::is_same<COMPLEX_T, double>::value) {
+        detail::on_device_native(u, (int64_t)(height * width), [&](void* d) {
+            detail::check(olx_fresnel_c128(detail::ctx(), d, (int64_t)height, (int64_t)width, dy,
+                                           dx, lambda, z, 0), "FresnelProp");
+        });
+        return;
+    }
     detail::on_device(u, (int64_t)(height * width), [&](void* d) {
         detail::check(olx_fresnel_c64(detail::ctx(), d, (int64_t)height, (int64_t)width, dy, dx,
                                       lambda, z, 0), "FresnelProp");
@@ -153,6 +189,14 @@ void FresnelProp(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, size_t height, s
 template <typename PREC_T = double, typename COMPLEX_T>
 void FresnelPropBandLimit(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, size_t height,
                           size_t width, PREC_T dy, PREC_T dx, PREC_T lambda, PREC_T z) {
+    if constexpr (std::is_same<COMPLEX_T, double>::value) {
+        detail::on_device_native(u, (int64_t)(height * width), [&](void* d) {
+            detail::check(olx_fresnel_c128(detail::ctx(), d, (int64_t)height, (int64_t)width, dy,
+                                           dx, lambda, z, OLX_FRESNEL_BL_CPU),
+                          "FresnelPropBandLimit");
+        });
+        return;
+    }
     detail::on_device(u, (int64_t)(height * width), [&](void* d) {
         detail::check(olx_fresnel_c64(detail::ctx(), d, (int64_t)height, (int64_t)width, dy, dx,
                                       lambda, z, OLX_FRESNEL_BL_CPU), "FresnelPropBandLimit");
@@ -256,6 +300,13 @@ void Prop(std::unique_ptr<std::complex<COMPLEX_T>[]>& u, int ny, int nx, PREC_T 
           PREC_T lambda, PREC_T d, PROPMODE propmode = PROPMODE::AUTO) {
     const int mode = propmode == ASM ? OLX_MODE_ASM : propmode == FRESNEL ? OLX_MODE_FRESNEL
                                                                           : OLX_MODE_AUTO;
+    if constexpr (std::is_same<COMPLEX_T, double>::value) {
+        detail::on_device_native(u, (int64_t)ny * nx, [&](void* dd) {
+            detail::check(olx_prop_c128(detail::ctx(), dd, ny, nx, dy, dx, lambda, d, mode, nullptr),
+                          "Prop");
+        });
+        return;
+    }
     detail::on_device(u, (int64_t)ny * nx, [&](void* dd) {
         detail::check(olx_prop_c64(detail::ctx(), dd, ny, nx, dy, dx, lambda, d, mode, nullptr),
                       "Prop");

@@ -5,9 +5,9 @@
 //
 // Host arrays are staged to HBM and the result is copied back.  Data movement
 // (zeropadding, del_zero, cut, fftshift) and the elementwise complex128 products and
-// scalings are exact for every element type; the 2-D FFT computes in complex64
-// (complex<double> data is rounded to complex64 for the transform, as the
-// propagation overloads do).  Functions that return a new array (zeropadding,
+// scalings are exact for every element type; the 2-D FFT computes complex<float> data
+// in complex64 and complex<double> data in complex128 (FP64 throughout, sides up to
+// 8192).  Functions that return a new array (zeropadding,
 // del_zero, cut and their device twins) move it into dst, which may be src, as the
 // reference does.  Errors throw std::runtime_error.
 #ifndef OLUTILS_H
@@ -80,7 +80,10 @@ void fft_host(std::unique_ptr<std::complex<T>[]>& src, std::unique_ptr<std::comp
         if (!dst) dst = std::make_unique<std::complex<T>[]>((size_t)(ny * nx));
         for (int64_t i = 0; i < ny * nx; i++) dst[i] = src[i];
     }
-    on_device(dst, ny * nx, [&](void* d) { check(olx_fft2d_c64(ctx(), d, ny, nx, sign), "fft"); });
+    if constexpr (std::is_same<T, double>::value)
+        on_device_native(dst, ny * nx, [&](void* d) { check(olx_fft2d_c128(ctx(), d, ny, nx, sign), "fft"); });
+    else
+        on_device(dst, ny * nx, [&](void* d) { check(olx_fft2d_c64(ctx(), d, ny, nx, sign), "fft"); });
 }
 
 }  // namespace detail

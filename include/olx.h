@@ -155,6 +155,21 @@ int olx_asm_c64_noalloc(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double 
  * gFresnelProp (include/golprop.h:167-198), gFresnelPropBandLimit (:258-302). */
 int olx_fresnel_c64(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
                     double lambda, double z, uint32_t flags);
+/* complex128 twins of olx_asm_c64 / olx_fresnel_c64 / olx_prop_c64 / olx_fft2d_c64:
+ * FP64 throughout (transforms, twiddles, transfer function and spectra), the
+ * reference's fftw3 (double) path for std::complex<double> fields (include/olprop.h:
+ * 125-170, 227-248, 656-682; include/olutils.h:19-66).  d_u holds interleaved double
+ * pairs.  Propagation sides in [2, 4096], the FFT's in [2, 8192], padded / transform
+ * lengths with prime factors 2, 3, 5, 7, 11, 13 only; flags: ASM BANDLIMIT, ADJOINT,
+ * EXTRA_INV_M2; Fresnel FRESNEL_BL_CPU / FRESNEL_BL_GPU, ADJOINT.  Evanescent waves
+ * are 0 (the reference: NaN). */
+int olx_asm_c128(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
+                 double lambda, double z, uint32_t flags);
+int olx_fresnel_c128(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
+                     double lambda, double z, uint32_t flags);
+int olx_prop_c128(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy, double dx,
+                  double lambda, double z, int mode, int* chosen);
+int olx_fft2d_c128(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, int sign);
 /* Batched propagation with fused factor fields.  d_u holds nb fields of ny x nx,
  * field k at d_u + k ny nx; each is replaced in place by post * Prop(pre * u_k),
  * where Prop is olx_asm_c64 (mode OLX_MODE_ASM, its flags) or olx_fresnel_c64

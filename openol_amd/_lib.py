@@ -68,6 +68,10 @@ SIGNATURES = {
     "olx_asm_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_asm_c64_noalloc": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
     "olx_fresnel_c64": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_asm_c128": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_fresnel_c128": [_p, _p, _i64, _i64, _d, _d, _d, _d, _u32],
+    "olx_prop_c128": [_p, _p, _i64, _i64, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
+    "olx_fft2d_c128": [_p, _p, _i64, _i64, _i],
     "olx_prop_batch_c64": [_p, _i, _p, _i64, _i64, _i64, _d, _d, _d, _d, _u32, _p, _p, _p, _u32],
     "olx_arss_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _u32],
     "olx_shifted_asm_c64": [_p, _p, _p, _i64, _i64, _d, _d, _d, _d, _d, _d, _u32],

@@ -27,27 +27,33 @@ def _ptr(t):
     return _p(t.data_ptr())
 
 
-def _field(u, height, width, name="u"):
+def _field(u, height, width, name="u", c128=False):
+    """Checks a field tensor; with c128, complex128 is accepted too and the entry-point
+    suffix ("c64" / "c128") is returned."""
     if not isinstance(u, torch.Tensor) or not u.is_cuda:
         raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be a CUDA (HIP) tensor")
-    if u.dtype != torch.complex64:
-        raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be complex64, got {u.dtype}")
+    ok = (torch.complex64, torch.complex128) if c128 else (torch.complex64,)
+    if u.dtype not in ok:
+        raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be {' or '.join(map(str, ok))}, got {u.dtype}")
     if not u.is_contiguous():
         raise _lib.OlxInvalid(_lib.EINVAL, f"{name} must be contiguous")
     if u.dim() != 2 or tuple(u.shape) != (int(height), int(width)):
         raise _lib.OlxInvalid(_lib.EINVAL, f"{name} shape {tuple(u.shape)} != ({height}, {width})")
-    return u
+    return ("c128" if u.dtype == torch.complex128 else "c64") if c128 else u
 
 
 # ------------------------------------------------------------------ propagation
 def AsmProp(u, height, width, dy, dx, lam, z, bandlimit=False, adjoint=False, prec_f32=False):
     """ol::AsmProp (include/olprop.h:227-248): in place, net scale 1.  prec_f32 computes
-    the transfer function with the reference's PREC_T = float arithmetic."""
-    _field(u, height, width)
+    the transfer function with the reference's PREC_T = float arithmetic (complex64).
+    A complex128 field runs the FP64 passes (the reference's fftw3 path)."""
+    suf = _field(u, height, width, c128=True)
+    if suf == "c128" and prec_f32:
+        raise _lib.OlxInvalid(_lib.EINVAL, "AsmProp: prec_f32 takes complex64 fields")
     flags = (_lib.BANDLIMIT if bandlimit else 0) | (_lib.ADJOINT if adjoint else 0) | \
         (_lib.PREC_F32 if prec_f32 else 0)
-    check(_lib.call("olx_asm_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z, flags),
-          "AsmProp")
+    check(_lib.call(f"olx_asm_{suf}", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
+                    flags), "AsmProp")
     return u
 
 
@@ -80,27 +86,27 @@ def mAsmProp(u, height, width, dy, dx, lam, z):
     """ol::mAsmProp (include/molprop.h:80-101): the angular spectrum propagation whose
     transforms the reference stages through host memory, net scale 1/M^2 (it scales
     before and after the inverse transform).  The field stays in HBM here."""
-    _field(u, height, width)
-    check(_lib.call("olx_asm_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
-                                  _lib.EXTRA_INV_M2), "mAsmProp")
+    suf = _field(u, height, width, c128=True)
+    check(_lib.call(f"olx_asm_{suf}", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
+                    _lib.EXTRA_INV_M2), "mAsmProp")
     return u
 
 
 def FresnelProp(u, height, width, dy, dx, lam, z, adjoint=False):
     """ol::FresnelProp (include/olprop.h:125-148): circconv(pad u, h) / M^2, in place."""
-    _field(u, height, width)
-    check(_lib.call("olx_fresnel_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
-                            _lib.ADJOINT if adjoint else 0), "FresnelProp")
+    suf = _field(u, height, width, c128=True)
+    check(_lib.call(f"olx_fresnel_{suf}", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
+                    _lib.ADJOINT if adjoint else 0), "FresnelProp")
     return u
 
 
 def FresnelPropBandLimit(u, height, width, dy, dx, lam, z, variant="cpu"):
     """ol::FresnelPropBandLimit (include/olprop.h:150-170, per-axis window) or, with
     variant="gpu", gFresnelPropBandLimit (include/golprop.h:258-302, x-limit on both axes)."""
-    _field(u, height, width)
+    suf = _field(u, height, width, c128=True)
     flags = {"cpu": _lib.FRESNEL_BL_CPU, "gpu": _lib.FRESNEL_BL_GPU}[variant]
-    check(_lib.call("olx_fresnel_c64", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
-                            flags), "FresnelPropBandLimit")
+    check(_lib.call(f"olx_fresnel_{suf}", u.device.index, _ptr(u), height, width, dy, dx, lam, z,
+                    flags), "FresnelPropBandLimit")
     return u
 
 
@@ -231,10 +237,10 @@ def TiltedReferenceProp(u, ny, nx, dy, dx, lam, z, rotate, inp=BILINEAR, propmod
 def Prop(u, ny, nx, dy, dx, lam, d, propmode=AUTO):
     """ol::Prop (include/olprop.h:656-682).  Returns the mode used (ASM, FRESNEL or
     FRESNEL_BL when AUTO falls back to the band-limited Fresnel)."""
-    _field(u, ny, nx)
+    suf = _field(u, ny, nx, c128=True)
     ch = ctypes.c_int(-1)
-    check(_lib.call("olx_prop_c64", u.device.index, _ptr(u), ny, nx, dy, dx, lam, d, propmode,
-                         ctypes.byref(ch)), "Prop")
+    check(_lib.call(f"olx_prop_{suf}", u.device.index, _ptr(u), ny, nx, dy, dx, lam, d, propmode,
+                    ctypes.byref(ch)), "Prop")
     return ch.value
 
 
@@ -249,18 +255,20 @@ def FresnelPropCheck(ny, nx, dy, dx, lam, z):
 
 
 def fft(u):
-    """Unnormalised forward 2-D FFT in place (ol::fft, include/olutils.h:19-41)."""
-    _field(u, u.shape[-2], u.shape[-1])
-    check(_lib.call("olx_fft2d_c64", u.device.index, _ptr(u), u.shape[-2], u.shape[-1],
-                                    -1), "fft")
+    """Unnormalised forward 2-D FFT in place (ol::fft, include/olutils.h:19-41);
+    complex128 tensors in FP64."""
+    suf = _field(u, u.shape[-2], u.shape[-1], c128=True)
+    check(_lib.call(f"olx_fft2d_{suf}", u.device.index, _ptr(u), u.shape[-2], u.shape[-1], -1),
+          "fft")
     return u
 
 
 def ifft(u):
-    """Unnormalised backward 2-D FFT in place (ol::ifft, include/olutils.h:44-66)."""
-    _field(u, u.shape[-2], u.shape[-1])
-    check(_lib.call("olx_fft2d_c64", u.device.index, _ptr(u), u.shape[-2], u.shape[-1],
-                                    1), "ifft")
+    """Unnormalised backward 2-D FFT in place (ol::ifft, include/olutils.h:44-66);
+    complex128 tensors in FP64."""
+    suf = _field(u, u.shape[-2], u.shape[-1], c128=True)
+    check(_lib.call(f"olx_fft2d_{suf}", u.device.index, _ptr(u), u.shape[-2], u.shape[-1], 1),
+          "ifft")
     return u
 
 

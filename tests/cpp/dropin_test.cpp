@@ -54,6 +54,23 @@ int main(int argc, char** argv) {
         for (size_t i = 0; i < (size_t)n * n; i++) ud[i] = std::complex<double>(field[i]);
         ol::AsmProp(ud, n, n, p, p, lambda, 0.05);
         save(dir + "/asm_c128.bin", ud.get(), (size_t)n * n);
+        auto fresh_d = [&] {
+            auto v = std::make_unique<std::complex<double>[]>((size_t)n * n);
+            for (size_t i = 0; i < (size_t)n * n; i++) v[i] = std::complex<double>(field[i]);
+            return v;
+        };
+        ud = fresh_d();
+        ol::FresnelProp(ud, n, n, p, p, lambda, 0.2);
+        save(dir + "/fresnel_c128.bin", ud.get(), (size_t)n * n);
+        ud = fresh_d();
+        ol::FresnelPropBandLimit(ud, n, n, p, p, lambda, 0.002);
+        save(dir + "/fresnel_bl_c128.bin", ud.get(), (size_t)n * n);
+        ud = fresh_d();
+        ol::Prop(ud, n, n, p, p, lambda, 0.05, ol::AUTO);
+        save(dir + "/prop_c128.bin", ud.get(), (size_t)n * n);
+        ud = fresh_d();
+        ol::mAsmProp(ud, n, n, p, p, lambda, 0.05);
+        save(dir + "/masm_c128.bin", ud.get(), (size_t)n * n);
         u = fresh();
         ol::FresnelProp(u, n, n, p, p, lambda, 0.2);
         save(dir + "/fresnel.bin", u.get(), (size_t)n * n);

@@ -43,7 +43,19 @@ def test_dropin_program_matches_oracle(tmp_path):
     pf, lf, zf = (float(np.float32(v)) for v in (3.74e-6, 532e-9, 0.1))
     assert rel(f4k, olref.fresnel_prop(u4k, pf, pf, lf, zf)) < 1e-5
     assert rel(rd("asm"), olref.asm_prop(u0, P, P, LAM, 0.05)) < 1e-5
-    assert rel(rd("asm_c128", np.complex128), olref.asm_prop(u0.astype(np.complex128), P, P, LAM, 0.05)) < 1e-5
+    # complex<double> fields: the FP64 passes against the FP64 oracle (the fftw3 path);
+    # 1e-9 for ASM, whose 1e5-cycle phase carries ~1e-11 cycle of FP64 rounding
+    u0d = u0.astype(np.complex128)
+    assert rel(rd("asm_c128", np.complex128), olref.asm_prop(u0d, P, P, LAM, 0.05)) < 1e-9
+    assert rel(rd("fresnel_c128", np.complex128), olref.fresnel_prop(u0d, P, P, LAM, 0.2)) < 1e-12
+    assert rel(rd("fresnel_bl_c128", np.complex128),
+               olref.fresnel_prop(u0d, P, P, LAM, 0.002, olref.FRESNEL_BL_CPU)) < 1e-12
+    chosen = olref.prop(u0, P, P, LAM, 0.05)[1]
+    ref = olref.asm_prop(u0d, P, P, LAM, 0.05) if chosen == 0 else olref.fresnel_prop(
+        u0d, P, P, LAM, 0.05, 0 if chosen == 1 else olref.FRESNEL_BL_CPU)
+    assert rel(rd("prop_c128", np.complex128), ref) < 1e-9
+    m2 = (2 * n) ** 2
+    assert rel(rd("masm_c128", np.complex128), olref.asm_prop(u0d, P, P, LAM, 0.05) / m2) < 1e-9
     assert rel(rd("fresnel"), olref.fresnel_prop(u0, P, P, LAM, 0.2)) < 1e-5
     assert rel(rd("fresnel_bl"), olref.fresnel_prop(u0, P, P, LAM, 0.002, olref.FRESNEL_BL_CPU)) < 1e-5
     assert rel(rd("prop"), olref.prop(u0, P, P, LAM, 0.05)[0]) < 1e-5

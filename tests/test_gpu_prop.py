@@ -283,8 +283,8 @@ def test_invalid_arguments_raise():
         ol().FresnelProp(u, 16, 16, 8e-6, 8e-6, LAM, 0.0)
     with pytest.raises(OlxInvalid):
         ol().AsmProp(u, 16, 8, 8e-6, 8e-6, LAM, 0.1)  # shape mismatch
-    with pytest.raises(OlxInvalid):
-        ol().AsmProp(u.to(torch.complex128), 16, 16, 8e-6, 8e-6, LAM, 0.1)
+    with pytest.raises(OlxInvalid):  # a real tensor (complex128 runs the FP64 passes)
+        ol().AsmProp(u.real.contiguous(), 16, 16, 8e-6, 8e-6, LAM, 0.1)
     with pytest.raises(OlxInvalid):
         ol().Prop(u, 16, 16, 8e-6, 8e-6, LAM, 0.1, propmode=7)
 
@@ -307,3 +307,48 @@ def test_launch_timing_sampling():
         rep = _lib.timing_report()
         assert all(rep[k][0] == per_pass and rep[k][1] > 0 for k in names), rep
     assert L.olx_ctx_enable_timing(c, -2) != 0
+
+
+@pytest.mark.parametrize("mode", ["asm_bl", "asm_bl_adjoint", "asm", "fresnel", "fresnel_bl",
+                                  "shifted", "shifted_window", "asm_prec_f32"])
+def test_column_pass_8192_all_modes(mode):
+    """Fields of 4096 rows (padded columns of 8192) take the wave-split column pass
+    (8 x 1024 four-step): every transfer-function mode against the oracle, on 4096 x
+    1024 (fast to check on the CPU)."""
+    ny, nx, d = 4096, 1024, 8e-6
+    print(f"column pass 8192: {mode}", flush=True)
+    u0 = synth.random_field(ny, nx, seed=41)
+    u = dev(u0)
+    if mode.startswith("asm"):
+        bl, adj, f32 = "bl" in mode, "adjoint" in mode, "f32" in mode
+        ol().AsmProp(u, ny, nx, d, d, LAM, 0.1, bandlimit=bl, adjoint=adj, prec_f32=f32)
+        flags = (olref.BANDLIMIT if bl else 0) | (olref.ADJOINT if adj else 0) | \
+            (olref.PREC_F32 if f32 else 0)
+        ref = olref.asm_prop(u0, d, d, LAM, 0.1, flags)
+        got = u
+    elif mode.startswith("fresnel"):
+        if mode == "fresnel":
+            ol().FresnelProp(u, ny, nx, d, d, LAM, 0.4)
+            ref = olref.fresnel_prop(u0, d, d, LAM, 0.4, 0)
+        else:
+            ol().FresnelPropBandLimit(u, ny, nx, d, d, LAM, 0.4)
+            ref = olref.fresnel_prop(u0, d, d, LAM, 0.4, olref.FRESNEL_BL_CPU)
+        got = u
+    else:
+        got = torch.empty_like(u)
+        win = mode == "shifted_window"
+        ol().gshiftedAsmProp(u, got, ny, nx, d, d, LAM, 0.1, 2e-4, -3e-4, bandlimit=win)
+        ref = olref.shifted_asm(u0, d, d, LAM, 0.1, 2e-4, -3e-4, olref.SHIFT_WINDOW if win else 0)
+    tol = 2e-5 if mode == "asm_prec_f32" else TOL
+    assert rel(got.cpu().numpy(), ref) < tol
+
+
+def test_column_pass_8192_batch():
+    """Two stacked 4096-row fields in one call: the column pass takes field blockIdx.y."""
+    ny, nx, d = 4096, 1024, 8e-6
+    u0 = np.stack([synth.random_field(ny, nx, seed=42 + k) for k in range(2)])
+    u = torch.from_numpy(u0).cuda()
+    ol().prop_batch(u, d, d, LAM, 0.1, ol().ASM, olref.BANDLIMIT)
+    for k in range(2):
+        ref = olref.asm_prop(u0[k], d, d, LAM, 0.1, olref.BANDLIMIT)
+        assert rel(u[k].cpu().numpy(), ref) < TOL
