@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--n", type=int, default=4096, help="field side (metric: 4096)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the ASM metric")
-    ap.add_argument("--only", choices=["ingest", "next", "cgh"], default=None,
+    ap.add_argument("--only", choices=["ingest", "next", "cgh", "sizes"], default=None,
                     help="run one secondary workload alone and print its figures (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (for PMC byte-counter calibration)")
@@ -417,6 +417,7 @@ def extras(ol, lib, torch):
     out.update(multiplane(torch))
     out.update(ingest_encode(ol, torch))
     out.update(next_rows(ol, torch))
+    out.update(sizes_batch(ol, torch))
     arss_s = (t2 - t1) / reps
     out["arss_4096_ms"] = round(arss_s * 1e3, 3)
     out["arss_4096_GBps_80N2"] = round(80 * N * N / arss_s / 1e9, 1)
@@ -517,6 +518,61 @@ def next_rows(ol, torch):
     return out
 
 
+def sizes_batch(ol, torch):
+    """The reference's demo sizes (mixed-radix passes), batched small fields, the lens
+    fused into the first pass, and the complex128 passes: device time per call (HIP
+    events)."""
+    from openol_amd import _lib
+    out = {}
+    rng = np.random.default_rng(9)
+
+    def field(ny, nx, dt=np.complex64):
+        return torch.from_numpy((rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx)))
+                                .astype(dt)).cuda()
+    # tests/Prop_test.cpp:46: FresnelProp(u, 2160, 3840, 3.74e-6, 3.74e-6, 532e-9, d)
+    f = field(2160, 3840)
+    ms = _event_ms(torch, lambda: ol.FresnelProp(f, 2160, 3840, 3.74e-6, 3.74e-6, LAM, 0.1), 10)
+    out["fresnel_2160x3840_ms"] = round(ms, 3)
+    out["fresnel_2160x3840_GBps_80N2"] = round(80 * 2160 * 3840 / ms / 1e6, 1)
+    g = field(1080, 1920)
+    ms = _event_ms(torch, lambda: ol.AsmProp(g, 1080, 1920, PITCH, PITCH, LAM, 0.05, bandlimit=True), 10)
+    out["asm_1080x1920_ms"] = round(ms, 3)
+    del f, g
+    # 16 fields of 1024^2: one batched call against 16 calls
+    nb, n = 16, 1024
+    b = field(nb * n, n).view(nb, n, n)
+    ms = _event_ms(torch, lambda: ol.prop_batch(b, PITCH, PITCH, LAM, 0.1, ol.ASM, _lib.BANDLIMIT), 10)
+    out["asm_1024_batch16_props_s"] = round(nb / ms * 1e3, 1)
+
+    def singles():
+        for k in range(nb):
+            ol.AsmProp(b[k], n, n, PITCH, PITCH, LAM, 0.1, bandlimit=True)
+    ms = _event_ms(torch, singles, 10)
+    out["asm_1024_single_props_s"] = round(nb / ms * 1e3, 1)
+    del b
+    # a lens array multiplied in by pass A against a separate multiply, at 4096^2
+    N = 4096
+    u = field(N, N)
+    lens = ol.Lensarry(N, N, PITCH, 32 * PITCH, 0.05, LAM).gen_complex(ol.PLAY)
+    uv = u.view(1, N, N)
+    ms = _event_ms(torch, lambda: ol.prop_batch(uv, PITCH, PITCH, LAM, 0.1, ol.ASM, _lib.BANDLIMIT,
+                                                pre=lens), 10)
+    out["asm_4096_fused_lens_ms"] = round(ms, 3)
+
+    def separate():
+        ol.mult(u, lens, u)
+        ol.AsmProp(u, N, N, PITCH, PITCH, LAM, 0.1, bandlimit=True)
+    ms = _event_ms(torch, separate, 10)
+    out["asm_4096_lens_then_asm_ms"] = round(ms, 3)
+    del u, uv, lens
+    # complex128 (the reference's fftw3 path) at configuration 1's size
+    d = field(1024, 1024, np.complex128)
+    ms = _event_ms(torch, lambda: ol.AsmProp(d, 1024, 1024, PITCH, PITCH, LAM, 0.1, bandlimit=True), 10)
+    out["asm_c128_1024_ms"] = round(ms, 3)
+    out["asm_c128_1024_GBps_160N2"] = round(160 * 1024 * 1024 / ms / 1e6, 1)
+    return out
+
+
 def layered(ol, torch):
     """Layered-Fresnel CGH on MFMA: 1,000,000 points in 64 layers over z in
     [0.10, 0.20) m, 4096^2 hologram, FP16 and split-BF16 factors; against the
@@ -614,6 +670,9 @@ def main():
         return
     if args.only == "cgh":
         print(json.dumps({"direct_cgh": direct_cgh(ol, torch)}), flush=True)
+        return
+    if args.only == "sizes":
+        print(json.dumps({"sizes_batch": sizes_batch(ol, torch)}), flush=True)
         return
     n = args.n
     dt, rep = bench_asm(ol, lib, torch, n, args.steps, args.warmup, settle_first=True)
