@@ -1015,14 +1015,15 @@ __global__ void k_phigrad(const float2* __restrict__ u, const float2* __restrict
     }
 }
 
-// Wext: a caller-provided intermediate of at least ny x 2nx complex64 values in
+// Wext: a caller-provided intermediate of at least ny x (2nx rounded up to 8) complex64 values in
 // place of the context's workspace (which is then neither grown nor touched)
 static int run_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a,
                           float2* Wext = nullptr) {
     const int64_t Mx = 2 * (int64_t)a.nx, My = 2 * (int64_t)a.ny;
     float2* W = Wext;
     if (!W) {
-        OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.rows * (size_t)Mx));
+        // Mx rounded up to whole 8-column tiles (the mixed-radix passes' layout)
+        OLX_TRY(ctx->work.reserve(sizeof(float2) * (size_t)a.rows * (size_t)((Mx + 7) & ~7)));
         W = (float2*)ctx->work.p;
     }
     a.scale = (float)(a.extra_scale / ((double)Mx * (double)My));
@@ -1271,7 +1272,7 @@ int olx::noalloc_init(int device) {
 extern "C" int olx_asm_c64_noalloc(olx_ctx* ctx, void* d_u, int64_t ny, int64_t nx, double dy,
                                    double dx, double lambda, double z, uint32_t flags) {
     OLX_TRY(check_field(ctx, d_u, ny, nx));
-    const size_t need = sizeof(float2) * (size_t)ny * 2 * (size_t)nx;
+    const size_t need = sizeof(float2) * (size_t)ny * (size_t)((2 * nx + 7) & ~7);
     if (need > NOALLOC_BYTES)
         OLX_FAIL(OLX_EINVAL, "asm noalloc: %lld x %lld needs %zu workspace bytes, above the "
                  "static %zu (at most 2048 x 2048 elements)", (long long)ny, (long long)nx, need,

@@ -11,11 +11,15 @@
 // use the symmetric-pair DFT.  Each stage loads every butterfly a thread owns into
 // registers, waits, then writes, so one LDS image per transform suffices.
 //
-// The three propagation passes mirror the power-of-two ones over a row-major
-// intermediate W of ny x Mx values: A pads, transforms and stores rows; B loads
-// columns (ny of My values nonzero), transforms, multiplies by the transfer
-// function, transforms back and stores the centre rows; C transforms rows back,
-// crops, scales and applies the post-chirp.  Zero padding is centred at offset
+// The three propagation passes mirror the power-of-two ones over an intermediate W
+// of ny x Mx values in column tiles of 8 (tile g = m / 8 holds columns 8 g .. 8 g + 7
+// of every row contiguously, W[(g rows + h) 8 + m % 8]): A pads, transforms and
+// stores rows; B loads columns (ny of My values nonzero), transforms, multiplies by
+// the transfer function, transforms back and stores the centre rows; C transforms
+// rows back, crops, scales and applies the post-chirp.  With tiles, the column pass's
+// one value per row shares its 64-byte line with the seven neighbouring columns,
+// which neighbouring work-groups read at the same time (a row-major W puts every
+// value in a line of its own).  Zero padding is centred at offset
 // (M - n) / 2 = n / 2, as the reference's zeropadding / del_zero.
 #include <climits>
 #include <cstring>
@@ -152,9 +156,10 @@ __device__ __forceinline__ void mr_stage(float2* lds, int LP, int nb, const MrSt
     __syncthreads();
 }
 
-// the whole transform of nb LDS-resident sequences (natural order in and out)
+// the whole transform of nb LDS-resident sequences (natural order in and out);
+// inlined: as a called function it kept its own frame and spilled (580 B of scratch)
 template <bool INV, int NT, int LCAP>
-__device__ void mr_run(float2* lds, int LP, int nb, const MrPlan* __restrict__ P, int tid) {
+__device__ __forceinline__ void mr_run(float2* lds, int LP, int nb, const MrPlan* __restrict__ P, int tid) {
     const int nst = P->nst;
     const float2* tw = P->tw;
     for (int s = 0; s < nst; s++) {
@@ -177,6 +182,11 @@ __device__ void mr_run(float2* lds, int LP, int nb, const MrPlan* __restrict__ P
 template <int LCAP>
 constexpr int mr_lds() { return LCAP + LCAP / 16 + 32; }
 __host__ __device__ inline int mr_lp(int L) { return L + L / 16 + 1; }
+
+// element (row, m) of the tiled intermediate over `rows` rows
+__device__ __forceinline__ size_t mrw(int rows, int row, int m) {
+    return ((size_t)(m >> 3) * rows + row) * 8 + (m & 7);
+}
 
 // ------------------------------------------------------------------ pass A
 template <int NT, int LCAP>
@@ -204,7 +214,7 @@ k_mr_rows_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
         if (row >= a.rows) break;
-        for (int i = tid; i < L; i += NT) W[(size_t)row * L + i] = lds[tr * LP + lpad(i)];
+        for (int i = tid; i < L; i += NT) W[mrw(a.rows, row, i)] = lds[tr * LP + lpad(i)];
     }
 }
 
@@ -216,13 +226,13 @@ k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P,
     const int tid = threadIdx.x, My = P->L, LP = mr_lp(My), ny = a.ny, off = ny / 2;
     const int Mx = 2 * a.nx;
     const int col0 = xcd_remap(blockIdx.x, gridDim.x) * nb;
-    W += (size_t)blockIdx.y * ny * Mx;  // this field's rows
+    const int fo = blockIdx.y * ny;  // this field's first row
     for (int tr = 0; tr < nb; tr++) {
         const int col = col0 + tr;
         for (int i = tid; i < My; i += NT) {
             const int h = i - off;
             float2 x = make_float2(0.f, 0.f);
-            if (col < Mx && h >= 0 && h < ny) x = W[(size_t)h * Mx + col];
+            if (col < Mx && h >= 0 && h < ny) x = W[mrw(a.rows, fo + h, col)];
             lds[tr * LP + lpad(i)] = x;
         }
     }
@@ -242,7 +252,7 @@ k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P,
     for (int tr = 0; tr < nb; tr++) {
         const int col = col0 + tr;
         if (col >= Mx) break;
-        for (int h = tid; h < ny; h += NT) W[(size_t)h * Mx + col] = lds[tr * LP + lpad(h + off)];
+        for (int h = tid; h < ny; h += NT) W[mrw(a.rows, fo + h, col)] = lds[tr * LP + lpad(h + off)];
     }
 }
 
@@ -256,7 +266,7 @@ k_mr_rows_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
         for (int i = tid; i < L; i += NT)
-            lds[tr * LP + lpad(i)] = row < a.rows ? W[(size_t)row * L + i] : make_float2(0.f, 0.f);
+            lds[tr * LP + lpad(i)] = row < a.rows ? W[mrw(a.rows, row, i)] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     mr_run<true, NT, LCAP>(lds, LP, nb, P, tid);
@@ -392,11 +402,16 @@ static int mr_nb(int L) {
     return nb < 1 ? 1 : (nb > 8 ? 8 : nb);
 }
 
-// calls f(integral NT, integral LCAP) for the work-group shape of length L
+// calls f(integral NT, integral LCAP) for the work-group shape of length L.  A stage
+// holds each thread's ceil(LCAP / R / NT) butterflies in registers between its LDS
+// read and write, so long transforms take 1024 threads: with 256 the radix-3 stage of
+// an 8192-value image held 33 values per thread and the kernel spilled (289 VGPRs,
+// one wave per SIMD).
 template <class F>
 static void mr_shape(int L, F&& f) {
-    if (L <= 8192) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
-    else f(std::integral_constant<int, 512>(), std::integral_constant<int, 16384>());
+    if (L <= 2048) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
+    else if (L <= 8192) f(std::integral_constant<int, 1024>(), std::integral_constant<int, 8192>());
+    else f(std::integral_constant<int, 1024>(), std::integral_constant<int, 16384>());
 }
 
 int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, float2* W) {
