@@ -396,6 +396,134 @@ __device__ __forceinline__ void run_stages_pair(float2* va, float2* vb, int t, f
     }
 }
 
+// ---------------------------------------------------------------- pipelined pair
+// Stage S twiddles of one thread, split in two steps so that the table loads can be
+// issued an exchange before their use: TwBase holds the loaded w1, w2, w4, w8 of each
+// butterfly, TwFull the R twiddles formed from them (the products of apply_twiddles2,
+// in the same order, so the results are bitwise those of run_stages_pair).
+template <class P, int S, bool INV, int TWLEN>
+struct TwBase {
+    static constexpr int R = P::radix(S), NS = P::ns(S), NB = P::E / R;
+    static constexpr int NL = NS > 1 ? (R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4) : 0;
+    float2 l[NB][NL > 0 ? NL : 1];
+    __device__ __forceinline__ void load(int t, const float2* __restrict__ tw) {
+        if constexpr (NL > 0) {
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int k = (t + P::T * b) & (NS - 1);
+                const int base = k * (TWLEN / (NS * R));
+#pragma unroll
+                for (int i = 0; i < NL; i++) l[b][i] = tw[(1 << i) * base];
+            }
+        }
+    }
+};
+
+template <class P, int S, bool INV, int TWLEN>
+struct TwFull {
+    static constexpr int R = P::radix(S), NS = P::ns(S), NB = P::E / R;
+    float2 w[NB][R];  // w[b][0] = 1 is not stored or applied
+    __device__ __forceinline__ void expand(const TwBase<P, S, INV, TWLEN>& tb) {
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                float2 w1 = tb.l[b][0];
+                if (INV) w1 = cconj(w1);
+                w[b][1] = w1;
+                if constexpr (R >= 4) {
+                    float2 w2 = tb.l[b][1];
+                    if (INV) w2 = cconj(w2);
+                    w[b][2] = w2;
+                    w[b][3] = cmul(w1, w2);
+                }
+                if constexpr (R >= 8) {
+                    float2 w4 = tb.l[b][2];
+                    if (INV) w4 = cconj(w4);
+                    w[b][4] = w4;
+                    w[b][5] = cmul(w1, w4);
+                    w[b][6] = cmul(w[b][2], w4);
+                    w[b][7] = cmul(w[b][3], w4);
+                }
+                if constexpr (R == 16) {
+                    float2 w8 = tb.l[b][3];
+                    if (INV) w8 = cconj(w8);
+                    w[b][8] = w8;
+#pragma unroll
+                    for (int r = 1; r < 8; r++) w[b][8 + r] = cmul(w[b][r], w8);
+                }
+            }
+        }
+    }
+    // twiddles then DFT on every butterfly of one value set
+    __device__ __forceinline__ void compute(float2* v) const {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b * R + r] = cmul(v[b * R + r], w[b][r]);
+            }
+            dft<R, INV>(v + b * R);
+        }
+    }
+};
+
+// run_stages_pair with the exchanges pipelined.  LDS holds one value set at a
+// time, so each exchange is two rounds (write, barrier, read, barrier); here the
+// set not in LDS computes during the other set's writes:
+//   write a(S) | compute b(S) ; barrier ; read a(S+1) ; barrier ;
+//   write b(S) | compute a(S+1) ; barrier ; read b(S+1) ; barrier ; ...
+// and stage S+1's twiddle loads are issued at the start of the exchange before it.
+// The twiddles of a stage are formed once and kept for the second set.
+// sched_barrier pins each computation on its side of the work-group barrier (the
+// scheduler otherwise sinks the arithmetic below it, serialising LDS and VALU).
+// Computing half of a set on each side of the read barrier instead measured slower:
+// the read rounds leave the VALU to the CU's other work-group.
+// On entry both sets hold stage-0 inputs; on return both hold last-stage outputs,
+// and the last LDS reads are followed by a barrier.
+template <class P, bool INV, int TWLEN, int STRIDE, int S>
+__device__ __forceinline__ void pipe_rest(float2* va, float2* vb, int t, int tt, float2* lds,
+                                          int off, const float2* __restrict__ tw,
+                                          const TwFull<P, S, INV, TWLEN>& ws) {
+    if constexpr (S + 1 < P::NST) {
+        TwBase<P, S + 1, INV, TWLEN> tb;
+        tb.load(tt, tw);
+        stage_store_lds<P, S, STRIDE>(va, t, lds, off);
+        ws.compute(vb);
+        __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        stage_load_lds<P, S + 1, STRIDE>(va, t, lds, off);
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        stage_store_lds<P, S, STRIDE>(vb, t, lds, off);
+        TwFull<P, S + 1, INV, TWLEN> wn;
+        wn.expand(tb);
+        wn.compute(va);
+        __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        stage_load_lds<P, S + 1, STRIDE>(vb, t, lds, off);
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        pipe_rest<P, INV, TWLEN, STRIDE, S + 1>(va, vb, t, tt, lds, off, tw, wn);
+    } else {
+        ws.compute(vb);
+    }
+}
+
+template <class P, bool INV, int TWLEN, int STRIDE>
+__device__ __forceinline__ void run_stages_pipe(float2* va, float2* vb, int t, float2* lds,
+                                                int off, const float2* __restrict__ tw) {
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    TwBase<P, 0, INV, TWLEN> tb;
+    tb.load(tt, tw);
+    TwFull<P, 0, INV, TWLEN> w0;
+    w0.expand(tb);
+    w0.compute(va);
+    pipe_rest<P, INV, TWLEN, STRIDE, 0>(va, vb, t, tt, lds, off, tw, w0);
+}
+
 // element index held in v[b R + r] after run_stages (last stage: NS R = L, so the
 // output index is j + r NS with j = t + T b < NS)
 template <class P>

@@ -321,7 +321,7 @@ k_rows_pad_fwd2(const float2* __restrict__ src, float2* __restrict__ W, PropArgs
             va[b * R0 + r] = xa;
             vb[b * R0 + r] = xb;
         }
-    run_stages_pair<P, false, TWN, KR>(va, vb, t, lds, c, tw);
+    run_stages_pipe<P, false, TWN, KR>(va, vb, t, lds, c, tw);
     // element t + D of row h sits at widx(h, t) + D ny (D a multiple of the tile width)
     const uint32_t wo = (uint32_t)(widx(a, ha, t) * sizeof(float2));
     const uint32_t setb = (uint32_t)(KR * a.cbl * sizeof(float2));
@@ -366,7 +366,7 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
             va[b * R0 + r] = xa;
             vb[b * R0 + r] = xb;
         }
-    run_stages_pair<PF, false, TWN, KR>(va, vb, t, lds, c, tw);
+    run_stages_pipe<PF, false, TWN, KR>(va, vb, t, lds, c, tw);
     // transfer function on the spectrum held in registers
     if constexpr (MODE == K_TABLE) {
         const float2 sxa = a.sx[cola], sxb = a.sx[colb];
@@ -454,7 +454,7 @@ k_cols_conv2(float2* __restrict__ W, PropArgs a, const float2* __restrict__ tw) 
             wb[b * R0I + r] = vb[spec_slot<PF, PI>(b, r)];
         }
     // (the forward transform ended with a barrier after its last LDS reads)
-    run_stages_pair<PI, true, TWN, KR>(wa, wb, t, lds, c, tw);
+    run_stages_pipe<PI, true, TWN, KR>(wa, wb, t, lds, c, tw);
 #pragma unroll
     for (int b = 0; b < E / RLI; b++)
 #pragma unroll
