@@ -146,7 +146,7 @@ struct Plan {
     static constexpr int N16 = L < 16 ? 0 : LOG / 4;
     static constexpr int REM = L < 16 ? L : (1 << (LOG % 4));
     static constexpr int NST = N16 + (REM > 1 ? 1 : 0);
-    static constexpr int LP = L + L / 16;  // padded LDS length
+    static constexpr int LP = L + L / 16;  // padded LDS length (the largest padding, lpadn<4>)
     __host__ __device__ static constexpr int radix(int s) {
         if (REM == 1) return 16;
         if (L < 16) return L;
@@ -159,7 +159,7 @@ struct Plan {
     }
 };
 
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }  // lpadn<4>
 
 // twiddles W_{NS R}^{k r}, r = 1..R-1, from the length-TWN table (forward sign)
 template <int R, bool INV>
@@ -289,35 +289,54 @@ __device__ __forceinline__ void stage_compute2(float2* va, float2* vb, int t,
     }
 }
 
-// LDS positions.  lpad(x + y) = lpad(x) + lpad(y) whenever x % 16 + y % 16 < 16,
-// which holds for every split below when T (threads per transform) is a multiple
-// of 16: each access is then one per-thread base plus a compile-time offset, so
-// the compiler needs a single address register per stage instead of one per value.
+// LDS positions: element i of a transform sits at lpadn<SH>(i) = i + (i >> SH), one
+// float2 of padding every 2^SH elements (lpad: SH = 4).  lpadn(x + y) = lpadn(x) +
+// lpadn(y) whenever x % 2^SH + y % 2^SH < 2^SH, which holds for every split below
+// when T (threads per transform) is a multiple of 2^SH: each access is then one
+// per-thread base plus a compile-time offset, so the compiler needs a single address
+// register per stage instead of one per value.
 //
+// The period against bank conflicts: gfx950's LDS serves a b64 access 32 lanes per
+// cycle from 64 banks, so the 32 lanes of a half-wave must hit 32 distinct 8-byte
+// bank pairs.  With one transform per LDS round (STRIDE 1), the contiguous reads of
+// every stage are conflict-free only with SH = 5 (with SH = 4 lanes 16-31 are shifted
+// by one and wrap onto lanes 0-1), and only the second stage's writes then conflict,
+// 2-way; modelled over an 8192-point transform, 1792 cycles per half-wave group
+// against 2560 (SH = 4) and 1536 ideal.  With transforms interleaved (STRIDE >= 2),
+// SH = 4 is the better choice.
+template <int STRIDE>
+constexpr int pad_shift() {
+    return STRIDE == 1 ? 5 : 4;
+}
+template <int SH>
+__device__ __forceinline__ constexpr int lpadn(int i) {
+    return i + (i >> SH);
+}
+
 // stage S output (b, r) goes to position (t - k) R + k + r NS + T R b, k = t mod NS
 // (T is a multiple of NS, so k does not depend on b)
-template <class P, int S>
+template <class P, int S, int SH = 4>
 __device__ __forceinline__ int store_base(int t) {
     constexpr int R = P::radix(S), NS = P::ns(S);
     const int k = t & (NS - 1);
-    return lpad((t - k) * R) + lpad(k);
+    return lpadn<SH>((t - k) * R) + lpadn<SH>(k);
 }
-template <class P, int S>
+template <class P, int S, int SH = 4>
 __device__ __forceinline__ constexpr int store_off(int b, int r) {
     constexpr int R = P::radix(S), NS = P::ns(S);
-    return (r * NS + ((r * NS) >> 4)) + (P::T * R * b + ((P::T * R * b) >> 4));
+    return lpadn<SH>(r * NS) + lpadn<SH>(P::T * R * b);
 }
 
-// store stage S outputs to LDS (element i of this transform at off + lpad(i)*STRIDE)
-template <class P, int S, int STRIDE>
+// store stage S outputs to LDS (element i of this transform at off + lpadn<SH>(i)*STRIDE)
+template <class P, int S, int STRIDE, int SH = 4>
 __device__ __forceinline__ void stage_store_lds(const float2* v, int t, float2* lds, int off) {
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::E / R;
-    if constexpr (P::T % 16 == 0) {
-        float2* q = lds + off + store_base<P, S>(t) * STRIDE;
+    if constexpr (P::T % (1 << SH) == 0) {
+        float2* q = lds + off + store_base<P, S, SH>(t) * STRIDE;
 #pragma unroll
         for (int b = 0; b < NB; b++)
 #pragma unroll
-            for (int r = 0; r < R; r++) q[store_off<P, S>(b, r) * STRIDE] = v[b * R + r];
+            for (int r = 0; r < R; r++) q[store_off<P, S, SH>(b, r) * STRIDE] = v[b * R + r];
     } else {
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -325,30 +344,27 @@ __device__ __forceinline__ void stage_store_lds(const float2* v, int t, float2* 
             int k = j & (NS - 1);
             int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[off + lpad(base + r * NS) * STRIDE] = v[b * R + r];
+            for (int r = 0; r < R; r++) lds[off + lpadn<SH>(base + r * NS) * STRIDE] = v[b * R + r];
         }
     }
 }
 
 // load stage S inputs from LDS: (b, r) reads element t + T b + r L/R
-template <class P, int S, int STRIDE>
+template <class P, int S, int STRIDE, int SH = 4>
 __device__ __forceinline__ void stage_load_lds(float2* v, int t, const float2* lds, int off) {
     constexpr int R = P::radix(S), NB = P::E / R;
-    if constexpr (P::T % 16 == 0) {
-        const float2* q = lds + off + lpad(t) * STRIDE;
+    if constexpr (P::T % (1 << SH) == 0) {
+        const float2* q = lds + off + lpadn<SH>(t) * STRIDE;
 #pragma unroll
         for (int b = 0; b < NB; b++)
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                const int o = P::T * b + r * (P::L / R);
-                v[b * R + r] = q[(o + (o >> 4)) * STRIDE];
-            }
+            for (int r = 0; r < R; r++) v[b * R + r] = q[lpadn<SH>(P::T * b + r * (P::L / R)) * STRIDE];
     } else {
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             int j = t + P::T * b;
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b * R + r] = lds[off + lpad(j + r * (P::L / R)) * STRIDE];
+            for (int r = 0; r < R; r++) v[b * R + r] = lds[off + lpadn<SH>(j + r * (P::L / R)) * STRIDE];
         }
     }
 }
@@ -382,14 +398,14 @@ __device__ __forceinline__ void run_stages_pair(float2* va, float2* vb, int t, f
     asm volatile("" : "+v"(tt));
     if constexpr (S == 0) stage_compute2<P, 0, INV, TWLEN>(va, vb, tt, tw);
     if constexpr (S + 1 < P::NST) {
-        stage_store_lds<P, S, STRIDE>(va, t, lds, off);
+        stage_store_lds<P, S, STRIDE, pad_shift<STRIDE>()>(va, t, lds, off);
         __syncthreads();
-        stage_load_lds<P, S + 1, STRIDE>(va, t, lds, off);
+        stage_load_lds<P, S + 1, STRIDE, pad_shift<STRIDE>()>(va, t, lds, off);
         __syncthreads();
-        stage_store_lds<P, S, STRIDE>(vb, t, lds, off);
+        stage_store_lds<P, S, STRIDE, pad_shift<STRIDE>()>(vb, t, lds, off);
         __syncthreads();
         // set b's reads overlap the next stage's twiddles and set a's arithmetic
-        stage_load_lds<P, S + 1, STRIDE>(vb, t, lds, off);
+        stage_load_lds<P, S + 1, STRIDE, pad_shift<STRIDE>()>(vb, t, lds, off);
         stage_compute2<P, S + 1, INV, TWLEN>(va, vb, tt, tw);
         __syncthreads();
         run_stages_pair<P, INV, TWLEN, STRIDE, S + 1>(va, vb, t, lds, off, tw);
@@ -487,22 +503,22 @@ __device__ __forceinline__ void pipe_rest(float2* va, float2* vb, int t, int tt,
     if constexpr (S + 1 < P::NST) {
         TwBase<P, S + 1, INV, TWLEN> tb;
         tb.load(tt, tw);
-        stage_store_lds<P, S, STRIDE>(va, t, lds, off);
+        stage_store_lds<P, S, STRIDE, pad_shift<STRIDE>()>(va, t, lds, off);
         ws.compute(vb);
         __builtin_amdgcn_sched_barrier(0);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        stage_load_lds<P, S + 1, STRIDE>(va, t, lds, off);
+        stage_load_lds<P, S + 1, STRIDE, pad_shift<STRIDE>()>(va, t, lds, off);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        stage_store_lds<P, S, STRIDE>(vb, t, lds, off);
+        stage_store_lds<P, S, STRIDE, pad_shift<STRIDE>()>(vb, t, lds, off);
         TwFull<P, S + 1, INV, TWLEN> wn;
         wn.expand(tb);
         wn.compute(va);
         __builtin_amdgcn_sched_barrier(0);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        stage_load_lds<P, S + 1, STRIDE>(vb, t, lds, off);
+        stage_load_lds<P, S + 1, STRIDE, pad_shift<STRIDE>()>(vb, t, lds, off);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
         pipe_rest<P, INV, TWLEN, STRIDE, S + 1>(va, vb, t, tt, lds, off, tw, wn);
