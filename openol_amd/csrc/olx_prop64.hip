@@ -52,6 +52,22 @@ constexpr double c_sin(double x) {
 }
 constexpr double c_cos(double x) { return c_sin(x + kPi64 / 2); }
 
+// cos / sin of 2 pi m / R, m in [0, R), evaluated by the compiler; the odd-radix DFT
+// indexes it with its unrolled loop counters, so every entry folds to an immediate
+// (the series called there directly is not a constant expression and ran at run time)
+template <int R>
+struct OddRoots64 {
+    double c[R], s[R];
+    constexpr OddRoots64() : c(), s() {
+        for (int m = 0; m < R; m++) {
+            c[m] = c_cos(2.0 * kPi64 * m / R);
+            s[m] = c_sin(2.0 * kPi64 * m / R);
+        }
+    }
+};
+template <int R>
+__device__ constexpr OddRoots64<R> kOddRoots64{};
+
 // multiply by W_R^m (forward e^{-2 pi i m / R}); m, R compile-time
 template <int M, int R, bool INV>
 __device__ __forceinline__ double2 dtw(double2 a) {
@@ -135,8 +151,8 @@ __device__ __forceinline__ void ddft(double2* v) {
             double2 A = v[0], B = make_double2(0.0, 0.0);
 #pragma unroll
             for (int n = 1; n <= H; n++) {
-                const double c = c_cos(2.0 * kPi64 * ((n * k) % R) / R);
-                const double s = c_sin(2.0 * kPi64 * ((n * k) % R) / R);
+                const double c = kOddRoots64<R>.c[(n * k) % R];
+                const double s = kOddRoots64<R>.s[(n * k) % R];
                 A = make_double2(fma(a[n - 1].x, c, A.x), fma(a[n - 1].y, c, A.y));
                 B = make_double2(fma(b[n - 1].x, s, B.x), fma(b[n - 1].y, s, B.y));
             }

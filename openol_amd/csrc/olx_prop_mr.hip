@@ -71,6 +71,23 @@ constexpr double ct_sin(double x) {
 }
 constexpr double ct_cos(double x) { return ct_sin(x + kPi / 2); }
 
+// cos / sin of 2 pi m / R for m in [0, R), evaluated by the compiler.  The DFT below
+// indexes it with its unrolled loop counters, so every entry folds to an immediate
+// (calling ct_cos / ct_sin there directly is not a constant expression, and the
+// series then ran at run time: 132 FP64 divisions in the column kernel).
+template <int R>
+struct OddRoots {
+    float c[R], s[R];
+    constexpr OddRoots() : c(), s() {
+        for (int m = 0; m < R; m++) {
+            c[m] = (float)ct_cos(2.0 * kPi * m / R);
+            s[m] = (float)ct_sin(2.0 * kPi * m / R);
+        }
+    }
+};
+template <int R>
+__device__ constexpr OddRoots<R> kOddRoots{};
+
 // R-point DFT in registers, natural-order output; forward e^{-2 pi i nk/R}
 template <int R, bool INV>
 __device__ __forceinline__ void dft_any(float2* v) {
@@ -95,8 +112,8 @@ __device__ __forceinline__ void dft_any(float2* v) {
             float2 A = v[0], B = make_float2(0.f, 0.f);
 #pragma unroll
             for (int n = 1; n <= H; n++) {
-                const float c = (float)ct_cos(2.0 * kPi * ((n * k) % R) / R);
-                const float s = (float)ct_sin(2.0 * kPi * ((n * k) % R) / R);
+                const float c = kOddRoots<R>.c[(n * k) % R];
+                const float s = kOddRoots<R>.s[(n * k) % R];
                 A = make_float2(fmaf(a[n - 1].x, c, A.x), fmaf(a[n - 1].y, c, A.y));
                 B = make_float2(fmaf(b[n - 1].x, s, B.x), fmaf(b[n - 1].y, s, B.y));
             }
@@ -158,7 +175,10 @@ __device__ __forceinline__ void mr_stage(float2* lds, int LP, int nb, const MrSt
 
 // the whole transform of nb LDS-resident sequences (natural order in and out);
 // inlined: as a called function it kept its own frame and spilled (580 B of scratch)
-template <bool INV, int NT, int LCAP>
+// P7: the plan may hold radix 7, 11 or 13.  Their DFTs set the register budget of the
+// whole switch (1024-thread kernels spilled 207 VGPRs with them), so lengths without
+// those factors run a copy of the switch that leaves them out.
+template <bool INV, int NT, int LCAP, bool P7>
 __device__ __forceinline__ void mr_run(float2* lds, int LP, int nb, const MrPlan* __restrict__ P, int tid) {
     const int nst = P->nst;
     const float2* tw = P->tw;
@@ -169,10 +189,16 @@ __device__ __forceinline__ void mr_run(float2* lds, int LP, int nb, const MrPlan
             case 3: mr_stage<3, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
             case 4: mr_stage<4, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
             case 5: mr_stage<5, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
-            case 7: mr_stage<7, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 7:
+                if constexpr (P7) mr_stage<7, INV, NT, LCAP>(lds, LP, nb, S, tw, tid);
+                break;
             case 8: mr_stage<8, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
-            case 11: mr_stage<11, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
-            case 13: mr_stage<13, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 11:
+                if constexpr (P7) mr_stage<11, INV, NT, LCAP>(lds, LP, nb, S, tw, tid);
+                break;
+            case 13:
+                if constexpr (P7) mr_stage<13, INV, NT, LCAP>(lds, LP, nb, S, tw, tid);
+                break;
             default: mr_stage<16, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
         }
     }
@@ -189,7 +215,7 @@ __device__ __forceinline__ size_t mrw(int rows, int row, int m) {
 }
 
 // ------------------------------------------------------------------ pass A
-template <int NT, int LCAP>
+template <int NT, int LCAP, bool P7>
 __global__ void __launch_bounds__(NT)
 k_mr_rows_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P, int nb) {
     __shared__ float2 lds[mr_lds<LCAP>()];
@@ -210,7 +236,7 @@ k_mr_rows_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a
         }
     }
     __syncthreads();
-    mr_run<false, NT, LCAP>(lds, LP, nb, P, tid);
+    mr_run<false, NT, LCAP, P7>(lds, LP, nb, P, tid);
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
         if (row >= a.rows) break;
@@ -219,7 +245,7 @@ k_mr_rows_fwd(const float2* __restrict__ src, float2* __restrict__ W, PropArgs a
 }
 
 // ------------------------------------------------------------------ pass B
-template <int NT, int LCAP, int MODE>
+template <int NT, int LCAP, int MODE, bool P7>
 __global__ void __launch_bounds__(NT)
 k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P, int nb) {
     __shared__ float2 lds[mr_lds<LCAP>()];
@@ -237,7 +263,7 @@ k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P,
         }
     }
     __syncthreads();
-    mr_run<false, NT, LCAP>(lds, LP, nb, P, tid);
+    mr_run<false, NT, LCAP, P7>(lds, LP, nb, P, tid);
     for (int tr = 0; tr < nb; tr++) {
         const int col = col0 + tr;
         if (col >= Mx) break;
@@ -248,7 +274,7 @@ k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P,
         }
     }
     __syncthreads();
-    mr_run<true, NT, LCAP>(lds, LP, nb, P, tid);
+    mr_run<true, NT, LCAP, P7>(lds, LP, nb, P, tid);
     for (int tr = 0; tr < nb; tr++) {
         const int col = col0 + tr;
         if (col >= Mx) break;
@@ -257,7 +283,7 @@ k_mr_cols_conv(float2* __restrict__ W, PropArgs a, const MrPlan* __restrict__ P,
 }
 
 // ------------------------------------------------------------------ pass C
-template <int NT, int LCAP>
+template <int NT, int LCAP, bool P7>
 __global__ void __launch_bounds__(NT)
 k_mr_rows_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a, const MrPlan* __restrict__ P, int nb) {
     __shared__ float2 lds[mr_lds<LCAP>()];
@@ -269,7 +295,7 @@ k_mr_rows_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a
             lds[tr * LP + lpad(i)] = row < a.rows ? W[mrw(a.rows, row, i)] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    mr_run<true, NT, LCAP>(lds, LP, nb, P, tid);
+    mr_run<true, NT, LCAP, P7>(lds, LP, nb, P, tid);
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
         if (row >= a.rows) break;
@@ -288,7 +314,7 @@ k_mr_rows_inv(const float2* __restrict__ W, float2* __restrict__ dst, PropArgs a
 }
 
 // ------------------------------------------------------------------ plain 2-D FFT
-template <int NT, int LCAP, bool INV>
+template <int NT, int LCAP, bool INV, bool P7>
 __global__ void __launch_bounds__(NT)
 k_mr_fft_rows(float2* __restrict__ u, int rows, const MrPlan* __restrict__ P, int nb) {
     __shared__ float2 lds[mr_lds<LCAP>()];
@@ -300,7 +326,7 @@ k_mr_fft_rows(float2* __restrict__ u, int rows, const MrPlan* __restrict__ P, in
             lds[tr * LP + lpad(i)] = row < rows ? u[(size_t)row * L + i] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    mr_run<INV, NT, LCAP>(lds, LP, nb, P, tid);
+    mr_run<INV, NT, LCAP, P7>(lds, LP, nb, P, tid);
     for (int tr = 0; tr < nb; tr++) {
         const int row = row0 + tr;
         if (row >= rows) break;
@@ -308,7 +334,7 @@ k_mr_fft_rows(float2* __restrict__ u, int rows, const MrPlan* __restrict__ P, in
     }
 }
 
-template <int NT, int LCAP, bool INV>
+template <int NT, int LCAP, bool INV, bool P7>
 __global__ void __launch_bounds__(NT)
 k_mr_fft_cols(float2* __restrict__ u, int cols, const MrPlan* __restrict__ P, int nb) {
     __shared__ float2 lds[mr_lds<LCAP>()];
@@ -320,7 +346,7 @@ k_mr_fft_cols(float2* __restrict__ u, int cols, const MrPlan* __restrict__ P, in
         lds[tr * LP + lpad(i)] = col < cols ? u[(size_t)i * cols + col] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    mr_run<INV, NT, LCAP>(lds, LP, nb, P, tid);
+    mr_run<INV, NT, LCAP, P7>(lds, LP, nb, P, tid);
     for (int e = tid; e < nb * L; e += NT) {
         const int i = e / nb, tr = e - i * nb, col = col0 + tr;
         if (col < cols) u[(size_t)i * cols + col] = lds[tr * LP + lpad(i)];
@@ -408,10 +434,19 @@ static int mr_nb(int L) {
 // an 8192-value image held 33 values per thread and the kernel spilled (289 VGPRs,
 // one wave per SIMD).
 template <class F>
-static void mr_shape(int L, F&& f) {
+static void mr_shape2(int L, F&& f) {
     if (L <= 2048) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
     else if (L <= 8192) f(std::integral_constant<int, 1024>(), std::integral_constant<int, 8192>());
     else f(std::integral_constant<int, 1024>(), std::integral_constant<int, 16384>());
+}
+// also passes P7 (whether L has a factor 7, 11 or 13) as a compile-time constant
+template <class F>
+static void mr_shape(int L, F&& f) {
+    const bool p7 = L % 7 == 0 || L % 11 == 0 || L % 13 == 0;
+    mr_shape2(L, [&](auto NT, auto LC) {
+        if (p7) f(NT, LC, std::true_type());
+        else f(NT, LC, std::false_type());
+    });
 }
 
 int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, float2* W) {
@@ -422,33 +457,33 @@ int mr_three_pass(olx_ctx* ctx, const float2* src, float2* dst, PropArgs& a, flo
     hipStream_t s = ctx->stream;
     {
         TimedLaunch tl(ctx, "prop_rows_pad_fwd");
-        mr_shape(Mx, [&](auto NT, auto LC) {
+        mr_shape(Mx, [&](auto NT, auto LC, auto P7) {
             const int nb = mr_nb<LC.value>(Mx);
-            k_mr_rows_fwd<NT.value, LC.value><<<(a.rows + nb - 1) / nb, NT.value, 0, s>>>(src, W, a, px, nb);
+            k_mr_rows_fwd<NT.value, LC.value, P7.value><<<(a.rows + nb - 1) / nb, NT.value, 0, s>>>(src, W, a, px, nb);
         });
     }
     OLX_TRY(check_launch("mixed-radix pass A"));
     {
         TimedLaunch tl(ctx, "prop_cols_conv");
-        mr_shape(My, [&](auto NT, auto LC) {
+        mr_shape(My, [&](auto NT, auto LC, auto P7) {
             const int nb = mr_nb<LC.value>(My);
             const dim3 grid((Mx + nb - 1) / nb, a.nb);
             if (a.mode == K_ASM)
-                k_mr_cols_conv<NT.value, LC.value, K_ASM><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+                k_mr_cols_conv<NT.value, LC.value, K_ASM, P7.value><<<grid, NT.value, 0, s>>>(W, a, py, nb);
             else if (a.mode == K_ASHIFT)
-                k_mr_cols_conv<NT.value, LC.value, K_ASHIFT><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+                k_mr_cols_conv<NT.value, LC.value, K_ASHIFT, P7.value><<<grid, NT.value, 0, s>>>(W, a, py, nb);
             else if (a.mode == K_ASM32)
-                k_mr_cols_conv<NT.value, LC.value, K_ASM32><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+                k_mr_cols_conv<NT.value, LC.value, K_ASM32, P7.value><<<grid, NT.value, 0, s>>>(W, a, py, nb);
             else
-                k_mr_cols_conv<NT.value, LC.value, K_TABLE><<<grid, NT.value, 0, s>>>(W, a, py, nb);
+                k_mr_cols_conv<NT.value, LC.value, K_TABLE, P7.value><<<grid, NT.value, 0, s>>>(W, a, py, nb);
         });
     }
     OLX_TRY(check_launch("mixed-radix pass B"));
     {
         TimedLaunch tl(ctx, "prop_rows_crop_inv");
-        mr_shape(Mx, [&](auto NT, auto LC) {
+        mr_shape(Mx, [&](auto NT, auto LC, auto P7) {
             const int nb = mr_nb<LC.value>(Mx);
-            k_mr_rows_inv<NT.value, LC.value><<<(a.rows + nb - 1) / nb, NT.value, 0, s>>>(W, dst, a, px, nb);
+            k_mr_rows_inv<NT.value, LC.value, P7.value><<<(a.rows + nb - 1) / nb, NT.value, 0, s>>>(W, dst, a, px, nb);
         });
     }
     return check_launch("mixed-radix pass C");
@@ -461,21 +496,21 @@ int mr_fft2d(olx_ctx* ctx, float2* u, int64_t ny, int64_t nx, bool inv) {
     hipStream_t s = ctx->stream;
     {
         TimedLaunch tl(ctx, "fft_rows");
-        mr_shape((int)nx, [&](auto NT, auto LC) {
+        mr_shape((int)nx, [&](auto NT, auto LC, auto P7) {
             const int nb = mr_nb<LC.value>((int)nx);
             const int grid = (int)((ny + nb - 1) / nb);
-            if (inv) k_mr_fft_rows<NT.value, LC.value, true><<<grid, NT.value, 0, s>>>(u, (int)ny, px, nb);
-            else k_mr_fft_rows<NT.value, LC.value, false><<<grid, NT.value, 0, s>>>(u, (int)ny, px, nb);
+            if (inv) k_mr_fft_rows<NT.value, LC.value, true, P7.value><<<grid, NT.value, 0, s>>>(u, (int)ny, px, nb);
+            else k_mr_fft_rows<NT.value, LC.value, false, P7.value><<<grid, NT.value, 0, s>>>(u, (int)ny, px, nb);
         });
     }
     OLX_TRY(check_launch("mixed-radix fft rows"));
     {
         TimedLaunch tl(ctx, "fft_cols");
-        mr_shape((int)ny, [&](auto NT, auto LC) {
+        mr_shape((int)ny, [&](auto NT, auto LC, auto P7) {
             const int nb = mr_nb<LC.value>((int)ny);
             const int grid = (int)((nx + nb - 1) / nb);
-            if (inv) k_mr_fft_cols<NT.value, LC.value, true><<<grid, NT.value, 0, s>>>(u, (int)nx, py, nb);
-            else k_mr_fft_cols<NT.value, LC.value, false><<<grid, NT.value, 0, s>>>(u, (int)nx, py, nb);
+            if (inv) k_mr_fft_cols<NT.value, LC.value, true, P7.value><<<grid, NT.value, 0, s>>>(u, (int)nx, py, nb);
+            else k_mr_fft_cols<NT.value, LC.value, false, P7.value><<<grid, NT.value, 0, s>>>(u, (int)nx, py, nb);
         });
     }
     return check_launch("mixed-radix fft cols");
