@@ -436,7 +436,13 @@ static int mr_nb(int L) {
 template <class F>
 static void mr_shape2(int L, F&& f) {
     if (L <= 2048) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
-    else if (L <= 8192) f(std::integral_constant<int, 1024>(), std::integral_constant<int, 8192>());
+    else if (L <= 8192) {
+        static const char* e = getenv("OLX_MRNT");
+        const int nt = e ? atoi(e) : 512;
+        if (nt == 256) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
+        else if (nt == 1024) f(std::integral_constant<int, 1024>(), std::integral_constant<int, 8192>());
+        else f(std::integral_constant<int, 512>(), std::integral_constant<int, 8192>());
+    }
     else f(std::integral_constant<int, 1024>(), std::integral_constant<int, 16384>());
 }
 // also passes P7 (whether L has a factor 7, 11 or 13) as a compile-time constant
