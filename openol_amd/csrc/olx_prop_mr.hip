@@ -437,10 +437,14 @@ template <class F>
 static void mr_shape2(int L, F&& f) {
     if (L <= 2048) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
     else if (L <= 8192) {
+        // 512 threads; lengths up to 4352 hold two transforms per work-group (capacity
+        // 8704), so the radix-16 stages of a 4320-point column keep more threads busy.
+        // OLX_MRNT = 256, 512 or 1024 (capacity 8192) overrides for tuning.
         static const char* e = getenv("OLX_MRNT");
-        const int nt = e ? atoi(e) : 512;
+        const int nt = e ? atoi(e) : 0;
         if (nt == 256) f(std::integral_constant<int, 256>(), std::integral_constant<int, 8192>());
         else if (nt == 1024) f(std::integral_constant<int, 1024>(), std::integral_constant<int, 8192>());
+        else if (nt == 0 && L <= 4352) f(std::integral_constant<int, 512>(), std::integral_constant<int, 8704>());
         else f(std::integral_constant<int, 512>(), std::integral_constant<int, 8192>());
     }
     else f(std::integral_constant<int, 1024>(), std::integral_constant<int, 16384>());
