@@ -119,12 +119,16 @@ __device__ __forceinline__ long long block_excl_scan(long long v, long long* par
 
 __global__ void __launch_bounds__(SCAN_T)
 k_scan_blocks(const int* __restrict__ counts, int* __restrict__ offs, long long* __restrict__ bsum,
-              int ntiles) {
+              int ntiles, int* __restrict__ ndense, int* __restrict__ dlist, int small) {
     __shared__ long long part[2 * SCAN_T / 64];
     const int i0 = blockIdx.x * SCAN_B + threadIdx.x * SCAN_V;
     int c[SCAN_V];
 #pragma unroll
-    for (int k = 0; k < SCAN_V; k++) c[k] = i0 + k < ntiles ? counts[i0 + k] : 0;
+    for (int k = 0; k < SCAN_V; k++) {
+        c[k] = i0 + k < ntiles ? counts[i0 + k] : 0;
+        // tiles whose list passes the sparse launch's sort space, for the dense launch
+        if (c[k] > small) dlist[atomicAdd(ndense, 1)] = i0 + k;
+    }
     long long v = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_V; k++) v += c[k];
@@ -165,6 +169,10 @@ k_scan_add(int* __restrict__ offs, const long long* __restrict__ bbase, int ntil
 }
 
 constexpr int WRP_SORT_CAP = 4096;
+// sort space of the launch for sparse tiles (see k_wrp_accum): at 200,000 points into
+// 4096^2 (about 27 points per tile) 0.41 ms against 0.53 ms with 4096 for every tile
+// (2048: 0.45 ms; 512: 0.41 ms)
+constexpr int WRP_SMALL_CAP = 1024;
 
 struct WrpPt {     // per-point constants of one tile's list, staged in LDS
     double x, y;
@@ -218,22 +226,26 @@ __device__ __forceinline__ bool wrp_touches(const olx_point& q, int tx, int ty, 
     return tx >= x0 && tx <= x1 && ty >= y0 && ty <= y1;
 }
 
-template <bool SQUARE>
-__global__ void __launch_bounds__(WRP_NT)
-k_wrp_accum(const olx_point* __restrict__ pts, int n, const int* __restrict__ offs,
-            const int* __restrict__ list, float2* __restrict__ U, int ny, int nx, double p,
-            double lambda, double plane_z, double plim) {
-    __shared__ int ids[WRP_SORT_CAP];
-    __shared__ WrpPt sp[WRP_NT];
-    __shared__ int wcnt[WRP_NT / 64];
+// Tiles whose list holds LO < count <= CAP entries are summed from their sorted list;
+// with DENSE, tiles beyond CAP scan the whole cloud; other tiles are left to the other
+// launch.  Two launches share the tiles: <1024, -1, false> takes the lists of up to 1024
+// points with 4 KiB of sort space, so that more work-groups fit per CU (LDS-bound with
+// 16 KiB), and <4096, 1024, true> the rest.  Every tile sums in point-index order either
+// way, so the split does not change the result.
+template <bool SQUARE, int CAP, int LO, bool DENSE>
+__device__ __forceinline__ void wrp_tile(int tile, int* ids, WrpPt* sp, int* wcnt,
+                                         const olx_point* __restrict__ pts, int n,
+                                         const int* __restrict__ offs, const int* __restrict__ list,
+                                         float2* __restrict__ U, int ny, int nx, double p,
+                                         double lambda, double plane_z, double plim) {
     const int tiles_x = (nx + WT - 1) / WT;
-    const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     const int w = tx * WT + threadIdx.x % WT, h0 = ty * WT + threadIdx.x / WT, h1 = h0 + WT / 2;
     const int beg = offs[tile], cnt = offs[tile + 1] - beg;
+    if (cnt <= LO || (!DENSE && cnt > CAP)) return;  // the other launch's tile (uniform)
     const double xw = p * w, yh0 = p * h0, yh1 = p * h1;
     float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
-    const bool sorted = cnt <= WRP_SORT_CAP;
+    const bool sorted = cnt <= CAP;
     if (sorted) {
         int np2 = 1;
         while (np2 < cnt) np2 <<= 1;
@@ -304,6 +316,28 @@ k_wrp_accum(const olx_point* __restrict__ pts, int n, const int* __restrict__ of
     if (w < nx && h1 < ny) U[(size_t)h1 * nx + w] = make_float2(re1, im1);
 }
 
+// one work-group per tile, or (LIST) a grid-stride loop over the nt tiles listed in
+// dlist (the dense launch, which runs only when there are such tiles)
+template <bool SQUARE, int CAP, int LO, bool DENSE, bool LIST>
+__global__ void __launch_bounds__(WRP_NT)
+k_wrp_accum(const olx_point* __restrict__ pts, int n, const int* __restrict__ offs,
+            const int* __restrict__ list, float2* __restrict__ U, int ny, int nx, double p,
+            double lambda, double plane_z, double plim, int nt, const int* __restrict__ dlist) {
+    __shared__ int ids[CAP];
+    __shared__ WrpPt sp[WRP_NT];
+    __shared__ int wcnt[WRP_NT / 64];
+    if constexpr (LIST) {
+        for (int e = blockIdx.x; e < nt; e += gridDim.x) {
+            wrp_tile<SQUARE, CAP, LO, DENSE>(dlist[e], ids, sp, wcnt, pts, n, offs, list, U, ny, nx,
+                                             p, lambda, plane_z, plim);
+            __syncthreads();  // the next tile reuses the LDS arrays
+        }
+    } else {
+        wrp_tile<SQUARE, CAP, LO, DENSE>(blockIdx.x, ids, sp, wcnt, pts, n, offs, list, U, ny, nx, p,
+                                         lambda, plane_z, plim);
+    }
+}
+
 }  // namespace olx
 
 using namespace olx;
@@ -341,29 +375,32 @@ extern "C" int olx_wrp_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     const int tiles_x = (int)((nx + WT - 1) / WT), tiles_y = (int)((ny + WT - 1) / WT);
     const int ntiles = tiles_x * tiles_y;
     const int nblk = (ntiles + SCAN_B - 1) / SCAN_B;
-    const size_t ints = ((size_t)3 * ntiles + 2 + 1) & ~(size_t)1;  // keep the 64-bit part aligned
+    const size_t ints = ((size_t)4 * ntiles + 2 + 1) & ~(size_t)1;  // keep the 64-bit part aligned
     OLX_TRY(ctx->wrp_counts.reserve(sizeof(int) * ints + sizeof(long long) * (size_t)nblk));
     int* counts = (int*)ctx->wrp_counts.p;
-    int* offs = counts + ntiles;       // ntiles + 1
-    int* cursor = offs + ntiles + 1;   // ntiles
+    int* offs = counts + ntiles;       // ntiles + 1 (the total last)
+    int* ndense = offs + ntiles + 1;   // 1, beside the total: one copy reads both
+    int* cursor = ndense + 1;          // ntiles
+    int* dlist = cursor + ntiles;      // ntiles
     long long* bsum = (long long*)(counts + ints);  // nblk
     hipStream_t s = ctx->stream;
     OLX_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (size_t)ntiles, s));
-    OLX_HIP(hipMemsetAsync(cursor, 0, sizeof(int) * (size_t)ntiles, s));
+    OLX_HIP(hipMemsetAsync(ndense, 0, sizeof(int) * ((size_t)ntiles + 1), s));
     const int nb = (int)((n + 255) / 256);
     {
         TimedLaunch tl(ctx, "wrp_bin");
         if (n > 0)
             k_wrp_count<<<nb, 256, 0, s>>>(d_pts, (int)n, counts, (int)ny, (int)nx, p, lambda,
                                            plane_z, p_limit, square);
-        k_scan_blocks<<<nblk, SCAN_T, 0, s>>>(counts, offs, bsum, ntiles);
+        k_scan_blocks<<<nblk, SCAN_T, 0, s>>>(counts, offs, bsum, ntiles, ndense, dlist, WRP_SMALL_CAP);
         k_scan_top<<<1, SCAN_T, 0, s>>>(bsum, nblk, offs + ntiles);
         k_scan_add<<<(ntiles + 255) / 256, 256, 0, s>>>(offs, bsum, ntiles);
     }
     OLX_TRY(check_launch("wrp count/scan"));
-    int total = 0;
-    OLX_HIP(hipMemcpyAsync(&total, offs + ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
+    int tot2[2] = {0, 0};  // (tile, point) pairs, dense tiles
+    OLX_HIP(hipMemcpyAsync(tot2, offs + ntiles, sizeof tot2, hipMemcpyDeviceToHost, s));
     OLX_HIP(hipStreamSynchronize(s));
+    const int total = tot2[0], nd = tot2[1];
     if (total < 0) OLX_FAIL(OLX_EUNSUPPORTED, "wrp: more than 2^31 (tile, point) pairs");
     OLX_TRY(ctx->wrp_lists.reserve(sizeof(int) * (size_t)std::max(total, 1)));
     int* list = (int*)ctx->wrp_lists.p;
@@ -376,12 +413,18 @@ extern "C" int olx_wrp_c64(olx_ctx* ctx, const olx_point* d_pts, int64_t n, void
     OLX_TRY(check_launch("wrp fill"));
     {
         TimedLaunch tl(ctx, "wrp_accum");
-        if (square)
-            k_wrp_accum<true><<<ntiles, WRP_NT, 0, s>>>(d_pts, (int)n, offs, list, (float2*)d_U,
-                                                     (int)ny, (int)nx, p, lambda, plane_z, p_limit);
-        else
-            k_wrp_accum<false><<<ntiles, WRP_NT, 0, s>>>(d_pts, (int)n, offs, list, (float2*)d_U,
-                                                      (int)ny, (int)nx, p, lambda, plane_z, p_limit);
+        auto go = [&](auto SQ) {
+            constexpr bool Q = decltype(SQ)::value;
+            k_wrp_accum<Q, WRP_SMALL_CAP, -1, false, false><<<ntiles, WRP_NT, 0, s>>>(
+                d_pts, (int)n, offs, list, (float2*)d_U, (int)ny, (int)nx, p, lambda, plane_z, p_limit,
+                ntiles, nullptr);
+            if (nd > 0)
+                k_wrp_accum<Q, WRP_SORT_CAP, WRP_SMALL_CAP, true, true><<<std::min(nd, 4096), WRP_NT, 0, s>>>(
+                    d_pts, (int)n, offs, list, (float2*)d_U, (int)ny, (int)nx, p, lambda, plane_z,
+                    p_limit, nd, dlist);
+        };
+        if (square) go(std::true_type());
+        else go(std::false_type());
     }
     return check_launch("wrp accumulate");
 }
