@@ -112,6 +112,12 @@ float2* table(olx_ctx* ctx, const std::string& key, size_t count, bool* fresh, i
 const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa, int64_t lim,
                              int adj, int* rc, double qb = 0.0);
 
+// forward FP64 DFT (exponent -2 pi i nk/L) of one sequence of length L in place, on the
+// context's stream (olx_prop64.hip): O(L log L) in one work-group's LDS.  Returns
+// OLX_EUNSUPPORTED, without recording an error, when L is above 8192 or has a prime
+// factor above 13; the caller then takes another route.
+int fft64_one(olx_ctx* ctx, double2* u, int L);
+
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // mixed-radix transforms (olx_prop_mr.hip): whether length L (2 .. 16384) factors
