@@ -490,6 +490,10 @@ def next_rows(ol, torch):
     ms = _event_ms(torch, lambda: ol.FresnelPropBandLimit(f, N, N, PITCH, PITCH, LAM, 0.3,
                                                           variant="cpu"), 10)
     out["fresnel_bl_4096_ms"] = round(ms, 3)
+    # a depth sweep: every call a new distance, so both axis spectra are built each time
+    zs = iter(0.2 + 1e-4 * k for k in range(1000))
+    ms = _event_ms(torch, lambda: ol.FresnelProp(f, N, N, PITCH, PITCH, LAM, next(zs)), 10)
+    out["fresnel_4096_new_z_ms"] = round(ms, 3)
     del f
     ms = _event_ms(torch, lambda: ol.gshiftedAsmProp(u, u, N, N, PITCH, PITCH, LAM, 0.01, 1e-3, -2e-3,
                                                      bandlimit=True), 10)
