@@ -240,6 +240,43 @@ def test_wrp_dense_tiles_deterministic(square):
         assert torch.equal(U, U2)
 
 
+@pytest.mark.parametrize("square", [False, True])
+def test_wrp_mid_density_tiles(square):
+    """Tiles whose lists hold between 1024 and 4096 entries: the accumulate runs the
+    tiles of up to 1024 entries in one launch and lists the longer ones for a second
+    launch with the larger sort space.  4,000 points in a 64 x 64-pixel patch (the
+    windows' bounding boxes, estimated here, put several tiles in that range and none
+    above it), plus 1,000 over the 128 x 160 plane.  Sampled pixels against the FP64
+    oracle; two runs bit for bit alike."""
+    ny, nx = 128, 160
+    a = cloud(4000, 79, 64, 64, 0.151, 0.155)
+    b = cloud(1000, 80, nx, ny, 0.151, 0.155)
+    pts = {k: np.concatenate([a[k] + (40 * P if k in ("x", "y") else 0), b[k]]) for k in a}
+    # bounding-box estimate of the tile list lengths (16 x 16 tiles): the same windows
+    # as the kernel's tile test, W = |dz| tan(asin(lambda / 2p))
+    s = LAM / (2 * P)
+    W = np.abs(pts["z"] - 0.15) * s / np.sqrt(1 - s * s)
+    counts = []
+    for ty in range(ny // 16):
+        for tx in range(nx // 16):
+            x0, x1, y0, y1 = 16 * tx * P, (16 * tx + 15) * P, 16 * ty * P, (16 * ty + 15) * P
+            counts.append(int(np.sum((pts["x"] + W >= x0) & (pts["x"] - W <= x1) &
+                                     (pts["y"] + W >= y0) & (pts["y"] - W <= y1))))
+    assert sum(1200 < c < 3500 for c in counts) >= 4 and max(counts) < 3500
+    o = obj_of(pts, with_ap=False)
+    U = torch.empty((ny, nx), dtype=torch.complex64, device="cuda")
+    ol().WRPStep1(o, U, ny, nx, P, LAM, 0.15, square=square)
+    rng = np.random.default_rng(6)
+    ph, pw = rng.integers(0, ny, 1500), rng.integers(0, nx, 1500)
+    ph[:300], pw[:300] = rng.integers(40, 104, 300), rng.integers(40, 104, 300)  # in the patch
+    ref = olref.wrp_pixels(pts["x"], pts["y"], pts["z"], ph, pw, P, LAM, 0.15, square=square)
+    assert rel(U.cpu().numpy()[ph, pw], ref) < 1e-5
+    for _ in range(2):
+        U2 = torch.empty_like(U)
+        ol().WRPStep1(o, U2, ny, nx, P, LAM, 0.15, square=square)
+        assert torch.equal(U, U2)
+
+
 def test_wrp_large_plane_matches_windows():
     """A 33024^2 plane (4.26 M tiles of 16 x 16: the tile-count scan spans 1040 blocks
     and two rounds of its top level) with clusters of points in three far-apart
