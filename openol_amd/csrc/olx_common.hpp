@@ -78,6 +78,7 @@ struct olx_ctx {
     olx::DevBuf split_acc;  // split propagation: the summed destination field
     olx::DevBuf mp_work;    // multi-plane loss: exp(i phi), one plane's field, partial sums
     olx::DevBuf cgh_part;   // direct CGH split over point slices: one partial plane per slice
+    olx::DevBuf fft64_tmp;  // FP64 1-D FFT above 8192 points: the even and odd halves
     // timing
     bool timing = false;
     int timing_period = 1;   // time every timing_period-th launch of each kernel
@@ -113,9 +114,10 @@ const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa,
                              int adj, int* rc, double qb = 0.0);
 
 // forward FP64 DFT (exponent -2 pi i nk/L) of one sequence of length L in place, on the
-// context's stream (olx_prop64.hip): O(L log L) in one work-group's LDS.  Returns
-// OLX_EUNSUPPORTED, without recording an error, when L is above 8192 or has a prime
-// factor above 13; the caller then takes another route.
+// context's stream (olx_prop64.hip): O(L log L) in one work-group's LDS, lengths above
+// 8192 split once into even and odd halves.  Returns OLX_EUNSUPPORTED, without
+// recording an error, when L (or L/2 above 8192) has a prime factor above 13 or L is
+// above 16384; the caller then takes another route.
 int fft64_one(olx_ctx* ctx, double2* u, int L);
 
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }

@@ -880,7 +880,7 @@ const float2* spectrum_table(olx_ctx* ctx, int M, double d, double K, double qa,
     if ((*rc = ctx->scratch2.reserve(sizeof(double2) * M)) != OLX_OK) return nullptr;
     double2* tmp = (double2*)ctx->scratch2.p;
     // the chirp's FFT in FP64 (a 8192-point table: tens of microseconds against about
-    // 2 ms for the direct DFT, which takes the lengths the FFT does not: above 8192)
+    // 2 ms for the direct DFT, which takes the lengths the FFT does not: above 16384)
     if (M % 2 == 0) {
         k_chirp_f64<<<(M + 255) / 256, 256, 0, ctx->stream>>>(tmp, M, d, K, qa, qb, 0.0, lim, 1);
         if (fft64_one(ctx, tmp, M) == OLX_OK) {

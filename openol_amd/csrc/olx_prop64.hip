@@ -569,13 +569,44 @@ int run64_prop(olx_ctx* ctx, double2* u, Args64& a) {
 
 }  // namespace
 
+// radix-2 split of a length-2H sequence: eo[0, H) its even samples, eo[H, 2H) its odd ones
+__global__ void k64_split2(const double2* __restrict__ in, double2* __restrict__ eo, int H) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H) return;
+    eo[i] = in[2 * i];
+    eo[H + i] = in[2 * i + 1];
+}
+// out[k] = E[k] + W^k O[k], out[k + H] = E[k] - W^k O[k], W = exp(-2 pi i / 2H)
+__global__ void k64_join2(double2* __restrict__ out, const double2* __restrict__ eo, int H) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= H) return;
+    double s, c;
+    sincospi(-(double)k / (double)H, &s, &c);
+    const double2 e = eo[k], o = eo[H + k];
+    const double2 wo = make_double2(o.x * c - o.y * s, o.x * s + o.y * c);
+    out[k] = make_double2(e.x + wo.x, e.y + wo.y);
+    out[k + H] = make_double2(e.x - wo.x, e.y - wo.y);
+}
+
 int fft64_one(olx_ctx* ctx, double2* u, int L) {
     int radix[ST_MAX], nst = 0;
-    if (L < 2 || L > LMAX64 || !factor64(L, radix, &nst)) return OLX_EUNSUPPORTED;
+    hipStream_t s = ctx->stream;
+    if (L >= 2 && L <= LMAX64 && factor64(L, radix, &nst)) {
+        const Plan64* P = nullptr;
+        OLX_TRY(plan64(ctx, L, &P));
+        k64_fft_rows<false><<<1, NT64, 0, s>>>(u, 1, P, 1);
+        return check_launch("FP64 FFT");
+    }
+    const int H = L / 2;
+    if (L % 2 != 0 || H > LMAX64 || L > 2 * LMAX64 || !factor64(H, radix, &nst)) return OLX_EUNSUPPORTED;
+    OLX_TRY(ctx->fft64_tmp.reserve(sizeof(double2) * (size_t)L));
+    double2* eo = (double2*)ctx->fft64_tmp.p;
     const Plan64* P = nullptr;
-    OLX_TRY(plan64(ctx, L, &P));
-    k64_fft_rows<false><<<1, NT64, 0, ctx->stream>>>(u, 1, P, 1);
-    return check_launch("FP64 FFT");
+    OLX_TRY(plan64(ctx, H, &P));
+    k64_split2<<<(H + 255) / 256, 256, 0, s>>>(u, eo, H);
+    k64_fft_rows<false><<<2, NT64, 0, s>>>(eo, 2, P, 1);
+    k64_join2<<<(H + 255) / 256, 256, 0, s>>>(u, eo, H);
+    return check_launch("FP64 FFT (split)");
 }
 
 }  // namespace olx

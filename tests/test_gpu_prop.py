@@ -312,9 +312,10 @@ def test_launch_timing_sampling():
 @pytest.mark.parametrize("mode", ["asm_bl", "asm_bl_adjoint", "asm", "fresnel", "fresnel_bl",
                                   "shifted", "shifted_window", "asm_prec_f32"])
 def test_column_pass_8192_all_modes(mode):
-    """Fields of 4096 rows (padded columns of 8192) take the wave-split column pass
-    (8 x 1024 four-step): every transfer-function mode against the oracle, on 4096 x
-    1024 (fast to check on the CPU)."""
+    """Fields of 4096 rows (padded columns of 8192: one transform per LDS round, the
+    pipelined exchanges, the forward transform with its remainder radix first): every
+    transfer-function mode against the oracle, on 4096 x 1024 (fast to check on the
+    CPU)."""
     ny, nx, d = 4096, 1024, 8e-6
     print(f"column pass 8192: {mode}", flush=True)
     u0 = synth.random_field(ny, nx, seed=41)
@@ -341,6 +342,21 @@ def test_column_pass_8192_all_modes(mode):
         ref = olref.shifted_asm(u0, d, d, LAM, 0.1, 2e-4, -3e-4, olref.SHIFT_WINDOW if win else 0)
     tol = 2e-5 if mode == "asm_prec_f32" else TOL
     assert rel(got.cpu().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("variant,flags", [(None, 0), ("gpu", olref.FRESNEL_BL_GPU)])
+def test_fresnel_16384_spectrum(variant, flags):
+    """Padded columns of 16384: the row spectrum table is the FP64 FFT of the chirp
+    split once into two 8192-point halves (8192 above the single-group limit).  8192 x 64
+    against the oracle's direct sums."""
+    ny, nx, d, z = 8192, 64, 8e-6, 0.05
+    u0 = synth.random_field(ny, nx, seed=43)
+    u = torch.from_numpy(u0).cuda()
+    if variant is None:
+        ol().FresnelProp(u, ny, nx, d, d, LAM, z)
+    else:
+        ol().FresnelPropBandLimit(u, ny, nx, d, d, LAM, z, variant=variant)
+    assert rel(u.cpu().numpy(), olref.fresnel_prop(u0, d, d, LAM, z, flags)) < TOL
 
 
 def test_column_pass_8192_batch():
