@@ -227,8 +227,10 @@ __device__ __forceinline__ void stage64(double2* lds, int LP, int nb, const Stag
     __syncthreads();
 }
 
+// inlined into each kernel: as a called function it kept a 156-byte stack frame in
+// scratch memory
 template <bool INV>
-__device__ void run64(double2* lds, int LP, int nb, const Plan64* __restrict__ P, int tid) {
+__device__ __forceinline__ void run64(double2* lds, int LP, int nb, const Plan64* __restrict__ P, int tid) {
     const double2* tw = P->tw;
     for (int s = 0; s < P->nst; s++) {
         const Stage64 S = P->st[s];
