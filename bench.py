@@ -659,8 +659,9 @@ def multiplane(torch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "decomposition is described in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); a propagation "
+              "sharded over GPUs (row bands, an all-to-all transpose between the row and "
+              "column passes) is not run here.", file=sys.stderr)
         sys.exit(2)
     import torch
     import openol_amd as ol
