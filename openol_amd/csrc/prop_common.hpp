@@ -113,8 +113,15 @@ __device__ __forceinline__ float2 asm_h_f32(const PropArgs& a, float uf, bool uc
     const bool keep = (!ucut) & ((!a.bandlimit) | (fabs(vd) * 2 < a.vw)) & (arg >= 0.f);
     const float w = sqrtf(fmaxf(arg, 0.f));
     const float ph = (float)(2.0 * M_PI * (double)w * (double)a.zf);
-    float s, c;
-    sincosf(ph, &s, &c);
+    // sin / cos of that float phase (up to about 1e6 rad): reduced to revolutions mod 1
+    // in FP64 (1e-10 rad), then the hardware sine and cosine (about 1e-7 absolute, far
+    // below the 0.06 rad spacing of float phases of that size, which is what this mode
+    // reproduces).  sincosf's large-argument reduction needed so many registers that the
+    // column pass spilled, and an FP64 sincos more still.
+    double cyc = (double)ph * 0.15915494309189535;
+    cyc -= rint(cyc);
+    const float t = (float)cyc;
+    const float s = __builtin_amdgcn_sinf(t), c = __builtin_amdgcn_cosf(t);
     const float m = keep ? 1.f : 0.f;
     return make_float2(c * m, (a.adjoint ? -s : s) * m);
 }
