@@ -161,6 +161,22 @@ struct Plan {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }  // lpadn<4>
 
+// w1, w2, w4, w8 of a radix-16 stage (W_{16 NS}^{k}, ^{2k}, ^{4k}, ^{8k}) from the stage
+// table after the 16384-entry twiddle table: entry 4 (NS - 1) + 4 k + i, two contiguous
+// 16-byte loads per lane.  The table's entries are bitwise the twiddle table's at
+// index 2^i k 16384 / (16 NS); read from the twiddle table, lanes of one wave touched
+// up to 64 cache lines per load.
+template <int NS>
+__device__ __forceinline__ void stage16_twiddles(int k, const float2* __restrict__ tw, float2* w) {
+    const float4* st = reinterpret_cast<const float4*>(tw + 16384 + 4 * (NS - 1)) + 2 * k;
+    const float4 a = st[0], b = st[1];
+    w[0] = make_float2(a.x, a.y);
+    w[1] = make_float2(a.z, a.w);
+    w[2] = make_float2(b.x, b.y);
+    w[3] = make_float2(b.z, b.w);
+}
+
+
 // twiddles W_{NS R}^{k r}, r = 1..R-1, from the length-TWN table (forward sign)
 template <int R, bool INV>
 __device__ __forceinline__ void apply_twiddles(float2* v, int base, const float2* __restrict__ tw) {
@@ -257,6 +273,39 @@ __device__ __forceinline__ void apply_twiddles2(float2* a, float2* b, int base,
     }
 }
 
+// the products of apply_twiddles(2) for R = 16 from w1, w2, w4, w8 already loaded
+template <bool INV, bool TWO>
+__device__ __forceinline__ void apply16_from(float2* a, float2* b, const float2* l) {
+    float2 w1 = l[0], w2 = l[1], w4 = l[2], w8 = l[3];
+    if (INV) { w1 = cconj(w1); w2 = cconj(w2); w4 = cconj(w4); w8 = cconj(w8); }
+    a[1] = cmul(a[1], w1); if constexpr (TWO) b[1] = cmul(b[1], w1);
+    a[2] = cmul(a[2], w2); if constexpr (TWO) b[2] = cmul(b[2], w2);
+    a[4] = cmul(a[4], w4); if constexpr (TWO) b[4] = cmul(b[4], w4);
+    const float2 w3 = cmul(w1, w2);
+    a[3] = cmul(a[3], w3); if constexpr (TWO) b[3] = cmul(b[3], w3);
+    const float2 w5 = cmul(w1, w4);
+    a[5] = cmul(a[5], w5); if constexpr (TWO) b[5] = cmul(b[5], w5);
+    const float2 w6 = cmul(w2, w4);
+    a[6] = cmul(a[6], w6); if constexpr (TWO) b[6] = cmul(b[6], w6);
+    const float2 w7 = cmul(w3, w4);
+    a[7] = cmul(a[7], w7); if constexpr (TWO) b[7] = cmul(b[7], w7);
+    a[8] = cmul(a[8], w8); if constexpr (TWO) b[8] = cmul(b[8], w8);
+    float2 w = cmul(w1, w8);
+    a[9] = cmul(a[9], w); if constexpr (TWO) b[9] = cmul(b[9], w);
+    w = cmul(w2, w8);
+    a[10] = cmul(a[10], w); if constexpr (TWO) b[10] = cmul(b[10], w);
+    w = cmul(w3, w8);
+    a[11] = cmul(a[11], w); if constexpr (TWO) b[11] = cmul(b[11], w);
+    w = cmul(w4, w8);
+    a[12] = cmul(a[12], w); if constexpr (TWO) b[12] = cmul(b[12], w);
+    w = cmul(w5, w8);
+    a[13] = cmul(a[13], w); if constexpr (TWO) b[13] = cmul(b[13], w);
+    w = cmul(w6, w8);
+    a[14] = cmul(a[14], w); if constexpr (TWO) b[14] = cmul(b[14], w);
+    w = cmul(w7, w8);
+    a[15] = cmul(a[15], w); if constexpr (TWO) b[15] = cmul(b[15], w);
+}
+
 // twiddle + DFT for every butterfly of stage S held by thread t
 template <class P, int S, bool INV, int TWLEN>
 __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw) {
@@ -266,7 +315,13 @@ __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __
         if constexpr (NS > 1) {
             int j = t + P::T * b;
             int k = j & (NS - 1);
-            apply_twiddles<R, INV>(v + b * R, k * (TWLEN / (NS * R)), tw);
+            if constexpr (R == 16 && TWLEN == 16384 && NS <= 1024) {
+                float2 l[4];
+                stage16_twiddles<NS>(k, tw, l);
+                apply16_from<INV, false>(v + b * R, nullptr, l);
+            } else {
+                apply_twiddles<R, INV>(v + b * R, k * (TWLEN / (NS * R)), tw);
+            }
         }
         dft<R, INV>(v + b * R);
     }
@@ -282,7 +337,13 @@ __device__ __forceinline__ void stage_compute2(float2* va, float2* vb, int t,
         if constexpr (NS > 1) {
             int j = t + P::T * b;
             int k = j & (NS - 1);
-            apply_twiddles2<R, INV>(va + b * R, vb + b * R, k * (TWLEN / (NS * R)), tw);
+            if constexpr (R == 16 && TWLEN == 16384 && NS <= 1024) {
+                float2 l[4];
+                stage16_twiddles<NS>(k, tw, l);
+                apply16_from<INV, true>(va + b * R, vb + b * R, l);
+            } else {
+                apply_twiddles2<R, INV>(va + b * R, vb + b * R, k * (TWLEN / (NS * R)), tw);
+            }
         }
         dft<R, INV>(va + b * R);
         dft<R, INV>(vb + b * R);
@@ -427,9 +488,13 @@ struct TwBase {
 #pragma unroll
             for (int b = 0; b < NB; b++) {
                 const int k = (t + P::T * b) & (NS - 1);
-                const int base = k * (TWLEN / (NS * R));
+                if constexpr (R == 16 && TWLEN == 16384 && NS <= 1024) {
+                    stage16_twiddles<NS>(k, tw, l[b]);
+                } else {
+                    const int base = k * (TWLEN / (NS * R));
 #pragma unroll
-                for (int i = 0; i < NL; i++) l[b][i] = tw[(1 << i) * base];
+                    for (int i = 0; i < NL; i++) l[b][i] = tw[(1 << i) * base];
+                }
             }
         }
     }

@@ -91,6 +91,10 @@ struct olx_ctx {
 namespace olx {
 
 constexpr int TWN = 16384;  // twiddle table length = largest transform length
+// radix-16 stage tables after the twiddle table: for every NS = 2^p <= 1024, entries
+// 4 (NS - 1) + 4 k + i = W_{16 NS}^{2^i k} (k < NS, i < 4), so that a stage's w1, w2, w4,
+// w8 are two contiguous 16-byte loads per lane (fft_core.hpp)
+constexpr int TWS = 8192;
 constexpr size_t TABLE_CAP = 64;  // cached tables kept per context
 
 // timing bracket around launches (no-op unless ctx->timing)

@@ -112,6 +112,21 @@ __global__ void k_twiddle(float2* tw, int n) {
     tw[m] = make_float2((float)c, (float)s);
 }
 
+// the radix-16 stage tables (olx::TWS): the same FP64 expression as k_twiddle for the
+// table index each entry stands for, so the values are bitwise those of the table
+__global__ void k_twiddle_stages(float2* st, int n) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 4 * 2047) return;
+    const int q = idx / 4;
+    const int p = 31 - __clz(q + 1);  // NS = 2^p: q in [NS - 1, 2 NS - 1)
+    const int ns = 1 << p, off = idx - 4 * (ns - 1);
+    const int k = off / 4, i = off % 4;
+    const int m = (k << i) * (n / (16 * ns));
+    double s, c;
+    sincospi(-2.0 * (double)m / (double)n, &s, &c);
+    st[idx] = make_float2((float)c, (float)s);
+}
+
 // plain copy, 16 bytes per lane.  Each work-group copies a contiguous span in 16 KiB
 // blocks, four independent loads per thread before their stores (a grid-stride
 // loop with one load per iteration reached 4.56 TB/s at 1 GiB).
@@ -152,7 +167,7 @@ extern "C" int olx_ctx_create(olx_ctx** out, int device) {
     OLX_HIP(hipSetDevice(device));
     olx_ctx* ctx = new olx_ctx();
     ctx->device = device;
-    if (hipMalloc((void**)&ctx->twiddle, sizeof(float2) * TWN) != hipSuccess) {
+    if (hipMalloc((void**)&ctx->twiddle, sizeof(float2) * (TWN + TWS)) != hipSuccess) {
         delete ctx;
         OLX_FAIL(OLX_ENOMEM, "twiddle table allocation failed");
     }
@@ -162,6 +177,7 @@ extern "C" int olx_ctx_create(olx_ctx** out, int device) {
         return rc;
     }
     k_twiddle<<<TWN / 256, 256>>>(ctx->twiddle, TWN);
+    k_twiddle_stages<<<TWS / 256, 256>>>(ctx->twiddle + TWN, TWN);
     if (hipDeviceSynchronize() != hipSuccess) {
         (void)hipFree(ctx->twiddle);
         delete ctx;
