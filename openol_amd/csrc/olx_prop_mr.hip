@@ -38,13 +38,17 @@ struct MrStage {
     int tws;      // twiddle stride L / (ns R)
     uint32_t mns; // fast-division magic of ns (unused when ns == 1)
     uint32_t mlr; // fast-division magic of lr
+    int off;      // this stage's twiddles in the stage table: k (R - 1) + r - 1 from here
 };
 
 // a plan lives in device memory right after its length's twiddle table, so kernels
 // index its stages through a pointer (uniform scalar loads)
 struct MrPlan {
     int L, nst;
-    const float2* tw;  // W_L^m = exp(-2 pi i m / L), m in [0, L)
+    const float2* tw;    // W_L^m = exp(-2 pi i m / L), m in [0, L)
+    const float2* stab;  // per stage (ns > 1): W_L^{k r tws} for k < ns, r = 1 .. R-1, k-major:
+                         // a butterfly's R - 1 twiddles are contiguous (from tw, lanes touched
+                         // scattered lines)
     MrStage st[MR_MAXST];
 };
 
@@ -134,7 +138,7 @@ __device__ __forceinline__ void dft_any(float2* v) {
 // one Stockham stage over nb transforms (LDS stride LP apart) of nb * lr butterflies
 template <int R, bool INV, int NT, int LCAP>
 __device__ __forceinline__ void mr_stage(float2* lds, int LP, int nb, const MrStage& S,
-                                         const float2* __restrict__ tw, int tid) {
+                                         const float2* __restrict__ stab, int tid) {
     constexpr int Q = (LCAP / R + NT - 1) / NT;
     const int tot = nb * S.lr;
     float2 v[Q][R];
@@ -156,9 +160,10 @@ __device__ __forceinline__ void mr_stage(float2* lds, int LP, int nb, const MrSt
             const int tr = fdiv(g, S.lr, S.mlr), j = g - tr * S.lr;
             const int k = j - fdiv(j, S.ns, S.mns) * S.ns;
             if (S.ns > 1) {
+                const float2* wk = stab + S.off + k * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    float2 w = tw[k * r * S.tws];
+                    float2 w = wk[r - 1];
                     if (INV) w = cconj(w);
                     v[q][r] = cmul(v[q][r], w);
                 }
@@ -181,7 +186,7 @@ __device__ __forceinline__ void mr_stage(float2* lds, int LP, int nb, const MrSt
 template <bool INV, int NT, int LCAP, bool P7>
 __device__ __forceinline__ void mr_run(float2* lds, int LP, int nb, const MrPlan* __restrict__ P, int tid) {
     const int nst = P->nst;
-    const float2* tw = P->tw;
+    const float2* tw = P->stab;
     for (int s = 0; s < nst; s++) {
         const MrStage S = P->st[s];
         switch (S.R) {
@@ -363,6 +368,28 @@ __global__ void k_mr_twiddle(float2* __restrict__ tw, int L, MrPlan plan, MrPlan
     tw[m] = make_float2((float)c, (float)s);
 }
 
+// the stage tables: entry S.off + k (R - 1) + r - 1 = W_L^{k r tws}, by the same FP64
+// expression as k_mr_twiddle at that index (bitwise the table's value)
+__global__ void k_mr_stage_tab(float2* __restrict__ stab, int L, MrPlan plan, int total) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    int s = -1;
+    for (int q = 0; q < plan.nst; q++) {
+        const MrStage& Q = plan.st[q];
+        if (Q.ns > 1 && e >= Q.off && e < Q.off + Q.ns * (Q.R - 1)) {
+            s = q;
+            break;
+        }
+    }
+    if (s < 0) return;
+    const MrStage& S = plan.st[s];
+    const int loc = e - S.off, k = loc / (S.R - 1), r = loc % (S.R - 1) + 1;
+    const int m = k * r * S.tws;
+    double sn, cs;
+    sincospi(-2.0 * (double)m / (double)L, &sn, &cs);
+    stab[e] = make_float2((float)cs, (float)sn);
+}
+
 // ------------------------------------------------------------------ host side
 // Stage radices of L: sixteens then the remaining 8 / 4 / 2, then 3, 5, 7, 11, 13.
 // Returns false when L has another prime factor or needs too many stages.
@@ -394,16 +421,28 @@ static int mr_plan(olx_ctx* ctx, int L, const MrPlan** out) {
     bool fresh = false;
     int rc = OLX_OK;
     constexpr size_t plan_f2 = (sizeof(MrPlan) + sizeof(float2) - 1) / sizeof(float2);
-    float2* tw = table(ctx, key, (size_t)L + plan_f2, &fresh, &rc);
+    // stage-table length: sum over the stages with ns > 1 of ns (R - 1)
+    int ntab = 0;
+    {
+        int ns = 1;
+        for (int s = 0; s < nst; s++) {
+            if (ns > 1) ntab += ns * (radix[s] - 1);
+            ns *= radix[s];
+        }
+    }
+    const size_t tabf2 = ((size_t)ntab + 1) & ~(size_t)1;  // keep the plan 16-byte aligned
+    float2* tw = table(ctx, key, (size_t)L + tabf2 + plan_f2, &fresh, &rc);
     if (!tw) return rc;
-    MrPlan* dplan = reinterpret_cast<MrPlan*>(tw + L);
+    float2* stab = tw + L;
+    MrPlan* dplan = reinterpret_cast<MrPlan*>(stab + tabf2);
     if (fresh) {
         MrPlan P;
         memset(&P, 0, sizeof P);
         P.L = L;
         P.nst = nst;
         P.tw = tw;
-        int ns = 1;
+        P.stab = stab;
+        int ns = 1, off = 0;
         for (int s = 0; s < nst; s++) {
             MrStage& S = P.st[s];
             S.R = radix[s];
@@ -412,9 +451,13 @@ static int mr_plan(olx_ctx* ctx, int L, const MrPlan** out) {
             S.tws = L / (ns * S.R);
             S.mns = fdiv_magic(ns);
             S.mlr = fdiv_magic(S.lr);
+            S.off = off;
+            if (ns > 1) off += ns * (S.R - 1);
             ns *= S.R;
         }
         k_mr_twiddle<<<(L + 255) / 256, 256, 0, ctx->stream>>>(tw, L, P, dplan);
+        if (ntab > 0)
+            k_mr_stage_tab<<<(ntab + 255) / 256, 256, 0, ctx->stream>>>(stab, L, P, ntab);
         OLX_TRY(check_launch("mixed-radix plan"));
     }
     *out = dplan;
