@@ -110,9 +110,9 @@ def bench_asm(ol, lib, torch, n, steps, warmup, settle_first=False):
 # shapes, measured with rocprofv3 --pmc: HBM bytes = FETCH_SIZE x 2 + WRITE_SIZE x 1
 # (the factors a plain 512 MiB copy kernel reads back), and SQ_INSTS_VALU.
 MEASURED_4096 = {
-    "prop_rows_pad_fwd": {"traffic": 404.2e6, "valu_instr": 24.95e6},
-    "prop_cols_conv": {"traffic": 539.2e6, "valu_instr": 113.12e6},
-    "prop_rows_crop_inv": {"traffic": 404.3e6, "valu_instr": 25.58e6},
+    "prop_rows_pad_fwd": {"traffic": 403.2e6, "valu_instr": 24.94e6},
+    "prop_cols_conv": {"traffic": 539.5e6, "valu_instr": 113.12e6},
+    "prop_rows_crop_inv": {"traffic": 403.3e6, "valu_instr": 25.59e6},
 }
 # wave64 VALU instructions per second: 1024 SIMDs x 2.4 GHz / 4 cycles (no packed FP32)
 VALU_PEAK = 1024 * 2.4e9 / 4
