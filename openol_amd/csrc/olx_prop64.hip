@@ -176,10 +176,13 @@ constexpr int NT64 = 512;
 
 struct Stage64 {
     int R, ns, lr, tws;
+    int off;  // this stage's twiddles in the stage table: k (R - 1) + r - 1 from here
 };
 struct Plan64 {
     int L, nst;
-    const double2* tw;  // W_L^m, m in [0, L)
+    const double2* tw;    // W_L^m, m in [0, L)
+    const double2* stab;  // per stage (ns > 1): W_L^{k r tws}, k-major, a butterfly's R - 1
+                          // twiddles contiguous
     Stage64 st[ST_MAX];
 };
 
@@ -210,9 +213,10 @@ __device__ __forceinline__ void stage64(double2* lds, int LP, int nb, const Stag
             const int tr = g / S.lr, j = g - tr * S.lr;
             const int k = j % S.ns;
             if (S.ns > 1) {
+                const double2* wk = tw + S.off + k * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    double2 w = tw[k * r * S.tws];
+                    double2 w = wk[r - 1];
                     if (INV) w = dconj(w);
                     v[q][r] = dmul(v[q][r], w);
                 }
@@ -231,7 +235,7 @@ __device__ __forceinline__ void stage64(double2* lds, int LP, int nb, const Stag
 // scratch memory
 template <bool INV>
 __device__ __forceinline__ void run64(double2* lds, int LP, int nb, const Plan64* __restrict__ P, int tid) {
-    const double2* tw = P->tw;
+    const double2* tw = P->stab;
     for (int s = 0; s < P->nst; s++) {
         const Stage64 S = P->st[s];
         switch (S.R) {
@@ -392,6 +396,28 @@ k64_fft_cols(double2* __restrict__ u, int cols, const Plan64* __restrict__ P, in
     }
 }
 
+// the stage tables: entry S.off + k (R - 1) + r - 1 = W_L^{k r tws}, by the same FP64
+// expression as k64_twiddle at that index
+__global__ void k64_stage_tab(double2* __restrict__ stab, int L, Plan64 plan, int total) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    int s = -1;
+    for (int q = 0; q < plan.nst; q++) {
+        const Stage64& Q = plan.st[q];
+        if (Q.ns > 1 && e >= Q.off && e < Q.off + Q.ns * (Q.R - 1)) {
+            s = q;
+            break;
+        }
+    }
+    if (s < 0) return;
+    const Stage64& S = plan.st[s];
+    const int loc = e - S.off, k = loc / (S.R - 1), r = loc % (S.R - 1) + 1;
+    const int m = k * r * S.tws;
+    double sn, cs;
+    sincospi(-2.0 * (double)m / (double)L, &sn, &cs);
+    stab[e] = make_double2(cs, sn);
+}
+
 __global__ void k64_twiddle(double2* __restrict__ tw, int L, Plan64 plan, Plan64* __restrict__ dst) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m == 0) *dst = plan;
@@ -447,28 +473,42 @@ int plan64(olx_ctx* ctx, int L, const Plan64** out) {
     snprintf(key, sizeof key, "plan64|%d", L);
     bool fresh = false;
     int rc = OLX_OK;
-    // L double2 twiddles (2 L float2) followed by the plan
+    // L double2 twiddles (2 L float2), the stage tables, then the plan
     const size_t plan_f2 = (sizeof(Plan64) + sizeof(float2) - 1) / sizeof(float2);
-    float2* mem = table(ctx, key, 2 * (size_t)L + plan_f2, &fresh, &rc);
+    int ntab = 0;
+    {
+        int ns = 1;
+        for (int s = 0; s < nst; s++) {
+            if (ns > 1) ntab += ns * (radix[s] - 1);
+            ns *= radix[s];
+        }
+    }
+    float2* mem = table(ctx, key, 2 * (size_t)L + 2 * (size_t)ntab + plan_f2, &fresh, &rc);
     if (!mem) return rc;
     double2* tw = reinterpret_cast<double2*>(mem);
-    Plan64* dplan = reinterpret_cast<Plan64*>(mem + 2 * (size_t)L);
+    double2* stab = tw + L;
+    Plan64* dplan = reinterpret_cast<Plan64*>(mem + 2 * (size_t)L + 2 * (size_t)ntab);
     if (fresh) {
         Plan64 P;
         memset(&P, 0, sizeof P);
         P.L = L;
         P.nst = nst;
         P.tw = tw;
-        int ns = 1;
+        P.stab = stab;
+        int ns = 1, off = 0;
         for (int s = 0; s < nst; s++) {
             Stage64& S = P.st[s];
             S.R = radix[s];
             S.ns = ns;
             S.lr = L / S.R;
             S.tws = L / (ns * S.R);
+            S.off = off;
+            if (ns > 1) off += ns * (S.R - 1);
             ns *= S.R;
         }
         k64_twiddle<<<(L + 255) / 256, 256, 0, ctx->stream>>>(tw, L, P, dplan);
+        if (ntab > 0)
+            k64_stage_tab<<<(ntab + 255) / 256, 256, 0, ctx->stream>>>(stab, L, P, ntab);
         OLX_TRY(check_launch("complex128 plan"));
     }
     *out = dplan;
