@@ -87,7 +87,10 @@ def test_prop_and_masm_c128():
     assert olref.rel_l2(u.cpu().numpy(), ref) < TOL_ASM
 
 
-@pytest.mark.parametrize("shape", [(8, 8), (90, 150), (1024, 512), (7, 13), (8192, 16)])
+# 154 = 2 * 7 * 11, 1001 = 7 * 11 * 13 and 4004 = 4 * 7 * 11 * 13 put radix 7, 11 and 13 in
+# stages after the first, where their twiddles come from the per-stage tables
+@pytest.mark.parametrize("shape", [(8, 8), (90, 150), (1024, 512), (7, 13), (8192, 16),
+                                   (154, 1001), (4004, 8)])
 def test_fft_c128_matches_numpy(shape):
     u0 = field(*shape, 73)
     u = torch.from_numpy(u0).cuda()
