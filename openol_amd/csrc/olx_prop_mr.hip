@@ -7,8 +7,11 @@
 // the earlier radices takes butterfly j in [0, L/R) from x[j + r L/R], multiplies
 // by W_{NS R}^{k r} (k = j mod NS, from an FP64-built table of W_L), applies the
 // R-point DFT and writes y[(j - k) R + k + r NS], leaving natural order after the
-// last stage.  Radix 16 (remainder 8, 4, 2) takes the factors of two; odd primes
-// use the symmetric-pair DFT.  Each stage loads every butterfly a thread owns into
+// last stage.  Radix 16 (remainder 8, 4, 2) takes the factors of two; the odd
+// factors go to radix 9 while 9 divides, then one radix 15 where 15 divides, then
+// 3, 5, 7, 11, 13, all by the symmetric-pair DFT (valid for any odd R).  Each stage
+// is an LDS round trip, so fewer, larger odd radices cost less than their extra
+// arithmetic: 4320 = 16 2 9 15 takes four stages where 16 2 3 3 3 5 took six.  Each stage loads every butterfly a thread owns into
 // registers, waits, then writes, so one LDS image per transform suffices.
 //
 // The three propagation passes mirror the power-of-two ones over an intermediate W
@@ -198,6 +201,8 @@ __device__ __forceinline__ void mr_run(float2* lds, int LP, int nb, const MrPlan
                 if constexpr (P7) mr_stage<7, INV, NT, LCAP>(lds, LP, nb, S, tw, tid);
                 break;
             case 8: mr_stage<8, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 9: mr_stage<9, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
+            case 15: mr_stage<15, INV, NT, LCAP>(lds, LP, nb, S, tw, tid); break;
             case 11:
                 if constexpr (P7) mr_stage<11, INV, NT, LCAP>(lds, LP, nb, S, tw, tid);
                 break;
@@ -391,13 +396,16 @@ __global__ void k_mr_stage_tab(float2* __restrict__ stab, int L, MrPlan plan, in
 }
 
 // ------------------------------------------------------------------ host side
-// Stage radices of L: sixteens then the remaining 8 / 4 / 2, then 3, 5, 7, 11, 13.
+// Stage radices of L: sixteens then the remaining 8 / 4 / 2, then nines, one 15,
+// then 3, 5, 7, 11, 13.
 // Returns false when L has another prime factor or needs too many stages.
 static bool mr_factor(int L, int* radix, int* nst) {
     int n = L, k = 0;
     while (n % 16 == 0 && k < MR_MAXST) { radix[k++] = 16; n /= 16; }
     for (int r : {8, 4, 2})
         if (n % r == 0 && k < MR_MAXST) { radix[k++] = r; n /= r; }
+    while (n % 9 == 0 && k < MR_MAXST) { radix[k++] = 9; n /= 9; }
+    if (n % 15 == 0 && k < MR_MAXST) { radix[k++] = 15; n /= 15; }
     for (int p : {3, 5, 7, 11, 13})
         while (n % p == 0 && k < MR_MAXST) { radix[k++] = p; n /= p; }
     *nst = k;

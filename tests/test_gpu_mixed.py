@@ -35,7 +35,7 @@ def rel(a, b):
 
 @pytest.mark.parametrize("shape", [(12, 18), (30, 45), (7, 13), (2, 3), (1080, 1920),
                                    (4320, 30), (6, 16384), (15360, 4), (121, 169),
-                                   (154, 1001)])
+                                   (154, 1001), (81, 675)])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft2d_mixed_matches_numpy(shape, sign):
     u0 = synth.random_field(*shape, seed=70)
