@@ -245,6 +245,8 @@ __device__ __forceinline__ void run64(double2* lds, int LP, int nb, const Plan64
             case 5: stage64<5, INV>(lds, LP, nb, S, tw, tid); break;
             case 7: stage64<7, INV>(lds, LP, nb, S, tw, tid); break;
             case 8: stage64<8, INV>(lds, LP, nb, S, tw, tid); break;
+            case 9: stage64<9, INV>(lds, LP, nb, S, tw, tid); break;
+            case 15: stage64<15, INV>(lds, LP, nb, S, tw, tid); break;
             case 11: stage64<11, INV>(lds, LP, nb, S, tw, tid); break;
             case 13: stage64<13, INV>(lds, LP, nb, S, tw, tid); break;
             default: stage64<16, INV>(lds, LP, nb, S, tw, tid); break;
@@ -458,6 +460,8 @@ bool factor64(int L, int* radix, int* nst) {
     while (n % 16 == 0 && k < ST_MAX) { radix[k++] = 16; n /= 16; }
     for (int r : {8, 4, 2})
         if (n % r == 0 && k < ST_MAX) { radix[k++] = r; n /= r; }
+    while (n % 9 == 0 && k < ST_MAX) { radix[k++] = 9; n /= 9; }
+    if (n % 15 == 0 && k < ST_MAX) { radix[k++] = 15; n /= 15; }
     for (int p : {3, 5, 7, 11, 13})
         while (n % p == 0 && k < ST_MAX) { radix[k++] = p; n /= p; }
     *nst = k;
